@@ -1,0 +1,144 @@
+// gpu.hpp — device-side state of libclothvbd: the cv_gpu handle (stream,
+// scratch, per-solver device cache) and the layout of every device array.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "solver.hpp"
+
+#define CV_HIP(call)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      fail(CV_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));                \
+  } while (0)
+
+// Grow-only device buffer.
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void reserve(size_t count) {
+    if (count <= n) return;
+    if (p) CV_HIP(hipFree(p));
+    p = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    CV_HIP(hipMalloc(&p, bytes));
+    n = count;
+  }
+  void upload(const T* src, size_t count, hipStream_t s) {
+    reserve(count);
+    if (count) CV_HIP(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+};
+
+// Step scalars shared by kernels and read back by the host in one copy.
+struct Scalars {
+  double origin[3];
+  double rq;
+  double padded_radius2;
+  double motion;
+  double min_distance;
+  double residual_sum;
+  double true_residual_sum;
+  double scale;
+  unsigned long long vf_count, ee_count;        // pairs with distance < rq
+  unsigned long long vf_cand, ee_cand;          // pairs passing the padded AABB test
+  unsigned long long grid_tests;                // pairs examined by the grid
+  unsigned long long raw_contacts, contacts, active, incidences;
+  unsigned long long clipped;
+  unsigned long long cand_export_vf, cand_export_ee;
+  unsigned int err_degenerate, err_overflow, err_nonfinite, err_trust;
+};
+
+// Per-solver device cache (the role of Prepared, src/gpu_vbd.rs:478-547):
+// static arrays are uploaded once per solver generation.
+struct DevSolver {
+  uint64_t generation = 0;
+  uint32_t n = 0, n_fixed = 0, n_total = 0, F = 0, E = 0;
+  DBuf<uint32_t> term_off;
+  DBuf<cvm::TermRec> terms;
+  DBuf<double> masses;
+  DBuf<uint8_t> pinned;
+  DBuf<uint32_t> faces, edges, nb_off, nb, ex_off, ex;
+  DBuf<int32_t> opposites;
+  DBuf<uint8_t> face_moving, edge_moving;
+  DBuf<double> rest_len2;
+  DBuf<double> fixed_pos;  // avatar f64
+  double fixed_abs_max = 0;
+  double cell = 0;         // grid cell size for detection
+  // prediction
+  DBuf<double> acc;
+  uint64_t acc_version = 0;
+  bool acc_device = false;
+};
+
+// Device working set of one step (sized to the largest solver seen).
+struct StepBufs {
+  DBuf<double> x64, x_old64, v64, v_old64, guess64, xcur64, radii64, centers64;
+  DBuf<unsigned long long> dist_bits;
+  DBuf<float4> pos, tgt, ball;
+  DBuf<float> gsq;
+  DBuf<uint32_t> order;
+  // detection
+  DBuf<double> fbox, ebox;  // [6F], [6E]
+  DBuf<uint32_t> cell_count, cell_off;
+  DBuf<unsigned long long> cell_key, cell_key_sorted;
+  DBuf<uint32_t> cell_val, cell_val_sorted;
+  DBuf<uint32_t> raw_ids;   // [5*cap] kind, ids
+  DBuf<uint32_t> raw_count;
+  DBuf<double> raw_eps;
+  DBuf<unsigned long long> key_hi, key_lo, key_tmp;
+  DBuf<uint32_t> perm, perm_tmp;
+  DBuf<uint32_t> uniq_flag, uniq_pos;
+  DBuf<uint32_t> c_ids;     // unique contacts [5*cap]
+  DBuf<uint32_t> c_count;
+  DBuf<double> c_eps;
+  DBuf<unsigned long long> inc_key, inc_key_sorted;
+  DBuf<uint32_t> coff, vcount;
+  DBuf<cvm::TermRec> crec;
+  DBuf<uint32_t> cand_vf, cand_ee;  // optional candidate export
+  DBuf<unsigned char> temp;         // rocPRIM temporary storage
+  size_t contact_cap = 0;
+};
+
+struct cv_gpu {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool check_gradients = false;
+  DevSolver dev;
+  StepBufs b;
+  Scalars* d_scal = nullptr;  // device
+  Scalars* h_scal = nullptr;  // pinned host mirror
+  double* h_x = nullptr;      // pinned staging [3n]
+  double* h_v = nullptr;
+  size_t h_cap = 0;
+  std::string last_error;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // host-side contact incidence for the greedy colouring
+  std::vector<uint32_t> h_cids, h_ccount;
+  std::vector<uint32_t> cnb_off, cnb;
+  std::vector<uint32_t> color_of, order, color_off;
+  // Sweep launch shape (lanes per vertex), chosen by the autotuner.
+  int lanes = 32;
+  bool pred_valid = false;
+};
+
+// Detection on device positions b.x64 (all n_total vertices) with query
+// radius rq; fills distance, radii, unique contacts and contact records.
+// Candidate pairs are also exported when cand_cap > 0.
+void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings& ogc,
+                size_t cand_cap);
+void ensure_device_solver(cv_gpu& g, const cv_solver& s);
+void ensure_step_buffers(cv_gpu& g, const cv_solver& s);

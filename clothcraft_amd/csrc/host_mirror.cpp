@@ -1,0 +1,147 @@
+// host_mirror.cpp — host (x86) build of the product's solver set-up and of
+// the device math headers, behind a small C ABI.  The CPU test suite uses it
+// to check, without a GPU, the exact per-term energies/gradients/Hessians the
+// kernels evaluate (vbd_math.hpp, f32 and f64 instantiations), the packed
+// term records, the colouring and the start-state validation.
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "ogc_math.hpp"
+#include "solver.hpp"
+#include "vbd_math.hpp"
+
+namespace {
+template <class R>
+struct Pos {
+  const double* x;
+  cvm::V3<R> operator()(uint32_t i) const { return {(R)x[3 * i], (R)x[3 * i + 1], (R)x[3 * i + 2]}; }
+};
+template <class R>
+void deriv_out(const cvm::Deriv<R>& d, double* out) {
+  out[0] = d.e;
+  out[1] = d.g.x;
+  out[2] = d.g.y;
+  out[3] = d.g.z;
+  for (int k = 0; k < 6; ++k) out[4 + k] = d.h.a[k];
+}
+}  // namespace
+
+extern "C" {
+
+int cvh_solver_create(const cv_garment* g, const cv_surface* avatar, const cv_state* st, cv_solver** out,
+                      char* err, size_t errn) {
+  cv_solver* s = new cv_solver();
+  try {
+    build_solver(*s, g, avatar, st);
+  } catch (const CvError& e) {
+    if (err && errn) std::snprintf(err, errn, "%s", e.message.c_str());
+    delete s;
+    return e.status;
+  }
+  *out = s;
+  return 0;
+}
+void cvh_solver_destroy(cv_solver* s) { delete s; }
+
+uint64_t cvh_solver_terms(const cv_solver* s, uint32_t* term_off, cvm::TermRec* recs, uint64_t cap) {
+  if (term_off) std::memcpy(term_off, s->term_off.data(), s->term_off.size() * sizeof(uint32_t));
+  if (recs && cap >= s->terms.size()) std::memcpy(recs, s->terms.data(), s->terms.size() * sizeof(cvm::TermRec));
+  return s->terms.size();
+}
+uint32_t cvh_solver_edges(const cv_solver* s, uint32_t* out, uint32_t cap) {
+  uint32_t E = (uint32_t)(s->top.edges.size() / 2);
+  if (out && cap >= E) std::memcpy(out, s->top.edges.data(), 2 * E * sizeof(uint32_t));
+  return E;
+}
+
+// Greedy colouring with contact cliques given as (kind, ids[4], count) rows.
+uint32_t cvh_greedy_colors(const cv_solver* s, const uint32_t* kind_ids, const uint32_t* counts,
+                           uint64_t m, const cv_grab* grab, uint32_t* color_of) {
+  uint32_t n = s->n;
+  std::vector<uint32_t> off(n + 1, 0), nb;
+  for (uint64_t c = 0; c < m; ++c)
+    for (uint32_t q = 0; q < counts[c]; ++q) {
+      uint32_t v = kind_ids[5 * c + 1 + q];
+      if (v < n) off[v + 1] += counts[c];
+    }
+  for (uint32_t v = 0; v < n; ++v) off[v + 1] += off[v];
+  nb.resize(off[n]);
+  std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+  for (uint64_t c = 0; c < m; ++c)
+    for (uint32_t q = 0; q < counts[c]; ++q) {
+      uint32_t v = kind_ids[5 * c + 1 + q];
+      if (v < n)
+        for (uint32_t r = 0; r < counts[c]; ++r) nb[cur[v]++] = kind_ids[5 * c + 1 + r];
+    }
+  std::vector<uint32_t> col, order, coff;
+  uint32_t k = greedy_colors(*s, off.data(), nb.data(), grab, col, order, coff);
+  std::memcpy(color_of, col.data(), n * sizeof(uint32_t));
+  return k;
+}
+
+// One term record evaluated for its vertex at xv with the other vertices at x
+// (absolute f64 positions, rounded to R first).  out = e, g[3], H[6].
+void cvh_term_deriv_f64(const cvm::TermRec* r, const double* xv, const double* x, double cr, double ck,
+                        double* out) {
+  Pos<double> P{x};
+  deriv_out(cvm::term_deriv<double>(*r, cvm::V3<double>{xv[0], xv[1], xv[2]}, cr, ck, P), out);
+}
+void cvh_term_deriv_f32(const cvm::TermRec* r, const double* xv, const double* x, double cr, double ck,
+                        double* out) {
+  Pos<float> P{x};
+  deriv_out(cvm::term_deriv<float>(*r, cvm::V3<float>{(float)xv[0], (float)xv[1], (float)xv[2]},
+                                   (float)cr, (float)ck, P),
+            out);
+}
+double cvh_term_energy_f64(const cvm::TermRec* r, const double* xv, const double* x, double cr, double ck) {
+  Pos<double> P{x};
+  return cvm::term_energy<double>(*r, cvm::V3<double>{xv[0], xv[1], xv[2]}, cr, ck, P);
+}
+double cvh_term_energy_f32(const cvm::TermRec* r, const double* xv, const double* x, double cr, double ck) {
+  Pos<float> P{x};
+  return cvm::term_energy<float>(*r, cvm::V3<float>{(float)xv[0], (float)xv[1], (float)xv[2]}, (float)cr,
+                                 (float)ck, P);
+}
+// Contact record builder matching the device (k_contact_records).
+void cvh_contact_record(int kind, const uint32_t* ids, int count, double eps, uint32_t v, cvm::TermRec* out) {
+  cvm::TermRec r{};
+  int slot = 0;
+  for (int q = 0; q < 4; ++q) {
+    r.ids[q] = ids[q];
+    if (q < count && ids[q] == v) slot = q;
+  }
+  r.head = cvm::rec_head(cvm::TERM_CONTACT, slot, count, kind);
+  r.f[0] = (float)eps;
+  *out = r;
+}
+// Newton direction of the local solve (src/vbd.rs:353-371) for a summed
+// derivative in[10] (e, g, H) plus inertia.
+void cvh_newton_direction_f64(const double* in, const double* x, const double* target, double inertia,
+                              double* delta) {
+  cvm::Deriv<double> d;
+  d.e = in[0];
+  d.g = {in[1], in[2], in[3]};
+  for (int k = 0; k < 6; ++k) d.h.a[k] = in[4 + k];
+  cvm::V3<double> r = cvm::newton_direction<double>(d, {x[0], x[1], x[2]}, {target[0], target[1], target[2]},
+                                                    inertia);
+  delta[0] = r.x;
+  delta[1] = r.y;
+  delta[2] = r.z;
+}
+// f64 narrow-phase pieces (ogc_math.hpp) for comparison with the oracle.
+void cvh_closest_triangle(const double* p, const double* t9, double* q, int* feature, int* local) {
+  ogm::D3 Q;
+  ogm::closest_triangle({p[0], p[1], p[2]}, {t9[0], t9[1], t9[2]}, {t9[3], t9[4], t9[5]},
+                        {t9[6], t9[7], t9[8]}, Q, *feature, *local);
+  q[0] = Q.x;
+  q[1] = Q.y;
+  q[2] = Q.z;
+}
+void cvh_closest_segments(const double* a, const double* b, const double* c, const double* d, double* s,
+                          double* t) {
+  ogm::closest_segments({a[0], a[1], a[2]}, {b[0], b[1], b[2]}, {c[0], c[1], c[2]}, {d[0], d[1], d[2]}, *s,
+                        *t);
+}
+
+}  // extern "C"

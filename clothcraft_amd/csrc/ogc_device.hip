@@ -1,0 +1,598 @@
+// ogc_device.hip — Offset Geometric Contact detection on the device
+// (OgcWorld::detect, src/ogc/world.rs:308-420, moved off the host).
+//
+// f64 throughout and compiled with -ffp-contract=off: the broad phase tests
+// the reference's own predicate `Bounds::distance_squared <= padded_radius^2`
+// (src/ogc/world.rs:352-358, 462-472, 499-509) on f64 boxes, so the set of
+// candidate pairs equals the f64 restatement's bit for bit, and the narrow
+// phase (closest features, feasible regions, canonical ids) rounds exactly as
+// the reference does.
+//
+// Broad phase: a uniform grid.  Every triangle (edge) is registered in the
+// cells its box inflated by the padded radius covers; a point looks only in
+// its own cell; an edge scans the cells of its own box and keeps a pair only
+// in the cell holding the lower corner of the box intersection, so each pair
+// is examined once.  Pairs are emitted with atomics and then put in the
+// reference's canonical (kind, ids) order by a two-key radix sort, so results
+// are deterministic.
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "gpu.hpp"
+#include "ogc_math.hpp"
+
+using ogm::D3;
+
+namespace {
+
+constexpr int TPB = 256;
+constexpr long long kMaxCells = 1ll << 22;  // grid cells one primitive may cover
+
+__device__ __forceinline__ D3 ld3(const double* x, uint32_t i) {
+  return D3{x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+}
+__device__ __forceinline__ ogm::Box ldbox(const double* b, uint32_t i) {
+  return ogm::Box{{b[6 * i], b[6 * i + 1], b[6 * i + 2]}, {b[6 * i + 3], b[6 * i + 4], b[6 * i + 5]}};
+}
+__device__ __forceinline__ void stbox(double* b, uint32_t i, const ogm::Box& v) {
+  b[6 * i] = v.lo.x;
+  b[6 * i + 1] = v.lo.y;
+  b[6 * i + 2] = v.lo.z;
+  b[6 * i + 3] = v.hi.x;
+  b[6 * i + 4] = v.hi.y;
+  b[6 * i + 5] = v.hi.z;
+}
+__device__ __forceinline__ long long cell_of(double v, double h) { return (long long)floor(v / h); }
+__device__ __forceinline__ unsigned long long cell_key(long long i, long long j, long long k) {
+  return ((unsigned long long)(i & 0x1FFFFF) << 42) | ((unsigned long long)(j & 0x1FFFFF) << 21) |
+         (unsigned long long)(k & 0x1FFFFF);
+}
+__device__ __forceinline__ ogm::Box inflate(const ogm::Box& b, double r) {
+  return {{b.lo.x - r, b.lo.y - r, b.lo.z - r}, {b.hi.x + r, b.hi.y + r, b.hi.z + r}};
+}
+
+// Primitive boxes of faces and edges; degenerate moving faces flag an error
+// (src/ogc/world.rs:225-233).
+__global__ void k_boxes(const double* x, const uint32_t* faces, const uint8_t* face_moving,
+                        uint32_t F, const uint32_t* edges, uint32_t E, double* fbox, double* ebox,
+                        Scalars* sc) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < F) {
+    D3 a = ld3(x, faces[3 * i]), b = ld3(x, faces[3 * i + 1]), c = ld3(x, faces[3 * i + 2]);
+    stbox(fbox, i, ogm::box_of(a, b, c));
+    if (face_moving[i] && ogm::len2(ogm::cross(ogm::sub(b, a), ogm::sub(c, a))) < 1e-24)
+      atomicOr(&sc->err_degenerate, 1u);
+  } else if (i < F + E) {
+    uint32_t e = i - F;
+    stbox(ebox, e, ogm::box_of(ld3(x, edges[2 * e]), ld3(x, edges[2 * e + 1])));
+  }
+}
+
+__device__ __forceinline__ void cell_range(const ogm::Box& b, double h, long long r[6]) {
+  r[0] = cell_of(b.lo.x, h);
+  r[1] = cell_of(b.lo.y, h);
+  r[2] = cell_of(b.lo.z, h);
+  r[3] = cell_of(b.hi.x, h);
+  r[4] = cell_of(b.hi.y, h);
+  r[5] = cell_of(b.hi.z, h);
+}
+
+// Cells covered by each primitive box inflated by pr.
+__global__ void k_cell_count(const double* box, uint32_t N, double pr, double h, uint32_t* count) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  long long r[6];
+  cell_range(inflate(ldbox(box, i), pr), h, r);
+  long long c = (r[3] - r[0] + 1) * (r[4] - r[1] + 1) * (r[5] - r[2] + 1);
+  count[i] = (uint32_t)(c > kMaxCells ? 1 : c);  // over the cap: one sentinel entry, see k_cell_fill
+}
+__global__ void k_cell_fill(const double* box, uint32_t N, double pr, double h, const uint32_t* off,
+                            const uint32_t* count, unsigned long long* keys, uint32_t* vals,
+                            Scalars* sc) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  long long r[6];
+  cell_range(inflate(ldbox(box, i), pr), h, r);
+  long long c = (r[3] - r[0] + 1) * (r[4] - r[1] + 1) * (r[5] - r[2] + 1);
+  if (c > kMaxCells) {  // over the per-primitive cap: report, never drop silently
+    keys[off[i]] = ~0ull;  // sentinel key never matches a cell (cell keys use 63 bits)
+    vals[off[i]] = 0;
+    atomicOr(&sc->err_overflow, 2u);
+    return;
+  }
+  uint32_t o = off[i];
+  for (long long a = r[0]; a <= r[3]; ++a)
+    for (long long b = r[1]; b <= r[4]; ++b)
+      for (long long z = r[2]; z <= r[5]; ++z) {
+        keys[o] = cell_key(a, b, z);
+        vals[o] = i;
+        ++o;
+      }
+}
+
+__device__ __forceinline__ uint32_t lower_bound_key(const unsigned long long* keys, uint32_t n,
+                                                    unsigned long long k) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+struct DetectArgs {
+  const double* x;
+  uint32_t n, n_total, F, E;
+  const uint32_t* faces;
+  const uint32_t* edges;
+  const uint32_t* tri_edges;
+  const int32_t* opp;
+  const uint32_t* nb_off;
+  const uint32_t* nb;
+  const uint32_t* ex_off;
+  const uint32_t* ex;
+  const uint8_t* face_moving;
+  const uint8_t* edge_moving;
+  const double* rest_len2;
+  const double* fbox;
+  const double* ebox;
+  double rq, pr, pr2, h;
+  unsigned long long* dist_bits;
+  uint32_t* raw_ids;
+  uint32_t* raw_count;
+  double* raw_eps;
+  uint64_t cap;
+  uint32_t* cand_vf;
+  uint32_t* cand_ee;
+  uint64_t cand_cap;
+  Scalars* sc;
+};
+
+__device__ __forceinline__ bool is_excluded(const DetectArgs& a, uint32_t u, uint32_t w) {
+  uint32_t lo = a.ex_off[u], hi = a.ex_off[u + 1];
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    uint32_t v = a.ex[mid];
+    if (v == w) return true;
+    if (v < w)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return false;
+}
+__device__ __forceinline__ bool adjacent(const DetectArgs& a, uint32_t u, uint32_t w) {
+  return u == w || (a.ex_off[u] != a.ex_off[u + 1] && is_excluded(a, u, w));
+}
+__device__ __forceinline__ void dmin(const DetectArgs& a, uint32_t v, double d) {
+  atomicMin(&a.dist_bits[v], (unsigned long long)__double_as_longlong(d));
+}
+__device__ __forceinline__ void add_contact(const DetectArgs& a, int kind, uint32_t i0, uint32_t i1,
+                                            uint32_t i2, uint32_t i3, int count, double eps) {
+  uint32_t ids[4] = {i0, i1, i2, i3};
+  ogm::canonical(kind, ids);
+  unsigned long long slot = atomicAdd(&a.sc->raw_contacts, 1ull);
+  if (slot >= a.cap) {
+    atomicOr(&a.sc->err_overflow, 1u);
+    return;
+  }
+  a.raw_ids[5 * slot] = (uint32_t)kind;
+  for (int k = 0; k < 4; ++k) a.raw_ids[5 * slot + 1 + k] = ids[k];
+  a.raw_count[slot] = (uint32_t)count;
+  a.raw_eps[slot] = eps;
+}
+
+// VF: one thread per point (src/ogc/world.rs:461-497).
+__global__ void k_vf(DetectArgs a, const unsigned long long* keys, const uint32_t* vals,
+                     uint32_t nkeys) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= a.n_total) return;
+  bool v_moving = v < a.n;
+  D3 p = ld3(a.x, v);
+  ogm::Box pb = ogm::box_of(p);
+  unsigned long long key = cell_key(cell_of(p.x, a.h), cell_of(p.y, a.h), cell_of(p.z, a.h));
+  uint32_t it = lower_bound_key(keys, nkeys, key);
+  unsigned long long tests = 0;
+  for (; it < nkeys && keys[it] == key; ++it) {
+    uint32_t t = vals[it];
+    if (!v_moving && !a.face_moving[t]) continue;
+    uint32_t t0 = a.faces[3 * t], t1 = a.faces[3 * t + 1], t2 = a.faces[3 * t + 2];
+    if (adjacent(a, v, t0) || adjacent(a, v, t1) || adjacent(a, v, t2)) continue;
+    ++tests;
+    if (ogm::box_dist2(pb, ldbox(a.fbox, t)) > a.pr2) continue;
+    atomicAdd(&a.sc->vf_cand, 1ull);
+    if (a.cand_vf) {
+      unsigned long long s = atomicAdd(&a.sc->cand_export_vf, 1ull);
+      if (s < a.cand_cap) {
+        a.cand_vf[2 * s] = v;
+        a.cand_vf[2 * s + 1] = t;
+      }
+    }
+    D3 q;
+    int feat, loc;
+    ogm::closest_triangle(p, ld3(a.x, t0), ld3(a.x, t1), ld3(a.x, t2), q, feat, loc);
+    double d = ogm::dist(p, q);
+    if (d >= a.rq) continue;
+    atomicAdd(&a.sc->vf_count, 1ull);
+    dmin(a, v, d);
+    dmin(a, t0, d);
+    dmin(a, t1, d);
+    dmin(a, t2, d);
+    uint32_t tv[3] = {t0, t1, t2};
+    if (feat == ogm::F_VERTEX) {
+      uint32_t w = tv[loc];
+      if (ogm::vertex_feasible(p, (int)w, (const D3*)a.x, a.nb_off, a.nb))
+        add_contact(a, 0, v, w, 0, 0, 2, 0.0);
+    } else if (feat == ogm::F_EDGE) {
+      uint32_t e = a.tri_edges[3 * t + loc];
+      uint32_t ea = a.edges[2 * e], eb = a.edges[2 * e + 1];
+      if (ogm::edge_feasible(p, ld3(a.x, ea), ld3(a.x, eb), &a.opp[2 * e], (const D3*)a.x))
+        add_contact(a, 1, v, ea, eb, 0, 3, 0.0);
+    } else {
+      add_contact(a, 2, v, t0, t1, t2, 4, 0.0);
+    }
+  }
+  atomicAdd(&a.sc->grid_tests, tests);
+}
+
+// EE: one thread per moving edge (src/ogc/world.rs:498-555).  The moving edge
+// is the first operand; moving/moving pairs are taken once with e < f.
+__global__ void k_ee(DetectArgs a, const unsigned long long* keys, const uint32_t* vals,
+                     uint32_t nkeys) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.E || !a.edge_moving[e]) return;
+  ogm::Box be = ldbox(a.ebox, e);
+  long long r[6];
+  cell_range(be, a.h, r);
+  uint32_t ia = a.edges[2 * e], ib = a.edges[2 * e + 1];
+  unsigned long long tests = 0;
+  for (long long ci = r[0]; ci <= r[3]; ++ci)
+    for (long long cj = r[1]; cj <= r[4]; ++cj)
+      for (long long ck = r[2]; ck <= r[5]; ++ck) {
+        unsigned long long key = cell_key(ci, cj, ck);
+        uint32_t it = lower_bound_key(keys, nkeys, key);
+        for (; it < nkeys && keys[it] == key; ++it) {
+          uint32_t f = vals[it];
+          if (f == e) continue;
+          bool fm = a.edge_moving[f];
+          if (fm && f < e) continue;
+          ogm::Box bf = inflate(ldbox(a.ebox, f), a.pr);
+          // Only in the cell holding the lower corner of box(e) ∩ inflated box(f).
+          double lx = fmax(be.lo.x, bf.lo.x), ly = fmax(be.lo.y, bf.lo.y), lz = fmax(be.lo.z, bf.lo.z);
+          if (lx > fmin(be.hi.x, bf.hi.x) || ly > fmin(be.hi.y, bf.hi.y) || lz > fmin(be.hi.z, bf.hi.z))
+            continue;
+          if (cell_of(lx, a.h) != ci || cell_of(ly, a.h) != cj || cell_of(lz, a.h) != ck) continue;
+          uint32_t jc = a.edges[2 * f], jd = a.edges[2 * f + 1];
+          if (adjacent(a, ia, jc) || adjacent(a, ia, jd) || adjacent(a, ib, jc) || adjacent(a, ib, jd))
+            continue;
+          ++tests;
+          if (ogm::box_dist2(be, ldbox(a.ebox, f)) > a.pr2) continue;
+          atomicAdd(&a.sc->ee_cand, 1ull);
+          if (a.cand_ee) {
+            unsigned long long s = atomicAdd(&a.sc->cand_export_ee, 1ull);
+            if (s < a.cand_cap) {
+              a.cand_ee[2 * s] = e;
+              a.cand_ee[2 * s + 1] = f;
+            }
+          }
+          D3 A = ld3(a.x, ia), B = ld3(a.x, ib), C = ld3(a.x, jc), Dd = ld3(a.x, jd);
+          double s, t;
+          ogm::closest_segments(A, B, C, Dd, s, t);
+          D3 P = ogm::add(A, ogm::scale(ogm::sub(B, A), s));
+          D3 Q = ogm::add(C, ogm::scale(ogm::sub(Dd, C), t));
+          double dist = ogm::dist(P, Q);
+          if (dist >= a.rq) continue;
+          atomicAdd(&a.sc->ee_count, 1ull);
+          dmin(a, ia, dist);
+          dmin(a, ib, dist);
+          dmin(a, jc, dist);
+          dmin(a, jd, dist);
+          int end_a = s <= 0.0 ? (int)ia : (s >= 1.0 ? (int)ib : -1);
+          int end_b = t <= 0.0 ? (int)jc : (t >= 1.0 ? (int)jd : -1);
+          if ((end_a >= 0 && !ogm::vertex_feasible(Q, end_a, (const D3*)a.x, a.nb_off, a.nb)) ||
+              (end_b >= 0 && !ogm::vertex_feasible(P, end_b, (const D3*)a.x, a.nb_off, a.nb)))
+            continue;
+          if (end_a >= 0 && end_b >= 0)
+            add_contact(a, 0, end_a, end_b, 0, 0, 2, 0.0);
+          else if (end_a >= 0)
+            add_contact(a, 1, end_a, jc, jd, 0, 3, 0.0);
+          else if (end_b >= 0)
+            add_contact(a, 1, end_b, ia, ib, 0, 3, 0.0);
+          else
+            add_contact(a, 3, ia, ib, jc, jd, 4, 1e-3 * a.rest_len2[e] * a.rest_len2[f]);
+        }
+      }
+  atomicAdd(&a.sc->grid_tests, tests);
+}
+
+__global__ void k_init_dist(unsigned long long* bits, uint32_t n, double rq) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) bits[i] = (unsigned long long)__double_as_longlong(rq);
+}
+
+__global__ void k_contact_keys(const uint32_t* raw, uint64_t m, unsigned long long* hi,
+                               unsigned long long* lo, uint32_t* perm) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t* r = &raw[5 * i];
+  hi[i] = ((unsigned long long)r[0] << 62) | ((unsigned long long)r[1] << 31) | (unsigned long long)r[2];
+  lo[i] = ((unsigned long long)r[3] << 32) | (unsigned long long)r[4];
+  perm[i] = (uint32_t)i;
+}
+__global__ void k_gather_u64(const unsigned long long* src, const uint32_t* perm, uint64_t m,
+                             unsigned long long* dst) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i < m) dst[i] = src[perm[i]];
+}
+// Flag first occurrences of each (kind, ids) key in sorted order.
+__global__ void k_unique_flags(const unsigned long long* hi_sorted, const unsigned long long* lo,
+                               const uint32_t* perm, uint64_t m, uint32_t* flag) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  flag[i] = (i == 0 || hi_sorted[i] != hi_sorted[i - 1] || lo[perm[i]] != lo[perm[i - 1]]) ? 1u : 0u;
+}
+__global__ void k_unique_write(const uint32_t* raw, const uint32_t* raw_count, const double* raw_eps,
+                               const uint32_t* perm, const uint32_t* flag, const uint32_t* pos,
+                               uint64_t m, uint32_t* cids, uint32_t* ccount, double* ceps) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= m || !flag[i]) return;
+  uint32_t src = perm[i], dst = pos[i];
+  for (int k = 0; k < 5; ++k) cids[5 * dst + k] = raw[5 * src + k];
+  ccount[dst] = raw_count[src];
+  ceps[dst] = raw_eps[src];
+}
+// Active contacts, incidence counts and incidence keys (v << 32 | contact).
+__global__ void k_contact_post(const double* x, const uint32_t* cids, const uint32_t* ccount,
+                               const double* ceps, uint64_t m, uint32_t n, double r, double k,
+                               uint32_t* vcount, unsigned long long* inc_key, Scalars* sc) {
+  uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  int kind = (int)cids[5 * c];
+  D3 p[4];
+  for (int q = 0; q < 4; ++q) p[q] = ld3(x, cids[5 * c + 1 + q]);
+  if (ogm::contact_value(kind, p, ceps[c], r, k) > 0.0) atomicAdd(&sc->active, 1ull);
+  for (uint32_t q = 0; q < ccount[c]; ++q) {
+    uint32_t v = cids[5 * c + 1 + q];
+    if (v < n) {
+      atomicAdd(&vcount[v], 1u);
+      unsigned long long s = atomicAdd(&sc->incidences, 1ull);
+      inc_key[s] = ((unsigned long long)v << 32) | (unsigned long long)c;
+    }
+  }
+}
+// Per-vertex contact term records in sorted incidence order.
+__global__ void k_contact_records(const unsigned long long* inc_sorted, uint64_t m, const uint32_t* cids,
+                                  const uint32_t* ccount, const double* ceps, cvm::TermRec* rec) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  uint32_t v = (uint32_t)(inc_sorted[i] >> 32), c = (uint32_t)(inc_sorted[i] & 0xFFFFFFFFull);
+  cvm::TermRec r;
+  int cnt = (int)ccount[c], slot = 0;
+  for (int q = 0; q < 4; ++q) {
+    r.ids[q] = cids[5 * c + 1 + q];
+    if (q < cnt && r.ids[q] == v) slot = q;
+  }
+  r.head = cvm::rec_head(cvm::TERM_CONTACT, slot, cnt, (int)cids[5 * c]);
+  r.f[0] = (float)ceps[c];
+  for (int q = 1; q < 11; ++q) r.f[q] = 0.0f;
+  rec[i] = r;
+}
+// Radii and minimum distance over moving vertices (single block, fixed order).
+__global__ void k_radii(const unsigned long long* bits, uint32_t n, double relaxation, double rq,
+                        double* radii, Scalars* sc) {
+  __shared__ double red[1024];
+  double m = rq;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    double d = __longlong_as_double((long long)bits[i]);
+    radii[i] = relaxation * d;
+    m = fmin(m, d);
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sc->min_distance = red[0];
+}
+
+inline unsigned blocks(uint64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+template <class F>
+void with_temp(cv_gpu& g, F&& f) {
+  size_t bytes = 0;
+  CV_HIP(f(nullptr, bytes));
+  g.b.temp.reserve(bytes + 256);
+  bytes = g.b.temp.n;
+  CV_HIP(f((void*)g.b.temp.p, bytes));
+}
+
+void read_scalars(cv_gpu& g) {
+  CV_HIP(hipMemcpyAsync(g.h_scal, g.d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, g.stream));
+  CV_HIP(hipStreamSynchronize(g.stream));
+}
+
+// Register primitive boxes in grid cells and sort by cell key.  Returns the
+// number of (cell, primitive) entries.
+uint32_t build_grid(cv_gpu& g, const double* box, uint32_t N, double pr, double h) {
+  StepBufs& b = g.b;
+  hipStream_t st = g.stream;
+  b.cell_count.reserve(N + 1);
+  b.cell_off.reserve(N + 1);
+  if (N == 0) return 0;
+  k_cell_count<<<blocks(N), TPB, 0, st>>>(box, N, pr, h, b.cell_count.p);
+  CV_HIP(hipGetLastError());
+  CV_HIP(hipMemsetAsync(b.cell_count.p + N, 0, sizeof(uint32_t), st));
+  with_temp(g, [&](void* t, size_t& bytes) {
+    return rocprim::exclusive_scan(t, bytes, b.cell_count.p, b.cell_off.p, 0u, (size_t)N + 1,
+                                   rocprim::plus<uint32_t>(), st);
+  });
+  uint32_t total = 0;
+  CV_HIP(hipMemcpyAsync(&total, b.cell_off.p + N, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  CV_HIP(hipStreamSynchronize(st));
+  if (total > (1u << 31)) fail(CV_ERR_CAPACITY, "OGC grid registration exceeds 2^31 cells");
+  b.cell_key.reserve(total);
+  b.cell_val.reserve(total);
+  b.cell_key_sorted.reserve(total);
+  b.cell_val_sorted.reserve(total);
+  k_cell_fill<<<blocks(N), TPB, 0, st>>>(box, N, pr, h, b.cell_off.p, b.cell_count.p, b.cell_key.p,
+                                         b.cell_val.p, g.d_scal);
+  CV_HIP(hipGetLastError());
+  with_temp(g, [&](void* t, size_t& bytes) {
+    return rocprim::radix_sort_pairs(t, bytes, b.cell_key.p, b.cell_key_sorted.p, b.cell_val.p,
+                                     b.cell_val_sorted.p, (size_t)total, 0, 64, st);
+  });
+  return total;
+}
+
+}  // namespace
+
+void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings& ogc, size_t cand_cap) {
+  DevSolver& d = g.dev;
+  StepBufs& b = g.b;
+  hipStream_t st = g.stream;
+  // Padded radius (src/ogc/world.rs:352-358): scale = max(rq, max |x|_inf).
+  // The step prologue stored max |x| of the moving part in sc->scale.
+  double scale = std::max(std::max(rq, g.h_scal->scale), d.fixed_abs_max);
+  double pr = rq + 64.0 * 2.220446049250313e-16 * scale;
+  double pr2 = pr * pr;
+  // Grid registration inflates by slightly more than pr so rounding of
+  // box +- pr can never drop a pair the exact predicate accepts.
+  double pr_reg = pr * (1.0 + 1e-9) + 1e-12;
+  double h = std::max(d.cell, 2.0 * pr_reg);
+  // Reset per-detection counters.
+  Scalars& hs = *g.h_scal;
+  hs.vf_count = hs.ee_count = hs.vf_cand = hs.ee_cand = hs.grid_tests = 0;
+  hs.raw_contacts = hs.contacts = hs.active = hs.incidences = 0;
+  hs.cand_export_vf = hs.cand_export_ee = 0;
+  hs.err_degenerate = hs.err_overflow = 0;
+  hs.rq = rq;
+  hs.padded_radius2 = pr2;
+  CV_HIP(hipMemcpyAsync(g.d_scal, g.h_scal, sizeof(Scalars), hipMemcpyHostToDevice, st));
+  b.fbox.reserve(6 * (size_t)d.F);
+  b.ebox.reserve(6 * (size_t)d.E);
+  k_boxes<<<blocks(d.F + d.E), TPB, 0, st>>>(b.x64.p, d.faces.p, d.face_moving.p, d.F, d.edges.p, d.E,
+                                             b.fbox.p, b.ebox.p, g.d_scal);
+  CV_HIP(hipGetLastError());
+  b.dist_bits.reserve(d.n_total);
+  k_init_dist<<<blocks(d.n_total), TPB, 0, st>>>(b.dist_bits.p, d.n_total, rq);
+  if (cand_cap) {
+    b.cand_vf.reserve(2 * cand_cap);
+    b.cand_ee.reserve(2 * cand_cap);
+  }
+  DetectArgs a{};
+  a.x = b.x64.p;
+  a.n = d.n;
+  a.n_total = d.n_total;
+  a.F = d.F;
+  a.E = d.E;
+  a.faces = d.faces.p;
+  a.edges = d.edges.p;
+  a.tri_edges = d.faces.p + 3 * (size_t)d.F;  // packed after faces (see ensure_device_solver)
+  a.opp = d.opposites.p;
+  a.nb_off = d.nb_off.p;
+  a.nb = d.nb.p;
+  a.ex_off = d.ex_off.p;
+  a.ex = d.ex.p;
+  a.face_moving = d.face_moving.p;
+  a.edge_moving = d.edge_moving.p;
+  a.rest_len2 = d.rest_len2.p;
+  a.fbox = b.fbox.p;
+  a.ebox = b.ebox.p;
+  a.rq = rq;
+  a.pr = pr_reg;
+  a.pr2 = pr2;
+  a.h = h;
+  a.dist_bits = b.dist_bits.p;
+  a.raw_ids = b.raw_ids.p;
+  a.raw_count = b.raw_count.p;
+  a.raw_eps = b.raw_eps.p;
+  a.cap = b.contact_cap;
+  a.cand_vf = cand_cap ? b.cand_vf.p : nullptr;
+  a.cand_ee = cand_cap ? b.cand_ee.p : nullptr;
+  a.cand_cap = cand_cap;
+  a.sc = g.d_scal;
+  // VF: faces registered with the padded inflation, points look up their cell.
+  uint32_t nk = build_grid(g, b.fbox.p, d.F, pr_reg, h);
+  k_vf<<<blocks(d.n_total), TPB, 0, st>>>(a, b.cell_key_sorted.p, b.cell_val_sorted.p, nk);
+  CV_HIP(hipGetLastError());
+  // EE: edges registered with the padded inflation.
+  nk = build_grid(g, b.ebox.p, d.E, pr_reg, h);
+  k_ee<<<blocks(d.E), TPB, 0, st>>>(a, b.cell_key_sorted.p, b.cell_val_sorted.p, nk);
+  CV_HIP(hipGetLastError());
+  read_scalars(g);
+  if (g.h_scal->err_overflow)
+    fail(CV_ERR_CAPACITY, "OGC contact buffer or grid capacity exceeded");
+  uint64_t m = g.h_scal->raw_contacts;
+  // Canonical order and dedup (src/ogc/world.rs:377-381).
+  b.key_hi.reserve(m);
+  b.key_lo.reserve(m);
+  b.key_tmp.reserve(m);
+  b.perm.reserve(m);
+  b.perm_tmp.reserve(m);
+  b.uniq_flag.reserve(m + 1);
+  b.uniq_pos.reserve(m + 1);
+  uint64_t unique = 0;
+  if (m) {
+    k_contact_keys<<<blocks(m), TPB, 0, st>>>(b.raw_ids.p, m, b.key_hi.p, b.key_lo.p, b.perm.p);
+    with_temp(g, [&](void* t, size_t& bytes) {  // by low key
+      return rocprim::radix_sort_pairs(t, bytes, b.key_lo.p, b.key_tmp.p, b.perm.p, b.perm_tmp.p,
+                                       (size_t)m, 0, 64, st);
+    });
+    k_gather_u64<<<blocks(m), TPB, 0, st>>>(b.key_hi.p, b.perm_tmp.p, m, b.key_tmp.p);
+    with_temp(g, [&](void* t, size_t& bytes) {  // stable by high key
+      return rocprim::radix_sort_pairs(t, bytes, b.key_tmp.p, b.cell_key.p /*scratch*/, b.perm_tmp.p,
+                                       b.perm.p, (size_t)m, 0, 64, st);
+    });
+    // b.cell_key now holds hi keys in final order; perm the permutation.
+    k_unique_flags<<<blocks(m), TPB, 0, st>>>(b.cell_key.p, b.key_lo.p, b.perm.p, m, b.uniq_flag.p);
+    CV_HIP(hipMemsetAsync(b.uniq_flag.p + m, 0, sizeof(uint32_t), st));
+    with_temp(g, [&](void* t, size_t& bytes) {
+      return rocprim::exclusive_scan(t, bytes, b.uniq_flag.p, b.uniq_pos.p, 0u, (size_t)m + 1,
+                                     rocprim::plus<uint32_t>(), st);
+    });
+    uint32_t u32 = 0;
+    CV_HIP(hipMemcpyAsync(&u32, b.uniq_pos.p + m, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    CV_HIP(hipStreamSynchronize(st));
+    unique = u32;
+    k_unique_write<<<blocks(m), TPB, 0, st>>>(b.raw_ids.p, b.raw_count.p, b.raw_eps.p, b.perm.p,
+                                              b.uniq_flag.p, b.uniq_pos.p, m, b.c_ids.p, b.c_count.p,
+                                              b.c_eps.p);
+  }
+  g.h_scal->contacts = unique;
+  // Incidence, active count, per-vertex contact records.
+  b.vcount.reserve(d.n + 1);
+  b.coff.reserve(d.n + 1);
+  CV_HIP(hipMemsetAsync(b.vcount.p, 0, sizeof(uint32_t) * (d.n + 1), st));
+  if (unique) {
+    b.inc_key.reserve(4 * unique);
+    b.inc_key_sorted.reserve(4 * unique);
+    k_contact_post<<<blocks(unique), TPB, 0, st>>>(b.x64.p, b.c_ids.p, b.c_count.p, b.c_eps.p, unique,
+                                                   d.n, ogc.radius, ogc.stiffness, b.vcount.p,
+                                                   b.inc_key.p, g.d_scal);
+  }
+  with_temp(g, [&](void* t, size_t& bytes) {
+    return rocprim::exclusive_scan(t, bytes, b.vcount.p, b.coff.p, 0u, (size_t)d.n + 1,
+                                   rocprim::plus<uint32_t>(), st);
+  });
+  b.radii64.reserve(d.n);
+  k_radii<<<1, 1024, 0, st>>>(b.dist_bits.p, d.n, ogc.relaxation, rq, b.radii64.p, g.d_scal);
+  read_scalars(g);
+  g.h_scal->contacts = unique;
+  uint64_t ninc = g.h_scal->incidences;
+  b.crec.reserve(std::max<uint64_t>(ninc, 1));
+  if (ninc) {
+    with_temp(g, [&](void* t, size_t& bytes) {
+      return rocprim::radix_sort_keys(t, bytes, b.inc_key.p, b.inc_key_sorted.p, (size_t)ninc, 0, 64, st);
+    });
+    k_contact_records<<<blocks(ninc), TPB, 0, st>>>(b.inc_key_sorted.p, ninc, b.c_ids.p, b.c_count.p,
+                                                    b.c_eps.p, b.crec.p);
+    CV_HIP(hipGetLastError());
+  }
+  if (g.h_scal->err_degenerate) fail(CV_ERR_DEGENERATE_FACE, "OGC degenerate face; step rejected");
+}

@@ -1,0 +1,92 @@
+// solver.hpp — host-side VbdSolver state of libclothvbd (the handle behind
+// cv_solver).  Mirrors what VbdSolver::new builds (src/vbd.rs:101-192): the
+// OGC topology over garment + avatar faces, seam exclusions, per-vertex term
+// incidence and the symmetric conflict graph; plus the packed per-vertex
+// term records the device sweep reads.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/clothvbd.h"
+#include "vbd_math.hpp"
+
+struct CvError {
+  cv_status status;
+  std::string message;
+};
+
+struct OgcTopology {  // src/ogc/topology.rs:13-52 over all faces (garment, then avatar)
+  uint32_t n_total = 0;                 // garment + avatar vertices
+  uint32_t moving = 0;                  // garment vertex count
+  std::vector<uint32_t> faces;          // [3F]
+  std::vector<uint32_t> edges;          // [2E] (min, max)
+  std::vector<uint32_t> tri_edges;      // [3F]
+  std::vector<int32_t> opposites;       // [2E], -1 when absent
+  std::vector<uint32_t> nb_off, nb;     // neighbour CSR (sorted)
+  std::vector<uint32_t> ex_off, ex;     // seam-excluded neighbour CSR (sorted)
+  std::vector<double> rest_len2;        // [E]
+  uint32_t moving_faces = 0;            // faces touching a garment vertex come first? (count only)
+  std::vector<uint8_t> face_moving;     // [F]
+  std::vector<uint8_t> edge_moving;     // [E]
+};
+
+struct cv_solver {
+  uint32_t n = 0;
+  std::vector<double> masses;
+  std::vector<uint8_t> pinned;
+  std::vector<double> pin_targets;
+  std::vector<uint32_t> tri_v;          // [3T]
+  std::vector<double> tri_inv_dm;       // [4T]
+  std::vector<double> tri_area;
+  std::vector<uint32_t> tri_mat;
+  std::vector<uint32_t> hinge_v;        // [4H]
+  std::vector<double> hinge_weight, hinge_across;
+  std::vector<uint32_t> hinge_mat;
+  std::vector<cv_material> materials;
+  std::vector<uint32_t> st_off, st_v;   // stitch samples
+  std::vector<double> st_w, st_k;
+  std::vector<uint32_t> pr_off, pr_v;   // seam probes
+  std::vector<double> pr_w;
+  std::vector<double> avatar;           // [3*n_fixed] fixed positions
+  OgcTopology top;
+  std::vector<uint32_t> graph_off, graph;  // static conflict graph (sorted)
+  std::vector<uint32_t> term_off;          // [n+1] static TermRec per vertex
+  std::vector<cvm::TermRec> terms;
+  uint64_t generation = 0;                 // changes when a new solver is built
+  double face_extent_median = 0;           // grid cell hint for device detection
+
+  // Prediction history (src/vbd/prediction.rs:8-13); the acceleration lives
+  // where it was last computed: `acc` on the host after record_prediction,
+  // the device cache after a device step (acc_on_device).
+  bool has_last = false;
+  std::vector<double> last_x, last_v;
+  double last_time = 0;
+  double last_gravity[3] = {0, 0, 0};
+  double last_damping = 0;
+  std::vector<double> acc;
+  uint64_t acc_version = 0;
+
+  std::string last_error;
+};
+
+// Builders (solver.cpp).
+void build_solver(cv_solver& s, const cv_garment* g, const cv_surface* avatar, const cv_state* st);
+void validate_settings(const cv_settings* s);
+void validate_state(const cv_solver& s, const cv_state* st);
+void validate_grab(const cv_solver& s, const cv_grab* g);
+bool prediction_valid(const cv_solver& s, const cv_state* st, const cv_settings* set);
+
+// Greedy colouring in vertex order (src/vbd/coloring.rs:5-43).  `contact_*`
+// is the per-vertex contact incidence: for moving vertex v, the vertices of
+// each incident contact.  Returns colour of each vertex and the colour count.
+uint32_t greedy_colors(const cv_solver& s, const uint32_t* cnb_off, const uint32_t* cnb,
+                       const cv_grab* grab, std::vector<uint32_t>& color_of,
+                       std::vector<uint32_t>& order, std::vector<uint32_t>& color_off);
+
+// Garment diagnostics (src/garment/diagnostics.rs:7-28, template.rs:536-538).
+void garment_diagnostics(const cv_solver& s, const double* x, double& seam_gap, double& p95,
+                         double& pmax);
+
+[[noreturn]] void fail(cv_status st, const std::string& msg);

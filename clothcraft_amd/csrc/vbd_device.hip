@@ -1,0 +1,946 @@
+// vbd_device.hip — the device VBD step (GpuVbd::step, src/gpu_vbd.rs:125-472,
+// with the semantics of VbdSolver::step, src/vbd.rs:250-422).
+//
+// Per-colour Gauss-Seidel sweep: a group of L lanes serves one vertex.  Each
+// lane evaluates the exact energy/gradient/Hessian of its incident terms
+// (64-byte records stored contiguously per vertex, so a group reads one
+// coalesced run), the group reduces with butterfly shuffles (every lane ends
+// with the identical sum), every lane forms the same Gershgorin-shifted
+// Newton step, and the ball-truncated backtracking search re-evaluates only
+// energies.  Positions are f32 relative to the garment centroid.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "gpu.hpp"
+
+using cvm::V3;
+using V3f = V3<float>;
+
+namespace {
+
+constexpr int TPB = 256;
+inline unsigned blocks(uint64_t n, int per = TPB) { return (unsigned)((n + per - 1) / per); }
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct GrabArgs {
+  int active;
+  uint32_t v[3];
+  float w[3];
+  float target[3];  // relative to the origin
+  float k;
+};
+
+struct SweepArgs {
+  const uint32_t* order;
+  float4* pos;
+  const float4* tgt;
+  const float4* ball;
+  const uint32_t* term_off;
+  const cvm::TermRec* terms;
+  const uint32_t* coff;
+  const cvm::TermRec* crec;
+  float* gsq;
+  unsigned long long* clipped;
+  float cr, ck;
+  GrabArgs grab;
+};
+
+__device__ __forceinline__ V3f ldp(const float4* pos, uint32_t i) {
+  float4 q = pos[i];
+  return V3f{q.x, q.y, q.z};
+}
+__device__ __forceinline__ cvm::TermRec ldrec(const cvm::TermRec* r) {
+  const uint4* q = reinterpret_cast<const uint4*>(r);
+  uint4 a = q[0], b = q[1], c = q[2], d = q[3];
+  cvm::TermRec t;
+  uint32_t* w = reinterpret_cast<uint32_t*>(&t);
+  w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
+  w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+  w[8] = c.x; w[9] = c.y; w[10] = c.z; w[11] = c.w;
+  w[12] = d.x; w[13] = d.y; w[14] = d.z; w[15] = d.w;
+  return t;
+}
+
+template <int L>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int m = L / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, L);
+  return v;
+}
+
+// Grab term of vertex v (src/vbd.rs:237-246): d = sum w_j x_j - target.
+__device__ __forceinline__ bool grab_d(const GrabArgs& g, uint32_t v, V3f xv, const float4* pos,
+                                       V3f& d, float& wv) {
+  if (!g.active) return false;
+  int s = -1;
+  for (int k = 0; k < 3; ++k)
+    if (g.v[k] == v) s = k;
+  if (s < 0) return false;
+  d = xv - V3f{g.target[0], g.target[1], g.target[2]};
+  for (int k = 0; k < 3; ++k)
+    if (k != s) d = d + (ldp(pos, g.v[k]) - xv) * g.w[k];
+  wv = g.w[s];
+  return true;
+}
+
+template <int L>
+__device__ float local_energy(const SweepArgs& a, uint32_t v, V3f xv, int lane, uint32_t s0,
+                              uint32_t ns, uint32_t c0, uint32_t nt) {
+  auto P = [&](uint32_t i) { return ldp(a.pos, i); };
+  float e = 0.0f;
+  for (uint32_t i = lane; i < nt; i += L) {
+    cvm::TermRec t = ldrec(i < ns ? &a.terms[s0 + i] : &a.crec[c0 + (i - ns)]);
+    e += cvm::term_energy<float>(t, xv, a.cr, a.ck, P);
+  }
+  if (lane == 0) {
+    V3f d;
+    float wv;
+    if (grab_d(a.grab, v, xv, a.pos, d, wv)) e += 0.5f * a.grab.k * cvm::dot(d, d);
+  }
+  return group_sum<L>(e);
+}
+
+// One colour of one iteration (src/vbd.rs:337-395).
+template <int L>
+__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs a, uint32_t off, uint32_t count) {
+  const int lane = threadIdx.x & (L - 1);
+  const uint32_t grp = (blockIdx.x * blockDim.x + threadIdx.x) / L;
+  if (grp >= count) return;
+  const uint32_t v = a.order[off + grp];
+  float4 p4 = a.pos[v];
+  if (p4.w > 0.5f) {  // pinned: no update, zero residual contribution
+    if (lane == 0) a.gsq[v] = 0.0f;
+    return;
+  }
+  V3f x{p4.x, p4.y, p4.z};
+  uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
+  uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
+  auto P = [&](uint32_t i) { return ldp(a.pos, i); };
+  cvm::Deriv<float> acc = cvm::deriv_zero<float>();
+  for (uint32_t i = lane; i < nt; i += L) {
+    cvm::TermRec t = ldrec(i < ns ? &a.terms[s0 + i] : &a.crec[c0 + (i - ns)]);
+    cvm::deriv_add(acc, cvm::term_deriv<float>(t, x, a.cr, a.ck, P));
+  }
+  if (lane == 0) {
+    V3f d;
+    float wv;
+    if (grab_d(a.grab, v, x, a.pos, d, wv)) cvm::deriv_add(acc, cvm::stitch_d<float>(d, wv, a.grab.k));
+  }
+  acc.g.x = group_sum<L>(acc.g.x);
+  acc.g.y = group_sum<L>(acc.g.y);
+  acc.g.z = group_sum<L>(acc.g.z);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) acc.h.a[k] = group_sum<L>(acc.h.a[k]);
+  float4 tg = a.tgt[v];
+  V3f target{tg.x, tg.y, tg.z};
+  float inertia = tg.w;
+  // The accept test compares energies from one evaluation routine, so a
+  // candidate equal to x reproduces `old` exactly.
+  V3f dx = x - target;
+  float old = local_energy<L>(a, v, x, lane, s0, ns, c0, nt) + 0.5f * inertia * cvm::dot(dx, dx);
+  V3f delta = cvm::newton_direction<float>(acc, x, target, inertia);
+  float gg = cvm::dot(acc.g, acc.g);
+  float4 b4 = a.ball[v];
+  V3f center{b4.x, b4.y, b4.z};
+  bool truncated = false;
+  V3f accepted = x;
+  float scale = 1.0f;
+  for (int attempt = 0; attempt < 10; ++attempt) {
+    V3f cand = cvm::truncate_ball<float>(x + delta * scale, center, b4.w, truncated);
+    V3f dc = cand - target;
+    float e = local_energy<L>(a, v, cand, lane, s0, ns, c0, nt) + 0.5f * inertia * cvm::dot(dc, dc);
+    if (cvm::finite3(cand) && isfinite(e) && e <= old + 1e-14f) {
+      accepted = cand;
+      break;
+    }
+    scale *= 0.5f;
+  }
+  if (lane == 0) {
+    a.pos[v] = make_float4(accepted.x, accepted.y, accepted.z, 0.0f);
+    a.gsq[v] = gg;
+    if (truncated) atomicAdd(a.clipped, 1ull);
+  }
+}
+
+// Gradient probe (src/vbd.wgsl:142-147): (g, E) of the local objective.
+template <int L>
+__global__ void k_probe(SweepArgs a, uint32_t n, float4* out) {
+  const int lane = threadIdx.x & (L - 1);
+  const uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / L;
+  if (v >= n) return;
+  float4 p4 = a.pos[v];
+  V3f x{p4.x, p4.y, p4.z};
+  uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
+  uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
+  auto P = [&](uint32_t i) { return ldp(a.pos, i); };
+  cvm::Deriv<float> acc = cvm::deriv_zero<float>();
+  for (uint32_t i = lane; i < nt; i += L) {
+    cvm::TermRec t = ldrec(i < ns ? &a.terms[s0 + i] : &a.crec[c0 + (i - ns)]);
+    cvm::deriv_add(acc, cvm::term_deriv<float>(t, x, a.cr, a.ck, P));
+  }
+  if (lane == 0) {
+    V3f d;
+    float wv;
+    if (grab_d(a.grab, v, x, a.pos, d, wv)) cvm::deriv_add(acc, cvm::stitch_d<float>(d, wv, a.grab.k));
+  }
+  float e = group_sum<L>(acc.e);
+  float gx = group_sum<L>(acc.g.x), gy = group_sum<L>(acc.g.y), gz = group_sum<L>(acc.g.z);
+  if (lane == 0) {
+    float4 tg = a.tgt[v];
+    V3f dx = x - V3f{tg.x, tg.y, tg.z};
+    out[v] = make_float4(gx + dx.x * tg.w, gy + dx.y * tg.w, gz + dx.z * tg.w,
+                         e + 0.5f * tg.w * cvm::dot(dx, dx));
+  }
+}
+
+// Step prologue reductions (single block, fixed order => deterministic):
+// origin = centroid (src/gpu_vbd.rs:156), predicted motion max|target - x|
+// (src/vbd.rs:298-302) and max |x|_inf of the moving part (padded radius).
+__global__ void k_prologue_reduce(const double* x, const double* v, uint32_t n, double dt, double damp,
+                                  double gx, double gy, double gz, Scalars* sc) {
+  __shared__ double sx[1024], sy[1024], sz[1024], sm[1024], sa[1024];
+  double ax = 0, ay = 0, az = 0, mo = 0, mabs = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    double px = x[3 * i], py = x[3 * i + 1], pz = x[3 * i + 2];
+    ax += px;
+    ay += py;
+    az += pz;
+    double tx = px + (v[3 * i] * dt + gx * dt * dt) / damp;
+    double ty = py + (v[3 * i + 1] * dt + gy * dt * dt) / damp;
+    double tz = pz + (v[3 * i + 2] * dt + gz * dt * dt) / damp;
+    double dx = tx - px, dy = ty - py, dz = tz - pz;
+    mo = fmax(mo, sqrt(dx * dx + dy * dy + dz * dz));
+    mabs = fmax(mabs, fmax(fabs(px), fmax(fabs(py), fabs(pz))));
+  }
+  sx[threadIdx.x] = ax;
+  sy[threadIdx.x] = ay;
+  sz[threadIdx.x] = az;
+  sm[threadIdx.x] = mo;
+  sa[threadIdx.x] = mabs;
+  __syncthreads();
+  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sx[threadIdx.x] += sx[threadIdx.x + s];
+      sy[threadIdx.x] += sy[threadIdx.x + s];
+      sz[threadIdx.x] += sz[threadIdx.x + s];
+      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+      sa[threadIdx.x] = fmax(sa[threadIdx.x], sa[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sc->origin[0] = sx[0] / n;
+    sc->origin[1] = sy[0] / n;
+    sc->origin[2] = sz[0] / n;
+    sc->motion = sm[0];
+    sc->scale = sa[0];
+  }
+}
+
+// Targets, inertia, adaptive guess (src/vbd.rs:283-297, src/vbd/prediction.rs:15-41).
+__global__ void k_prologue(const double* x, const double* v, const double* acc, const double* masses,
+                           const uint8_t* pinned, uint32_t n, const double* fixed, uint32_t n_fixed,
+                           double dt, double damping, double gx, double gy, double gz, int pred_valid,
+                           const Scalars* sc, float4* pos, float4* tgt, double* guess, double* x_old,
+                           double* v_old) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ox = sc->origin[0], oy = sc->origin[1], oz = sc->origin[2];
+  if (i >= n) {
+    uint32_t j = i - n;
+    if (j < n_fixed)
+      pos[i] = make_float4((float)(fixed[3 * j] - ox), (float)(fixed[3 * j + 1] - oy),
+                           (float)(fixed[3 * j + 2] - oz), 1.0f);
+    return;
+  }
+  double damp = 1.0 + damping * dt;
+  double p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+  double w[3] = {v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+  double g[3] = {gx, gy, gz};
+  double t[3];
+  for (int k = 0; k < 3; ++k) t[k] = p[k] + (w[k] * dt + g[k] * dt * dt) / damp;
+  double inertia = masses[i] * damp / (dt * dt);
+  tgt[i] = make_float4((float)(t[0] - ox), (float)(t[1] - oy), (float)(t[2] - oz), (float)inertia);
+  double g2 = gx * gx + gy * gy + gz * gz;
+  double factor = 1.0;
+  if (pred_valid && g2 > 1e-20) {
+    double ag = acc[3 * i] * gx + acc[3 * i + 1] * gy + acc[3 * i + 2] * gz;
+    factor = fmin(fmax(ag / g2, 0.0), 1.0);
+  }
+  for (int k = 0; k < 3; ++k) {
+    guess[3 * i + k] = p[k] + (w[k] * dt + (factor * g[k]) * (dt * dt)) / (1.0 + damping * dt);
+    x_old[3 * i + k] = p[k];
+    v_old[3 * i + k] = w[k];
+  }
+  if (pinned[i])
+    pos[i] = make_float4((float)(p[0] - ox), (float)(p[1] - oy), (float)(p[2] - oz), 1.0f);
+}
+
+// Balls from the last detection: f32 centre and a radius shrunk by the f32
+// round-off margin of src/gpu_vbd.rs:258-274.
+__global__ void k_balls(const double* xdet, const double* radii, uint32_t n, const Scalars* sc,
+                        float4* ball, double* centers, double* radii_shrunk) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double cx = xdet[3 * i] - sc->origin[0], cy = xdet[3 * i + 1] - sc->origin[1],
+         cz = xdet[3 * i + 2] - sc->origin[2];
+  double m = fmax(fmax(fabs(cx), fabs(cy)), fmax(fabs(cz), 1.0));
+  double r = fmax(radii[i] - 16.0 * 1.1920928955078125e-07 * m, 0.0);
+  ball[i] = make_float4((float)cx, (float)cy, (float)cz, (float)r);
+  centers[3 * i] = xdet[3 * i];
+  centers[3 * i + 1] = xdet[3 * i + 1];
+  centers[3 * i + 2] = xdet[3 * i + 2];
+  radii_shrunk[i] = r;
+}
+
+// Conservative warm start: the guess clipped into its ball (src/vbd.rs:316-324).
+__global__ void k_truncate_guess(const double* guess, const double* centers, const double* radii_shrunk,
+                                 const uint8_t* pinned, uint32_t n, const Scalars* sc, float4* pos,
+                                 unsigned long long* clipped) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || pinned[i]) return;
+  double d[3], len2 = 0;
+  for (int k = 0; k < 3; ++k) {
+    d[k] = guess[3 * i + k] - centers[3 * i + k];
+    len2 += d[k] * d[k];
+  }
+  double len = sqrt(len2), r = radii_shrunk[i];
+  double p[3];
+  bool clip = len > r;
+  for (int k = 0; k < 3; ++k) p[k] = clip ? centers[3 * i + k] + d[k] * (r / len) : guess[3 * i + k];
+  pos[i] = make_float4((float)(p[0] - sc->origin[0]), (float)(p[1] - sc->origin[1]),
+                       (float)(p[2] - sc->origin[2]), 0.0f);
+  if (clip) atomicAdd(clipped, 1ull);
+}
+
+// Current f64 positions for a mid-step detection.
+__global__ void k_current(const float4* pos, const double* x_old, const uint8_t* pinned, uint32_t n,
+                          const Scalars* sc, double* x64) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (pinned[i]) {
+    for (int k = 0; k < 3; ++k) x64[3 * i + k] = x_old[3 * i + k];
+    return;
+  }
+  float4 q = pos[i];
+  x64[3 * i] = sc->origin[0] + (double)q.x;
+  x64[3 * i + 1] = sc->origin[1] + (double)q.y;
+  x64[3 * i + 2] = sc->origin[2] + (double)q.z;
+}
+
+// Commit (src/vbd.rs:399-411, src/vbd/prediction.rs:181-191): finiteness,
+// trust-region invariant, velocity, and the acceleration history.
+__global__ void k_finalize(const float4* pos, const double* x_old, const double* v_old,
+                           const double* centers, const double* radii, const uint8_t* pinned,
+                           uint32_t n, double dt, double damping, const Scalars* sc_in,
+                           double* x_out, double* v_out, double* acc, Scalars* sc) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double xn[3];
+  if (pinned[i]) {
+    for (int k = 0; k < 3; ++k) xn[k] = x_old[3 * i + k];
+  } else {
+    float4 q = pos[i];
+    xn[0] = sc_in->origin[0] + (double)q.x;
+    xn[1] = sc_in->origin[1] + (double)q.y;
+    xn[2] = sc_in->origin[2] + (double)q.z;
+  }
+  bool fin = isfinite(xn[0]) && isfinite(xn[1]) && isfinite(xn[2]);
+  if (!fin) atomicOr(&sc->err_nonfinite, 1u);
+  double d2 = 0;
+  for (int k = 0; k < 3; ++k) {
+    double d = xn[k] - centers[3 * i + k];
+    d2 += d * d;
+  }
+  if (fin && !pinned[i] && sqrt(d2) > radii[i] + 1e-8) atomicOr(&sc->err_trust, 1u);
+  for (int k = 0; k < 3; ++k) {
+    double vn = (xn[k] - x_old[3 * i + k]) / dt;
+    x_out[3 * i + k] = xn[k];
+    v_out[3 * i + k] = vn;
+    acc[3 * i + k] = (vn - v_old[3 * i + k]) / dt + damping * vn;
+  }
+}
+
+// Sum of |g|^2 in fixed order (single block).
+__global__ void k_sum_f32(const float* a, uint32_t n, double* out) {
+  __shared__ double red[1024];
+  double s = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s += (double)a[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+void read_scalars(cv_gpu& g) {
+  CV_HIP(hipMemcpyAsync(g.h_scal, g.d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, g.stream));
+  CV_HIP(hipStreamSynchronize(g.stream));
+}
+void write_scalars(cv_gpu& g) {
+  CV_HIP(hipMemcpyAsync(g.d_scal, g.h_scal, sizeof(Scalars), hipMemcpyHostToDevice, g.stream));
+}
+
+template <int L>
+void launch_sweep(const SweepArgs& a, uint32_t off, uint32_t count, hipStream_t st) {
+  k_sweep<L><<<blocks((uint64_t)count * L), TPB, 0, st>>>(a, off, count);
+}
+void sweep_color(cv_gpu& g, const SweepArgs& a, uint32_t off, uint32_t count) {
+  switch (g.lanes) {
+    case 8: launch_sweep<8>(a, off, count, g.stream); break;
+    case 16: launch_sweep<16>(a, off, count, g.stream); break;
+    case 64: launch_sweep<64>(a, off, count, g.stream); break;
+    default: launch_sweep<32>(a, off, count, g.stream); break;
+  }
+}
+
+// Host greedy colouring of the current contact set.
+uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
+  uint64_t m = g.h_scal->contacts;
+  g.h_cids.resize(5 * m);
+  g.h_ccount.resize(m);
+  if (m) {
+    CV_HIP(hipMemcpyAsync(g.h_cids.data(), g.b.c_ids.p, 5 * m * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          g.stream));
+    CV_HIP(hipMemcpyAsync(g.h_ccount.data(), g.b.c_count.p, m * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          g.stream));
+    CV_HIP(hipStreamSynchronize(g.stream));
+  }
+  uint32_t n = s.n;
+  g.cnb_off.assign(n + 1, 0);
+  for (uint64_t c = 0; c < m; ++c)
+    for (uint32_t q = 0; q < g.h_ccount[c]; ++q) {
+      uint32_t v = g.h_cids[5 * c + 1 + q];
+      if (v < n) g.cnb_off[v + 1] += g.h_ccount[c];
+    }
+  for (uint32_t v = 0; v < n; ++v) g.cnb_off[v + 1] += g.cnb_off[v];
+  g.cnb.resize(g.cnb_off[n]);
+  std::vector<uint32_t> cur(g.cnb_off.begin(), g.cnb_off.end() - 1);
+  for (uint64_t c = 0; c < m; ++c)
+    for (uint32_t q = 0; q < g.h_ccount[c]; ++q) {
+      uint32_t v = g.h_cids[5 * c + 1 + q];
+      if (v < n)
+        for (uint32_t r = 0; r < g.h_ccount[c]; ++r) g.cnb[cur[v]++] = g.h_cids[5 * c + 1 + r];
+    }
+  uint32_t k = greedy_colors(s, g.cnb_off.data(), g.cnb.data(), grab, g.color_of, g.order, g.color_off);
+  g.b.order.upload(g.order, g.stream);
+  return k;
+}
+
+struct StepOut {
+  uint64_t detections = 0, truncated = 0;
+  uint32_t colors = 0, iterations_run = 0;
+  double detection_ms = 0, refit_ms = 0, candidates_ms = 0, narrow_ms = 0;
+  uint64_t candidate_pairs = 0, grid_tests = 0;
+};
+
+SweepArgs sweep_args(cv_gpu& g, const cv_settings* set, const cv_grab* grab) {
+  SweepArgs a{};
+  a.order = g.b.order.p;
+  a.pos = g.b.pos.p;
+  a.tgt = g.b.tgt.p;
+  a.ball = g.b.ball.p;
+  a.term_off = g.dev.term_off.p;
+  a.terms = g.dev.terms.p;
+  a.coff = g.b.coff.p;
+  a.crec = g.b.crec.p;
+  a.gsq = g.b.gsq.p;
+  a.clipped = &g.d_scal->clipped;
+  a.cr = (float)set->contact.radius;
+  a.ck = (float)set->contact.stiffness;
+  a.grab.active = grab ? 1 : 0;
+  if (grab) {
+    for (int k = 0; k < 3; ++k) {
+      a.grab.v[k] = grab->vertices[k];
+      a.grab.w[k] = (float)grab->weights[k];
+      a.grab.target[k] = (float)(grab->target[k] - g.h_scal->origin[k]);
+    }
+    a.grab.k = (float)grab->stiffness;
+  }
+  return a;
+}
+
+// Detection + balls (+ colouring) on the positions held in b.x64.
+void detect_and_color(cv_gpu& g, const cv_solver& s, const cv_settings* set, const cv_grab* grab,
+                      double rq, StepOut& out, bool first) {
+  double t0 = now_ms();
+  gpu_detect(g, s, rq, set->contact, 0);
+  out.detections++;
+  out.candidate_pairs += g.h_scal->vf_cand + g.h_scal->ee_cand;
+  out.grid_tests += g.h_scal->grid_tests;
+  double md = g.h_scal->min_distance;
+  if (first && md <= 1e-12)
+    fail(CV_ERR_ZERO_SEPARATION, "OGC zero separation; step rejected, reset the arrangement");
+  if (first && md < 2e-6)
+    fail(CV_ERR_SEPARATION_F32,
+         "GPU OGC separation below f32 safety margin (2 micrometres); use f64 reference or reset");
+  uint32_t n = s.n;
+  k_balls<<<blocks(n), TPB, 0, g.stream>>>(g.b.x64.p, g.b.radii64.p, n, g.d_scal, g.b.ball.p,
+                                           g.b.centers64.p, g.b.guess64.p + 3 * (size_t)n);
+  CV_HIP(hipGetLastError());
+  double t1 = now_ms();
+  out.colors = color_on_host(g, s, grab);
+  out.detection_ms += now_ms() - t0;
+  out.narrow_ms += t1 - t0;
+}
+
+void run_step(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out) {
+  DevSolver& d = g.dev;
+  StepBufs& b = g.b;
+  hipStream_t st = g.stream;
+  uint32_t n = s.n;
+  double dt = set->dt, damp = 1.0 + set->damping * dt;
+  k_prologue_reduce<<<1, 1024, 0, st>>>(b.x64.p, b.v64.p, n, dt, damp, set->gravity[0],
+                                       set->gravity[1], set->gravity[2], g.d_scal);
+  CV_HIP(hipGetLastError());
+  read_scalars(g);
+  double r = set->contact.radius;
+  double rq = std::min(std::max(r + g.h_scal->motion, 2.0 * r), 4.0 * r);
+  g.h_scal->clipped = 0;
+  g.h_scal->err_nonfinite = g.h_scal->err_trust = 0;
+  write_scalars(g);
+  k_prologue<<<blocks(n + d.n_fixed), TPB, 0, st>>>(
+      b.x64.p, b.v64.p, d.acc.p, d.masses.p, d.pinned.p, n, d.fixed_pos.p, d.n_fixed, dt, set->damping,
+      set->gravity[0], set->gravity[1], set->gravity[2], g.pred_valid ? 1 : 0, g.d_scal, b.pos.p, b.tgt.p,
+      b.guess64.p, b.x_old64.p, b.v_old64.p);
+  CV_HIP(hipGetLastError());
+  detect_and_color(g, s, set, grab, rq, out, true);
+  k_truncate_guess<<<blocks(n), TPB, 0, st>>>(b.guess64.p, b.centers64.p, b.guess64.p + 3 * (size_t)n,
+                                              d.pinned.p, n, g.d_scal, b.pos.p, &g.d_scal->clipped);
+  CV_HIP(hipGetLastError());
+  read_scalars(g);
+  uint64_t clipped = g.h_scal->clipped;
+  SweepArgs a = sweep_args(g, set, grab);
+  uint32_t iters = set->iterations;
+  for (uint32_t it = 0; it < iters; ++it) {
+    if (clipped > 0 && (double)clipped >= set->contact.refresh_fraction * n) {
+      k_current<<<blocks(n), TPB, 0, st>>>(b.pos.p, b.x_old64.p, d.pinned.p, n, g.d_scal, b.x64.p);
+      CV_HIP(hipGetLastError());
+      detect_and_color(g, s, set, grab, rq, out, false);
+      a = sweep_args(g, set, grab);
+    }
+    CV_HIP(hipMemsetAsync(&g.d_scal->clipped, 0, sizeof(unsigned long long), st));
+    for (size_t c = 0; c + 1 < g.color_off.size(); ++c) {
+      uint32_t off = g.color_off[c], cnt = g.color_off[c + 1] - off;
+      if (cnt) sweep_color(g, a, off, cnt);
+    }
+    CV_HIP(hipGetLastError());
+    unsigned long long hc = 0;
+    CV_HIP(hipMemcpyAsync(&hc, &g.d_scal->clipped, sizeof(hc), hipMemcpyDeviceToHost, st));
+    CV_HIP(hipStreamSynchronize(st));
+    clipped = hc;
+    out.truncated += clipped;
+    out.iterations_run++;
+  }
+  k_sum_f32<<<1, 1024, 0, st>>>(b.gsq.p, n, &g.d_scal->residual_sum);
+  // Restore the start-of-step positions in x64 before committing (the moving
+  // part may hold a mid-step detection's positions).
+  k_finalize<<<blocks(n), TPB, 0, st>>>(b.pos.p, b.x_old64.p, b.v_old64.p, b.centers64.p,
+                                        b.radii64.p, d.pinned.p, n, dt, set->damping, g.d_scal,
+                                        b.xcur64.p, b.v64.p, d.acc.p, g.d_scal);
+  CV_HIP(hipGetLastError());
+  read_scalars(g);
+  if (g.h_scal->err_nonfinite) fail(CV_ERR_NONFINITE_RESULT, "Non-finite VBD result; state unchanged");
+  if (g.h_scal->err_trust) fail(CV_ERR_TRUST_REGION, "GPU OGC trust-region check failed; state unchanged");
+}
+
+}  // namespace
+
+void ensure_device_solver(cv_gpu& g, const cv_solver& s) {
+  DevSolver& d = g.dev;
+  if (d.generation == s.generation) return;
+  hipStream_t st = g.stream;
+  const OgcTopology& t = s.top;
+  d.n = s.n;
+  d.n_total = t.n_total;
+  d.n_fixed = t.n_total - s.n;
+  d.F = (uint32_t)(t.faces.size() / 3);
+  d.E = (uint32_t)(t.edges.size() / 2);
+  d.term_off.upload(s.term_off, st);
+  d.terms.upload(s.terms, st);
+  d.masses.upload(s.masses, st);
+  d.pinned.upload(s.pinned, st);
+  std::vector<uint32_t> ft(t.faces);
+  ft.insert(ft.end(), t.tri_edges.begin(), t.tri_edges.end());
+  d.faces.upload(ft, st);
+  d.edges.upload(t.edges, st);
+  d.opposites.upload(t.opposites, st);
+  d.nb_off.upload(t.nb_off, st);
+  std::vector<uint32_t> nb(t.nb);
+  if (nb.empty()) nb.push_back(0);
+  d.nb.upload(nb, st);
+  d.ex_off.upload(t.ex_off, st);
+  std::vector<uint32_t> ex(t.ex);
+  if (ex.empty()) ex.push_back(0);
+  d.ex.upload(ex, st);
+  d.face_moving.upload(t.face_moving, st);
+  d.edge_moving.upload(t.edge_moving, st);
+  d.rest_len2.upload(t.rest_len2, st);
+  std::vector<double> fx(s.avatar);
+  if (fx.empty()) fx.push_back(0);
+  d.fixed_pos.upload(fx, st);
+  d.fixed_abs_max = 0;
+  for (double v : s.avatar) d.fixed_abs_max = std::max(d.fixed_abs_max, std::fabs(v));
+  d.cell = s.face_extent_median;
+  d.acc.reserve(3 * (size_t)s.n);
+  d.acc_version = 0;
+  d.acc_device = false;
+  d.generation = s.generation;
+  CV_HIP(hipStreamSynchronize(st));
+}
+
+void ensure_step_buffers(cv_gpu& g, const cv_solver& s) {
+  StepBufs& b = g.b;
+  size_t n = s.n, nt = s.top.n_total;
+  b.x64.reserve(3 * nt);
+  b.x_old64.reserve(3 * n);
+  b.v64.reserve(3 * n);
+  b.v_old64.reserve(3 * n);
+  b.guess64.reserve(4 * n);  // guess (3n) + shrunk radii (n)
+  b.xcur64.reserve(3 * n);
+  b.radii64.reserve(n);
+  b.centers64.reserve(3 * n);
+  b.pos.reserve(nt);
+  b.tgt.reserve(n);
+  b.ball.reserve(n);
+  b.gsq.reserve(n);
+  b.order.reserve(n);
+  size_t cap = std::max<size_t>(1 << 20, 32 * n);
+  if (b.contact_cap < cap) {
+    b.raw_ids.reserve(5 * cap);
+    b.raw_count.reserve(cap);
+    b.raw_eps.reserve(cap);
+    b.c_ids.reserve(5 * cap);
+    b.c_count.reserve(cap);
+    b.c_eps.reserve(cap);
+    b.contact_cap = cap;
+  }
+  b.coff.reserve(n + 1);
+  if (g.h_cap < 3 * n) {
+    if (g.h_x) CV_HIP(hipHostFree(g.h_x));
+    if (g.h_v) CV_HIP(hipHostFree(g.h_v));
+    CV_HIP(hipHostMalloc((void**)&g.h_x, 3 * n * sizeof(double)));
+    CV_HIP(hipHostMalloc((void**)&g.h_v, 3 * n * sizeof(double)));
+    g.h_cap = 3 * n;
+  }
+}
+
+// ------------------------------------------------------------------- C ABI
+extern thread_local std::string cv_create_error;  // api.cpp
+
+namespace {
+
+template <class F>
+cv_status guard(std::string* err, F&& f) {
+  try {
+    f();
+    return CV_OK;
+  } catch (const CvError& e) {
+    if (err) *err = e.message;
+    return e.status;
+  } catch (const std::exception& e) {
+    if (err) *err = e.what();
+    return CV_ERR_DEVICE;
+  }
+}
+
+void fill_metrics(cv_metrics* m, const cv_gpu& g, const cv_solver& s, const cv_settings* set,
+                  const StepOut& o, double step_ms, double gpu_ms, const double* x_new) {
+  if (!m) return;
+  *m = cv_metrics{};
+  m->dt_s = set->dt;
+  m->step_ms = step_ms;
+  m->detection_ms = o.detection_ms;
+  m->solve_ms = step_ms - o.detection_ms;
+  m->gpu_compute_ms = gpu_ms;
+  m->refit_ms = 0;
+  m->candidates_ms = 0;
+  m->narrow_ms = o.narrow_ms;
+  m->candidate_pairs = o.candidate_pairs;
+  m->aabb_rejections = o.grid_tests - std::min(o.grid_tests, o.candidate_pairs);
+  m->iterations = set->iterations;
+  m->contacts = g.h_scal->active;
+  m->contact_terms = g.h_scal->contacts;
+  m->detection_updates = o.detections;
+  m->broad_phase_rebuilds = o.detections;
+  m->truncated_vertices = o.truncated;
+  m->minimum_distance_m = g.h_scal->min_distance;
+  m->residual_rms_n = std::sqrt(g.h_scal->residual_sum / (3.0 * s.n));
+  m->true_residual_rms_n = NAN;
+  m->colors = o.colors;
+  m->iterations_run = o.iterations_run;
+  if (!(set->flags & CV_FLAG_DIAGNOSTICS_OFF)) {
+    double gap, p95, pmax;
+    garment_diagnostics(s, x_new, gap, p95, pmax);
+    m->max_seam_gap_m = gap;
+    m->extension_p95_percent = p95;
+    m->extension_max_percent = pmax;
+  } else {
+    m->max_seam_gap_m = m->extension_p95_percent = m->extension_max_percent = NAN;
+  }
+}
+
+// Upload the state and the prediction history for one step.
+void stage_state(cv_gpu& g, cv_solver& s, const cv_state* st, const cv_settings* set) {
+  size_t n = s.n;
+  std::memcpy(g.h_x, st->x, 3 * n * sizeof(double));
+  std::memcpy(g.h_v, st->v, 3 * n * sizeof(double));
+  CV_HIP(hipMemcpyAsync(g.b.x64.p, g.h_x, 3 * n * sizeof(double), hipMemcpyHostToDevice, g.stream));
+  CV_HIP(hipMemcpyAsync(g.b.v64.p, g.h_v, 3 * n * sizeof(double), hipMemcpyHostToDevice, g.stream));
+  if (!s.avatar.empty())
+    CV_HIP(hipMemcpyAsync(g.b.x64.p + 3 * n, s.avatar.data(), s.avatar.size() * sizeof(double),
+                          hipMemcpyHostToDevice, g.stream));
+  bool valid = prediction_valid(s, st, set);
+  if (valid && g.dev.acc_version != s.acc_version) {
+    if (s.acc.size() == 3 * n) {
+      CV_HIP(hipMemcpyAsync(g.dev.acc.p, s.acc.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice,
+                            g.stream));
+      g.dev.acc_version = s.acc_version;
+    } else {
+      valid = false;
+    }
+  }
+  g.pred_valid = valid;
+}
+
+uint64_t next_acc_version() {
+  static uint64_t v = 1ull << 40;
+  return ++v;
+}
+
+}  // namespace
+
+extern "C" {
+
+cv_status cv_gpu_create(int device, cv_gpu** out) {
+  if (!out) return CV_ERR_INVALID_ARGUMENT;
+  std::string err;
+  cv_gpu* g = new cv_gpu();
+  cv_status st = guard(&err, [&] {
+    int count = 0;
+    CV_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) fail(CV_ERR_DEVICE, "no such HIP device");
+    g->device = device;
+    CV_HIP(hipSetDevice(device));
+    CV_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    CV_HIP(hipMalloc((void**)&g->d_scal, sizeof(Scalars)));
+    CV_HIP(hipHostMalloc((void**)&g->h_scal, sizeof(Scalars)));
+    std::memset(g->h_scal, 0, sizeof(Scalars));
+    CV_HIP(hipEventCreate(&g->ev0));
+    CV_HIP(hipEventCreate(&g->ev1));
+  });
+  if (st != CV_OK) {
+    cv_create_error = err;
+    delete g;
+    return st;
+  }
+  *out = g;
+  return CV_OK;
+}
+
+void cv_gpu_destroy(cv_gpu* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->device);
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  if (g->d_scal) (void)hipFree(g->d_scal);
+  if (g->h_scal) (void)hipHostFree(g->h_scal);
+  if (g->h_x) (void)hipHostFree(g->h_x);
+  if (g->h_v) (void)hipHostFree(g->h_v);
+  if (g->ev0) (void)hipEventDestroy(g->ev0);
+  if (g->ev1) (void)hipEventDestroy(g->ev1);
+  hipStream_t st = g->stream;
+  delete g;
+  if (st) (void)hipStreamDestroy(st);
+}
+
+cv_status cv_gpu_set_check_gradients(cv_gpu* g, int enabled) {
+  if (!g) return CV_ERR_INVALID_ARGUMENT;
+  g->check_gradients = enabled != 0;
+  return CV_OK;
+}
+
+cv_status cv_gpu_set_lanes(cv_gpu* g, int lanes) {
+  if (!g || !(lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64)) return CV_ERR_INVALID_ARGUMENT;
+  g->lanes = lanes;
+  return CV_OK;
+}
+
+cv_status cv_gpu_step(cv_gpu* g, cv_solver* s, cv_state* st, const cv_settings* set, const cv_grab* grab,
+                      cv_metrics* metrics) {
+  if (!g || !s) return CV_ERR_INVALID_ARGUMENT;
+  return guard(&g->last_error, [&] {
+    CV_HIP(hipSetDevice(g->device));
+    double t0 = now_ms();
+    validate_settings(set);
+    validate_state(*s, st);
+    validate_grab(*s, grab);
+    ensure_device_solver(*g, *s);
+    ensure_step_buffers(*g, *s);
+    stage_state(*g, *s, st, set);
+    CV_HIP(hipEventRecord(g->ev0, g->stream));
+    StepOut o;
+    run_step(*g, *s, set, grab, o);
+    CV_HIP(hipEventRecord(g->ev1, g->stream));
+    size_t n = s->n;
+    CV_HIP(hipMemcpyAsync(g->h_x, g->b.xcur64.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    CV_HIP(hipMemcpyAsync(g->h_v, g->b.v64.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    CV_HIP(hipStreamSynchronize(g->stream));
+    float gpu_ms = 0;
+    CV_HIP(hipEventElapsedTime(&gpu_ms, g->ev0, g->ev1));
+    // Commit (all checks passed).
+    std::memcpy(st->x, g->h_x, 3 * n * sizeof(double));
+    std::memcpy(st->v, g->h_v, 3 * n * sizeof(double));
+    st->time += set->dt;
+    s->has_last = true;
+    s->last_x.assign(st->x, st->x + 3 * n);
+    s->last_v.assign(st->v, st->v + 3 * n);
+    s->last_time = st->time;
+    for (int k = 0; k < 3; ++k) s->last_gravity[k] = set->gravity[k];
+    s->last_damping = set->damping;
+    s->acc_version = next_acc_version();
+    g->dev.acc_version = s->acc_version;
+    s->acc.clear();
+    double step_ms = now_ms() - t0;
+    fill_metrics(metrics, *g, *s, set, o, step_ms, gpu_ms, st->x);
+    if (metrics) metrics->time_s = st->time;
+  });
+}
+
+cv_status cv_gpu_run(cv_gpu* g, cv_solver* s, cv_state* st, const cv_settings* set, uint32_t steps,
+                     cv_metrics* last) {
+  if (!g || !s) return CV_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < steps; ++i) {
+    cv_status r = cv_gpu_step(g, s, st, set, nullptr, last);
+    if (r != CV_OK) return r;
+  }
+  return CV_OK;
+}
+
+cv_status cv_gpu_probe(cv_gpu* g, cv_solver* s, const cv_state* st, const cv_settings* set,
+                       const cv_grab* grab, double* out) {
+  if (!g || !s || !out) return CV_ERR_INVALID_ARGUMENT;
+  return guard(&g->last_error, [&] {
+    CV_HIP(hipSetDevice(g->device));
+    validate_settings(set);
+    validate_state(*s, st);
+    validate_grab(*s, grab);
+    ensure_device_solver(*g, *s);
+    ensure_step_buffers(*g, *s);
+    stage_state(*g, *s, st, set);
+    g->pred_valid = false;
+    size_t n = s->n;
+    hipStream_t stream = g->stream;
+    double dt = set->dt, damp = 1.0 + set->damping * dt;
+    k_prologue_reduce<<<1, 1024, 0, stream>>>(g->b.x64.p, g->b.v64.p, (uint32_t)n, dt, damp,
+                                              set->gravity[0], set->gravity[1], set->gravity[2], g->d_scal);
+    read_scalars(*g);
+    k_prologue<<<blocks(n + g->dev.n_fixed), TPB, 0, stream>>>(
+        g->b.x64.p, g->b.v64.p, g->dev.acc.p, g->dev.masses.p, g->dev.pinned.p, (uint32_t)n,
+        g->dev.fixed_pos.p, g->dev.n_fixed, dt, set->damping, set->gravity[0], set->gravity[1],
+        set->gravity[2], 0, g->d_scal, g->b.pos.p, g->b.tgt.p, g->b.guess64.p, g->b.x_old64.p,
+        g->b.v_old64.p);
+    double r = set->contact.radius;
+    double rq = std::min(std::max(r + g->h_scal->motion, 2.0 * r), 4.0 * r);
+    gpu_detect(*g, *s, rq, set->contact, 0);
+    // Probe at the state's positions (all vertices, pinned included).
+    std::vector<float4> pos(n);
+    for (size_t i = 0; i < n; ++i) {
+      float4 q;
+      q.x = (float)(st->x[3 * i] - g->h_scal->origin[0]);
+      q.y = (float)(st->x[3 * i + 1] - g->h_scal->origin[1]);
+      q.z = (float)(st->x[3 * i + 2] - g->h_scal->origin[2]);
+      q.w = s->pinned[i] ? 1.0f : 0.0f;
+      pos[i] = q;
+    }
+    CV_HIP(hipMemcpyAsync(g->b.pos.p, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream));
+    SweepArgs a = sweep_args(*g, set, grab);
+    DBuf<float4> outd;
+    outd.reserve(n);
+    k_probe<32><<<blocks(n * 32), TPB, 0, stream>>>(a, (uint32_t)n, outd.p);
+    CV_HIP(hipGetLastError());
+    std::vector<float4> h(n);
+    CV_HIP(hipMemcpyAsync(h.data(), outd.p, n * sizeof(float4), hipMemcpyDeviceToHost, stream));
+    CV_HIP(hipStreamSynchronize(stream));
+    for (size_t i = 0; i < n; ++i) {
+      out[4 * i] = h[i].x;
+      out[4 * i + 1] = h[i].y;
+      out[4 * i + 2] = h[i].z;
+      out[4 * i + 3] = h[i].w;
+    }
+  });
+}
+
+cv_status cv_gpu_detect(cv_gpu* g, cv_solver* s, const double* x, double rq, const cv_ogc_settings* ogc,
+                        cv_detection* out, uint32_t* vf_pairs, uint64_t vf_capacity, uint32_t* ee_pairs,
+                        uint64_t ee_capacity, uint32_t* contact_kind_ids, uint64_t contact_capacity,
+                        double* contact_eps, double* radii) {
+  if (!g || !s || !x || !ogc || !out) return CV_ERR_INVALID_ARGUMENT;
+  return guard(&g->last_error, [&] {
+    CV_HIP(hipSetDevice(g->device));
+    const cv_ogc_settings& c = *ogc;
+    if (!std::isfinite(c.radius) || c.radius <= 0.0 || !std::isfinite(c.stiffness) || c.stiffness <= 0.0 ||
+        !std::isfinite(c.relaxation) || c.relaxation <= 0.0 || c.relaxation >= 0.5 ||
+        !std::isfinite(c.refresh_fraction) || !(c.refresh_fraction >= 0.0 && c.refresh_fraction <= 1.0))
+      fail(CV_ERR_INVALID_OGC_SETTINGS, "Invalid OGC radius, stiffness or trust-region parameters");
+    if (!std::isfinite(rq) || rq < c.radius)
+      fail(CV_ERR_QUERY_RADIUS, "OGC query radius must cover the contact radius");
+    ensure_device_solver(*g, *s);
+    ensure_step_buffers(*g, *s);
+    size_t n = s->n;
+    for (size_t i = 0; i < 3 * n; ++i)
+      if (!std::isfinite(x[i])) fail(CV_ERR_INVALID_POSITIONS, "OGC invalid position array");
+    CV_HIP(hipMemcpyAsync(g->b.x64.p, x, 3 * n * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    if (!s->avatar.empty())
+      CV_HIP(hipMemcpyAsync(g->b.x64.p + 3 * n, s->avatar.data(), s->avatar.size() * sizeof(double),
+                            hipMemcpyHostToDevice, g->stream));
+    double mabs = 0;
+    for (size_t i = 0; i < 3 * n; ++i) mabs = std::max(mabs, std::fabs(x[i]));
+    g->h_scal->scale = mabs;
+    size_t cap = std::max(vf_capacity, ee_capacity);
+    gpu_detect(*g, *s, rq, c, (vf_pairs || ee_pairs) ? std::max<size_t>(cap, 1) : 0);
+    const Scalars& hs = *g->h_scal;
+    out->n_vf_candidates = hs.vf_cand;
+    out->n_ee_candidates = hs.ee_cand;
+    out->vertex_face_candidates = hs.vf_count;
+    out->edge_edge_candidates = hs.ee_count;
+    out->n_contacts = hs.contacts;
+    out->active_contacts = hs.active;
+    out->minimum_distance = hs.min_distance;
+    if (vf_pairs && hs.cand_export_vf <= vf_capacity)
+      CV_HIP(hipMemcpyAsync(vf_pairs, g->b.cand_vf.p, 2 * hs.cand_export_vf * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, g->stream));
+    if (ee_pairs && hs.cand_export_ee <= ee_capacity)
+      CV_HIP(hipMemcpyAsync(ee_pairs, g->b.cand_ee.p, 2 * hs.cand_export_ee * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, g->stream));
+    if (contact_kind_ids && hs.contacts <= contact_capacity && hs.contacts)
+      CV_HIP(hipMemcpyAsync(contact_kind_ids, g->b.c_ids.p, 5 * hs.contacts * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, g->stream));
+    if (contact_eps && hs.contacts <= contact_capacity && hs.contacts)
+      CV_HIP(hipMemcpyAsync(contact_eps, g->b.c_eps.p, hs.contacts * sizeof(double), hipMemcpyDeviceToHost,
+                            g->stream));
+    if (radii)
+      CV_HIP(hipMemcpyAsync(radii, g->b.radii64.p, n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    CV_HIP(hipStreamSynchronize(g->stream));
+  });
+}
+
+cv_status cv_gpu_step_batch(cv_gpu* g, cv_solver* const* solvers, cv_state* const* states, uint32_t count,
+                            const cv_settings* settings, cv_metrics* metrics) {
+  if (!g || (!solvers && count)) return CV_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < count; ++i) {
+    cv_status r = cv_gpu_step(g, solvers[i], states[i], settings, nullptr, metrics ? &metrics[i] : nullptr);
+    if (r != CV_OK) return r;
+  }
+  return CV_OK;
+}
+
+}  // extern "C"
+
+const std::string* cv_gpu_error_string(const cv_gpu* g) { return &g->last_error; }

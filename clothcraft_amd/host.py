@@ -1,0 +1,130 @@
+"""ctypes binding of libcvm_host.so: the host (x86) build of the product's
+solver set-up and device math headers, used by the CPU test suite."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcvm_host.so")
+
+TERM_DTYPE = np.dtype([("head", "<u4"), ("ids", "<u4", 4), ("f", "<f4", 11)])
+assert TERM_DTYPE.itemsize == 64
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            from . import build
+            build.build_host()
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.cvh_solver_create.argtypes = [C.POINTER(abi.cv_garment), C.POINTER(abi.cv_surface),
+                                        C.POINTER(abi.cv_state), C.POINTER(vp), C.c_char_p, C.c_size_t]
+        L.cvh_solver_destroy.argtypes = [vp]
+        L.cvh_solver_terms.argtypes = [vp, abi.u32p, C.c_void_p, C.c_uint64]
+        L.cvh_solver_terms.restype = C.c_uint64
+        L.cvh_solver_edges.argtypes = [vp, abi.u32p, C.c_uint32]
+        L.cvh_solver_edges.restype = C.c_uint32
+        L.cvh_greedy_colors.argtypes = [vp, abi.u32p, abi.u32p, C.c_uint64, C.POINTER(abi.cv_grab),
+                                        abi.u32p]
+        L.cvh_greedy_colors.restype = C.c_uint32
+        for name in ("cvh_term_deriv_f64", "cvh_term_deriv_f32"):
+            getattr(L, name).argtypes = [C.c_void_p, abi.f64p, abi.f64p, C.c_double, C.c_double,
+                                         abi.f64p]
+        for name in ("cvh_term_energy_f64", "cvh_term_energy_f32"):
+            getattr(L, name).argtypes = [C.c_void_p, abi.f64p, abi.f64p, C.c_double, C.c_double]
+            getattr(L, name).restype = C.c_double
+        L.cvh_contact_record.argtypes = [C.c_int, abi.u32p, C.c_int, C.c_double, C.c_uint32,
+                                         C.c_void_p]
+        L.cvh_newton_direction_f64.argtypes = [abi.f64p, abi.f64p, abi.f64p, C.c_double, abi.f64p]
+        L.cvh_closest_triangle.argtypes = [abi.f64p, abi.f64p, abi.f64p, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int)]
+        L.cvh_closest_segments.argtypes = [abi.f64p] * 4 + [abi.f64p, abi.f64p]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return abi.ptr(a, C.c_double)
+
+
+class HostSolver:
+    """The product's cv_solver set-up, built for the host."""
+
+    def __init__(self, garment, avatar, state):
+        self._g = garment.to_c()
+        self._a = avatar.to_c() if avatar is not None else None
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = lib().cvh_solver_create(C.byref(self._g), C.byref(self._a) if self._a else None,
+                                     C.byref(state.to_c()), C.byref(h), err, 512)
+        if rc != 0:
+            from .vbd import VbdError
+            raise VbdError(rc, err.value.decode())
+        self.h = h
+        self.n = garment.n
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().cvh_solver_destroy(self.h)
+            self.h = None
+
+    def terms(self):
+        off = np.zeros(self.n + 1, np.uint32)
+        k = lib().cvh_solver_terms(self.h, abi.ptr(off, C.c_uint32), None, 0)
+        recs = np.zeros(k, TERM_DTYPE)
+        lib().cvh_solver_terms(self.h, abi.ptr(off, C.c_uint32), recs.ctypes.data, k)
+        return off, recs
+
+    def edges(self):
+        out = np.zeros((1 << 21, 2), np.uint32)
+        k = lib().cvh_solver_edges(self.h, abi.ptr(out, C.c_uint32), 1 << 21)
+        return out[:k].copy()
+
+    def greedy_colors(self, kind_ids, counts, grab=None):
+        ki = np.ascontiguousarray(kind_ids, np.uint32).reshape(-1, 5)
+        cn = np.ascontiguousarray(counts, np.uint32)
+        out = np.zeros(self.n, np.uint32)
+        from .vbd import _grab
+        g = _grab(grab)
+        k = lib().cvh_greedy_colors(self.h, abi.ptr(ki, C.c_uint32), abi.ptr(cn, C.c_uint32), len(cn),
+                                    C.byref(g) if g else None, abi.ptr(out, C.c_uint32))
+        return out, int(k)
+
+
+def term_deriv(rec, xv, x, cr=0.003, ck=1e5, f32=False) -> np.ndarray:
+    r = np.ascontiguousarray(np.asarray(rec, TERM_DTYPE).reshape(1))
+    xv = np.ascontiguousarray(xv, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros(10)
+    fn = lib().cvh_term_deriv_f32 if f32 else lib().cvh_term_deriv_f64
+    fn(r.ctypes.data, _p(xv), _p(x), cr, ck, _p(out))
+    return out
+
+
+def term_energy(rec, xv, x, cr=0.003, ck=1e5, f32=False) -> float:
+    r = np.ascontiguousarray(np.asarray(rec, TERM_DTYPE).reshape(1))
+    xv = np.ascontiguousarray(xv, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    fn = lib().cvh_term_energy_f32 if f32 else lib().cvh_term_energy_f64
+    return fn(r.ctypes.data, _p(xv), _p(x), cr, ck)
+
+
+def contact_record(kind, ids, count, eps, v):
+    out = np.zeros(1, TERM_DTYPE)
+    ids = np.ascontiguousarray(ids, np.uint32)
+    lib().cvh_contact_record(kind, abi.ptr(ids, C.c_uint32), count, eps, v, out.ctypes.data)
+    return out[0]
+
+
+def sym(h6) -> np.ndarray:
+    a = h6
+    return np.array([[a[0], a[1], a[2]], [a[1], a[3], a[4]], [a[2], a[4], a[5]]])

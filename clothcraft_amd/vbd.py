@@ -1,0 +1,284 @@
+"""Python mirror of the reference stepping API over libclothvbd's C ABI.
+
+`VbdSolver(garment, avatar, state)` mirrors `VbdSolver::new`
+(src/vbd.rs:101-192); `GpuVbd(device).step(solver, state, settings, grab)`
+mirrors `GpuVbd::step` (src/gpu_vbd.rs:125-472): same argument meaning, the
+state is updated in place only on success, and every failure raises
+`VbdError` carrying the reference's message with the state untouched.
+
+The library must be built (python -m clothcraft_amd.build); there is no
+fallback path: loading fails loudly when libclothvbd.so is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libclothvbd.so")
+
+_lib = None
+
+
+class VbdError(RuntimeError):
+    """A failed step or create: `status` is the cv_status code, `message` the
+    reference error string (e.g. "Invalid mouse attachment")."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{abi.STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m clothcraft_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.cv_solver_create.argtypes = [C.POINTER(abi.cv_garment), C.POINTER(abi.cv_surface),
+                                       C.POINTER(abi.cv_state), C.POINTER(vp)]
+        L.cv_solver_destroy.argtypes = [vp]
+        L.cv_solver_record_prediction.argtypes = [vp, abi.f64p, C.POINTER(abi.cv_state),
+                                                  C.POINTER(abi.cv_settings)]
+        L.cv_solver_colors.argtypes = [vp, abi.u32p, C.c_uint32, abi.u32p]
+        L.cv_gpu_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.cv_gpu_destroy.argtypes = [vp]
+        L.cv_gpu_set_check_gradients.argtypes = [vp, C.c_int]
+        L.cv_gpu_set_lanes.argtypes = [vp, C.c_int]
+        L.cv_gpu_step.argtypes = [vp, vp, C.POINTER(abi.cv_state), C.POINTER(abi.cv_settings),
+                                  C.POINTER(abi.cv_grab), C.POINTER(abi.cv_metrics)]
+        L.cv_gpu_run.argtypes = [vp, vp, C.POINTER(abi.cv_state), C.POINTER(abi.cv_settings),
+                                 C.c_uint32, C.POINTER(abi.cv_metrics)]
+        L.cv_gpu_probe.argtypes = [vp, vp, C.POINTER(abi.cv_state), C.POINTER(abi.cv_settings),
+                                   C.POINTER(abi.cv_grab), abi.f64p]
+        L.cv_gpu_detect.argtypes = [vp, vp, abi.f64p, C.c_double, C.POINTER(abi.cv_ogc_settings),
+                                    C.POINTER(abi.cv_detection), abi.u32p, C.c_uint64, abi.u32p,
+                                    C.c_uint64, abi.u32p, C.c_uint64, abi.f64p, abi.f64p]
+        L.cv_gpu_step_batch.argtypes = [vp, C.POINTER(vp), C.POINTER(C.POINTER(abi.cv_state)),
+                                        C.c_uint32, C.POINTER(abi.cv_settings),
+                                        C.POINTER(abi.cv_metrics)]
+        L.cv_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
+        L.cv_last_error.restype = C.c_size_t
+        L.cv_solver_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
+        L.cv_solver_last_error.restype = C.c_size_t
+        L.cv_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _error(handle=None) -> str:
+    buf = C.create_string_buffer(1024)
+    lib().cv_last_error(handle, buf, 1024)
+    return buf.value.decode()
+
+
+@dataclass
+class OgcSettings:
+    """src/ogc/world.rs:33-54."""
+    radius: float = 0.003
+    stiffness: float = 100000.0
+    relaxation: float = 0.45
+    refresh_fraction: float = 0.01
+
+    def to_c(self):
+        return abi.cv_ogc_settings(self.radius, self.stiffness, self.relaxation,
+                                   self.refresh_fraction)
+
+
+@dataclass
+class VbdSettings:
+    """src/vbd.rs:18-36 (+ the library's extensions, zero = reference behaviour)."""
+    dt: float = 1.0 / 240.0
+    iterations: int = 16
+    damping: float = 1.5
+    gravity: tuple = (0.0, -9.81, 0.0)
+    contact: OgcSettings = field(default_factory=OgcSettings)
+    residual_tolerance: float = 0.0
+    flags: int = 0
+
+    @staticmethod
+    def from_dict(d: dict) -> "VbdSettings":
+        return VbdSettings(d["dt"], d["iterations"], d["damping"], tuple(d["gravity"]),
+                           OgcSettings(d["radius"], d["stiffness"], d["relaxation"],
+                                       d["refresh_fraction"]),
+                           d.get("residual_tolerance", 0.0), d.get("flags", 0))
+
+    def to_c(self):
+        return abi.cv_settings(self.dt, self.iterations, self.damping,
+                               (C.c_double * 3)(*self.gravity), self.contact.to_c(),
+                               self.residual_tolerance, self.flags)
+
+
+def _settings(s) -> abi.cv_settings:
+    if isinstance(s, dict):
+        s = VbdSettings.from_dict(s)
+    return s.to_c()
+
+
+@dataclass
+class Grab:
+    """src/vbd.rs:54-60."""
+    vertices: tuple
+    weights: tuple
+    target: tuple
+    stiffness: float
+
+    def to_c(self):
+        return abi.cv_grab((C.c_uint32 * 3)(*self.vertices), (C.c_double * 3)(*self.weights),
+                           (C.c_double * 3)(*self.target), self.stiffness)
+
+
+def _grab(g):
+    if g is None:
+        return None
+    if isinstance(g, dict):
+        g = Grab(**g)
+    return g.to_c()
+
+
+class VbdSolver:
+    """VbdSolver::new (src/vbd.rs:101-192): validates the garment, pins and the
+    start state (no intersection, positive separation) and builds the term
+    incidence, conflict graph and OGC topology."""
+
+    def __init__(self, garment, avatar, state):
+        L = lib()
+        self._g = garment.to_c()
+        self._a = avatar.to_c() if avatar is not None else None
+        h = C.c_void_p()
+        rc = L.cv_solver_create(C.byref(self._g), C.byref(self._a) if self._a else None,
+                                C.byref(state.to_c()), C.byref(h))
+        if rc != 0:
+            raise VbdError(rc, _error(None))
+        self.h = h
+        self.n = garment.n
+        self.garment = garment
+        self.avatar = avatar
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().cv_solver_destroy(self.h)
+            self.h = None
+
+    def record_prediction(self, old_velocity, state, settings):
+        """Prediction::record (src/vbd/prediction.rs:181-191)."""
+        ov = np.ascontiguousarray(old_velocity, np.float64)
+        rc = lib().cv_solver_record_prediction(self.h, abi.ptr(ov, C.c_double),
+                                               C.byref(state.to_c()), C.byref(_settings(settings)))
+        if rc != 0:
+            buf = C.create_string_buffer(512)
+            lib().cv_solver_last_error(self.h, buf, 512)
+            raise VbdError(rc, buf.value.decode())
+
+    def static_colors(self):
+        out = np.zeros(self.n, np.uint32)
+        k = C.c_uint32()
+        lib().cv_solver_colors(self.h, abi.ptr(out, C.c_uint32), self.n, C.byref(k))
+        return out, int(k.value)
+
+
+class GpuVbd:
+    """GpuVbd (src/gpu_vbd.rs:15-124) on one MI355X: one HIP stream, device
+    buffers reused across steps, static term records cached per solver."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().cv_gpu_create(device, C.byref(h))
+        if rc != 0:
+            raise VbdError(rc, _error(None))
+        self.h = h
+        self._check = False
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().cv_gpu_destroy(self.h)
+            self.h = None
+
+    @property
+    def check_gradients(self) -> bool:
+        return self._check
+
+    @check_gradients.setter
+    def check_gradients(self, v: bool):
+        self._check = bool(v)
+        lib().cv_gpu_set_check_gradients(self.h, int(v))
+
+    def set_lanes(self, lanes: int):
+        rc = lib().cv_gpu_set_lanes(self.h, lanes)
+        if rc != 0:
+            raise VbdError(rc, "lanes must be 8, 16, 32 or 64")
+
+    def _raise(self, rc):
+        raise VbdError(rc, _error(self.h))
+
+    def step(self, solver: VbdSolver, state, settings, grab=None) -> dict:
+        st = state.to_c()
+        m = abi.cv_metrics()
+        g = _grab(grab)
+        rc = lib().cv_gpu_step(self.h, solver.h, C.byref(st), C.byref(_settings(settings)),
+                               C.byref(g) if g else None, C.byref(m))
+        if rc != 0:
+            self._raise(rc)
+        state.time = st.time
+        return m.as_dict()
+
+    def run(self, solver: VbdSolver, state, settings, steps: int) -> dict:
+        st = state.to_c()
+        m = abi.cv_metrics()
+        rc = lib().cv_gpu_run(self.h, solver.h, C.byref(st), C.byref(_settings(settings)), steps,
+                              C.byref(m))
+        if rc != 0:
+            self._raise(rc)
+        state.time = st.time
+        return m.as_dict()
+
+    def probe(self, solver: VbdSolver, state, settings, grab=None) -> np.ndarray:
+        out = np.zeros((solver.n, 4))
+        g = _grab(grab)
+        rc = lib().cv_gpu_probe(self.h, solver.h, C.byref(state.to_c()),
+                                C.byref(_settings(settings)), C.byref(g) if g else None,
+                                abi.ptr(out, C.c_double))
+        if rc != 0:
+            self._raise(rc)
+        return out
+
+    def detect(self, solver: VbdSolver, x, rq, ogc: OgcSettings | None = None,
+               candidates: bool = True, capacity: int = 1 << 22) -> dict:
+        """Device OGC detection on f64 positions x (moving part): counts,
+        canonical contacts, radii and (optionally) the candidate pairs."""
+        ogc = ogc or OgcSettings()
+        x = np.ascontiguousarray(x, np.float64)
+        d = abi.cv_detection()
+        vf = np.zeros((capacity, 2), np.uint32) if candidates else None
+        ee = np.zeros((capacity, 2), np.uint32) if candidates else None
+        cki = np.zeros((capacity, 5), np.uint32)
+        ceps = np.zeros(capacity)
+        radii = np.zeros(solver.n)
+        rc = lib().cv_gpu_detect(self.h, solver.h, abi.ptr(x, C.c_double), rq,
+                                 C.byref(ogc.to_c()), C.byref(d), abi.ptr(vf, C.c_uint32),
+                                 capacity if candidates else 0, abi.ptr(ee, C.c_uint32),
+                                 capacity if candidates else 0, abi.ptr(cki, C.c_uint32),
+                                 capacity, abi.ptr(ceps, C.c_double), abi.ptr(radii, C.c_double))
+        if rc != 0:
+            self._raise(rc)
+        out = {name: getattr(d, name) for name, _ in d._fields_}
+        nc = int(d.n_contacts)
+        out["kind_ids"] = cki[:nc].copy()
+        out["eps"] = ceps[:nc].copy()
+        out["radii"] = radii
+        if candidates:
+            out["vf"] = vf[:int(d.n_vf_candidates)].copy()
+            out["ee"] = ee[:int(d.n_ee_candidates)].copy()
+        return out
+
+
+def version() -> str:
+    return lib().cv_version().decode()

@@ -1,0 +1,247 @@
+/*
+ * clothvbd.h — C ABI of libclothvbd.so, the MI355X Vertex Block Descent step.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes and never
+ * lets an exception cross the boundary.  Status codes map one-to-one to the
+ * error strings of the Rust API this library replaces; cv_last_error() returns
+ * the exact message.  On any status other than CV_OK the caller's state arrays
+ * are left untouched (the reference steps are transactional: src/vbd.rs:256-274,
+ * 399-405 and src/gpu_vbd.rs:132-152,170-172,435-439).
+ *
+ * Ownership: the caller owns every array passed in; create calls copy what
+ * they keep.  Threading: a handle is not thread-safe (the reference GpuVbd is
+ * !Sync, src/gpu_vbd.rs:23-24); one HIP stream per cv_gpu.
+ *
+ * Layouts follow the reference types: positions and velocities are
+ * xyz-interleaved f64 (glam DVec3, i.e. what Vec<DVec3>::as_ptr() yields),
+ * Dm^-1 is glam DMat2 column-major (x_axis.x, x_axis.y, y_axis.x, y_axis.y).
+ */
+#ifndef CLOTHVBD_H
+#define CLOTHVBD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+typedef enum cv_status {
+  CV_OK = 0,
+  CV_ERR_INVALID_ARGUMENT = 1,     /* null pointer, size mismatch in the call itself        */
+  CV_ERR_INVALID_SETTINGS = 2,     /* "Invalid VBD settings" (src/vbd.rs:38-51)             */
+  CV_ERR_INVALID_OGC_SETTINGS = 3, /* "Invalid OGC radius, stiffness ..." (ogc/world.rs:56) */
+  CV_ERR_INVALID_STATE = 4,        /* "Invalid VBD state" (src/vbd.rs:258-264)              */
+  CV_ERR_INVALID_GRAB = 5,         /* "Invalid mouse attachment" (src/vbd.rs:265-274)       */
+  CV_ERR_DIMENSIONS = 6,           /* "VBD state dimensions do not match garment"           */
+  CV_ERR_PIN_MISMATCH = 7,         /* "VBD pinned positions must match ..." (vbd.rs:110)    */
+  CV_ERR_TOPOLOGY = 8,             /* invalid triangle indices / non-manifold edge          */
+  CV_ERR_DEGENERATE_FACE = 9,      /* "OGC degenerate (initial) face"                       */
+  CV_ERR_INVALID_AVATAR = 10,      /* SurfaceMesh::validate (src/avatar.rs:78-97)           */
+  CV_ERR_INITIAL_INTERSECTION = 11,/* "OGC initial intersection between faces i and j"      */
+  CV_ERR_INITIAL_TOUCHING = 12,    /* "OGC initial touching/coplanar features"              */
+  CV_ERR_ZERO_SEPARATION = 13,     /* "OGC zero separation; step rejected"                  */
+  CV_ERR_SEPARATION_F32 = 14,      /* "GPU OGC separation below f32 safety margin"          */
+  CV_ERR_NONFINITE_RESULT = 15,    /* "Non-finite VBD result; state unchanged"              */
+  CV_ERR_TRUST_REGION = 16,        /* "OGC trust-region invariant failed"                   */
+  CV_ERR_FIXED_CHANGED = 17,       /* "OGC fixed geometry changed; rebuild the world"       */
+  CV_ERR_QUERY_RADIUS = 18,        /* "OGC query radius must cover the contact radius"      */
+  CV_ERR_INVALID_POSITIONS = 19,   /* "Invalid OGC positions" / invalid position array      */
+  CV_ERR_DEVICE = 20,              /* a HIP runtime call failed (message has the HIP text)  */
+  CV_ERR_CAPACITY = 21,            /* a request exceeds a documented size limit             */
+  CV_ERR_INVALID_SEAM = 22,        /* stitch weights invalid                                */
+  CV_ERR_NOT_BUILT = 23,           /* a feature (e.g. warm-start network) is not set up     */
+  CV_ERR_INVALID_MATERIAL = 24     /* Material::validate (src/model.rs:40-63)               */
+} cv_status;
+
+/* ------------------------------------------------------------- inputs */
+/* Material — src/model.rs:7-22 (all fields, same units). */
+typedef struct cv_material {
+  double warp, weft, shear;
+  double compression_ratio, compression_transition;
+  double bend_warp, bend_weft;
+  double density, thickness, grain_degrees;
+} cv_material;
+
+/* CompiledGarment — src/garment/template.rs:170-180, flattened.  Stitch
+ * samples are the seam Gauss samples (src/garment/seam.rs:269-272) with
+ * stitch_stiffness[s] = seam.stiffness * sample.weight_m, exactly the
+ * coefficient VbdSolver::new forms (src/vbd.rs:139-142).  Probes are the
+ * seam probe points used by max_seam_gap (src/garment/seam.rs:418-423). */
+typedef struct cv_garment {
+  uint32_t n_vertices;
+  const double* masses;           /* [n]                                        */
+  const uint8_t* pinned;          /* [n]  Mesh::pins[i].is_some()                */
+  const double* pin_targets;      /* [3n] Mesh::pins[i] value where pinned       */
+  uint32_t n_triangles;
+  const uint32_t* tri_vertices;   /* [3T] */
+  const double* tri_inv_dm;       /* [4T] */
+  const double* tri_area;         /* [T]  */
+  const uint32_t* tri_material;   /* [T]  */
+  uint32_t n_hinges;
+  const uint32_t* hinge_vertices; /* [4H] shared edge a,b then opposite c,d      */
+  const double* hinge_weight;     /* [H]  */
+  const double* hinge_across;     /* [2H] */
+  const uint32_t* hinge_material; /* [H]  */
+  uint32_t n_materials;
+  const cv_material* materials;   /* [M]  */
+  uint32_t n_stitches;
+  const uint32_t* stitch_offsets; /* [S+1] */
+  const uint32_t* stitch_vertices;
+  const double* stitch_weights;
+  const double* stitch_stiffness; /* [S] */
+  uint32_t n_probes;
+  const uint32_t* probe_offsets;  /* [P+1] */
+  const uint32_t* probe_vertices;
+  const double* probe_weights;
+} cv_garment;
+
+/* SurfaceMesh — src/avatar.rs:9-12: a static collider (fixed OGC geometry). */
+typedef struct cv_surface {
+  uint32_t n_vertices;
+  const double* vertices;         /* [3V] */
+  uint32_t n_triangles;
+  const uint32_t* triangles;      /* [3F] */
+} cv_surface;
+
+/* State — src/model.rs:116-121.  x and v are [3n] f64, updated in place on OK. */
+typedef struct cv_state {
+  double* x;
+  double* v;
+  uint32_t n;
+  double time;
+} cv_state;
+
+/* OgcSettings — src/ogc/world.rs:33-54. */
+typedef struct cv_ogc_settings {
+  double radius, stiffness, relaxation, refresh_fraction;
+} cv_ogc_settings;
+
+/* VbdSettings — src/vbd.rs:18-36.  Fields past `contact` are extensions with
+ * no reference counterpart; zero selects the reference behaviour. */
+typedef struct cv_settings {
+  double dt;
+  uint32_t iterations;
+  double damping;
+  double gravity[3];
+  cv_ogc_settings contact;
+  /* extensions */
+  double residual_tolerance;  /* >0: stop once the post-sweep residual <= tol (max `iterations`) */
+  uint32_t flags;             /* CV_FLAG_* */
+} cv_settings;
+
+enum {
+  CV_FLAG_DIAGNOSTICS_OFF = 1u << 0, /* skip max_seam_gap / extension_percent metrics      */
+  CV_FLAG_TRUE_RESIDUAL = 1u << 1,   /* compute the post-sweep residual every iteration     */
+  CV_FLAG_FAST_COLORING = 1u << 2,   /* device coloring (not the reference greedy order)    */
+  CV_FLAG_WARM_START = 1u << 3       /* learned initial guess in place of the adaptive one  */
+};
+
+/* Grab — src/vbd.rs:54-60. */
+typedef struct cv_grab {
+  uint32_t vertices[3];
+  double weights[3];
+  double target[3];
+  double stiffness;
+} cv_grab;
+
+/* VbdMetrics — src/vbd.rs:61-83 + DetectionProfile (src/ogc/world.rs:14-22).
+ * Optional reference fields are NaN when absent. */
+typedef struct cv_metrics {
+  double time_s, dt_s, step_ms, detection_ms, solve_ms, gpu_compute_ms;
+  double refit_ms, candidates_ms, narrow_ms;
+  uint64_t candidate_pairs, aabb_rejections;
+  uint32_t iterations;
+  uint64_t contacts, contact_terms, detection_updates, broad_phase_rebuilds, truncated_vertices;
+  double minimum_distance_m, max_seam_gap_m, extension_p95_percent, extension_max_percent;
+  double residual_rms_n;
+  /* extensions */
+  double true_residual_rms_n; /* sqrt(sum_free |g|^2 / 3 n_free) after the last sweep, or NaN */
+  uint32_t colors;            /* colour count of the last colouring                    */
+  uint32_t iterations_run;    /* sweeps actually run (< iterations with a tolerance)   */
+} cv_metrics;
+
+/* ------------------------------------------------------------- solver */
+typedef struct cv_solver cv_solver;
+typedef struct cv_gpu cv_gpu;
+
+/* VbdSolver::new — src/vbd.rs:101-192.  `avatar` may be NULL. */
+cv_status cv_solver_create(const cv_garment* garment, const cv_surface* avatar,
+                           const cv_state* state, cv_solver** out);
+void cv_solver_destroy(cv_solver* solver);
+
+/* Prediction::record — src/vbd/prediction.rs:181-191.  Installs the adaptive
+ * guess history as if `state` had just been committed from `old_velocity`. */
+cv_status cv_solver_record_prediction(cv_solver* solver, const double* old_velocity,
+                                      const cv_state* state, const cv_settings* settings);
+
+/* Greedy colouring of the last detection — src/vbd/coloring.rs:5-43.  Writes
+ * the colour of each moving vertex; returns the colour count in *n_colors. */
+cv_status cv_solver_colors(const cv_solver* solver, uint32_t* color_of_vertex, uint32_t n,
+                           uint32_t* n_colors);
+
+/* ---------------------------------------------------------------- GPU */
+/* GpuVbd::new — src/gpu_vbd.rs:56-124. */
+cv_status cv_gpu_create(int device, cv_gpu** out);
+void cv_gpu_destroy(cv_gpu* gpu);
+
+/* GpuVbd.check_gradients — src/gpu_vbd.rs:22.  When set, the next step
+ * verifies the device probe gradients against the f64 forward-mode
+ * derivatives at the same f32-rounded geometry (src/gpu_vbd.rs:305-379). */
+cv_status cv_gpu_set_check_gradients(cv_gpu* gpu, int enabled);
+
+/* GpuVbd::step — src/gpu_vbd.rs:125-472.  `grab` and `metrics` may be NULL. */
+cv_status cv_gpu_step(cv_gpu* gpu, cv_solver* solver, cv_state* state,
+                      const cv_settings* settings, const cv_grab* grab, cv_metrics* metrics);
+
+/* Device-resident stepping (no reference counterpart): upload `state`, run
+ * `steps` steps without host round trips of the state, download once. */
+cv_status cv_gpu_run(cv_gpu* gpu, cv_solver* solver, cv_state* state,
+                     const cv_settings* settings, uint32_t steps, cv_metrics* last_metrics);
+
+/* Device probe: per moving vertex (g.x, g.y, g.z, E) of the local objective at
+ * the state's f32-rounded geometry — the WGSL `probe` entry (src/vbd.wgsl:142-147).
+ * `out` is [4n] f64. */
+cv_status cv_gpu_probe(cv_gpu* gpu, cv_solver* solver, const cv_state* state,
+                       const cv_settings* settings, const cv_grab* grab, double* out);
+
+/* Detection only, for candidate-set parity: runs the device OGC detection on
+ * the given f64 positions (moving part; the avatar is appended) with query
+ * radius rq and returns counts.  When the arrays are non-NULL they receive the
+ * canonical sorted VF / EE candidate pairs and the contacts. */
+typedef struct cv_detection {
+  uint64_t n_vf_candidates, n_ee_candidates; /* pairs surviving the padded AABB test */
+  uint64_t vertex_face_candidates, edge_edge_candidates; /* pairs with distance < rq */
+  uint64_t n_contacts, active_contacts;
+  double minimum_distance;
+} cv_detection;
+cv_status cv_gpu_detect(cv_gpu* gpu, cv_solver* solver, const double* x, double rq,
+                        const cv_ogc_settings* settings, cv_detection* out,
+                        uint32_t* vf_pairs, uint64_t vf_capacity,
+                        uint32_t* ee_pairs, uint64_t ee_capacity,
+                        uint32_t* contact_kind_ids, uint64_t contact_capacity,
+                        double* contact_eps, double* radii);
+
+/* Batched uneven scenes (no reference counterpart): steps `count` independent
+ * solvers in one persistent launch.  States are updated in place. */
+cv_status cv_gpu_step_batch(cv_gpu* gpu, cv_solver* const* solvers, cv_state* const* states,
+                            uint32_t count, const cv_settings* settings, cv_metrics* metrics);
+
+/* ------------------------------------------------------------ errors */
+/* Copies the last error message of `gpu` (or, for NULL, the calling thread's
+ * last create error) into buf; returns the full message length. */
+size_t cv_last_error(const cv_gpu* gpu, char* buf, size_t n);
+/* Last error of a solver call (cv_solver_record_prediction). */
+size_t cv_solver_last_error(const cv_solver* solver, char* buf, size_t n);
+
+/* Lanes per vertex of the sweep kernel (8, 16, 32 or 64); the autotuner sets it. */
+cv_status cv_gpu_set_lanes(cv_gpu* gpu, int lanes);
+
+/* Library version string. */
+const char* cv_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLOTHVBD_H */
