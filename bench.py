@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Throughput of the device VBD step on the sheet-on-sphere workload.
+
+Workload: a 1 m x 1 m sheet of 128 x 128 quads (16,641 vertices, 32,768
+triangles, 32,258 hinges) dropped from 0.3 m onto a static icosphere collider
+(radius 0.25 m, subdivision 4, 5,120 triangles); StVK mu 2e4 N/m scaled per
+triangle by a seeded U[0.8, 1.2], lambda 8e3 N/m, bending 2e-5 N m, 0.2 kg/m^2;
+OGC contact radius 2 mm, stiffness 1e5 N/m, self-contact on; 20 VBD iterations
+per step, dt 1/60 s.  Synthetic, seeded geometry (no external data).
+
+One step is the complete VbdSolver::step of the reference (src/vbd.rs:194-436):
+prediction, OGC detection and colouring, 20 coloured Gauss-Seidel sweeps with
+the ball-truncated line search, mid-step re-detection, commit.  The timed
+region runs `--steps` steps device-resident after `--warmup` untimed steps
+(the sheet reaches the sphere after about 17 steps, so the default timed
+window is in contact).  `value` is steps/s from HIP events on the solver's
+stream around the device work, state already in HBM; the wall-clock rate
+including the one upload/download of the state is reported beside it.
+
+roofline: the sweep kernel (k_sweep), algorithmic bytes per launch at the f32
+minimum record sizes (64 B per free vertex + 72 per triangle, 52 per hinge,
+68 per stitch, 56 per contact incidence), divided by its mean launch time from
+HIP events recorded around every launch on the solver's stream.
+
+cpu_baseline: the f64 CPU restatement of VbdSolver::step (OpenMP within a
+colour) on the host, on a bounded sample: steps from the same state the timed
+window starts at, until about 15 s have elapsed.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+
+
+def _traffic():
+    """HBM bytes per sweep launch from the committed rocprofv3 counter runs
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "sweep_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(sc, state, settings, budget_s, threads):
+    """Time the oracle's f64 step on the host from `state` (bounded sample)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the CPU restatement: baseline only, never the product
+    st = state.copy()
+    ref = O.Solver(sc.garment, sc.avatar, st, threads=threads)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        ref.step(st, settings)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    return steps, el
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--precision", type=int, default=64, choices=(32, 64),
+                    help="sweep arithmetic (64: f64 default; 32: f32 opt-in)")
+    ap.add_argument("--lanes", type=int, default=32, choices=(8, 16, 32, 64))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host CPUs)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nx", type=int, default=128, help="quads per side (128 is the workload)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the step has no multi-GPU path in this build)", file=sys.stderr)
+        sys.exit(2)
+
+    from clothcraft_amd import GpuVbd, VbdSolver, scenes  # fails loudly if not built
+
+    sc = scenes.sheet_on_sphere(nx=args.nx)
+    settings = sc.settings
+    st = sc.state.copy()
+    gpu = GpuVbd(0)
+    gpu.set_precision(args.precision)
+    gpu.set_lanes(args.lanes)
+    sol = VbdSolver(sc.garment, sc.avatar, st)
+    print(f"[bench] {sc.name}: {sc.garment.n} vertices, warmup {args.warmup} steps", file=sys.stderr,
+          flush=True)
+    if args.warmup:
+        gpu.run(sol, st, settings, args.warmup)
+    start_state = st.copy()
+
+    print(f"[bench] timed: {args.steps} steps", file=sys.stderr, flush=True)
+    gpu.set_profiling(True)
+    t0 = time.perf_counter()
+    m = gpu.run(sol, st, settings, args.steps)
+    wall_s = time.perf_counter() - t0
+    gpu.set_profiling(False)
+    if not np.all(np.isfinite(st.x)):
+        raise RuntimeError("non-finite state after the timed steps")
+
+    K = args.steps
+    dev_s = m["gpu_compute_ms"] / 1e3
+    n_free = int(np.count_nonzero(sc.garment.pinned == 0))
+    vi = n_free * m["iterations_run"]  # vertex-iterations in the timed window
+    launches = max(m["sweep_launches"], 1)
+    sweep_s = m["sweep_ms"] / 1e3
+    avg_launch_s = sweep_s / launches
+    bytes_per_launch = m["sweep_bytes_min"] / launches
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+
+    out = {
+        "metric": "steps/s + vertex-iters/s at fixed residual; warm-start vs plain iters-to-tol",
+        "value": K / dev_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dev_s / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == 64 else "f32",
+        "data": "synthetic (seeded sheet and icosphere geometry)",
+        "config": {
+            "workload": f"sheet on sphere {args.nx}x{args.nx} quads: {sc.garment.n} vertices, "
+                        f"{len(sc.garment.tri_vertices)} triangles, icosphere r 0.25 m "
+                        f"({len(sc.avatar.triangles)} triangles), OGC r 2 mm k 1e5, "
+                        f"{settings['iterations']} iterations, dt 1/60, self-contact on",
+            "steps_timed": f"{args.warmup + 1}..{args.warmup + K} of the drop",
+            "iteration_mode": "fixed iterations (residual tolerance 0)",
+            "lanes_per_vertex": args.lanes,
+        },
+        "vertex_iters_per_s": vi / dev_s,
+        "vertex_iters_per_s_sweep_only": vi / sweep_s if sweep_s > 0 else None,
+        "steps_per_s_wall_incl_transfers": K / wall_s,
+        "residual_rms_n_last_step": m["residual_rms_n"],
+        "contacts_last_step": m["contact_terms"],
+        "colors_last_step": m["colors"],
+        "detections": m["detection_updates"],
+        "sweep_share_of_step": sweep_s / dev_s,
+        "roofline": {
+            "kernel": "k_sweep",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": _traffic(),
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_us": 1e6 * avg_launch_s,
+            "launches": m["sweep_launches"],
+            "bytes_per_launch_in_sweep_precision": m["sweep_bytes"] / launches,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        print(f"[bench] cpu baseline: oracle f64 step, {threads} threads, ~{args.cpu_seconds:.0f} s",
+              file=sys.stderr, flush=True)
+        cs, ce = cpu_baseline(sc, start_state, settings, args.cpu_seconds, threads)
+        out["cpu_baseline"] = {
+            "value": cs / ce,
+            "unit": "steps/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{cs} steps of the same workload from the state after step {args.warmup} "
+                      f"({ce:.1f} s, f64 C++ restatement of VbdSolver::step, OpenMP within a colour)",
+            "vertex_iters_per_s": cs * n_free * settings["iterations"] / ce,
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -69,6 +69,7 @@ struct DevSolver {
   uint32_t n = 0, n_fixed = 0, n_total = 0, F = 0, E = 0;
   DBuf<uint32_t> term_off;
   DBuf<cvm::TermRec> terms;
+  DBuf<cvm::TermRec64> terms_d;
   DBuf<double> masses;
   DBuf<uint8_t> pinned;
   DBuf<uint32_t> faces, edges, nb_off, nb, ex_off, ex;
@@ -88,7 +89,8 @@ struct DevSolver {
 struct StepBufs {
   DBuf<double> x64, x_old64, v64, v_old64, guess64, xcur64, radii64, centers64;
   DBuf<unsigned long long> dist_bits;
-  DBuf<float4> pos, tgt, ball;
+  DBuf<float4> pos, tgt, ball;        // f32 sweep
+  DBuf<double4> pos_d, tgt_d, ball_d;  // f64 sweep
   DBuf<float> gsq;
   DBuf<uint32_t> order;
   // detection
@@ -108,6 +110,7 @@ struct StepBufs {
   DBuf<unsigned long long> inc_key, inc_key_sorted;
   DBuf<uint32_t> coff, vcount;
   DBuf<cvm::TermRec> crec;
+  DBuf<cvm::TermRec64> crec_d;
   DBuf<uint32_t> cand_vf, cand_ee;  // optional candidate export
   DBuf<unsigned char> temp;         // rocPRIM temporary storage
   size_t contact_cap = 0;
@@ -133,6 +136,11 @@ struct cv_gpu {
   // Sweep launch shape (lanes per vertex), chosen by the autotuner.
   int lanes = 32;
   bool pred_valid = false;
+  // Live sweep-kernel timing (events around every sweep launch).
+  bool profile = false;
+  std::vector<hipEvent_t> sweep_events;
+  double static_bytes32 = 0, static_bytes64 = 0;  // algorithmic bytes of one sweep, static terms
+  bool f64 = true;  // sweep precision (f64 reproduces the f64 reference step)
 };
 
 // Detection on device positions b.x64 (all n_total vertices) with query

@@ -49,6 +49,11 @@ uint64_t cvh_solver_terms(const cv_solver* s, uint32_t* term_off, cvm::TermRec* 
   if (recs && cap >= s->terms.size()) std::memcpy(recs, s->terms.data(), s->terms.size() * sizeof(cvm::TermRec));
   return s->terms.size();
 }
+uint64_t cvh_solver_terms64(const cv_solver* s, cvm::TermRec64* recs, uint64_t cap) {
+  if (recs && cap >= s->terms64.size())
+    std::memcpy(recs, s->terms64.data(), s->terms64.size() * sizeof(cvm::TermRec64));
+  return s->terms64.size();
+}
 uint32_t cvh_solver_edges(const cv_solver* s, uint32_t* out, uint32_t cap) {
   uint32_t E = (uint32_t)(s->top.edges.size() / 2);
   if (out && cap >= E) std::memcpy(out, s->top.edges.data(), 2 * E * sizeof(uint32_t));
@@ -145,3 +150,85 @@ void cvh_closest_segments(const double* a, const double* b, const double* c, con
 }
 
 }  // extern "C"
+
+// Sequential host mirror of one colour of the device sweep (k_sweep) in R:
+// same record math, Newton step, truncation and accept test; the only
+// difference from the device is the order of the per-vertex term sums.
+namespace {
+template <class R>
+struct PosArr {
+  const R* p;  // [4 * n] xyz w
+  cvm::V3<R> operator()(uint32_t i) const { return {p[4 * i], p[4 * i + 1], p[4 * i + 2]}; }
+};
+template <class R>
+void sweep_color(uint32_t count, const uint32_t* order, R* pos, const R* tgt, const R* ball,
+                 const uint32_t* term_off, const cvm::TermRecT<R>* terms, const uint32_t* coff,
+                 const cvm::TermRecT<R>* crec, R cr, R ck, R* gsq, uint64_t* clipped, uint32_t* attempts) {
+  PosArr<R> P{pos};
+  std::vector<std::pair<uint32_t, cvm::V3<R>>> upd;
+  for (uint32_t gi = 0; gi < count; ++gi) {
+    uint32_t v = order[gi];
+    if (pos[4 * v + 3] > R(0.5)) {
+      gsq[v] = 0;
+      continue;
+    }
+    cvm::V3<R> x{pos[4 * v], pos[4 * v + 1], pos[4 * v + 2]};
+    uint32_t s0 = term_off[v], ns = term_off[v + 1] - s0, c0 = coff[v], nt = ns + (coff[v + 1] - c0);
+    cvm::Deriv<R> acc = cvm::deriv_zero<R>();
+    auto energy = [&](cvm::V3<R> p) {
+      R e = 0;
+      for (uint32_t i = 0; i < nt; ++i)
+        e += cvm::term_energy<R>(i < ns ? terms[s0 + i] : crec[c0 + i - ns], p, cr, ck, P);
+      return e;
+    };
+    for (uint32_t i = 0; i < nt; ++i)
+      cvm::deriv_add(acc, cvm::term_deriv<R>(i < ns ? terms[s0 + i] : crec[c0 + i - ns], x, cr, ck, P));
+    cvm::V3<R> target{tgt[4 * v], tgt[4 * v + 1], tgt[4 * v + 2]};
+    R inertia = tgt[4 * v + 3];
+    cvm::V3<R> dx = x - target;
+    R old = energy(x) + R(0.5) * inertia * cvm::dot(dx, dx);
+    cvm::V3<R> delta = cvm::newton_direction<R>(acc, x, target, inertia);
+    gsq[v] = cvm::dot(acc.g, acc.g);
+    cvm::V3<R> center{ball[4 * v], ball[4 * v + 1], ball[4 * v + 2]};
+    bool tr = false;
+    cvm::V3<R> accepted = x;
+    R scale = 1;
+    int a = 0;
+    for (; a < 10; ++a) {
+      cvm::V3<R> cand = cvm::truncate_ball<R>(x + delta * scale, center, ball[4 * v + 3], tr);
+      cvm::V3<R> dc = cand - target;
+      R e = energy(cand) + R(0.5) * inertia * cvm::dot(dc, dc);
+      if (cvm::finite3(cand) && std::isfinite(e) && e <= old + R(1e-14)) {
+        accepted = cand;
+        break;
+      }
+      scale *= R(0.5);
+    }
+    if (attempts) attempts[v] = a;
+    upd.push_back({v, accepted});
+    if (tr) (*clipped)++;
+  }
+  for (auto& [v, p] : upd) {
+    pos[4 * v] = p.x;
+    pos[4 * v + 1] = p.y;
+    pos[4 * v + 2] = p.z;
+  }
+}
+}  // namespace
+
+extern "C" {
+void cvh_sweep_color_f32(uint32_t count, const uint32_t* order, float* pos, const float* tgt, const float* ball,
+                         const uint32_t* term_off, const cvm::TermRec* terms, const uint32_t* coff,
+                         const cvm::TermRec* crec, double cr, double ck, float* gsq, uint64_t* clipped,
+                         uint32_t* attempts) {
+  sweep_color<float>(count, order, pos, tgt, ball, term_off, terms, coff, crec, (float)cr, (float)ck, gsq,
+                     clipped, attempts);
+}
+void cvh_sweep_color_f64(uint32_t count, const uint32_t* order, double* pos, const double* tgt,
+                         const double* ball, const uint32_t* term_off, const cvm::TermRec64* terms,
+                         const uint32_t* coff, const cvm::TermRec64* crec, double cr, double ck, double* gsq,
+                         uint64_t* clipped, uint32_t* attempts) {
+  sweep_color<double>(count, order, pos, tgt, ball, term_off, terms, coff, crec, cr, ck, gsq, clipped,
+                      attempts);
+}
+}

@@ -366,20 +366,21 @@ __global__ void k_contact_post(const double* x, const uint32_t* cids, const uint
   }
 }
 // Per-vertex contact term records in sorted incidence order.
+template <class F>
 __global__ void k_contact_records(const unsigned long long* inc_sorted, uint64_t m, const uint32_t* cids,
-                                  const uint32_t* ccount, const double* ceps, cvm::TermRec* rec) {
+                                  const uint32_t* ccount, const double* ceps, cvm::TermRecT<F>* rec) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   if (i >= m) return;
   uint32_t v = (uint32_t)(inc_sorted[i] >> 32), c = (uint32_t)(inc_sorted[i] & 0xFFFFFFFFull);
-  cvm::TermRec r;
+  cvm::TermRecT<F> r;
   int cnt = (int)ccount[c], slot = 0;
   for (int q = 0; q < 4; ++q) {
     r.ids[q] = cids[5 * c + 1 + q];
     if (q < cnt && r.ids[q] == v) slot = q;
   }
   r.head = cvm::rec_head(cvm::TERM_CONTACT, slot, cnt, (int)cids[5 * c]);
-  r.f[0] = (float)ceps[c];
-  for (int q = 1; q < 11; ++q) r.f[q] = 0.0f;
+  r.f[0] = (F)ceps[c];
+  for (int q = 1; q < 11; ++q) r.f[q] = F(0);
   rec[i] = r;
 }
 // Radii and minimum distance over moving vertices (single block, fixed order).
@@ -586,12 +587,17 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   g.h_scal->contacts = unique;
   uint64_t ninc = g.h_scal->incidences;
   b.crec.reserve(std::max<uint64_t>(ninc, 1));
+  b.crec_d.reserve(std::max<uint64_t>(ninc, 1));
   if (ninc) {
     with_temp(g, [&](void* t, size_t& bytes) {
       return rocprim::radix_sort_keys(t, bytes, b.inc_key.p, b.inc_key_sorted.p, (size_t)ninc, 0, 64, st);
     });
-    k_contact_records<<<blocks(ninc), TPB, 0, st>>>(b.inc_key_sorted.p, ninc, b.c_ids.p, b.c_count.p,
-                                                    b.c_eps.p, b.crec.p);
+    if (g.f64)
+      k_contact_records<double><<<blocks(ninc), TPB, 0, st>>>(b.inc_key_sorted.p, ninc, b.c_ids.p,
+                                                              b.c_count.p, b.c_eps.p, b.crec_d.p);
+    else
+      k_contact_records<float><<<blocks(ninc), TPB, 0, st>>>(b.inc_key_sorted.p, ninc, b.c_ids.p,
+                                                             b.c_count.p, b.c_eps.p, b.crec.p);
     CV_HIP(hipGetLastError());
   }
   if (g.h_scal->err_degenerate) fail(CV_ERR_DEGENERATE_FACE, "OGC degenerate face; step rejected");

@@ -410,7 +410,7 @@ void build_solver(cv_solver& s, const cv_garment* g, const cv_surface* avatar, c
   for (uint32_t i = 0; i < n; ++i) s.graph.insert(s.graph.end(), graph[i].begin(), graph[i].end());
 
   s.term_off.assign(n + 1, 0);
-  s.terms.clear();
+  s.terms64.clear();
   for (uint32_t v = 0; v < n; ++v) {
     for (uint32_t ti : tri_of[v]) {
       const uint32_t* tv = &s.tri_v[3 * ti];
@@ -423,22 +423,22 @@ void build_solver(cv_solver& s, const cv_garment* g, const cv_surface* avatar, c
       double cw[3] = {-ux - uy, ux, uy}, cu[3] = {-vx - vy, vx, vy};
       int slot = tv[0] == v ? 0 : (tv[1] == v ? 1 : 2);
       int o1 = (slot + 1) % 3, o2 = (slot + 2) % 3;
-      cvm::TermRec r{};
+      cvm::TermRec64 r{};
       r.head = cvm::rec_head(cvm::TERM_MEMBRANE, slot, 3, 0);
       r.ids[0] = tv[o1];
       r.ids[1] = tv[o2];
       r.ids[2] = r.ids[3] = 0;
-      r.f[0] = (float)cw[o1];
-      r.f[1] = (float)cw[o2];
-      r.f[2] = (float)cu[o1];
-      r.f[3] = (float)cu[o2];
-      r.f[4] = (float)s.tri_area[ti];
-      r.f[5] = (float)m.warp;
-      r.f[6] = (float)m.weft;
-      r.f[7] = (float)m.shear;
-      r.f[8] = (float)m.compression_ratio;
-      r.f[9] = (float)m.compression_transition;
-      s.terms.push_back(r);
+      r.f[0] = cw[o1];
+      r.f[1] = cw[o2];
+      r.f[2] = cu[o1];
+      r.f[3] = cu[o2];
+      r.f[4] = s.tri_area[ti];
+      r.f[5] = m.warp;
+      r.f[6] = m.weft;
+      r.f[7] = m.shear;
+      r.f[8] = m.compression_ratio;
+      r.f[9] = m.compression_transition;
+      s.terms64.push_back(r);
     }
     for (uint32_t hi : hinge_of[v]) {
       const uint32_t* hv = &s.hinge_v[4 * hi];
@@ -447,26 +447,32 @@ void build_solver(cv_solver& s, const cv_garment* g, const cv_surface* avatar, c
       double ax = s.hinge_across[2 * hi], ay = s.hinge_across[2 * hi + 1];
       double aw = ax * cs + ay * sn, af = -ax * sn + ay * cs;
       int slot = hv[0] == v ? 0 : (hv[1] == v ? 1 : (hv[2] == v ? 2 : 3));
-      cvm::TermRec r{};
+      cvm::TermRec64 r{};
       r.head = cvm::rec_head(cvm::TERM_HINGE, slot, 4, 0);
       for (int k = 0; k < 4; ++k) r.ids[k] = hv[k];
-      r.f[0] = (float)(s.hinge_weight[hi] * (m.bend_warp * aw * aw + m.bend_weft * af * af));
-      s.terms.push_back(r);
+      r.f[0] = (s.hinge_weight[hi] * (m.bend_warp * aw * aw + m.bend_weft * af * af));
+      s.terms64.push_back(r);
     }
     for (uint32_t si : st_of[v]) {
-      cvm::TermRec r{};
+      cvm::TermRec64 r{};
       int cnt = (int)(s.st_off[si + 1] - s.st_off[si]);
       int slot = 0;
       for (int k = 0; k < cnt; ++k) {
         r.ids[k] = s.st_v[s.st_off[si] + k];
-        r.f[k] = (float)s.st_w[s.st_off[si] + k];
+        r.f[k] = s.st_w[s.st_off[si] + k];
         if (r.ids[k] == v) slot = k;
       }
-      r.f[4] = (float)s.st_k[si];
+      r.f[4] = s.st_k[si];
       r.head = cvm::rec_head(cvm::TERM_STITCH, slot, cnt, 0);
-      s.terms.push_back(r);
+      s.terms64.push_back(r);
     }
-    s.term_off[v + 1] = (uint32_t)s.terms.size();
+    s.term_off[v + 1] = (uint32_t)s.terms64.size();
+  }
+  s.terms.resize(s.terms64.size());
+  for (size_t i = 0; i < s.terms64.size(); ++i) {
+    s.terms[i].head = s.terms64[i].head;
+    for (int k = 0; k < 4; ++k) s.terms[i].ids[k] = s.terms64[i].ids[k];
+    for (int k = 0; k < 11; ++k) s.terms[i].f[k] = (float)s.terms64[i].f[k];
   }
   static uint64_t counter = 0;
   s.generation = ++counter;

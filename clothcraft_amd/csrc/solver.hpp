@@ -53,7 +53,8 @@ struct cv_solver {
   OgcTopology top;
   std::vector<uint32_t> graph_off, graph;  // static conflict graph (sorted)
   std::vector<uint32_t> term_off;          // [n+1] static TermRec per vertex
-  std::vector<cvm::TermRec> terms;
+  std::vector<cvm::TermRec> terms;      // f32 coefficients (f32 sweep)
+  std::vector<cvm::TermRec64> terms64;  // f64 coefficients (f64 sweep)
   uint64_t generation = 0;                 // changes when a new solver is built
   double face_extent_median = 0;           // grid cell hint for device detection
 

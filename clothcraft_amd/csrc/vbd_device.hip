@@ -3,11 +3,15 @@
 //
 // Per-colour Gauss-Seidel sweep: a group of L lanes serves one vertex.  Each
 // lane evaluates the exact energy/gradient/Hessian of its incident terms
-// (64-byte records stored contiguously per vertex, so a group reads one
-// coalesced run), the group reduces with butterfly shuffles (every lane ends
-// with the identical sum), every lane forms the same Gershgorin-shifted
-// Newton step, and the ball-truncated backtracking search re-evaluates only
-// energies.  Positions are f32 relative to the garment centroid.
+// (records stored contiguously per vertex, so a group reads one coalesced
+// run), the group reduces with butterfly shuffles (every lane ends with the
+// identical sum), every lane forms the same Gershgorin-shifted Newton step,
+// and the ball-truncated backtracking search re-evaluates only energies.
+//
+// Positions are relative to the garment centroid.  The sweep is templated on
+// its scalar: f64 (default) reproduces the f64 reference step to round-off;
+// f32 is available as a faster mode whose strain resolution (one f32 ulp of
+// Green strain times the membrane stiffness) limits parity on stiff fabrics.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -16,7 +20,6 @@
 #include "gpu.hpp"
 
 using cvm::V3;
-using V3f = V3<float>;
 
 namespace {
 
@@ -28,174 +31,182 @@ double now_ms() {
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+template <class R>
+struct P4;
+template <>
+struct P4<float> {
+  using T = float4;
+  static __host__ __device__ T make(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
+};
+template <>
+struct P4<double> {
+  using T = double4;
+  static __host__ __device__ T make(double a, double b, double c, double d) { return make_double4(a, b, c, d); }
+};
+
+template <class R>
 struct GrabArgs {
   int active;
   uint32_t v[3];
-  float w[3];
-  float target[3];  // relative to the origin
-  float k;
+  R w[3];
+  R target[3];  // relative to the origin
+  R k;
 };
 
+template <class R>
 struct SweepArgs {
+  using T4 = typename P4<R>::T;
   const uint32_t* order;
-  float4* pos;
-  const float4* tgt;
-  const float4* ball;
+  T4* pos;
+  const T4* tgt;
+  const T4* ball;
   const uint32_t* term_off;
-  const cvm::TermRec* terms;
+  const cvm::TermRecT<R>* terms;
   const uint32_t* coff;
-  const cvm::TermRec* crec;
+  const cvm::TermRecT<R>* crec;
   float* gsq;
   unsigned long long* clipped;
-  float cr, ck;
-  GrabArgs grab;
+  R cr, ck;
+  GrabArgs<R> grab;
 };
 
-__device__ __forceinline__ V3f ldp(const float4* pos, uint32_t i) {
-  float4 q = pos[i];
-  return V3f{q.x, q.y, q.z};
-}
-__device__ __forceinline__ cvm::TermRec ldrec(const cvm::TermRec* r) {
-  const uint4* q = reinterpret_cast<const uint4*>(r);
-  uint4 a = q[0], b = q[1], c = q[2], d = q[3];
-  cvm::TermRec t;
-  uint32_t* w = reinterpret_cast<uint32_t*>(&t);
-  w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
-  w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
-  w[8] = c.x; w[9] = c.y; w[10] = c.z; w[11] = c.w;
-  w[12] = d.x; w[13] = d.y; w[14] = d.z; w[15] = d.w;
-  return t;
+template <class T4>
+__device__ __forceinline__ auto ldp(const T4* pos, uint32_t i) {
+  T4 q = pos[i];
+  using R = decltype(q.x);
+  return V3<R>{q.x, q.y, q.z};
 }
 
-template <int L>
-__device__ __forceinline__ float group_sum(float v) {
+template <int L, class R>
+__device__ __forceinline__ R group_sum(R v) {
 #pragma unroll
   for (int m = L / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, L);
   return v;
 }
 
 // Grab term of vertex v (src/vbd.rs:237-246): d = sum w_j x_j - target.
-__device__ __forceinline__ bool grab_d(const GrabArgs& g, uint32_t v, V3f xv, const float4* pos,
-                                       V3f& d, float& wv) {
+template <class R, class T4>
+__device__ __forceinline__ bool grab_d(const GrabArgs<R>& g, uint32_t v, V3<R> xv, const T4* pos, V3<R>& d,
+                                       R& wv) {
   if (!g.active) return false;
   int s = -1;
   for (int k = 0; k < 3; ++k)
     if (g.v[k] == v) s = k;
   if (s < 0) return false;
-  d = xv - V3f{g.target[0], g.target[1], g.target[2]};
+  d = xv - V3<R>{g.target[0], g.target[1], g.target[2]};
   for (int k = 0; k < 3; ++k)
     if (k != s) d = d + (ldp(pos, g.v[k]) - xv) * g.w[k];
   wv = g.w[s];
   return true;
 }
 
-template <int L>
-__device__ float local_energy(const SweepArgs& a, uint32_t v, V3f xv, int lane, uint32_t s0,
-                              uint32_t ns, uint32_t c0, uint32_t nt) {
+template <int L, class R>
+__device__ R local_energy(const SweepArgs<R>& a, uint32_t v, V3<R> xv, int lane, uint32_t s0, uint32_t ns,
+                          uint32_t c0, uint32_t nt) {
   auto P = [&](uint32_t i) { return ldp(a.pos, i); };
-  float e = 0.0f;
+  R e = R(0);
   for (uint32_t i = lane; i < nt; i += L) {
-    cvm::TermRec t = ldrec(i < ns ? &a.terms[s0 + i] : &a.crec[c0 + (i - ns)]);
-    e += cvm::term_energy<float>(t, xv, a.cr, a.ck, P);
+    const cvm::TermRecT<R>& t = i < ns ? a.terms[s0 + i] : a.crec[c0 + (i - ns)];
+    e += cvm::term_energy<R>(t, xv, a.cr, a.ck, P);
   }
   if (lane == 0) {
-    V3f d;
-    float wv;
-    if (grab_d(a.grab, v, xv, a.pos, d, wv)) e += 0.5f * a.grab.k * cvm::dot(d, d);
+    V3<R> d;
+    R wv;
+    if (grab_d(a.grab, v, xv, a.pos, d, wv)) e += R(0.5) * a.grab.k * cvm::dot(d, d);
   }
   return group_sum<L>(e);
 }
 
-// One colour of one iteration (src/vbd.rs:337-395).
-template <int L>
-__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs a, uint32_t off, uint32_t count) {
-  const int lane = threadIdx.x & (L - 1);
-  const uint32_t grp = (blockIdx.x * blockDim.x + threadIdx.x) / L;
-  if (grp >= count) return;
-  const uint32_t v = a.order[off + grp];
-  float4 p4 = a.pos[v];
-  if (p4.w > 0.5f) {  // pinned: no update, zero residual contribution
-    if (lane == 0) a.gsq[v] = 0.0f;
-    return;
-  }
-  V3f x{p4.x, p4.y, p4.z};
-  uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
-  uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
+template <int L, class R>
+__device__ cvm::Deriv<R> local_deriv(const SweepArgs<R>& a, uint32_t v, V3<R> x, int lane, uint32_t s0,
+                                     uint32_t ns, uint32_t c0, uint32_t nt) {
   auto P = [&](uint32_t i) { return ldp(a.pos, i); };
-  cvm::Deriv<float> acc = cvm::deriv_zero<float>();
+  cvm::Deriv<R> acc = cvm::deriv_zero<R>();
   for (uint32_t i = lane; i < nt; i += L) {
-    cvm::TermRec t = ldrec(i < ns ? &a.terms[s0 + i] : &a.crec[c0 + (i - ns)]);
-    cvm::deriv_add(acc, cvm::term_deriv<float>(t, x, a.cr, a.ck, P));
+    const cvm::TermRecT<R>& t = i < ns ? a.terms[s0 + i] : a.crec[c0 + (i - ns)];
+    cvm::deriv_add(acc, cvm::term_deriv<R>(t, x, a.cr, a.ck, P));
   }
   if (lane == 0) {
-    V3f d;
-    float wv;
-    if (grab_d(a.grab, v, x, a.pos, d, wv)) cvm::deriv_add(acc, cvm::stitch_d<float>(d, wv, a.grab.k));
+    V3<R> d;
+    R wv;
+    if (grab_d(a.grab, v, x, a.pos, d, wv)) cvm::deriv_add(acc, cvm::stitch_d<R>(d, wv, a.grab.k));
   }
+  acc.e = group_sum<L>(acc.e);
   acc.g.x = group_sum<L>(acc.g.x);
   acc.g.y = group_sum<L>(acc.g.y);
   acc.g.z = group_sum<L>(acc.g.z);
 #pragma unroll
   for (int k = 0; k < 6; ++k) acc.h.a[k] = group_sum<L>(acc.h.a[k]);
-  float4 tg = a.tgt[v];
-  V3f target{tg.x, tg.y, tg.z};
-  float inertia = tg.w;
+  return acc;
+}
+
+// One colour of one iteration (src/vbd.rs:337-395).
+template <int L, class R>
+__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uint32_t count) {
+  using T4 = typename P4<R>::T;
+  const int lane = threadIdx.x & (L - 1);
+  const uint32_t grp = (blockIdx.x * blockDim.x + threadIdx.x) / L;
+  if (grp >= count) return;
+  const uint32_t v = a.order[off + grp];
+  T4 p4 = a.pos[v];
+  if (p4.w > R(0.5)) {  // pinned: no update, zero residual contribution
+    if (lane == 0) a.gsq[v] = 0.0f;
+    return;
+  }
+  V3<R> x{p4.x, p4.y, p4.z};
+  uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
+  uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
+  cvm::Deriv<R> acc = local_deriv<L>(a, v, x, lane, s0, ns, c0, nt);
+  T4 tg = a.tgt[v];
+  V3<R> target{tg.x, tg.y, tg.z};
+  R inertia = tg.w;
   // The accept test compares energies from one evaluation routine, so a
   // candidate equal to x reproduces `old` exactly.
-  V3f dx = x - target;
-  float old = local_energy<L>(a, v, x, lane, s0, ns, c0, nt) + 0.5f * inertia * cvm::dot(dx, dx);
-  V3f delta = cvm::newton_direction<float>(acc, x, target, inertia);
-  float gg = cvm::dot(acc.g, acc.g);
-  float4 b4 = a.ball[v];
-  V3f center{b4.x, b4.y, b4.z};
+  V3<R> dx = x - target;
+  R old = local_energy<L>(a, v, x, lane, s0, ns, c0, nt) + R(0.5) * inertia * cvm::dot(dx, dx);
+  V3<R> delta = cvm::newton_direction<R>(acc, x, target, inertia);
+  R gg = cvm::dot(acc.g, acc.g);
+  T4 b4 = a.ball[v];
+  V3<R> center{b4.x, b4.y, b4.z};
   bool truncated = false;
-  V3f accepted = x;
-  float scale = 1.0f;
+  V3<R> accepted = x;
+  R scale = R(1);
   for (int attempt = 0; attempt < 10; ++attempt) {
-    V3f cand = cvm::truncate_ball<float>(x + delta * scale, center, b4.w, truncated);
-    V3f dc = cand - target;
-    float e = local_energy<L>(a, v, cand, lane, s0, ns, c0, nt) + 0.5f * inertia * cvm::dot(dc, dc);
-    if (cvm::finite3(cand) && isfinite(e) && e <= old + 1e-14f) {
+    V3<R> cand = cvm::truncate_ball<R>(x + delta * scale, center, b4.w, truncated);
+    V3<R> dc = cand - target;
+    R e = local_energy<L>(a, v, cand, lane, s0, ns, c0, nt) + R(0.5) * inertia * cvm::dot(dc, dc);
+    if (cvm::finite3(cand) && isfinite(e) && e <= old + R(1e-14)) {
       accepted = cand;
       break;
     }
-    scale *= 0.5f;
+    scale *= R(0.5);
   }
   if (lane == 0) {
-    a.pos[v] = make_float4(accepted.x, accepted.y, accepted.z, 0.0f);
-    a.gsq[v] = gg;
+    a.pos[v] = P4<R>::make(accepted.x, accepted.y, accepted.z, R(0));
+    a.gsq[v] = (float)gg;
     if (truncated) atomicAdd(a.clipped, 1ull);
   }
 }
 
 // Gradient probe (src/vbd.wgsl:142-147): (g, E) of the local objective.
-template <int L>
-__global__ void k_probe(SweepArgs a, uint32_t n, float4* out) {
+template <int L, class R>
+__global__ void k_probe(SweepArgs<R> a, uint32_t n, double* out) {
+  using T4 = typename P4<R>::T;
   const int lane = threadIdx.x & (L - 1);
   const uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / L;
   if (v >= n) return;
-  float4 p4 = a.pos[v];
-  V3f x{p4.x, p4.y, p4.z};
+  T4 p4 = a.pos[v];
+  V3<R> x{p4.x, p4.y, p4.z};
   uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
   uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
-  auto P = [&](uint32_t i) { return ldp(a.pos, i); };
-  cvm::Deriv<float> acc = cvm::deriv_zero<float>();
-  for (uint32_t i = lane; i < nt; i += L) {
-    cvm::TermRec t = ldrec(i < ns ? &a.terms[s0 + i] : &a.crec[c0 + (i - ns)]);
-    cvm::deriv_add(acc, cvm::term_deriv<float>(t, x, a.cr, a.ck, P));
-  }
+  cvm::Deriv<R> acc = local_deriv<L>(a, v, x, lane, s0, ns, c0, nt);
   if (lane == 0) {
-    V3f d;
-    float wv;
-    if (grab_d(a.grab, v, x, a.pos, d, wv)) cvm::deriv_add(acc, cvm::stitch_d<float>(d, wv, a.grab.k));
-  }
-  float e = group_sum<L>(acc.e);
-  float gx = group_sum<L>(acc.g.x), gy = group_sum<L>(acc.g.y), gz = group_sum<L>(acc.g.z);
-  if (lane == 0) {
-    float4 tg = a.tgt[v];
-    V3f dx = x - V3f{tg.x, tg.y, tg.z};
-    out[v] = make_float4(gx + dx.x * tg.w, gy + dx.y * tg.w, gz + dx.z * tg.w,
-                         e + 0.5f * tg.w * cvm::dot(dx, dx));
+    T4 tg = a.tgt[v];
+    V3<R> dx = x - V3<R>{tg.x, tg.y, tg.z};
+    out[4 * v] = acc.g.x + dx.x * tg.w;
+    out[4 * v + 1] = acc.g.y + dx.y * tg.w;
+    out[4 * v + 2] = acc.g.z + dx.z * tg.w;
+    out[4 * v + 3] = acc.e + R(0.5) * tg.w * cvm::dot(dx, dx);
   }
 }
 
@@ -244,18 +255,18 @@ __global__ void k_prologue_reduce(const double* x, const double* v, uint32_t n, 
 }
 
 // Targets, inertia, adaptive guess (src/vbd.rs:283-297, src/vbd/prediction.rs:15-41).
+template <class R>
 __global__ void k_prologue(const double* x, const double* v, const double* acc, const double* masses,
-                           const uint8_t* pinned, uint32_t n, const double* fixed, uint32_t n_fixed,
-                           double dt, double damping, double gx, double gy, double gz, int pred_valid,
-                           const Scalars* sc, float4* pos, float4* tgt, double* guess, double* x_old,
+                           const uint8_t* pinned, uint32_t n, const double* fixed, uint32_t n_fixed, double dt,
+                           double damping, double gx, double gy, double gz, int pred_valid, const Scalars* sc,
+                           typename P4<R>::T* pos, typename P4<R>::T* tgt, double* guess, double* x_old,
                            double* v_old) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   double ox = sc->origin[0], oy = sc->origin[1], oz = sc->origin[2];
   if (i >= n) {
     uint32_t j = i - n;
     if (j < n_fixed)
-      pos[i] = make_float4((float)(fixed[3 * j] - ox), (float)(fixed[3 * j + 1] - oy),
-                           (float)(fixed[3 * j + 2] - oz), 1.0f);
+      pos[i] = P4<R>::make((R)(fixed[3 * j] - ox), (R)(fixed[3 * j + 1] - oy), (R)(fixed[3 * j + 2] - oz), R(1));
     return;
   }
   double damp = 1.0 + damping * dt;
@@ -265,7 +276,7 @@ __global__ void k_prologue(const double* x, const double* v, const double* acc, 
   double t[3];
   for (int k = 0; k < 3; ++k) t[k] = p[k] + (w[k] * dt + g[k] * dt * dt) / damp;
   double inertia = masses[i] * damp / (dt * dt);
-  tgt[i] = make_float4((float)(t[0] - ox), (float)(t[1] - oy), (float)(t[2] - oz), (float)inertia);
+  tgt[i] = P4<R>::make((R)(t[0] - ox), (R)(t[1] - oy), (R)(t[2] - oz), (R)inertia);
   double g2 = gx * gx + gy * gy + gz * gz;
   double factor = 1.0;
   if (pred_valid && g2 > 1e-20) {
@@ -277,30 +288,36 @@ __global__ void k_prologue(const double* x, const double* v, const double* acc, 
     x_old[3 * i + k] = p[k];
     v_old[3 * i + k] = w[k];
   }
-  if (pinned[i])
-    pos[i] = make_float4((float)(p[0] - ox), (float)(p[1] - oy), (float)(p[2] - oz), 1.0f);
+  if (pinned[i]) pos[i] = P4<R>::make((R)(p[0] - ox), (R)(p[1] - oy), (R)(p[2] - oz), R(1));
 }
 
-// Balls from the last detection: f32 centre and a radius shrunk by the f32
-// round-off margin of src/gpu_vbd.rs:258-274.
+// Balls from the last detection.  In f32 the radius is shrunk by two ulps of
+// the coordinate magnitude so a truncated f32 point, converted back, never
+// leaves the f64 ball (the f32 safety margin of src/gpu_vbd.rs:258-274); the
+// f64 sweep uses the radius unchanged.
+template <class R>
 __global__ void k_balls(const double* xdet, const double* radii, uint32_t n, const Scalars* sc,
-                        float4* ball, double* centers, double* radii_shrunk) {
+                        typename P4<R>::T* ball, double* centers, double* radii_used) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double cx = xdet[3 * i] - sc->origin[0], cy = xdet[3 * i + 1] - sc->origin[1],
          cz = xdet[3 * i + 2] - sc->origin[2];
-  double m = fmax(fmax(fabs(cx), fabs(cy)), fmax(fabs(cz), 1.0));
-  double r = fmax(radii[i] - 16.0 * 1.1920928955078125e-07 * m, 0.0);
-  ball[i] = make_float4((float)cx, (float)cy, (float)cz, (float)r);
+  double r = radii[i];
+  if (sizeof(R) == 4) {
+    double m = fmax(fmax(fabs(cx), fabs(cy)), fabs(cz)) + r;
+    r = fmax(r - 2.0 * 1.1920928955078125e-07 * m, 0.0);
+  }
+  ball[i] = P4<R>::make((R)cx, (R)cy, (R)cz, (R)r);
   centers[3 * i] = xdet[3 * i];
   centers[3 * i + 1] = xdet[3 * i + 1];
   centers[3 * i + 2] = xdet[3 * i + 2];
-  radii_shrunk[i] = r;
+  radii_used[i] = r;
 }
 
 // Conservative warm start: the guess clipped into its ball (src/vbd.rs:316-324).
-__global__ void k_truncate_guess(const double* guess, const double* centers, const double* radii_shrunk,
-                                 const uint8_t* pinned, uint32_t n, const Scalars* sc, float4* pos,
+template <class R>
+__global__ void k_truncate_guess(const double* guess, const double* centers, const double* radii_used,
+                                 const uint8_t* pinned, uint32_t n, const Scalars* sc, typename P4<R>::T* pos,
                                  unsigned long long* clipped) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || pinned[i]) return;
@@ -309,17 +326,17 @@ __global__ void k_truncate_guess(const double* guess, const double* centers, con
     d[k] = guess[3 * i + k] - centers[3 * i + k];
     len2 += d[k] * d[k];
   }
-  double len = sqrt(len2), r = radii_shrunk[i];
+  double len = sqrt(len2), r = radii_used[i];
   double p[3];
   bool clip = len > r;
   for (int k = 0; k < 3; ++k) p[k] = clip ? centers[3 * i + k] + d[k] * (r / len) : guess[3 * i + k];
-  pos[i] = make_float4((float)(p[0] - sc->origin[0]), (float)(p[1] - sc->origin[1]),
-                       (float)(p[2] - sc->origin[2]), 0.0f);
+  pos[i] = P4<R>::make((R)(p[0] - sc->origin[0]), (R)(p[1] - sc->origin[1]), (R)(p[2] - sc->origin[2]), R(0));
   if (clip) atomicAdd(clipped, 1ull);
 }
 
 // Current f64 positions for a mid-step detection.
-__global__ void k_current(const float4* pos, const double* x_old, const uint8_t* pinned, uint32_t n,
+template <class R>
+__global__ void k_current(const typename P4<R>::T* pos, const double* x_old, const uint8_t* pinned, uint32_t n,
                           const Scalars* sc, double* x64) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -327,7 +344,7 @@ __global__ void k_current(const float4* pos, const double* x_old, const uint8_t*
     for (int k = 0; k < 3; ++k) x64[3 * i + k] = x_old[3 * i + k];
     return;
   }
-  float4 q = pos[i];
+  auto q = pos[i];
   x64[3 * i] = sc->origin[0] + (double)q.x;
   x64[3 * i + 1] = sc->origin[1] + (double)q.y;
   x64[3 * i + 2] = sc->origin[2] + (double)q.z;
@@ -335,20 +352,21 @@ __global__ void k_current(const float4* pos, const double* x_old, const uint8_t*
 
 // Commit (src/vbd.rs:399-411, src/vbd/prediction.rs:181-191): finiteness,
 // trust-region invariant, velocity, and the acceleration history.
-__global__ void k_finalize(const float4* pos, const double* x_old, const double* v_old,
-                           const double* centers, const double* radii, const uint8_t* pinned,
-                           uint32_t n, double dt, double damping, const Scalars* sc_in,
-                           double* x_out, double* v_out, double* acc, Scalars* sc) {
+template <class R>
+__global__ void k_finalize(const typename P4<R>::T* pos, const double* x_old, const double* v_old,
+                           const double* centers, const double* radii, const uint8_t* pinned, uint32_t n,
+                           double dt, double damping, double tol, double* x_out, double* v_out, double* acc,
+                           Scalars* sc) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double xn[3];
   if (pinned[i]) {
     for (int k = 0; k < 3; ++k) xn[k] = x_old[3 * i + k];
   } else {
-    float4 q = pos[i];
-    xn[0] = sc_in->origin[0] + (double)q.x;
-    xn[1] = sc_in->origin[1] + (double)q.y;
-    xn[2] = sc_in->origin[2] + (double)q.z;
+    auto q = pos[i];
+    xn[0] = sc->origin[0] + (double)q.x;
+    xn[1] = sc->origin[1] + (double)q.y;
+    xn[2] = sc->origin[2] + (double)q.z;
   }
   bool fin = isfinite(xn[0]) && isfinite(xn[1]) && isfinite(xn[2]);
   if (!fin) atomicOr(&sc->err_nonfinite, 1u);
@@ -357,7 +375,7 @@ __global__ void k_finalize(const float4* pos, const double* x_old, const double*
     double d = xn[k] - centers[3 * i + k];
     d2 += d * d;
   }
-  if (fin && !pinned[i] && sqrt(d2) > radii[i] + 1e-8) atomicOr(&sc->err_trust, 1u);
+  if (fin && !pinned[i] && sqrt(d2) > radii[i] + tol) atomicOr(&sc->err_trust, 1u);
   for (int k = 0; k < 3; ++k) {
     double vn = (xn[k] - x_old[3 * i + k]) / dt;
     x_out[3 * i + k] = xn[k];
@@ -388,11 +406,59 @@ void write_scalars(cv_gpu& g) {
   CV_HIP(hipMemcpyAsync(g.d_scal, g.h_scal, sizeof(Scalars), hipMemcpyHostToDevice, g.stream));
 }
 
-template <int L>
-void launch_sweep(const SweepArgs& a, uint32_t off, uint32_t count, hipStream_t st) {
-  k_sweep<L><<<blocks((uint64_t)count * L), TPB, 0, st>>>(a, off, count);
+// Device arrays of the sweep in precision R.
+template <class R>
+struct PBufs;
+template <>
+struct PBufs<float> {
+  static float4* pos(cv_gpu& g) { return g.b.pos.p; }
+  static float4* tgt(cv_gpu& g) { return g.b.tgt.p; }
+  static float4* ball(cv_gpu& g) { return g.b.ball.p; }
+  static const cvm::TermRec* terms(cv_gpu& g) { return g.dev.terms.p; }
+  static const cvm::TermRec* crec(cv_gpu& g) { return g.b.crec.p; }
+};
+template <>
+struct PBufs<double> {
+  static double4* pos(cv_gpu& g) { return g.b.pos_d.p; }
+  static double4* tgt(cv_gpu& g) { return g.b.tgt_d.p; }
+  static double4* ball(cv_gpu& g) { return g.b.ball_d.p; }
+  static const cvm::TermRec64* terms(cv_gpu& g) { return g.dev.terms_d.p; }
+  static const cvm::TermRec64* crec(cv_gpu& g) { return g.b.crec_d.p; }
+};
+
+template <class R>
+SweepArgs<R> sweep_args(cv_gpu& g, const cv_settings* set, const cv_grab* grab) {
+  SweepArgs<R> a{};
+  a.order = g.b.order.p;
+  a.pos = PBufs<R>::pos(g);
+  a.tgt = PBufs<R>::tgt(g);
+  a.ball = PBufs<R>::ball(g);
+  a.term_off = g.dev.term_off.p;
+  a.terms = PBufs<R>::terms(g);
+  a.coff = g.b.coff.p;
+  a.crec = PBufs<R>::crec(g);
+  a.gsq = g.b.gsq.p;
+  a.clipped = &g.d_scal->clipped;
+  a.cr = (R)set->contact.radius;
+  a.ck = (R)set->contact.stiffness;
+  a.grab.active = grab ? 1 : 0;
+  if (grab) {
+    for (int k = 0; k < 3; ++k) {
+      a.grab.v[k] = grab->vertices[k];
+      a.grab.w[k] = (R)grab->weights[k];
+      a.grab.target[k] = (R)(grab->target[k] - g.h_scal->origin[k]);
+    }
+    a.grab.k = (R)grab->stiffness;
+  }
+  return a;
 }
-void sweep_color(cv_gpu& g, const SweepArgs& a, uint32_t off, uint32_t count) {
+
+template <int L, class R>
+void launch_sweep(const SweepArgs<R>& a, uint32_t off, uint32_t count, hipStream_t st) {
+  k_sweep<L, R><<<blocks((uint64_t)count * L), TPB, 0, st>>>(a, off, count);
+}
+template <class R>
+void sweep_color(cv_gpu& g, const SweepArgs<R>& a, uint32_t off, uint32_t count) {
   switch (g.lanes) {
     case 8: launch_sweep<8>(a, off, count, g.stream); break;
     case 16: launch_sweep<16>(a, off, count, g.stream); break;
@@ -437,39 +503,16 @@ uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
 struct StepOut {
   uint64_t detections = 0, truncated = 0;
   uint32_t colors = 0, iterations_run = 0;
+  uint64_t sweep_launches = 0;
+  double sweep_ms = 0, sweep_bytes = 0, sweep_bytes_min = 0;
   double detection_ms = 0, refit_ms = 0, candidates_ms = 0, narrow_ms = 0;
   uint64_t candidate_pairs = 0, grid_tests = 0;
 };
 
-SweepArgs sweep_args(cv_gpu& g, const cv_settings* set, const cv_grab* grab) {
-  SweepArgs a{};
-  a.order = g.b.order.p;
-  a.pos = g.b.pos.p;
-  a.tgt = g.b.tgt.p;
-  a.ball = g.b.ball.p;
-  a.term_off = g.dev.term_off.p;
-  a.terms = g.dev.terms.p;
-  a.coff = g.b.coff.p;
-  a.crec = g.b.crec.p;
-  a.gsq = g.b.gsq.p;
-  a.clipped = &g.d_scal->clipped;
-  a.cr = (float)set->contact.radius;
-  a.ck = (float)set->contact.stiffness;
-  a.grab.active = grab ? 1 : 0;
-  if (grab) {
-    for (int k = 0; k < 3; ++k) {
-      a.grab.v[k] = grab->vertices[k];
-      a.grab.w[k] = (float)grab->weights[k];
-      a.grab.target[k] = (float)(grab->target[k] - g.h_scal->origin[k]);
-    }
-    a.grab.k = (float)grab->stiffness;
-  }
-  return a;
-}
-
 // Detection + balls (+ colouring) on the positions held in b.x64.
-void detect_and_color(cv_gpu& g, const cv_solver& s, const cv_settings* set, const cv_grab* grab,
-                      double rq, StepOut& out, bool first) {
+template <class R>
+void detect_and_color(cv_gpu& g, const cv_solver& s, const cv_settings* set, const cv_grab* grab, double rq,
+                      StepOut& out, bool first) {
   double t0 = now_ms();
   gpu_detect(g, s, rq, set->contact, 0);
   out.detections++;
@@ -478,12 +521,12 @@ void detect_and_color(cv_gpu& g, const cv_solver& s, const cv_settings* set, con
   double md = g.h_scal->min_distance;
   if (first && md <= 1e-12)
     fail(CV_ERR_ZERO_SEPARATION, "OGC zero separation; step rejected, reset the arrangement");
-  if (first && md < 2e-6)
+  if (first && sizeof(R) == 4 && md < 2e-6)
     fail(CV_ERR_SEPARATION_F32,
          "GPU OGC separation below f32 safety margin (2 micrometres); use f64 reference or reset");
   uint32_t n = s.n;
-  k_balls<<<blocks(n), TPB, 0, g.stream>>>(g.b.x64.p, g.b.radii64.p, n, g.d_scal, g.b.ball.p,
-                                           g.b.centers64.p, g.b.guess64.p + 3 * (size_t)n);
+  k_balls<R><<<blocks(n), TPB, 0, g.stream>>>(g.b.x64.p, g.b.radii64.p, n, g.d_scal, PBufs<R>::ball(g),
+                                              g.b.centers64.p, g.b.guess64.p + 3 * (size_t)n);
   CV_HIP(hipGetLastError());
   double t1 = now_ms();
   out.colors = color_on_host(g, s, grab);
@@ -491,14 +534,15 @@ void detect_and_color(cv_gpu& g, const cv_solver& s, const cv_settings* set, con
   out.narrow_ms += t1 - t0;
 }
 
-void run_step(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out) {
+template <class R>
+void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out) {
   DevSolver& d = g.dev;
   StepBufs& b = g.b;
   hipStream_t st = g.stream;
   uint32_t n = s.n;
   double dt = set->dt, damp = 1.0 + set->damping * dt;
-  k_prologue_reduce<<<1, 1024, 0, st>>>(b.x64.p, b.v64.p, n, dt, damp, set->gravity[0],
-                                       set->gravity[1], set->gravity[2], g.d_scal);
+  k_prologue_reduce<<<1, 1024, 0, st>>>(b.x64.p, b.v64.p, n, dt, damp, set->gravity[0], set->gravity[1],
+                                       set->gravity[2], g.d_scal);
   CV_HIP(hipGetLastError());
   read_scalars(g);
   double r = set->contact.radius;
@@ -506,31 +550,50 @@ void run_step(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* gr
   g.h_scal->clipped = 0;
   g.h_scal->err_nonfinite = g.h_scal->err_trust = 0;
   write_scalars(g);
-  k_prologue<<<blocks(n + d.n_fixed), TPB, 0, st>>>(
+  k_prologue<R><<<blocks(n + d.n_fixed), TPB, 0, st>>>(
       b.x64.p, b.v64.p, d.acc.p, d.masses.p, d.pinned.p, n, d.fixed_pos.p, d.n_fixed, dt, set->damping,
-      set->gravity[0], set->gravity[1], set->gravity[2], g.pred_valid ? 1 : 0, g.d_scal, b.pos.p, b.tgt.p,
-      b.guess64.p, b.x_old64.p, b.v_old64.p);
+      set->gravity[0], set->gravity[1], set->gravity[2], g.pred_valid ? 1 : 0, g.d_scal, PBufs<R>::pos(g),
+      PBufs<R>::tgt(g), b.guess64.p, b.x_old64.p, b.v_old64.p);
   CV_HIP(hipGetLastError());
-  detect_and_color(g, s, set, grab, rq, out, true);
-  k_truncate_guess<<<blocks(n), TPB, 0, st>>>(b.guess64.p, b.centers64.p, b.guess64.p + 3 * (size_t)n,
-                                              d.pinned.p, n, g.d_scal, b.pos.p, &g.d_scal->clipped);
+  detect_and_color<R>(g, s, set, grab, rq, out, true);
+  k_truncate_guess<R><<<blocks(n), TPB, 0, st>>>(b.guess64.p, b.centers64.p, b.guess64.p + 3 * (size_t)n,
+                                                 d.pinned.p, n, g.d_scal, PBufs<R>::pos(g), &g.d_scal->clipped);
   CV_HIP(hipGetLastError());
   read_scalars(g);
   uint64_t clipped = g.h_scal->clipped;
-  SweepArgs a = sweep_args(g, set, grab);
+  SweepArgs<R> a = sweep_args<R>(g, set, grab);
   uint32_t iters = set->iterations;
   for (uint32_t it = 0; it < iters; ++it) {
     if (clipped > 0 && (double)clipped >= set->contact.refresh_fraction * n) {
-      k_current<<<blocks(n), TPB, 0, st>>>(b.pos.p, b.x_old64.p, d.pinned.p, n, g.d_scal, b.x64.p);
+      k_current<R><<<blocks(n), TPB, 0, st>>>(PBufs<R>::pos(g), b.x_old64.p, d.pinned.p, n, g.d_scal, b.x64.p);
       CV_HIP(hipGetLastError());
-      detect_and_color(g, s, set, grab, rq, out, false);
-      a = sweep_args(g, set, grab);
+      detect_and_color<R>(g, s, set, grab, rq, out, false);
+      a = sweep_args<R>(g, set, grab);
     }
     CV_HIP(hipMemsetAsync(&g.d_scal->clipped, 0, sizeof(unsigned long long), st));
+    size_t nev = 0;
     for (size_t c = 0; c + 1 < g.color_off.size(); ++c) {
       uint32_t off = g.color_off[c], cnt = g.color_off[c + 1] - off;
-      if (cnt) sweep_color(g, a, off, cnt);
+      if (!cnt) continue;
+      if (g.profile) {
+        if (g.sweep_events.size() < 2 * (nev + 1)) {
+          hipEvent_t e0, e1;
+          CV_HIP(hipEventCreate(&e0));
+          CV_HIP(hipEventCreate(&e1));
+          g.sweep_events.push_back(e0);
+          g.sweep_events.push_back(e1);
+        }
+        CV_HIP(hipEventRecord(g.sweep_events[2 * nev], st));
+      }
+      sweep_color<R>(g, a, off, cnt);
+      if (g.profile) CV_HIP(hipEventRecord(g.sweep_events[2 * nev + 1], st));
+      nev += g.profile ? 1 : 0;
+      out.sweep_launches++;
     }
+    // Algorithmic bytes of this sweep: static terms + contact incidences.
+    out.sweep_bytes += sizeof(R) == 4 ? g.static_bytes32 + 56.0 * (double)g.h_scal->incidences
+                                      : g.static_bytes64 + 100.0 * (double)g.h_scal->incidences;
+    out.sweep_bytes_min += g.static_bytes32 + 56.0 * (double)g.h_scal->incidences;
     CV_HIP(hipGetLastError());
     unsigned long long hc = 0;
     CV_HIP(hipMemcpyAsync(&hc, &g.d_scal->clipped, sizeof(hc), hipMemcpyDeviceToHost, st));
@@ -538,17 +601,28 @@ void run_step(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* gr
     clipped = hc;
     out.truncated += clipped;
     out.iterations_run++;
+    for (size_t k = 0; k < nev; ++k) {
+      float ms = 0;
+      CV_HIP(hipEventElapsedTime(&ms, g.sweep_events[2 * k], g.sweep_events[2 * k + 1]));
+      out.sweep_ms += ms;
+    }
   }
   k_sum_f32<<<1, 1024, 0, st>>>(b.gsq.p, n, &g.d_scal->residual_sum);
-  // Restore the start-of-step positions in x64 before committing (the moving
-  // part may hold a mid-step detection's positions).
-  k_finalize<<<blocks(n), TPB, 0, st>>>(b.pos.p, b.x_old64.p, b.v_old64.p, b.centers64.p,
-                                        b.radii64.p, d.pinned.p, n, dt, set->damping, g.d_scal,
-                                        b.xcur64.p, b.v64.p, d.acc.p, g.d_scal);
+  // The trust-region check tolerance of the device path (src/gpu_vbd.rs:436).
+  k_finalize<R><<<blocks(n), TPB, 0, st>>>(PBufs<R>::pos(g), b.x_old64.p, b.v_old64.p, b.centers64.p,
+                                           b.radii64.p, d.pinned.p, n, dt, set->damping, 1e-8, b.xcur64.p,
+                                           b.v64.p, d.acc.p, g.d_scal);
   CV_HIP(hipGetLastError());
   read_scalars(g);
   if (g.h_scal->err_nonfinite) fail(CV_ERR_NONFINITE_RESULT, "Non-finite VBD result; state unchanged");
   if (g.h_scal->err_trust) fail(CV_ERR_TRUST_REGION, "GPU OGC trust-region check failed; state unchanged");
+}
+
+void run_step(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out) {
+  if (g.f64)
+    run_step_t<double>(g, s, set, grab, out);
+  else
+    run_step_t<float>(g, s, set, grab, out);
 }
 
 }  // namespace
@@ -565,6 +639,7 @@ void ensure_device_solver(cv_gpu& g, const cv_solver& s) {
   d.E = (uint32_t)(t.edges.size() / 2);
   d.term_off.upload(s.term_off, st);
   d.terms.upload(s.terms, st);
+  d.terms_d.upload(s.terms64, st);
   d.masses.upload(s.masses, st);
   d.pinned.upload(s.pinned, st);
   std::vector<uint32_t> ft(t.faces);
@@ -589,6 +664,27 @@ void ensure_device_solver(cv_gpu& g, const cv_solver& s) {
   d.fixed_abs_max = 0;
   for (double v : s.avatar) d.fixed_abs_max = std::max(d.fixed_abs_max, std::fabs(v));
   d.cell = s.face_extent_median;
+  // Algorithmic bytes of one vertex-iteration (each term's data and the
+  // neighbour positions read once, position written once):
+  //   f32: 64 fixed + 72 per triangle, 52 per hinge, 68 per stitch;
+  //   f64: 120 fixed + 136 per triangle, 92 per hinge, 124 per stitch.
+  // A pinned vertex reads its position only.
+  g.static_bytes32 = g.static_bytes64 = 0;
+  for (uint32_t v = 0; v < s.n; ++v) {
+    if (s.pinned[v]) {
+      g.static_bytes32 += 16;
+      g.static_bytes64 += 32;
+      continue;
+    }
+    double b32 = 64, b64 = 120;
+    for (uint32_t k = s.term_off[v]; k < s.term_off[v + 1]; ++k) {
+      uint32_t kind = cvm::rec_kind(s.terms[k].head);
+      b32 += kind == cvm::TERM_MEMBRANE ? 72 : (kind == cvm::TERM_HINGE ? 52 : 68);
+      b64 += kind == cvm::TERM_MEMBRANE ? 136 : (kind == cvm::TERM_HINGE ? 92 : 124);
+    }
+    g.static_bytes32 += b32;
+    g.static_bytes64 += b64;
+  }
   d.acc.reserve(3 * (size_t)s.n);
   d.acc_version = 0;
   d.acc_device = false;
@@ -610,6 +706,9 @@ void ensure_step_buffers(cv_gpu& g, const cv_solver& s) {
   b.pos.reserve(nt);
   b.tgt.reserve(n);
   b.ball.reserve(n);
+  b.pos_d.reserve(nt);
+  b.tgt_d.reserve(n);
+  b.ball_d.reserve(n);
   b.gsq.reserve(n);
   b.order.reserve(n);
   size_t cap = std::max<size_t>(1 << 20, 32 * n);
@@ -676,6 +775,12 @@ void fill_metrics(cv_metrics* m, const cv_gpu& g, const cv_solver& s, const cv_s
   m->true_residual_rms_n = NAN;
   m->colors = o.colors;
   m->iterations_run = o.iterations_run;
+  m->sweep_ms = g.profile ? o.sweep_ms : NAN;
+  m->f64 = g.f64 ? 1 : 0;
+  m->sweep_launches = o.sweep_launches;
+  m->sweep_bytes = o.sweep_bytes;
+  m->sweep_bytes_min = o.sweep_bytes_min;
+  m->steps = 1;
   if (!(set->flags & CV_FLAG_DIAGNOSTICS_OFF)) {
     double gap, p95, pmax;
     garment_diagnostics(s, x_new, gap, p95, pmax);
@@ -713,6 +818,35 @@ void stage_state(cv_gpu& g, cv_solver& s, const cv_state* st, const cv_settings*
 uint64_t next_acc_version() {
   static uint64_t v = 1ull << 40;
   return ++v;
+}
+
+template <class R>
+void probe_t(cv_gpu& g, cv_solver& s, const cv_state* st, const cv_settings* set, const cv_grab* grab,
+             double* out) {
+  size_t n = s.n;
+  hipStream_t stream = g.stream;
+  double dt = set->dt;
+  k_prologue<R><<<blocks(n + g.dev.n_fixed), TPB, 0, stream>>>(
+      g.b.x64.p, g.b.v64.p, g.dev.acc.p, g.dev.masses.p, g.dev.pinned.p, (uint32_t)n, g.dev.fixed_pos.p,
+      g.dev.n_fixed, dt, set->damping, set->gravity[0], set->gravity[1], set->gravity[2], 0, g.d_scal,
+      PBufs<R>::pos(g), PBufs<R>::tgt(g), g.b.guess64.p, g.b.x_old64.p, g.b.v_old64.p);
+  double r = set->contact.radius;
+  double rq = std::min(std::max(r + g.h_scal->motion, 2.0 * r), 4.0 * r);
+  gpu_detect(g, s, rq, set->contact, 0);
+  // Probe at the state's positions rounded to the sweep precision.
+  using T4 = typename P4<R>::T;
+  std::vector<T4> pos(n);
+  for (size_t i = 0; i < n; ++i)
+    pos[i] = P4<R>::make((R)(st->x[3 * i] - g.h_scal->origin[0]), (R)(st->x[3 * i + 1] - g.h_scal->origin[1]),
+                         (R)(st->x[3 * i + 2] - g.h_scal->origin[2]), s.pinned[i] ? R(1) : R(0));
+  CV_HIP(hipMemcpyAsync(PBufs<R>::pos(g), pos.data(), n * sizeof(T4), hipMemcpyHostToDevice, stream));
+  SweepArgs<R> a = sweep_args<R>(g, set, grab);
+  DBuf<double> outd;
+  outd.reserve(4 * n);
+  k_probe<32, R><<<blocks(n * 32), TPB, 0, stream>>>(a, (uint32_t)n, outd.p);
+  CV_HIP(hipGetLastError());
+  CV_HIP(hipMemcpyAsync(out, outd.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost, stream));
+  CV_HIP(hipStreamSynchronize(stream));
 }
 
 }  // namespace
@@ -772,8 +906,22 @@ cv_status cv_gpu_set_lanes(cv_gpu* g, int lanes) {
   return CV_OK;
 }
 
-cv_status cv_gpu_step(cv_gpu* g, cv_solver* s, cv_state* st, const cv_settings* set, const cv_grab* grab,
-                      cv_metrics* metrics) {
+cv_status cv_gpu_set_precision(cv_gpu* g, int bits) {
+  if (!g || (bits != 32 && bits != 64)) return CV_ERR_INVALID_ARGUMENT;
+  g->f64 = bits == 64;
+  return CV_OK;
+}
+
+cv_status cv_gpu_set_profiling(cv_gpu* g, int enabled) {
+  if (!g) return CV_ERR_INVALID_ARGUMENT;
+  g->profile = enabled != 0;
+  return CV_OK;
+}
+
+// Steps `steps` times; the state crosses PCIe once each way.  Each step has
+// the semantics of GpuVbd::step; on error the caller's state is untouched.
+static cv_status run_steps(cv_gpu* g, cv_solver* s, cv_state* st, const cv_settings* set, const cv_grab* grab,
+                           uint32_t steps, cv_metrics* metrics) {
   if (!g || !s) return CV_ERR_INVALID_ARGUMENT;
   return guard(&g->last_error, [&] {
     CV_HIP(hipSetDevice(g->device));
@@ -784,20 +932,39 @@ cv_status cv_gpu_step(cv_gpu* g, cv_solver* s, cv_state* st, const cv_settings* 
     ensure_device_solver(*g, *s);
     ensure_step_buffers(*g, *s);
     stage_state(*g, *s, st, set);
-    CV_HIP(hipEventRecord(g->ev0, g->stream));
-    StepOut o;
-    run_step(*g, *s, set, grab, o);
-    CV_HIP(hipEventRecord(g->ev1, g->stream));
     size_t n = s->n;
+    StepOut o, tot;
+    float gpu_ms = 0;
+    for (uint32_t k = 0; k < steps; ++k) {
+      o = StepOut{};
+      if (k > 0) {
+        // The previous step's commit is the new state: positions into x64,
+        // velocities already in v64, acceleration history on the device.
+        CV_HIP(hipMemcpyAsync(g->b.x64.p, g->b.xcur64.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice,
+                              g->stream));
+        g->pred_valid = true;
+        t0 = now_ms();
+      } else {
+        CV_HIP(hipEventRecord(g->ev0, g->stream));
+      }
+      run_step(*g, *s, set, grab, o);
+      tot.detections += o.detections;
+      tot.truncated += o.truncated;
+      tot.iterations_run += o.iterations_run;
+      tot.sweep_launches += o.sweep_launches;
+      tot.sweep_ms += o.sweep_ms;
+      tot.sweep_bytes += o.sweep_bytes;
+      tot.sweep_bytes_min += o.sweep_bytes_min;
+    }
+    CV_HIP(hipEventRecord(g->ev1, g->stream));
     CV_HIP(hipMemcpyAsync(g->h_x, g->b.xcur64.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     CV_HIP(hipMemcpyAsync(g->h_v, g->b.v64.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     CV_HIP(hipStreamSynchronize(g->stream));
-    float gpu_ms = 0;
     CV_HIP(hipEventElapsedTime(&gpu_ms, g->ev0, g->ev1));
     // Commit (all checks passed).
     std::memcpy(st->x, g->h_x, 3 * n * sizeof(double));
     std::memcpy(st->v, g->h_v, 3 * n * sizeof(double));
-    st->time += set->dt;
+    st->time += steps * set->dt;
     s->has_last = true;
     s->last_x.assign(st->x, st->x + 3 * n);
     s->last_v.assign(st->v, st->v + 3 * n);
@@ -808,19 +975,33 @@ cv_status cv_gpu_step(cv_gpu* g, cv_solver* s, cv_state* st, const cv_settings* 
     g->dev.acc_version = s->acc_version;
     s->acc.clear();
     double step_ms = now_ms() - t0;
+    if (steps > 1) {
+      // Run totals (see cv_gpu_run).
+      o.detections = tot.detections;
+      o.truncated = tot.truncated;
+      o.iterations_run = tot.iterations_run;
+      o.sweep_launches = tot.sweep_launches;
+      o.sweep_ms = tot.sweep_ms;
+      o.sweep_bytes = tot.sweep_bytes;
+      o.sweep_bytes_min = tot.sweep_bytes_min;
+    }
     fill_metrics(metrics, *g, *s, set, o, step_ms, gpu_ms, st->x);
-    if (metrics) metrics->time_s = st->time;
+    if (metrics) {
+      metrics->time_s = st->time;
+      metrics->steps = steps;
+    }
   });
+}
+
+cv_status cv_gpu_step(cv_gpu* g, cv_solver* s, cv_state* st, const cv_settings* set, const cv_grab* grab,
+                      cv_metrics* metrics) {
+  return run_steps(g, s, st, set, grab, 1, metrics);
 }
 
 cv_status cv_gpu_run(cv_gpu* g, cv_solver* s, cv_state* st, const cv_settings* set, uint32_t steps,
                      cv_metrics* last) {
-  if (!g || !s) return CV_ERR_INVALID_ARGUMENT;
-  for (uint32_t i = 0; i < steps; ++i) {
-    cv_status r = cv_gpu_step(g, s, st, set, nullptr, last);
-    if (r != CV_OK) return r;
-  }
-  return CV_OK;
+  if (steps == 0) return CV_OK;
+  return run_steps(g, s, st, set, nullptr, steps, last);
 }
 
 cv_status cv_gpu_probe(cv_gpu* g, cv_solver* s, const cv_state* st, const cv_settings* set,
@@ -841,39 +1022,10 @@ cv_status cv_gpu_probe(cv_gpu* g, cv_solver* s, const cv_state* st, const cv_set
     k_prologue_reduce<<<1, 1024, 0, stream>>>(g->b.x64.p, g->b.v64.p, (uint32_t)n, dt, damp,
                                               set->gravity[0], set->gravity[1], set->gravity[2], g->d_scal);
     read_scalars(*g);
-    k_prologue<<<blocks(n + g->dev.n_fixed), TPB, 0, stream>>>(
-        g->b.x64.p, g->b.v64.p, g->dev.acc.p, g->dev.masses.p, g->dev.pinned.p, (uint32_t)n,
-        g->dev.fixed_pos.p, g->dev.n_fixed, dt, set->damping, set->gravity[0], set->gravity[1],
-        set->gravity[2], 0, g->d_scal, g->b.pos.p, g->b.tgt.p, g->b.guess64.p, g->b.x_old64.p,
-        g->b.v_old64.p);
-    double r = set->contact.radius;
-    double rq = std::min(std::max(r + g->h_scal->motion, 2.0 * r), 4.0 * r);
-    gpu_detect(*g, *s, rq, set->contact, 0);
-    // Probe at the state's positions (all vertices, pinned included).
-    std::vector<float4> pos(n);
-    for (size_t i = 0; i < n; ++i) {
-      float4 q;
-      q.x = (float)(st->x[3 * i] - g->h_scal->origin[0]);
-      q.y = (float)(st->x[3 * i + 1] - g->h_scal->origin[1]);
-      q.z = (float)(st->x[3 * i + 2] - g->h_scal->origin[2]);
-      q.w = s->pinned[i] ? 1.0f : 0.0f;
-      pos[i] = q;
-    }
-    CV_HIP(hipMemcpyAsync(g->b.pos.p, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream));
-    SweepArgs a = sweep_args(*g, set, grab);
-    DBuf<float4> outd;
-    outd.reserve(n);
-    k_probe<32><<<blocks(n * 32), TPB, 0, stream>>>(a, (uint32_t)n, outd.p);
-    CV_HIP(hipGetLastError());
-    std::vector<float4> h(n);
-    CV_HIP(hipMemcpyAsync(h.data(), outd.p, n * sizeof(float4), hipMemcpyDeviceToHost, stream));
-    CV_HIP(hipStreamSynchronize(stream));
-    for (size_t i = 0; i < n; ++i) {
-      out[4 * i] = h[i].x;
-      out[4 * i + 1] = h[i].y;
-      out[4 * i + 2] = h[i].z;
-      out[4 * i + 3] = h[i].w;
-    }
+    if (g->f64)
+      probe_t<double>(*g, *s, st, set, grab, out);
+    else
+      probe_t<float>(*g, *s, st, set, grab, out);
   });
 }
 

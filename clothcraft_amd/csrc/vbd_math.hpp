@@ -27,6 +27,14 @@
 
 namespace cvm {
 
+#if !defined(__HIPCC__)
+using std::fabs;
+using std::fmin;
+using std::isfinite;
+using std::log;
+using std::sqrt;
+#endif
+
 template <class R>
 struct V3 {
   R x, y, z;
@@ -282,8 +290,8 @@ CV_HD R axial_e(R s, R k, R c, R delta) {
 // Record layout (f[]): a1 a2 | b1 b2 | area | warp weft shear | c delta.
 // a_i, b_i are the grain-rotated Dm^-1 coefficients of the two other vertices
 // (w = a1 (x1 - xv) + a2 (x2 - xv), v = b1 (x1 - xv) + b2 (x2 - xv)).
-template <class R>
-CV_HD R membrane_e(V3<R> xv, V3<R> x1, V3<R> x2, const float* f) {
+template <class R, class F>
+CV_HD R membrane_e(V3<R> xv, V3<R> x1, V3<R> x2, const F* f) {
   V3<R> e1 = x1 - xv, e2 = x2 - xv;
   V3<R> w = e1 * R(f[0]) + e2 * R(f[1]);
   V3<R> u = e1 * R(f[2]) + e2 * R(f[3]);
@@ -293,8 +301,8 @@ CV_HD R membrane_e(V3<R> xv, V3<R> x1, V3<R> x2, const float* f) {
   return R(f[4]) * (axial_e(ew, R(f[5]), R(f[8]), R(f[9])) + axial_e(ev, R(f[6]), R(f[8]), R(f[9])) +
                     R(0.5) * R(f[7]) * sh * sh);
 }
-template <class R>
-CV_HD Deriv<R> membrane_d(V3<R> xv, V3<R> x1, V3<R> x2, const float* f) {
+template <class R, class F>
+CV_HD Deriv<R> membrane_d(V3<R> xv, V3<R> x1, V3<R> x2, const F* f) {
   V3<R> e1 = x1 - xv, e2 = x2 - xv;
   V3<R> w = e1 * R(f[0]) + e2 * R(f[1]);
   V3<R> u = e1 * R(f[2]) + e2 * R(f[3]);
@@ -322,8 +330,8 @@ CV_HD Deriv<R> membrane_d(V3<R> xv, V3<R> x1, V3<R> x2, const float* f) {
 // --------------------------------------------------------------- bending
 // E = 1/2 kappa |n1/|n1| - n2/|n2||^2, n1 = (x1-x0)x(x2-x0), n2 = (x3-x0)x(x1-x0).
 // f[0] = kappa = weight (bend_warp aw^2 + bend_weft af^2) (src/energy.rs:40-45).
-template <class R>
-CV_HD R bending_e(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, const float* f) {
+template <class R, class F>
+CV_HD R bending_e(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, const F* f) {
   V3<R> e = x1 - x0;
   V3<R> n1 = cross(e, x2 - x0);
   V3<R> n2 = cross(x3 - x0, e);
@@ -343,8 +351,8 @@ CV_HD M3<R> unit_normal_curv(V3<R> nh, V3<R> y, R l) {
       t.m[i][j] = (R(3) * yn * n[i] * n[j] - yy[i] * n[j] - yy[j] * n[i] - (i == j ? yn : R(0))) * il2;
   return t;
 }
-template <class R>
-CV_HD Deriv<R> bending_d(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, int slot, const float* f) {
+template <class R, class F>
+CV_HD Deriv<R> bending_d(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, int slot, const F* f) {
   R kappa = R(f[0]);
   V3<R> e = x1 - x0, u = x2 - x0, w = x3 - x0;
   V3<R> n1 = cross(e, u);
@@ -522,12 +530,16 @@ enum : uint32_t {
   TERM_STITCH = 2,
   TERM_CONTACT = 3,
 };
-struct alignas(16) TermRec {
+template <class F>
+struct alignas(16) TermRecT {
   uint32_t head;     // kind | slot << 4 | count << 8 | contact kind << 12
   uint32_t ids[4];   // membrane: 2 others; hinge: x0..x3; stitch/contact: all vertices
-  float f[11];
+  F f[11];
 };
+using TermRec = TermRecT<float>;     // f32 sweep: 64 bytes
+using TermRec64 = TermRecT<double>;  // f64 sweep: 112 bytes
 static_assert(sizeof(TermRec) == 64, "TermRec must be 64 bytes");
+static_assert(sizeof(TermRec64) == 112, "TermRec64 must be 112 bytes");
 CV_HD uint32_t rec_kind(uint32_t h) { return h & 0xFu; }
 CV_HD int rec_slot(uint32_t h) { return (int)((h >> 4) & 0xFu); }
 CV_HD int rec_count(uint32_t h) { return (int)((h >> 8) & 0xFu); }
@@ -537,15 +549,15 @@ CV_HD uint32_t rec_head(uint32_t kind, int slot, int count, int ckind) {
 }
 
 // Position lookup abstraction: P(i) returns the relative position of vertex i.
-template <class R, class PosFn>
-CV_HD R term_energy(const TermRec& t, V3<R> xv, R cr, R ck, PosFn&& P) {
+template <class R, class F, class PosFn>
+CV_HD R term_energy(const TermRecT<F>& t, V3<R> xv, R cr, R ck, PosFn&& P) {
   uint32_t kind = rec_kind(t.head);
-  if (kind == TERM_MEMBRANE) return membrane_e<R>(xv, P(t.ids[0]), P(t.ids[1]), t.f);
+  if (kind == TERM_MEMBRANE) return membrane_e<R, F>(xv, P(t.ids[0]), P(t.ids[1]), t.f);
   if (kind == TERM_HINGE) {
     int s = rec_slot(t.head);
     V3<R> x[4];
     for (int i = 0; i < 4; ++i) x[i] = i == s ? xv : P(t.ids[i]);
-    return bending_e<R>(x[0], x[1], x[2], x[3], t.f);
+    return bending_e<R, F>(x[0], x[1], x[2], x[3], t.f);
   }
   if (kind == TERM_STITCH) {
     int s = rec_slot(t.head), n = rec_count(t.head);
@@ -559,15 +571,15 @@ CV_HD R term_energy(const TermRec& t, V3<R> xv, R cr, R ck, PosFn&& P) {
   for (int i = 0; i < 4; ++i) x[i] = i == s ? xv : P(t.ids[i]);
   return contact_e<R>(rec_ckind(t.head), x, R(t.f[0]), cr, ck);
 }
-template <class R, class PosFn>
-CV_HD Deriv<R> term_deriv(const TermRec& t, V3<R> xv, R cr, R ck, PosFn&& P) {
+template <class R, class F, class PosFn>
+CV_HD Deriv<R> term_deriv(const TermRecT<F>& t, V3<R> xv, R cr, R ck, PosFn&& P) {
   uint32_t kind = rec_kind(t.head);
-  if (kind == TERM_MEMBRANE) return membrane_d<R>(xv, P(t.ids[0]), P(t.ids[1]), t.f);
+  if (kind == TERM_MEMBRANE) return membrane_d<R, F>(xv, P(t.ids[0]), P(t.ids[1]), t.f);
   int s = rec_slot(t.head);
   if (kind == TERM_HINGE) {
     V3<R> x[4];
     for (int i = 0; i < 4; ++i) x[i] = i == s ? xv : P(t.ids[i]);
-    return bending_d<R>(x[0], x[1], x[2], x[3], s, t.f);
+    return bending_d<R, F>(x[0], x[1], x[2], x[3], s, t.f);
   }
   if (kind == TERM_STITCH) {
     int n = rec_count(t.head);

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "libcvm_host.so")
 
 TERM_DTYPE = np.dtype([("head", "<u4"), ("ids", "<u4", 4), ("f", "<f4", 11)])
 assert TERM_DTYPE.itemsize == 64
+TERM64_DTYPE = np.dtype({"names": ["head", "ids", "f"], "formats": ["<u4", ("<u4", 4), ("<f8", 11)],
+                         "offsets": [0, 4, 24], "itemsize": 112})
 
 _lib = None
 
@@ -84,6 +86,15 @@ class HostSolver:
         lib().cvh_solver_terms(self.h, abi.ptr(off, C.c_uint32), recs.ctypes.data, k)
         return off, recs
 
+    def terms64(self):
+        L = lib()
+        L.cvh_solver_terms64.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+        L.cvh_solver_terms64.restype = C.c_uint64
+        k = L.cvh_solver_terms64(self.h, None, 0)
+        recs = np.zeros(k, TERM64_DTYPE)
+        L.cvh_solver_terms64(self.h, recs.ctypes.data, k)
+        return recs
+
     def edges(self):
         out = np.zeros((1 << 21, 2), np.uint32)
         k = lib().cvh_solver_edges(self.h, abi.ptr(out, C.c_uint32), 1 << 21)
@@ -128,3 +139,23 @@ def contact_record(kind, ids, count, eps, v):
 def sym(h6) -> np.ndarray:
     a = h6
     return np.array([[a[0], a[1], a[2]], [a[1], a[3], a[4]], [a[2], a[4], a[5]]])
+
+
+def sweep_color(order, pos, tgt, ball, term_off, terms, coff, crec, cr, ck, gsq, f32=True):
+    """Host mirror of one device sweep launch (k_sweep) over `order`.
+    pos/tgt/ball are [n,4] arrays of the precision used (float32 or float64),
+    updated in place; returns (clipped count, per-vertex attempt index)."""
+    L = lib()
+    fn = L.cvh_sweep_color_f32 if f32 else L.cvh_sweep_color_f64
+    ct = C.c_float if f32 else C.c_double
+    fp = C.POINTER(ct)
+    fn.argtypes = [C.c_uint32, abi.u32p, fp, fp, fp, abi.u32p, C.c_void_p, abi.u32p, C.c_void_p,
+                   C.c_double, C.c_double, fp, C.POINTER(C.c_uint64), abi.u32p]
+    clipped = C.c_uint64(0)
+    att = np.zeros(len(pos), np.uint32)
+    order = np.ascontiguousarray(order, np.uint32)
+    fn(len(order), abi.ptr(order, C.c_uint32), abi.ptr(pos, ct), abi.ptr(tgt, ct), abi.ptr(ball, ct),
+       abi.ptr(term_off, C.c_uint32), terms.ctypes.data, abi.ptr(coff, C.c_uint32),
+       crec.ctypes.data if len(crec) else None, cr, ck, abi.ptr(gsq, ct), C.byref(clipped),
+       abi.ptr(att, C.c_uint32))
+    return int(clipped.value), att

@@ -47,7 +47,7 @@ class Material:
 
     @staticmethod
     def stvk(mu: float, lam: float, bend: float, density: float, grain: float = 0.0) -> "Material":
-        """The configs' "StVK mu, lambda" in the reference law: compression_ratio 1,
+        """An "StVK mu, lambda" material in the reference law: compression_ratio 1,
         warp = weft = 2 mu + lambda, shear = mu (isotropic StVK minus the
         lambda*E11*E22 Poisson coupling, which the orthotropic law has no term for)."""
         return Material(warp=2 * mu + lam, weft=2 * mu + lam, shear=mu, compression_ratio=1.0,
@@ -316,7 +316,7 @@ def icosphere(radius: float, subdivisions: int, center=(0.0, 0.0, 0.0)) -> Surfa
     return Surface(V, F)
 
 
-# ---------------------------------------------------------------- configs
+# ------------------------------------------------------------- workloads
 def default_settings(**kw) -> dict:
     """VbdSettings::default() (src/vbd.rs:26-36) with OgcSettings::default()."""
     s = dict(dt=1.0 / 240.0, iterations=16, damping=1.5, gravity=(0.0, -9.81, 0.0),
@@ -326,7 +326,7 @@ def default_settings(**kw) -> dict:
     return s
 
 
-def config0(seed: int = 0) -> Scene:
+def hanging_sheet(seed: int = 0) -> Scene:
     """Plain VBD sheet: 1.0 x 1.0 m, 32x32 quads (1,089 vertices, 2,048 triangles),
     hanging from the two top corners, mu 2e4, lambda 8e3, bending 2e-5, density 0.2,
     dt 1/60, 10 iterations, 120 steps, no collider.  Unpinned vertices jittered by
@@ -337,11 +337,11 @@ def config0(seed: int = 0) -> Scene:
     jit = rng.uniform(-1e-4, 1e-4, size=s.x.shape)
     jit[g.pinned.astype(bool)] = 0.0
     s.x = np.ascontiguousarray(s.x + jit)
-    return Scene("config0_hanging_sheet_32x32", g, s, None,
+    return Scene("hanging_sheet_32x32", g, s, None,
                  default_settings(dt=1.0 / 60.0, iterations=10), 120, {"seed": seed})
 
 
-def config1(seed: int = 1, nx: int = 128) -> Scene:
+def sheet_on_sphere(seed: int = 1, nx: int = 128) -> Scene:
     """Sheet on sphere: nx x nx quads over 1 m^2 (128: 16,641 vertices, 32,768
     triangles), horizontal, 0.3 m above the top of a static icosphere collider
     (radius 0.25 m at the origin, subdivision 4, 5,120 triangles).  Per-triangle
@@ -356,14 +356,14 @@ def config1(seed: int = 1, nx: int = 128) -> Scene:
     g.hinge_material = np.zeros(len(g.hinge_weight), np.uint32)
     s.x[:, 1] = 0.25 + 0.3
     sphere = icosphere(0.25, 4)
-    return Scene(f"config1_sheet_on_sphere_{nx}x{nx}", g, s, sphere,
+    return Scene(f"sheet_on_sphere_{nx}x{nx}", g, s, sphere,
                  default_settings(dt=1.0 / 60.0, iterations=20, radius=0.002, stiffness=1e5),
                  300, {"seed": seed, "sphere_center": (0.0, 0.0, 0.0), "sphere_radius": 0.25})
 
 
 def small_sheet_on_sphere(nx: int = 16, height: float = 0.255, seed: int = 3) -> Scene:
-    """A reduced config-1 scene (nx x nx quads) for fast parity tests."""
-    sc = config1(seed=seed, nx=nx)
+    """The sheet-on-sphere scene at nx x nx quads, placed at the given height."""
+    sc = sheet_on_sphere(seed=seed, nx=nx)
     sc.state.x[:, 1] = height
     sc.name = f"sheet_on_sphere_{nx}x{nx}"
     return sc

@@ -53,6 +53,8 @@ def lib():
         L.cv_gpu_destroy.argtypes = [vp]
         L.cv_gpu_set_check_gradients.argtypes = [vp, C.c_int]
         L.cv_gpu_set_lanes.argtypes = [vp, C.c_int]
+        L.cv_gpu_set_profiling.argtypes = [vp, C.c_int]
+        L.cv_gpu_set_precision.argtypes = [vp, C.c_int]
         L.cv_gpu_step.argtypes = [vp, vp, C.POINTER(abi.cv_state), C.POINTER(abi.cv_settings),
                                   C.POINTER(abi.cv_grab), C.POINTER(abi.cv_metrics)]
         L.cv_gpu_run.argtypes = [vp, vp, C.POINTER(abi.cv_state), C.POINTER(abi.cv_settings),
@@ -215,6 +217,15 @@ class GpuVbd:
         rc = lib().cv_gpu_set_lanes(self.h, lanes)
         if rc != 0:
             raise VbdError(rc, "lanes must be 8, 16, 32 or 64")
+
+    def set_precision(self, bits: int):
+        """64 (default, reference parity) or 32 (faster f32 sweep)."""
+        rc = lib().cv_gpu_set_precision(self.h, bits)
+        if rc != 0:
+            raise VbdError(rc, "precision must be 32 or 64")
+
+    def set_profiling(self, on: bool):
+        lib().cv_gpu_set_profiling(self.h, int(on))
 
     def _raise(self, rc):
         raise VbdError(rc, _error(self.h))

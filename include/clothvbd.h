@@ -160,6 +160,14 @@ typedef struct cv_metrics {
   double true_residual_rms_n; /* sqrt(sum_free |g|^2 / 3 n_free) after the last sweep, or NaN */
   uint32_t colors;            /* colour count of the last colouring                    */
   uint32_t iterations_run;    /* sweeps actually run (< iterations with a tolerance)   */
+  double sweep_ms;            /* summed device time of the sweep kernels (profiling on), else NaN */
+  uint64_t sweep_launches;    /* sweep kernel launches in the step                     */
+  double sweep_bytes;         /* algorithmic bytes those launches move in the sweep's precision */
+  uint32_t f64;               /* 1 when the sweep ran in f64, 0 for f32                */
+  double sweep_bytes_min;     /* the same launches priced at the f32 record sizes:
+                                 64 B per free vertex + 72 per triangle, 52 per hinge,
+                                 68 per stitch, 56 per contact incidence             */
+  uint32_t steps;             /* steps these metrics cover (1 for cv_gpu_step)         */
 } cv_metrics;
 
 /* ------------------------------------------------------------- solver */
@@ -196,7 +204,13 @@ cv_status cv_gpu_step(cv_gpu* gpu, cv_solver* solver, cv_state* state,
                       const cv_settings* settings, const cv_grab* grab, cv_metrics* metrics);
 
 /* Device-resident stepping (no reference counterpart): upload `state`, run
- * `steps` steps without host round trips of the state, download once. */
+ * `steps` steps without host round trips of the state, download once.  Each
+ * step has GpuVbd::step semantics; the run is transactional as a whole.
+ * `last` describes the last step, except gpu_compute_ms (device time from the
+ * first step's first kernel to the last step's last kernel, state already in
+ * HBM), iterations_run, sweep_ms, sweep_launches, sweep_bytes,
+ * sweep_bytes_min, detection_updates and truncated_vertices, which are totals
+ * over the run, and steps = `steps`. */
 cv_status cv_gpu_run(cv_gpu* gpu, cv_solver* solver, cv_state* state,
                      const cv_settings* settings, uint32_t steps, cv_metrics* last_metrics);
 
@@ -234,6 +248,13 @@ cv_status cv_gpu_step_batch(cv_gpu* gpu, cv_solver* const* solvers, cv_state* co
 size_t cv_last_error(const cv_gpu* gpu, char* buf, size_t n);
 /* Last error of a solver call (cv_solver_record_prediction). */
 size_t cv_solver_last_error(const cv_solver* solver, char* buf, size_t n);
+
+/* Sweep precision: 64 (default; reproduces the f64 reference step to
+ * round-off) or 32 (faster; strain resolution limits parity on stiff fabrics). */
+cv_status cv_gpu_set_precision(cv_gpu* gpu, int bits);
+
+/* Events around every sweep launch; metrics.sweep_ms then reports their sum. */
+cv_status cv_gpu_set_profiling(cv_gpu* gpu, int enabled);
 
 /* Lanes per vertex of the sweep kernel (8, 16, 32 or 64); the autotuner sets it. */
 cv_status cv_gpu_set_lanes(cv_gpu* gpu, int lanes);

@@ -556,3 +556,54 @@ def extension_percent(garment, x):
 def max_seam_gap(garment, x):
     g = garment.to_c()
     return lib().cvo_max_seam_gap(C.byref(g), _p(_f(x)))
+
+
+class Prediction:
+    """Prediction (src/vbd/prediction.rs:8-54)."""
+
+    def __init__(self):
+        L = lib()
+        L.cvo_prediction_create.restype = C.c_void_p
+        L.cvo_prediction_destroy.argtypes = [C.c_void_p]
+        L.cvo_prediction_guess.argtypes = [C.c_void_p, C.POINTER(abi.cv_state),
+                                           C.POINTER(abi.cv_settings), abi.f64p]
+        L.cvo_prediction_record.argtypes = [C.c_void_p, abi.f64p, C.POINTER(abi.cv_state),
+                                            C.POINTER(abi.cv_settings)]
+        self.h = L.cvo_prediction_create()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().cvo_prediction_destroy(self.h)
+            self.h = None
+
+    def guess(self, state, settings):
+        out = np.zeros_like(state.x)
+        lib().cvo_prediction_guess(self.h, C.byref(state.to_c()), C.byref(settings_c(settings)), _p(out))
+        return out
+
+    def record(self, old_v, state, settings):
+        ov = _f(old_v)
+        lib().cvo_prediction_record(self.h, _p(ov), C.byref(state.to_c()), C.byref(settings_c(settings)))
+
+
+def colors(graph, kind_ids, counts, grab=None):
+    """coloring::colors (src/vbd/coloring.rs:5-43) on an explicit graph."""
+    L = lib()
+    L.cvo_colors.argtypes = [C.c_uint32, abi.u32p, abi.u32p, abi.u32p, abi.u32p, C.c_uint64,
+                             C.POINTER(abi.cv_grab), abi.u32p]
+    L.cvo_colors.restype = C.c_uint32
+    n = len(graph)
+    off = np.zeros(n + 1, np.uint32)
+    for v in range(n):
+        off[v + 1] = off[v] + len(graph[v])
+    nb = np.array([u for row in graph for u in row] or [0], np.uint32)
+    ki = np.ascontiguousarray(np.asarray(kind_ids, np.uint32).reshape(-1, 5))
+    cn = np.ascontiguousarray(counts, np.uint32)
+    out = np.zeros(n, np.uint32)
+    g = grab_c(grab)
+    k = L.cvo_colors(n, abi.ptr(off, C.c_uint32), abi.ptr(nb, C.c_uint32), abi.ptr(ki, C.c_uint32),
+                     abi.ptr(cn, C.c_uint32), len(cn), C.byref(g) if g else None, abi.ptr(out, C.c_uint32))
+    groups = [[] for _ in range(k)]
+    for v in range(n):
+        groups[int(out[v])].append(v)
+    return groups

@@ -753,3 +753,51 @@ double cvo_max_seam_gap(const cv_garment* g, const double* x) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ prediction
+extern "C" {
+struct cvo_prediction {
+  Prediction p;
+};
+cvo_prediction* cvo_prediction_create() { return new cvo_prediction(); }
+void cvo_prediction_destroy(cvo_prediction* p) { delete p; }
+void cvo_prediction_guess(const cvo_prediction* p, const cv_state* st, const cv_settings* s, double* out) {
+  State state = state_of(st);
+  auto g = p->p.guess(state, settings_of(s));
+  for (size_t i = 0; i < g.size(); ++i) {
+    out[3 * i] = g[i].x;
+    out[3 * i + 1] = g[i].y;
+    out[3 * i + 2] = g[i].z;
+  }
+}
+void cvo_prediction_record(cvo_prediction* p, const double* old_v, const cv_state* st, const cv_settings* s) {
+  p->p.record(v3s(old_v, st->n), state_of(st), settings_of(s));
+}
+}
+
+// Greedy colouring (src/vbd/coloring.rs:5-43) on an explicit conflict graph
+// (CSR) and contact rows (kind, ids[4], count), with an optional grab.
+extern "C" uint32_t cvo_colors(uint32_t n, const uint32_t* g_off, const uint32_t* g_nb, const uint32_t* kind_ids,
+                               const uint32_t* counts, uint64_t m, const cv_grab* grab, uint32_t* color_of) {
+  std::vector<std::vector<int>> graph(n);
+  for (uint32_t v = 0; v < n; ++v)
+    for (uint32_t k = g_off[v]; k < g_off[v + 1]; ++k) graph[v].push_back((int)g_nb[k]);
+  ContactSet c;
+  c.incident.assign(n, {});
+  for (uint64_t i = 0; i < m; ++i) {
+    Contact ct{(int)kind_ids[5 * i],
+               {(int)kind_ids[5 * i + 1], (int)kind_ids[5 * i + 2], (int)kind_ids[5 * i + 3],
+                (int)kind_ids[5 * i + 4]},
+               (int)counts[i],
+               0.0};
+    c.contacts.push_back(ct);
+    for (int k = 0; k < ct.count; ++k)
+      if (ct.vertices[k] < (int)n) c.incident[ct.vertices[k]].push_back((int)i);
+  }
+  GrabT g;
+  if (grab) g = grab_of(grab);
+  auto cols = colors(graph, c, grab ? &g : nullptr);
+  for (size_t k = 0; k < cols.size(); ++k)
+    for (int v : cols[k]) color_of[v] = (uint32_t)k;
+  return (uint32_t)cols.size();
+}

@@ -1,0 +1,174 @@
+"""The product's host set-up and device math, checked on the CPU against the
+f64 oracle: term records, exact per-term derivatives (f64 and f32
+instantiations of the kernel code), colouring order and create-time errors."""
+import numpy as np
+import pytest
+
+import oracle as O
+from clothcraft_amd import host, scenes
+from clothcraft_amd.vbd import VbdError
+
+
+@pytest.fixture(scope="module")
+def sheet():
+    sc = scenes.sheet_on_sphere(nx=8)
+    rng = np.random.default_rng(0)
+    x = sc.state.x + rng.normal(scale=0.01, size=sc.state.x.shape)
+    return sc, x
+
+
+def test_term_records_cover_every_incidence(sheet):
+    sc, _ = sheet
+    g = sc.garment
+    hs = host.HostSolver(g, sc.avatar, sc.state)
+    off, recs = hs.terms()
+    tri_count = np.bincount(g.tri_vertices.reshape(-1), minlength=g.n)
+    hinge_count = np.bincount(g.hinge_vertices.reshape(-1), minlength=g.n)
+    assert np.array_equal(np.diff(off), tri_count + hinge_count)
+    kinds = recs["head"] & 0xF
+    for v in range(g.n):
+        k = kinds[off[v]:off[v + 1]]
+        # reference summation order: triangles, then hinges (src/vbd.rs:207-216)
+        assert np.all(k[:tri_count[v]] == 0) and np.all(k[tri_count[v]:] == 1)
+
+
+def test_f64_kernel_math_matches_oracle_jets(sheet):
+    """Per-vertex sums of the kernel's closed-form membrane/bending derivatives
+    equal the oracle's Jet<3> derivatives up to the f32 rounding of the stored
+    coefficients (~1e-7 relative); exact algebra errors would show at O(1)."""
+    sc, x = sheet
+    g = sc.garment
+    hs = host.HostSolver(g, sc.avatar, sc.state)
+    off, recs = hs.terms()
+    X = np.vstack([x, sc.avatar.vertices])
+    ref = O.Solver(g, sc.avatar, sc.state)
+    loc = ref.local_gradient(x, x, np.zeros(g.n), sc.settings, use_last_contacts=False)
+    for v in range(g.n):
+        tot = sum(host.term_deriv(recs[k], X[v], X) for k in range(off[v], off[v + 1]))
+        scale = np.abs(loc[v, :3]).max() + 1e-9
+        assert np.abs(tot[1:4] - loc[v, :3]).max() / scale < 2e-6
+        assert abs(tot[0] - loc[v, 3]) <= 1e-6 * abs(loc[v, 3]) + 1e-15
+
+
+def test_f64_kernel_hessians_match_finite_differences(sheet):
+    sc, x = sheet
+    g = sc.garment
+    hs = host.HostSolver(g, sc.avatar, sc.state)
+    off, recs = hs.terms()
+    X = np.vstack([x, sc.avatar.vertices])
+    for v in range(0, g.n, 5):
+        for k in range(off[v], off[v + 1]):
+            d0 = host.term_deriv(recs[k], X[v], X)
+            H = host.sym(d0[4:])
+            for ax in range(3):
+                e = np.zeros(3)
+                e[ax] = 1e-7
+                fd = (host.term_deriv(recs[k], X[v] + e, X)[1:4] - host.term_deriv(recs[k], X[v] - e, X)[1:4]) / 2e-7
+                assert np.abs(fd - H[:, ax]).max() <= 1e-6 * (np.abs(H).max() + 1e-9)
+            # energy route of the line search equals the derivative route's value
+            assert abs(host.term_energy(recs[k], X[v], X) - d0[0]) <= 1e-12 * abs(d0[0]) + 1e-18
+
+
+@pytest.mark.parametrize("kind,pts,count", [
+    (0, [[0.1, 0.1, 0.004], [0.1, 0.1, 0.0]], 2),
+    (1, [[0.1, 0.004, 0.002], [-1.0, 0.0, 0.0], [1.0, 0.0, 0.0]], 3),
+    (2, [[0.1, 0.1, 0.004], [-1.0, -1.0, 0.0], [1.0, -1.0, 0.0], [0.0, 1.0, 0.0]], 4),
+    (3, [[-1.0, 0.01, 0.004], [1.0, 0.0, 0.004], [0.0, -1.0, 0.0], [0.02, 1.0, 0.0]], 4),
+])
+def test_contact_kernel_matches_oracle_block(kind, pts, count):
+    x = np.zeros((4, 3))
+    x[:len(pts)] = pts
+    r, k, eps = 0.01, 1e5, 2.0 ** -6
+    for v in range(count):
+        e, gr, h = O.contact_block(kind, x, count, eps, v, r, k)
+        rec = host.contact_record(kind, np.arange(4), count, eps, v)
+        d = host.term_deriv(rec, x[v], x, r, k)
+        assert abs(d[0] - e) <= 1e-12 * abs(e) + 1e-18
+        assert np.abs(d[1:4] - gr).max() <= 1e-10 * (np.abs(gr).max() + 1e-12)
+        assert np.abs(host.sym(d[4:]) - h).max() <= 1e-8 * (np.abs(h).max() + 1e-12)
+        # f32 instantiation within the reference's probe tolerance (3 mN + 0.3 %)
+        d32 = host.term_deriv(rec, x[v], x, r, k, f32=True)
+        assert np.linalg.norm(d32[1:4] - gr) <= 0.003 + 0.003 * np.linalg.norm(gr)
+
+
+def test_colouring_matches_reference_order_with_contacts():
+    sc = scenes.small_sheet_on_sphere(16, 0.2535)
+    ref = O.Solver(sc.garment, sc.avatar, sc.state)
+    x = sc.state.x.copy()
+    x[:, 1] -= 0.0015  # bring the sheet within the query radius of the sphere top
+    cols, k, c = ref.detect_and_color(x, 0.008, O.ogc_c(radius=0.002))
+    assert len(c["kind_ids"]) > 0
+    hs = host.HostSolver(sc.garment, sc.avatar, sc.state)
+    counts = c["counts"]
+    mine, km = hs.greedy_colors(c["kind_ids"], counts)
+    assert km == k and np.array_equal(mine, cols)
+    grab = dict(vertices=(3, 40, 77), weights=(0.2, 0.3, 0.5), target=(0, 0, 0), stiffness=10.0)
+    cg, kg, _ = ref.detect_and_color(x, 0.008, O.ogc_c(radius=0.002), grab)
+    mg, kmg = hs.greedy_colors(c["kind_ids"], counts, grab)
+    assert kmg == kg and np.array_equal(mg, cg)
+
+
+def _expect_same_error(make):
+    with pytest.raises(O.OracleError) as a:
+        make(O.Solver)
+    with pytest.raises(VbdError) as b:
+        make(host.HostSolver)
+    assert a.value.status == b.value.status and a.value.message == b.value.message
+    return b.value
+
+
+def test_create_errors_match_the_reference_messages():
+    sc = scenes.hanging_sheet()
+    st = sc.state.copy()
+    st.x[0, 0] += 1e-6
+    e = _expect_same_error(lambda S: S(sc.garment, None, st))
+    assert "pinned positions" in e.message
+    sc = scenes.free_panel(2, 2)
+    bad = sc.state.copy()
+    bad.x = np.ascontiguousarray(bad.x[:-1])
+    bad.v = np.ascontiguousarray(bad.v[:-1])
+    _expect_same_error(lambda S: S(sc.garment, None, bad))
+    # an avatar triangle piercing the panel: initial intersection
+    p = sc.state.x
+    c = p.mean(axis=0)
+    av = scenes.Surface(np.array([c + [0.01, 0.01, -0.5], c + [0.01, 0.01, 0.5], c + [0.3, 0.0, 0.0]]),
+                        np.array([[0, 1, 2]], np.uint32))
+    e = _expect_same_error(lambda S: S(sc.garment, av, sc.state))
+    assert "initial intersection" in e.message
+    # degenerate avatar triangle
+    av = scenes.Surface(np.array([[0, 0, 5.0], [1, 0, 5.0], [2, 0, 5.0]]), np.array([[0, 1, 2]], np.uint32))
+    _expect_same_error(lambda S: S(sc.garment, av, sc.state))
+    # touching: avatar vertex on the panel
+    av = scenes.Surface(np.array([c, c + [0, 0, -0.2], c + [0.2, 0, -0.2]]), np.array([[0, 1, 2]], np.uint32))
+    _expect_same_error(lambda S: S(sc.garment, av, sc.state))
+
+
+def test_device_step_logic_replayed_on_cpu_matches_the_oracle():
+    """The device step replayed on the CPU with the kernel's own math
+    (host build of vbd_math.hpp): in f64 it reproduces the oracle step to
+    round-off; in f32 it stays within the 1e-5 relative parity bar on the
+    32x32 hanging sheet."""
+    from device_emulation import emulate_step
+    sc = scenes.hanging_sheet()
+    a = sc.state.copy()
+    O.Solver(sc.garment, None, a, threads=8).step(a, sc.settings)
+    diag = np.linalg.norm(a.x.max(0) - a.x.min(0))
+    x64 = emulate_step(sc, sc.state.copy(), f32=False)
+    assert np.abs(x64 - a.x).max() / diag < 1e-10
+    x32 = emulate_step(sc, sc.state.copy(), f32=True)
+    assert np.abs(x32 - a.x).max() / diag < 1e-5
+
+
+def test_f32_strain_resolution_limits_parity_on_stiff_sheets():
+    """Why the default sweep is f64: on a falling 32x32 sheet of the sheet-on-sphere
+    material, one f32 ulp of Green strain times 4.8e4 N/m is comparable to the
+    per-vertex weight, and the f32 replay drifts past 1e-5 while f64 stays
+    within it."""
+    from device_emulation import emulate_step
+    sc = scenes.small_sheet_on_sphere(32, 0.4)
+    a = sc.state.copy()
+    O.Solver(sc.garment, sc.avatar, a, threads=8).step(a, sc.settings)
+    diag = np.linalg.norm(a.x.max(0) - a.x.min(0))
+    e64 = np.abs(emulate_step(sc, sc.state.copy(), f32=False) - a.x).max() / diag
+    e32 = np.abs(emulate_step(sc, sc.state.copy(), f32=True) - a.x).max() / diag
+    assert e64 < 1e-5 < e32
