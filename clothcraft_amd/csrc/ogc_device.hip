@@ -346,6 +346,8 @@ __global__ void k_unique_write(const uint32_t* raw, const uint32_t* raw_count, c
   ccount[dst] = raw_count[src];
   ceps[dst] = raw_eps[src];
 }
+__global__ void k_store_u64(unsigned long long* dst, unsigned long long v) { *dst = v; }
+
 // Active contacts, incidence counts and incidence keys (v << 32 | contact).
 __global__ void k_contact_post(const double* x, const uint32_t* cids, const uint32_t* ccount,
                                const double* ceps, uint64_t m, uint32_t n, double r, double k,
@@ -565,6 +567,10 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
                                               b.uniq_flag.p, b.uniq_pos.p, m, b.c_ids.p, b.c_count.p,
                                               b.c_eps.p);
   }
+  // The unique count lives in the device scalars like every other counter,
+  // so later scalar read-backs keep it (passed by value: no host staging).
+  k_store_u64<<<1, 1, 0, st>>>(&g.d_scal->contacts, unique);
+  CV_HIP(hipGetLastError());
   g.h_scal->contacts = unique;
   // Incidence, active count, per-vertex contact records.
   b.vcount.reserve(d.n + 1);

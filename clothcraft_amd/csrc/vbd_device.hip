@@ -17,6 +17,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <hip/hip_ext.h>
+
 #include "gpu.hpp"
 
 using cvm::V3;
@@ -453,17 +455,26 @@ SweepArgs<R> sweep_args(cv_gpu& g, const cv_settings* set, const cv_grab* grab) 
   return a;
 }
 
+// With profiling on, the launch carries start/stop events that the runtime
+// stamps at the kernel's own begin and end (hipExtLaunchKernelGGL), so the
+// summed time is kernel execution only, as a kernel trace reports it.
 template <int L, class R>
-void launch_sweep(const SweepArgs<R>& a, uint32_t off, uint32_t count, hipStream_t st) {
-  k_sweep<L, R><<<blocks((uint64_t)count * L), TPB, 0, st>>>(a, off, count);
+void launch_sweep(const SweepArgs<R>& a, uint32_t off, uint32_t count, hipStream_t st, hipEvent_t e0,
+                  hipEvent_t e1) {
+  dim3 grid(blocks((uint64_t)count * L)), block(TPB);
+  if (e0)
+    hipExtLaunchKernelGGL(k_sweep<L, R>, grid, block, 0, st, e0, e1, 0, a, off, count);
+  else
+    k_sweep<L, R><<<grid, block, 0, st>>>(a, off, count);
 }
 template <class R>
-void sweep_color(cv_gpu& g, const SweepArgs<R>& a, uint32_t off, uint32_t count) {
+void sweep_color(cv_gpu& g, const SweepArgs<R>& a, uint32_t off, uint32_t count, hipEvent_t e0 = nullptr,
+                 hipEvent_t e1 = nullptr) {
   switch (g.lanes) {
-    case 8: launch_sweep<8>(a, off, count, g.stream); break;
-    case 16: launch_sweep<16>(a, off, count, g.stream); break;
-    case 64: launch_sweep<64>(a, off, count, g.stream); break;
-    default: launch_sweep<32>(a, off, count, g.stream); break;
+    case 8: launch_sweep<8>(a, off, count, g.stream, e0, e1); break;
+    case 16: launch_sweep<16>(a, off, count, g.stream, e0, e1); break;
+    case 64: launch_sweep<64>(a, off, count, g.stream, e0, e1); break;
+    default: launch_sweep<32>(a, off, count, g.stream, e0, e1); break;
   }
 }
 
@@ -583,11 +594,11 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
           g.sweep_events.push_back(e0);
           g.sweep_events.push_back(e1);
         }
-        CV_HIP(hipEventRecord(g.sweep_events[2 * nev], st));
+        sweep_color<R>(g, a, off, cnt, g.sweep_events[2 * nev], g.sweep_events[2 * nev + 1]);
+        ++nev;
+      } else {
+        sweep_color<R>(g, a, off, cnt);
       }
-      sweep_color<R>(g, a, off, cnt);
-      if (g.profile) CV_HIP(hipEventRecord(g.sweep_events[2 * nev + 1], st));
-      nev += g.profile ? 1 : 0;
       out.sweep_launches++;
     }
     // Algorithmic bytes of this sweep: static terms + contact incidences.

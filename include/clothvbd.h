@@ -238,7 +238,9 @@ cv_status cv_gpu_detect(cv_gpu* gpu, cv_solver* solver, const double* x, double 
                         double* contact_eps, double* radii);
 
 /* Batched uneven scenes (no reference counterpart): steps `count` independent
- * solvers in one persistent launch.  States are updated in place. */
+ * solvers with one call, each with cv_gpu_step semantics (metrics[i] for
+ * solver i).  Stops at the first failing solver and returns its status; the
+ * solvers before it have stepped.  States are updated in place. */
 cv_status cv_gpu_step_batch(cv_gpu* gpu, cv_solver* const* solvers, cv_state* const* states,
                             uint32_t count, const cv_settings* settings, cv_metrics* metrics);
 

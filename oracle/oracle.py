@@ -246,6 +246,8 @@ class Solver:
         return out, int(k)
 
     def detect_and_color(self, x_moving, rq, ogc=None, grab=None):
+        """OgcWorld::detect at x plus colors(); the set becomes the one
+        local_gradient(use_last_contacts=True) evaluates."""
         x = _f(x_moving)
         cols = np.zeros(self.n, np.uint32)
         k = C.c_uint32()

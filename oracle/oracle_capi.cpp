@@ -331,6 +331,7 @@ int cvo_solver_detect_and_color(cvo_solver* s, const double* x_moving, double rq
       for (int v : cols[ci]) color_of_vertex[v] = (uint32_t)ci;
     *n_colors = (uint32_t)cols.size();
     export_contacts(c, out);
+    s->s.last_contacts = std::move(c);  // local_gradient may then evaluate at this set
   });
 }
 
@@ -345,7 +346,8 @@ int cvo_solver_local_gradient(cvo_solver* s, const double* x_moving, const doubl
     auto x = s->s.world.positions(v3s(x_moving, n));
     ContactSet empty;
     empty.incident.assign(n, {});
-    const ContactSet& c = use_last_contacts ? s->s.last_contacts : empty;
+    bool have = use_last_contacts && s->s.last_contacts.incident.size() == (size_t)n;
+    const ContactSet& c = have ? s->s.last_contacts : empty;
     VbdSettings vs = settings_of(settings);
     GrabT g;
     if (grab) g = grab_of(grab);

@@ -152,14 +152,17 @@ def test_probe_gradients_match_oracle(gpu):
     ref = O.Solver(sc.garment, sc.avatar, st)
     sol = VbdSolver(sc.garment, sc.avatar, st)
     s = sc.settings
-    a = st.copy()
-    ref.step(a, dict(s, iterations=1))  # leaves the oracle's contact set at x
-    probe = gpu.probe(sol, st, s)
     dt, c = s["dt"], s["damping"]
     damp = 1 + c * dt
     target = st.x + (st.v * dt + np.array(s["gravity"]) * dt * dt) / damp
     inertia = sc.garment.masses * damp / dt ** 2
+    # The probe detects at x with the step's query radius (src/vbd.rs:297-304).
+    motion = np.linalg.norm(target - st.x, axis=1).max()
+    rq = min(max(s["radius"] + motion, 2 * s["radius"]), 4 * s["radius"])
+    ref.detect_and_color(st.x, rq, O.ogc_c(radius=s["radius"], stiffness=s["stiffness"]))
+    probe = gpu.probe(sol, st, s)
     loc = ref.local_gradient(st.x, target, inertia, s, use_last_contacts=True)
+    assert np.count_nonzero(ref.last_contacts()["counts"]) > 0
     err = np.linalg.norm(probe[:, :3] - loc[:, :3], axis=1)
     tol = 0.003 + 0.003 * np.linalg.norm(loc[:, :3], axis=1)
     assert np.all(err <= tol)
