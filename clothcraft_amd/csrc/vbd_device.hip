@@ -352,7 +352,7 @@ __global__ void k_current(const typename P4<R>::T* pos, const double* x_old, con
   x64[3 * i + 2] = sc->origin[2] + (double)q.z;
 }
 
-// Commit (src/vbd.rs:399-411, src/vbd/prediction.rs:181-191): finiteness,
+// Commit (src/vbd.rs:399-411, src/vbd/prediction.rs:43-54): finiteness,
 // trust-region invariant, velocity, and the acceleration history.
 template <class R>
 __global__ void k_finalize(const typename P4<R>::T* pos, const double* x_old, const double* v_old,

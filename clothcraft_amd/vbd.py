@@ -171,7 +171,7 @@ class VbdSolver:
             self.h = None
 
     def record_prediction(self, old_velocity, state, settings):
-        """Prediction::record (src/vbd/prediction.rs:181-191)."""
+        """Prediction::record (src/vbd/prediction.rs:43-54)."""
         ov = np.ascontiguousarray(old_velocity, np.float64)
         rc = lib().cv_solver_record_prediction(self.h, abi.ptr(ov, C.c_double),
                                                C.byref(state.to_c()), C.byref(_settings(settings)))

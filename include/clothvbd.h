@@ -179,7 +179,7 @@ cv_status cv_solver_create(const cv_garment* garment, const cv_surface* avatar,
                            const cv_state* state, cv_solver** out);
 void cv_solver_destroy(cv_solver* solver);
 
-/* Prediction::record — src/vbd/prediction.rs:181-191.  Installs the adaptive
+/* Prediction::record — src/vbd/prediction.rs:43-54.  Installs the adaptive
  * guess history as if `state` had just been committed from `old_velocity`. */
 cv_status cv_solver_record_prediction(cv_solver* solver, const double* old_velocity,
                                       const cv_state* state, const cv_settings* settings);
