@@ -176,6 +176,8 @@ def main():
         "residual_rms_n_last_step": m["residual_rms_n"],
         "contacts_last_step": m["contact_terms"],
         "colors_last_step": m["colors"],
+        "candidate_pairs_last_step": m["candidate_pairs"],
+        "grid_pairs_examined_last_step": m["candidate_pairs"] + m["aabb_rejections"],
         "detections": m["detection_updates"],
         "first_contact_by_step": None if first_contact is None else first_contact + 10,
         "sweep_share_of_step": sweep_s / dev_s,

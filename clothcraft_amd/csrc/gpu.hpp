@@ -58,9 +58,25 @@ struct Scalars {
   unsigned long long grid_tests;                // pairs examined by the grid
   unsigned long long raw_contacts, contacts, active, incidences;
   unsigned long long clipped;
+  unsigned long long truncated_total;  // clipped vertices summed over the step's iterations
+  unsigned int stop, stop_it;          // refresh needed after iteration stop_it (device decides)
+  unsigned int iters_run, pad_;
+  unsigned long long n_over;           // edges served by the overflow broad-phase kernel
   unsigned long long cand_export_vf, cand_export_ee;
   unsigned int err_degenerate, err_overflow, err_nonfinite, err_trust;
 };
+
+// MI355X dispatches the workgroups of a launch to its 8 XCDs round-robin
+// (block b runs on XCD b % 8), and each XCD has its own L2.  Kernels whose
+// neighbouring blocks share data remap the block index so every XCD receives
+// one contiguous range of logical blocks: a bijection of [0, nb).
+#ifdef __HIPCC__
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  constexpr uint32_t X = 8;
+  uint32_t x = b % X, k = b / X, q = nb / X, r = nb % X;
+  return x * q + (x < r ? x : r) + k;
+}
+#endif
 
 // Per-solver device cache (the role of Prepared, src/gpu_vbd.rs:478-547):
 // static arrays are uploaded once per solver generation.
@@ -97,7 +113,12 @@ struct StepBufs {
   DBuf<double> fbox, ebox;  // [6F], [6E]
   DBuf<uint32_t> cell_count, cell_off;
   DBuf<unsigned long long> cell_key, cell_key_sorted;
-  DBuf<uint32_t> cell_val, cell_val_sorted;
+  DBuf<uint32_t> cell_val, cell_val_sorted, cell_rank, cell_owner;
+  DBuf<double> cell_box_sorted;
+  DBuf<uint32_t> bcount, bstart;  // hash-grid buckets [2T+1]
+  DBuf<uint32_t> item_len, item_off, item_start, over, pair_owner;  // flattened broad phase
+  DBuf<unsigned long long> item_key;
+  uint64_t pair_cap = 0;
   DBuf<uint32_t> raw_ids;   // [5*cap] kind, ids
   DBuf<uint32_t> raw_count;
   DBuf<double> raw_eps;

@@ -8,13 +8,17 @@
 // phase (closest features, feasible regions, canonical ids) rounds exactly as
 // the reference does.
 //
-// Broad phase: a uniform grid.  Every triangle (edge) is registered in the
-// cells its box inflated by the padded radius covers; a point looks only in
-// its own cell; an edge scans the cells of its own box and keeps a pair only
-// in the cell holding the lower corner of the box intersection, so each pair
-// is examined once.  Pairs are emitted with atomics and then put in the
-// reference's canonical (kind, ids) order by a two-key radix sort, so results
-// are deterministic.
+// Broad phase: a uniform hash grid.  Every triangle (edge) is registered in
+// the cells its box inflated by the padded radius covers; entries are bucketed
+// by a hash of the cell key with a counting sort (one atomic rank per entry,
+// one scan, one scatter), and each bucket entry keeps its cell key so hash
+// collisions are filtered exactly.  A point looks only in its own cell; an
+// edge scans the cells of its own box and keeps a pair only in the cell
+// holding the lower corner of the box intersection, so each pair is examined
+// once.  Each query is served by a group of G lanes that stride over the
+// bucket's entries, so a launch has G x (points or edges) lanes in flight.
+// Pairs are emitted with atomics and then put in the reference's canonical
+// (kind, ids) order by a two-key radix sort, so results are deterministic.
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -79,50 +83,74 @@ __device__ __forceinline__ void cell_range(const ogm::Box& b, double h, long lon
   r[5] = cell_of(b.hi.z, h);
 }
 
-// Cells covered by each primitive box inflated by pr.
-__global__ void k_cell_count(const double* box, uint32_t N, double pr, double h, uint32_t* count) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  long long r[6];
-  cell_range(inflate(ldbox(box, i), pr), h, r);
-  long long c = (r[3] - r[0] + 1) * (r[4] - r[1] + 1) * (r[5] - r[2] + 1);
-  count[i] = (uint32_t)(c > kMaxCells ? 1 : c);  // over the cap: one sentinel entry, see k_cell_fill
+__device__ __forceinline__ uint32_t bucket_of(unsigned long long key, int log2t) {
+  return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - log2t));
 }
-__global__ void k_cell_fill(const double* box, uint32_t N, double pr, double h, const uint32_t* off,
-                            const uint32_t* count, unsigned long long* keys, uint32_t* vals,
-                            Scalars* sc) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  long long r[6];
-  cell_range(inflate(ldbox(box, i), pr), h, r);
-  long long c = (r[3] - r[0] + 1) * (r[4] - r[1] + 1) * (r[5] - r[2] + 1);
-  if (c > kMaxCells) {  // over the per-primitive cap: report, never drop silently
-    keys[off[i]] = ~0ull;  // sentinel key never matches a cell (cell keys use 63 bits)
-    vals[off[i]] = 0;
-    atomicOr(&sc->err_overflow, 2u);
-    return;
-  }
-  uint32_t o = off[i];
-  for (long long a = r[0]; a <= r[3]; ++a)
-    for (long long b = r[1]; b <= r[4]; ++b)
-      for (long long z = r[2]; z <= r[5]; ++z) {
-        keys[o] = cell_key(a, b, z);
-        vals[o] = i;
-        ++o;
-      }
+__device__ __forceinline__ ogm::Box prim_box(const double* fbox, const double* ebox, uint32_t F, uint32_t i) {
+  return i < F ? ldbox(fbox, i) : ldbox(ebox, i - F);
 }
 
-__device__ __forceinline__ uint32_t lower_bound_key(const unsigned long long* keys, uint32_t n,
-                                                    unsigned long long k) {
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (keys[mid] < k)
-      lo = mid + 1;
-    else
-      hi = mid;
+// Cells covered by each primitive box inflated by pr: faces 0..F-1, then
+// edges F..F+E-1.
+__global__ void k_reg_count(const double* fbox, const double* ebox, uint32_t F, uint32_t E, double pr,
+                            double h, uint32_t* count) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= F + E) return;
+  long long r[6];
+  cell_range(inflate(prim_box(fbox, ebox, F, i), pr), h, r);
+  long long c = (r[3] - r[0] + 1) * (r[4] - r[1] + 1) * (r[5] - r[2] + 1);
+  count[i] = (uint32_t)(c > kMaxCells ? 1 : c);  // over the cap: one sentinel entry, see k_reg_fill
+}
+// Owner primitive of every entry (entries of primitive i are off[i]..off[i+1]).
+__global__ void k_reg_owner(const uint32_t* off, uint32_t N, uint32_t* owner) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (uint32_t o = off[i]; o < off[i + 1]; ++o) owner[o] = i;
+}
+// One thread per entry: its cell key, primitive id and rank in its bucket
+// (one returning atomic per thread, all in flight together).  Faces use
+// buckets [0, T), edges [T, 2T).
+__global__ void k_reg_fill(const double* fbox, const double* ebox, uint32_t F, double pr, double h,
+                           const uint32_t* off, const uint32_t* owner, uint32_t M, int log2t,
+                           unsigned long long* keys, uint32_t* prims, uint32_t* ranks, uint32_t* bcount,
+                           Scalars* sc) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  uint32_t i = owner[j];
+  uint32_t table = i < F ? 0u : (1u << log2t), prim = i < F ? i : i - F;
+  long long r[6];
+  cell_range(inflate(prim_box(fbox, ebox, F, i), pr), h, r);
+  long long nx = r[3] - r[0] + 1, ny = r[4] - r[1] + 1, nz = r[5] - r[2] + 1;
+  unsigned long long key;
+  if (nx * ny * nz > kMaxCells) {  // over the per-primitive cap: report, never drop silently
+    key = ~0ull;                    // sentinel key never matches a cell (cell keys use 63 bits)
+    atomicOr(&sc->err_overflow, 2u);
+  } else {
+    long long k = (long long)(j - off[i]);
+    long long z = k % nz, y = (k / nz) % ny, x = k / (nz * ny);
+    key = cell_key(r[0] + x, r[1] + y, r[2] + z);
   }
-  return lo;
+  keys[j] = key;
+  prims[j] = prim;
+  ranks[j] = atomicAdd(&bcount[table + bucket_of(key, log2t)], 1u);
+}
+// Bucketed entries carry the primitive's (unpadded) box, so a query scans
+// contiguous memory and tests boxes without a dependent gather.
+__global__ void k_reg_scatter(const unsigned long long* keys, const uint32_t* prims, const uint32_t* ranks,
+                              uint32_t M, uint32_t face_entries, int log2t, const uint32_t* bstart,
+                              const double* fbox, const double* ebox, unsigned long long* skey,
+                              uint32_t* sprim, double* sbox) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  bool face = j < face_entries;
+  uint32_t b = (face ? 0u : (1u << log2t)) + bucket_of(keys[j], log2t);
+  uint32_t pos = bstart[b] + ranks[j];
+  uint32_t p = prims[j];
+  skey[pos] = keys[j];
+  sprim[pos] = p;
+  const double* src = face ? fbox + 6 * (size_t)p : ebox + 6 * (size_t)p;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sbox[6 * (size_t)pos + k] = src[k];
 }
 
 struct DetectArgs {
@@ -142,6 +170,11 @@ struct DetectArgs {
   const double* fbox;
   const double* ebox;
   double rq, pr, pr2, h;
+  const unsigned long long* skey;  // bucketed entries: cell key, primitive id and box
+  const uint32_t* sprim;
+  const double* sbox;
+  const uint32_t* bstart;          // [2T+1] bucket starts, faces then edges
+  int log2t;
   unsigned long long* dist_bits;
   uint32_t* raw_ids;
   uint32_t* raw_count;
@@ -169,8 +202,15 @@ __device__ __forceinline__ bool is_excluded(const DetectArgs& a, uint32_t u, uin
 __device__ __forceinline__ bool adjacent(const DetectArgs& a, uint32_t u, uint32_t w) {
   return u == w || (a.ex_off[u] != a.ex_off[u + 1] && is_excluded(a, u, w));
 }
+// Per-vertex minimum distance as the bit pattern of a non-negative double
+// (ordered like the value).  Values only decrease, so a plain read that is
+// already <= d (even a stale one) proves the atomic would change nothing:
+// most pairs skip it, which removes the same-address atomic contention of
+// vertices that appear in hundreds of candidate pairs.
 __device__ __forceinline__ void dmin(const DetectArgs& a, uint32_t v, double d) {
-  atomicMin(&a.dist_bits[v], (unsigned long long)__double_as_longlong(d));
+  unsigned long long bits = (unsigned long long)__double_as_longlong(d);
+  if (a.dist_bits[v] <= bits) return;
+  atomicMin(&a.dist_bits[v], bits);
 }
 __device__ __forceinline__ void add_contact(const DetectArgs& a, int kind, uint32_t i0, uint32_t i1,
                                             uint32_t i2, uint32_t i3, int count, double eps) {
@@ -187,127 +227,247 @@ __device__ __forceinline__ void add_contact(const DetectArgs& a, int kind, uint3
   a.raw_eps[slot] = eps;
 }
 
-// VF: one thread per point (src/ogc/world.rs:461-497).
-__global__ void k_vf(DetectArgs a, const unsigned long long* keys, const uint32_t* vals,
-                     uint32_t nkeys) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= a.n_total) return;
-  bool v_moving = v < a.n;
-  D3 p = ld3(a.x, v);
-  ogm::Box pb = ogm::box_of(p);
-  unsigned long long key = cell_key(cell_of(p.x, a.h), cell_of(p.y, a.h), cell_of(p.z, a.h));
-  uint32_t it = lower_bound_key(keys, nkeys, key);
-  unsigned long long tests = 0;
-  for (; it < nkeys && keys[it] == key; ++it) {
-    uint32_t t = vals[it];
-    if (!v_moving && !a.face_moving[t]) continue;
-    uint32_t t0 = a.faces[3 * t], t1 = a.faces[3 * t + 1], t2 = a.faces[3 * t + 2];
-    if (adjacent(a, v, t0) || adjacent(a, v, t1) || adjacent(a, v, t2)) continue;
-    ++tests;
-    if (ogm::box_dist2(pb, ldbox(a.fbox, t)) > a.pr2) continue;
-    atomicAdd(&a.sc->vf_cand, 1ull);
-    if (a.cand_vf) {
-      unsigned long long s = atomicAdd(&a.sc->cand_export_vf, 1ull);
-      if (s < a.cand_cap) {
-        a.cand_vf[2 * s] = v;
-        a.cand_vf[2 * s + 1] = t;
-      }
-    }
-    D3 q;
-    int feat, loc;
-    ogm::closest_triangle(p, ld3(a.x, t0), ld3(a.x, t1), ld3(a.x, t2), q, feat, loc);
-    double d = ogm::dist(p, q);
-    if (d >= a.rq) continue;
-    atomicAdd(&a.sc->vf_count, 1ull);
-    dmin(a, v, d);
-    dmin(a, t0, d);
-    dmin(a, t1, d);
-    dmin(a, t2, d);
-    uint32_t tv[3] = {t0, t1, t2};
-    if (feat == ogm::F_VERTEX) {
-      uint32_t w = tv[loc];
-      if (ogm::vertex_feasible(p, (int)w, (const D3*)a.x, a.nb_off, a.nb))
-        add_contact(a, 0, v, w, 0, 0, 2, 0.0);
-    } else if (feat == ogm::F_EDGE) {
-      uint32_t e = a.tri_edges[3 * t + loc];
-      uint32_t ea = a.edges[2 * e], eb = a.edges[2 * e + 1];
-      if (ogm::edge_feasible(p, ld3(a.x, ea), ld3(a.x, eb), &a.opp[2 * e], (const D3*)a.x))
-        add_contact(a, 1, v, ea, eb, 0, 3, 0.0);
-    } else {
-      add_contact(a, 2, v, t0, t1, t2, 4, 0.0);
-    }
-  }
-  atomicAdd(&a.sc->grid_tests, tests);
+// Sum of v over the wavefront, added to *dst once per wavefront.  Every lane
+// of the wave must reach the call (kernels using it never return early), so
+// the scalar counters see a few atomics per block instead of one per pair.
+__device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
 }
 
-// EE: one thread per moving edge (src/ogc/world.rs:498-555).  The moving edge
-// is the first operand; moving/moving pairs are taken once with e < f.
-__global__ void k_ee(DetectArgs a, const unsigned long long* keys, const uint32_t* vals,
-                     uint32_t nkeys) {
-  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= a.E || !a.edge_moving[e]) return;
-  ogm::Box be = ldbox(a.ebox, e);
-  long long r[6];
-  cell_range(be, a.h, r);
-  uint32_t ia = a.edges[2 * e], ib = a.edges[2 * e + 1];
-  unsigned long long tests = 0;
-  for (long long ci = r[0]; ci <= r[3]; ++ci)
-    for (long long cj = r[1]; cj <= r[4]; ++cj)
-      for (long long ck = r[2]; ck <= r[5]; ++ck) {
-        unsigned long long key = cell_key(ci, cj, ck);
-        uint32_t it = lower_bound_key(keys, nkeys, key);
-        for (; it < nkeys && keys[it] == key; ++it) {
-          uint32_t f = vals[it];
-          if (f == e) continue;
-          bool fm = a.edge_moving[f];
-          if (fm && f < e) continue;
-          ogm::Box bf = inflate(ldbox(a.ebox, f), a.pr);
-          // Only in the cell holding the lower corner of box(e) ∩ inflated box(f).
-          double lx = fmax(be.lo.x, bf.lo.x), ly = fmax(be.lo.y, bf.lo.y), lz = fmax(be.lo.z, bf.lo.z);
-          if (lx > fmin(be.hi.x, bf.hi.x) || ly > fmin(be.hi.y, bf.hi.y) || lz > fmin(be.hi.z, bf.hi.z))
-            continue;
-          if (cell_of(lx, a.h) != ci || cell_of(ly, a.h) != cj || cell_of(lz, a.h) != ck) continue;
-          uint32_t jc = a.edges[2 * f], jd = a.edges[2 * f + 1];
-          if (adjacent(a, ia, jc) || adjacent(a, ia, jd) || adjacent(a, ib, jc) || adjacent(a, ib, jd))
-            continue;
-          ++tests;
-          if (ogm::box_dist2(be, ldbox(a.ebox, f)) > a.pr2) continue;
-          atomicAdd(&a.sc->ee_cand, 1ull);
-          if (a.cand_ee) {
-            unsigned long long s = atomicAdd(&a.sc->cand_export_ee, 1ull);
-            if (s < a.cand_cap) {
-              a.cand_ee[2 * s] = e;
-              a.cand_ee[2 * s + 1] = f;
+struct PairCounts {
+  unsigned long long tests = 0, vf_cand = 0, vf_near = 0, ee_cand = 0, ee_near = 0;
+};
+
+// One vertex-face pair that shares a grid cell (src/ogc/world.rs:461-497):
+// padded box test, adjacency, closest point, distance, contact.
+__device__ __forceinline__ void vf_pair(const DetectArgs& a, uint32_t v, bool v_moving, bool v_ex, D3 p,
+                                        const ogm::Box& pb, uint32_t t, const ogm::Box& tb, PairCounts& c) {
+  ++c.tests;
+  if (ogm::box_dist2(pb, tb) > a.pr2) return;
+  if (!v_moving && !a.face_moving[t]) return;
+  uint32_t t0 = a.faces[3 * t], t1 = a.faces[3 * t + 1], t2 = a.faces[3 * t + 2];
+  if (v == t0 || v == t1 || v == t2) return;
+  if (v_ex && (is_excluded(a, v, t0) || is_excluded(a, v, t1) || is_excluded(a, v, t2))) return;
+  ++c.vf_cand;
+  if (a.cand_vf) {
+    unsigned long long s = atomicAdd(&a.sc->cand_export_vf, 1ull);
+    if (s < a.cand_cap) {
+      a.cand_vf[2 * s] = v;
+      a.cand_vf[2 * s + 1] = t;
+    }
+  }
+  D3 q;
+  int feat, loc;
+  ogm::closest_triangle(p, ld3(a.x, t0), ld3(a.x, t1), ld3(a.x, t2), q, feat, loc);
+  double d = ogm::dist(p, q);
+  if (d >= a.rq) return;
+  ++c.vf_near;
+  dmin(a, v, d);
+  dmin(a, t0, d);
+  dmin(a, t1, d);
+  dmin(a, t2, d);
+  if (feat == ogm::F_VERTEX) {
+    uint32_t w = loc == 0 ? t0 : (loc == 1 ? t1 : t2);
+    if (ogm::vertex_feasible(p, (int)w, (const D3*)a.x, a.nb_off, a.nb)) add_contact(a, 0, v, w, 0, 0, 2, 0.0);
+  } else if (feat == ogm::F_EDGE) {
+    uint32_t e = a.tri_edges[3 * t + loc];
+    uint32_t ea = a.edges[2 * e], eb = a.edges[2 * e + 1];
+    if (ogm::edge_feasible(p, ld3(a.x, ea), ld3(a.x, eb), &a.opp[2 * e], (const D3*)a.x))
+      add_contact(a, 1, v, ea, eb, 0, 3, 0.0);
+  } else {
+    add_contact(a, 2, v, t0, t1, t2, 4, 0.0);
+  }
+}
+
+// One edge-edge pair met in the grid cell `cell` (src/ogc/world.rs:498-555).
+// The moving edge e is the first operand; moving/moving pairs are taken once
+// with e < f, and only in the cell holding the lower corner of
+// box(e) ∩ inflated box(f), so each pair is examined once.
+__device__ __forceinline__ void ee_pair(const DetectArgs& a, uint32_t e, const ogm::Box& be, uint32_t ia,
+                                        uint32_t ib, bool ia_ex, bool ib_ex, uint32_t f, const ogm::Box& bfr,
+                                        unsigned long long cell, PairCounts& c) {
+  ++c.tests;
+  if (ogm::box_dist2(be, bfr) > a.pr2) return;
+  if (f == e) return;
+  if (a.edge_moving[f] && f < e) return;
+  ogm::Box bf = inflate(bfr, a.pr);
+  double lx = fmax(be.lo.x, bf.lo.x), ly = fmax(be.lo.y, bf.lo.y), lz = fmax(be.lo.z, bf.lo.z);
+  if (cell_key(cell_of(lx, a.h), cell_of(ly, a.h), cell_of(lz, a.h)) != cell) return;
+  uint32_t jc = a.edges[2 * f], jd = a.edges[2 * f + 1];
+  if (ia == jc || ia == jd || ib == jc || ib == jd) return;
+  if ((ia_ex && (is_excluded(a, ia, jc) || is_excluded(a, ia, jd))) ||
+      (ib_ex && (is_excluded(a, ib, jc) || is_excluded(a, ib, jd))))
+    return;
+  ++c.ee_cand;
+  if (a.cand_ee) {
+    unsigned long long s = atomicAdd(&a.sc->cand_export_ee, 1ull);
+    if (s < a.cand_cap) {
+      a.cand_ee[2 * s] = e;
+      a.cand_ee[2 * s + 1] = f;
+    }
+  }
+  D3 A = ld3(a.x, ia), B = ld3(a.x, ib), C = ld3(a.x, jc), Dd = ld3(a.x, jd);
+  double s, t;
+  ogm::closest_segments(A, B, C, Dd, s, t);
+  D3 P = ogm::add(A, ogm::scale(ogm::sub(B, A), s));
+  D3 Q = ogm::add(C, ogm::scale(ogm::sub(Dd, C), t));
+  double dist = ogm::dist(P, Q);
+  if (dist >= a.rq) return;
+  ++c.ee_near;
+  dmin(a, ia, dist);
+  dmin(a, ib, dist);
+  dmin(a, jc, dist);
+  dmin(a, jd, dist);
+  int end_a = s <= 0.0 ? (int)ia : (s >= 1.0 ? (int)ib : -1);
+  int end_b = t <= 0.0 ? (int)jc : (t >= 1.0 ? (int)jd : -1);
+  if ((end_a >= 0 && !ogm::vertex_feasible(Q, end_a, (const D3*)a.x, a.nb_off, a.nb)) ||
+      (end_b >= 0 && !ogm::vertex_feasible(P, end_b, (const D3*)a.x, a.nb_off, a.nb)))
+    return;
+  if (end_a >= 0 && end_b >= 0)
+    add_contact(a, 0, end_a, end_b, 0, 0, 2, 0.0);
+  else if (end_a >= 0)
+    add_contact(a, 1, end_a, jc, jd, 0, 3, 0.0);
+  else if (end_b >= 0)
+    add_contact(a, 1, end_b, ia, ib, 0, 3, 0.0);
+  else
+    add_contact(a, 3, ia, ib, jc, jd, 4, 1e-3 * a.rest_len2[e] * a.rest_len2[f]);
+}
+
+__device__ __forceinline__ void add_counts(const DetectArgs& a, const PairCounts& c) {
+  wave_add(&a.sc->grid_tests, c.tests);
+  wave_add(&a.sc->vf_cand, c.vf_cand);
+  wave_add(&a.sc->vf_count, c.vf_near);
+  wave_add(&a.sc->ee_cand, c.ee_cand);
+  wave_add(&a.sc->ee_count, c.ee_near);
+}
+
+// The broad phase is flattened so each thread tests one (query, entry) pair
+// with a short chain of dependent loads; a thread per query walking its
+// cells would serialise dozens of memory round trips.
+//
+// Query items: item i < n_total is point i's own cell in the face table; item
+// n_total + kEdgeCells*e + c is the c-th cell of moving edge e's box in the
+// edge table (unused slots have length 0).  An edge covering more cells than
+// kEdgeCells goes to the overflow list, served by k_ee_overflow.
+constexpr int kEdgeCells = 8;
+__global__ void k_items(DetectArgs a, uint32_t* len, uint32_t* start, unsigned long long* ikey, uint32_t* over,
+                        unsigned long long* n_over) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.n_total) {
+    D3 p = ld3(a.x, i);
+    unsigned long long key = cell_key(cell_of(p.x, a.h), cell_of(p.y, a.h), cell_of(p.z, a.h));
+    uint32_t bk = bucket_of(key, a.log2t);
+    uint32_t s0 = a.bstart[bk];
+    start[i] = s0;
+    len[i] = a.bstart[bk + 1] - s0;
+    ikey[i] = key;
+  } else if (i < a.n_total + a.E) {
+    uint32_t e = i - a.n_total;
+    size_t base = a.n_total + (size_t)kEdgeCells * e;
+    int c = 0;
+    if (a.edge_moving[e]) {
+      long long r[6];
+      cell_range(ldbox(a.ebox, e), a.h, r);
+      long long cnt = (r[3] - r[0] + 1) * (r[4] - r[1] + 1) * (r[5] - r[2] + 1);
+      if (cnt > kEdgeCells) {
+        over[atomicAdd(n_over, 1ull)] = e;
+      } else {
+        uint32_t table = 1u << a.log2t;
+        for (long long x = r[0]; x <= r[3]; ++x)
+          for (long long y = r[1]; y <= r[4]; ++y)
+            for (long long z = r[2]; z <= r[5]; ++z) {
+              unsigned long long key = cell_key(x, y, z);
+              uint32_t bk = table + bucket_of(key, a.log2t);
+              uint32_t s0 = a.bstart[bk];
+              start[base + c] = s0;
+              len[base + c] = a.bstart[bk + 1] - s0;
+              ikey[base + c] = key;
+              ++c;
             }
-          }
-          D3 A = ld3(a.x, ia), B = ld3(a.x, ib), C = ld3(a.x, jc), Dd = ld3(a.x, jd);
-          double s, t;
-          ogm::closest_segments(A, B, C, Dd, s, t);
-          D3 P = ogm::add(A, ogm::scale(ogm::sub(B, A), s));
-          D3 Q = ogm::add(C, ogm::scale(ogm::sub(Dd, C), t));
-          double dist = ogm::dist(P, Q);
-          if (dist >= a.rq) continue;
-          atomicAdd(&a.sc->ee_count, 1ull);
-          dmin(a, ia, dist);
-          dmin(a, ib, dist);
-          dmin(a, jc, dist);
-          dmin(a, jd, dist);
-          int end_a = s <= 0.0 ? (int)ia : (s >= 1.0 ? (int)ib : -1);
-          int end_b = t <= 0.0 ? (int)jc : (t >= 1.0 ? (int)jd : -1);
-          if ((end_a >= 0 && !ogm::vertex_feasible(Q, end_a, (const D3*)a.x, a.nb_off, a.nb)) ||
-              (end_b >= 0 && !ogm::vertex_feasible(P, end_b, (const D3*)a.x, a.nb_off, a.nb)))
-            continue;
-          if (end_a >= 0 && end_b >= 0)
-            add_contact(a, 0, end_a, end_b, 0, 0, 2, 0.0);
-          else if (end_a >= 0)
-            add_contact(a, 1, end_a, jc, jd, 0, 3, 0.0);
-          else if (end_b >= 0)
-            add_contact(a, 1, end_b, ia, ib, 0, 3, 0.0);
-          else
-            add_contact(a, 3, ia, ib, jc, jd, 4, 1e-3 * a.rest_len2[e] * a.rest_len2[f]);
-        }
       }
-  atomicAdd(&a.sc->grid_tests, tests);
+    }
+    for (; c < kEdgeCells; ++c) len[base + c] = 0;
+  }
+}
+// Owner item of every flattened pair slot; slots past `cap` raise the
+// capacity flag (the host grows the buffer and detects again).
+__global__ void k_item_owner(const uint32_t* off, uint32_t NI, uint32_t* owner, uint64_t cap, Scalars* sc) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NI) return;
+  uint32_t o0 = off[i], o1 = off[i + 1];
+  if (o1 > cap) {
+    atomicOr(&sc->err_overflow, 4u);
+    return;
+  }
+  for (uint32_t o = o0; o < o1; ++o) owner[o] = i;
+}
+// One (query item, bucket entry) pair per thread, grid-stride.
+// The pair range is split into 8 contiguous chunks, one per XCD (blocks b
+// with b % 8 == x run on XCD x), and each XCD's blocks stride through their
+// chunk: neighbouring slots read the same buckets from one L2, and every XCD
+// gets an equal share however many slots are in use.  gridDim.x is a
+// multiple of 8.
+__global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, const uint32_t* owner, const uint32_t* off,
+                                               const uint32_t* start, const unsigned long long* ikey,
+                                               uint32_t NI, uint64_t cap) {
+  uint64_t W = off[NI];
+  if (W > cap) W = 0;  // capacity flagged by k_item_owner: nothing is examined, the host retries
+  PairCounts cnt;
+  const uint32_t X = 8, x = blockIdx.x % X, nbx = gridDim.x / X;
+  const uint64_t lo = W * x / X, hi = W * (x + 1) / X, step = (uint64_t)nbx * blockDim.x;
+  for (uint64_t j = lo + (uint64_t)(blockIdx.x / X) * blockDim.x + threadIdx.x; j < hi; j += step) {
+    uint32_t item = owner[j];
+    uint32_t it = start[item] + (uint32_t)(j - off[item]);
+    unsigned long long key = ikey[item];
+    const double* bx = a.sbox + 6 * (size_t)it;
+    ogm::Box eb{{bx[0], bx[1], bx[2]}, {bx[3], bx[4], bx[5]}};
+    uint32_t prim = a.sprim[it];
+    if (a.skey[it] != key) continue;  // another cell hashed to the same bucket
+    if (item < a.n_total) {
+      uint32_t v = item;
+      D3 p = ld3(a.x, v);
+      vf_pair(a, v, v < a.n, a.ex_off[v] != a.ex_off[v + 1], p, ogm::box_of(p), prim, eb, cnt);
+    } else {
+      uint32_t e = (item - a.n_total) / kEdgeCells;
+      uint32_t ia = a.edges[2 * e], ib = a.edges[2 * e + 1];
+      ee_pair(a, e, ldbox(a.ebox, e), ia, ib, a.ex_off[ia] != a.ex_off[ia + 1], a.ex_off[ib] != a.ex_off[ib + 1],
+              prim, eb, key, cnt);
+    }
+  }
+  add_counts(a, cnt);
+}
+// Edges whose box covers more than kEdgeCells cells: a group of G lanes per
+// listed edge walks its cells (rare: edges much longer than the cell size).
+template <int G>
+__global__ __launch_bounds__(TPB) void k_ee_overflow(DetectArgs a, const uint32_t* over,
+                                                     const unsigned long long* n_over) {
+  PairCounts cnt;
+  uint32_t lane = threadIdx.x % G;
+  uint64_t groups = (uint64_t)gridDim.x * blockDim.x / G;
+  for (uint64_t gi = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G; gi < *n_over; gi += groups) {
+    uint32_t e = over[gi];
+    ogm::Box be = ldbox(a.ebox, e);
+    long long r[6];
+    cell_range(be, a.h, r);
+    uint32_t ia = a.edges[2 * e], ib = a.edges[2 * e + 1];
+    bool ia_ex = a.ex_off[ia] != a.ex_off[ia + 1], ib_ex = a.ex_off[ib] != a.ex_off[ib + 1];
+    uint32_t table = 1u << a.log2t;
+    for (long long ci = r[0]; ci <= r[3]; ++ci)
+      for (long long cj = r[1]; cj <= r[4]; ++cj)
+        for (long long ck = r[2]; ck <= r[5]; ++ck) {
+          unsigned long long key = cell_key(ci, cj, ck);
+          uint32_t bk = table + bucket_of(key, a.log2t);
+          uint32_t end = a.bstart[bk + 1];
+          for (uint32_t it = a.bstart[bk] + lane; it < end; it += G) {
+            if (a.skey[it] != key) continue;
+            const double* bx = a.sbox + 6 * (size_t)it;
+            ogm::Box bfr{{bx[0], bx[1], bx[2]}, {bx[3], bx[4], bx[5]}};
+            ee_pair(a, e, be, ia, ib, ia_ex, ib_ex, a.sprim[it], bfr, key, cnt);
+          }
+        }
+  }
+  add_counts(a, cnt);
 }
 
 __global__ void k_init_dist(unsigned long long* bits, uint32_t n, double rq) {
@@ -420,37 +580,57 @@ void read_scalars(cv_gpu& g) {
   CV_HIP(hipStreamSynchronize(g.stream));
 }
 
-// Register primitive boxes in grid cells and sort by cell key.  Returns the
-// number of (cell, primitive) entries.
-uint32_t build_grid(cv_gpu& g, const double* box, uint32_t N, double pr, double h) {
+// Register face and edge boxes (inflated by pr) in the hash grid: entries
+// bucketed by cell-key hash, faces in buckets [0, T) and edges in [T, 2T).
+// Returns log2(T).
+int build_grid(cv_gpu& g, uint32_t F, uint32_t E, double pr, double h, uint32_t& entries) {
   StepBufs& b = g.b;
   hipStream_t st = g.stream;
+  uint32_t N = F + E;
   b.cell_count.reserve(N + 1);
   b.cell_off.reserve(N + 1);
-  if (N == 0) return 0;
-  k_cell_count<<<blocks(N), TPB, 0, st>>>(box, N, pr, h, b.cell_count.p);
+  k_reg_count<<<blocks(N), TPB, 0, st>>>(b.fbox.p, b.ebox.p, F, E, pr, h, b.cell_count.p);
   CV_HIP(hipGetLastError());
   CV_HIP(hipMemsetAsync(b.cell_count.p + N, 0, sizeof(uint32_t), st));
   with_temp(g, [&](void* t, size_t& bytes) {
     return rocprim::exclusive_scan(t, bytes, b.cell_count.p, b.cell_off.p, 0u, (size_t)N + 1,
                                    rocprim::plus<uint32_t>(), st);
   });
-  uint32_t total = 0;
-  CV_HIP(hipMemcpyAsync(&total, b.cell_off.p + N, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  uint32_t tot[2] = {0, 0};  // entries of faces, all entries
+  CV_HIP(hipMemcpyAsync(&tot[0], b.cell_off.p + F, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  CV_HIP(hipMemcpyAsync(&tot[1], b.cell_off.p + N, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   CV_HIP(hipStreamSynchronize(st));
-  if (total > (1u << 31)) fail(CV_ERR_CAPACITY, "OGC grid registration exceeds 2^31 cells");
-  b.cell_key.reserve(total);
-  b.cell_val.reserve(total);
-  b.cell_key_sorted.reserve(total);
-  b.cell_val_sorted.reserve(total);
-  k_cell_fill<<<blocks(N), TPB, 0, st>>>(box, N, pr, h, b.cell_off.p, b.cell_count.p, b.cell_key.p,
-                                         b.cell_val.p, g.d_scal);
+  uint32_t M = tot[1];
+  entries = M;
+  if (M > (1u << 30)) fail(CV_ERR_CAPACITY, "OGC grid registration exceeds 2^30 cells");
+  int log2t = 10;
+  while (log2t < 28 && (1u << log2t) < M / 4) ++log2t;
+  size_t T2 = (size_t)2 << log2t;
+  b.cell_key.reserve(std::max<uint32_t>(M, 1));
+  b.cell_val.reserve(std::max<uint32_t>(M, 1));
+  b.cell_rank.reserve(std::max<uint32_t>(M, 1));
+  b.cell_key_sorted.reserve(std::max<uint32_t>(M, 1));
+  b.cell_val_sorted.reserve(std::max<uint32_t>(M, 1));
+  b.cell_owner.reserve(std::max<uint32_t>(M, 1));
+  b.cell_box_sorted.reserve(6 * (size_t)std::max<uint32_t>(M, 1));
+  b.bcount.reserve(T2 + 1);
+  b.bstart.reserve(T2 + 1);
+  CV_HIP(hipMemsetAsync(b.bcount.p, 0, (T2 + 1) * sizeof(uint32_t), st));
+  k_reg_owner<<<blocks(N), TPB, 0, st>>>(b.cell_off.p, N, b.cell_owner.p);
+  if (M)
+    k_reg_fill<<<blocks(M), TPB, 0, st>>>(b.fbox.p, b.ebox.p, F, pr, h, b.cell_off.p, b.cell_owner.p, M, log2t,
+                                          b.cell_key.p, b.cell_val.p, b.cell_rank.p, b.bcount.p, g.d_scal);
   CV_HIP(hipGetLastError());
   with_temp(g, [&](void* t, size_t& bytes) {
-    return rocprim::radix_sort_pairs(t, bytes, b.cell_key.p, b.cell_key_sorted.p, b.cell_val.p,
-                                     b.cell_val_sorted.p, (size_t)total, 0, 64, st);
+    return rocprim::exclusive_scan(t, bytes, b.bcount.p, b.bstart.p, 0u, T2 + 1, rocprim::plus<uint32_t>(),
+                                   st);
   });
-  return total;
+  if (M)
+    k_reg_scatter<<<blocks(M), TPB, 0, st>>>(b.cell_key.p, b.cell_val.p, b.cell_rank.p, M, tot[0], log2t,
+                                             b.bstart.p, b.fbox.p, b.ebox.p, b.cell_key_sorted.p,
+                                             b.cell_val_sorted.p, b.cell_box_sorted.p);
+  CV_HIP(hipGetLastError());
+  return log2t;
 }
 
 }  // namespace
@@ -520,15 +700,52 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   a.cand_ee = cand_cap ? b.cand_ee.p : nullptr;
   a.cand_cap = cand_cap;
   a.sc = g.d_scal;
-  // VF: faces registered with the padded inflation, points look up their cell.
-  uint32_t nk = build_grid(g, b.fbox.p, d.F, pr_reg, h);
-  k_vf<<<blocks(d.n_total), TPB, 0, st>>>(a, b.cell_key_sorted.p, b.cell_val_sorted.p, nk);
+  // Faces and edges registered with the padded inflation; points look up
+  // their own cell, edges the cells of their own box.
+  uint32_t grid_entries = 0;
+  a.log2t = build_grid(g, d.F, d.E, pr_reg, h, grid_entries);
+  a.skey = b.cell_key_sorted.p;
+  a.sprim = b.cell_val_sorted.p;
+  a.sbox = b.cell_box_sorted.p;
+  a.bstart = b.bstart.p;
+  // Flattened broad phase: query items, their pair counts, owners, pairs.
+  uint32_t NI = d.n_total + kEdgeCells * d.E;
+  b.item_len.reserve(NI + 1);
+  b.item_off.reserve(NI + 1);
+  b.item_start.reserve(NI);
+  b.item_key.reserve(NI);
+  b.over.reserve(std::max<uint32_t>(d.E, 1));
+  CV_HIP(hipMemsetAsync(b.item_len.p + NI, 0, sizeof(uint32_t), st));
+  CV_HIP(hipMemsetAsync(&g.d_scal->n_over, 0, sizeof(unsigned long long), st));
+  k_items<<<blocks(d.n_total + d.E), TPB, 0, st>>>(a, b.item_len.p, b.item_start.p, b.item_key.p, b.over.p,
+                                                   &g.d_scal->n_over);
   CV_HIP(hipGetLastError());
-  // EE: edges registered with the padded inflation.
-  nk = build_grid(g, b.ebox.p, d.E, pr_reg, h);
-  k_ee<<<blocks(d.E), TPB, 0, st>>>(a, b.cell_key_sorted.p, b.cell_val_sorted.p, nk);
+  with_temp(g, [&](void* t, size_t& bytes) {
+    return rocprim::exclusive_scan(t, bytes, b.item_len.p, b.item_off.p, 0u, (size_t)NI + 1,
+                                   rocprim::plus<uint32_t>(), st);
+  });
+  // Pair-slot capacity: grown from the last detection's need; an overflow is
+  // flagged on the device and the detection is run again with more room.
+  if (b.pair_cap == 0) b.pair_cap = std::max<uint64_t>(4ull * grid_entries + (1u << 20), 1u << 22);
+  b.pair_owner.reserve(b.pair_cap);
+  k_item_owner<<<blocks(NI), TPB, 0, st>>>(b.item_off.p, NI, b.pair_owner.p, b.pair_cap, g.d_scal);
+  CV_HIP(hipGetLastError());
+  k_pairs<<<8 * 256, TPB, 0, st>>>(
+      a, b.pair_owner.p, b.item_off.p, b.item_start.p, b.item_key.p, NI, b.pair_cap);
+  CV_HIP(hipGetLastError());
+  constexpr int G = 16;
+  k_ee_overflow<G><<<64, TPB, 0, st>>>(a, b.over.p, &g.d_scal->n_over);
   CV_HIP(hipGetLastError());
   read_scalars(g);
+  if (g.h_scal->err_overflow & 4u) {
+    // More (query, entry) pairs than slots: grow and detect again.
+    uint32_t need = 0;
+    CV_HIP(hipMemcpyAsync(&need, b.item_off.p + NI, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    CV_HIP(hipStreamSynchronize(st));
+    b.pair_cap = (uint64_t)need + need / 2 + 1024;
+    gpu_detect(g, s, rq, ogc, cand_cap);
+    return;
+  }
   if (g.h_scal->err_overflow)
     fail(CV_ERR_CAPACITY, "OGC contact buffer or grid capacity exceeded");
   uint64_t m = g.h_scal->raw_contacts;

@@ -68,6 +68,7 @@ struct SweepArgs {
   const cvm::TermRecT<R>* crec;
   float* gsq;
   unsigned long long* clipped;
+  const unsigned int* stop;  // set when a refresh is pending: the launch is a no-op
   R cr, ck;
   GrabArgs<R> grab;
 };
@@ -102,12 +103,30 @@ __device__ __forceinline__ bool grab_d(const GrabArgs<R>& g, uint32_t v, V3<R> x
   return true;
 }
 
+// The first term record of each lane is loaded once per vertex solve and kept
+// in registers for the derivative and every line-search energy; only the
+// neighbour positions are re-read (they are L1/L2 hits).  Holding the
+// positions in registers too exceeds the f64 register budget and spills.
+template <class R>
+struct LaneTerm {
+  cvm::TermRecT<R> rec;
+  bool live;
+};
+template <class R>
+__device__ __forceinline__ LaneTerm<R> load_lane_term(const SweepArgs<R>& a, int lane, uint32_t s0, uint32_t ns,
+                                                      uint32_t c0, uint32_t nt) {
+  LaneTerm<R> t;
+  t.live = (uint32_t)lane < nt;
+  if (t.live) t.rec = (uint32_t)lane < ns ? a.terms[s0 + lane] : a.crec[c0 + (lane - ns)];
+  return t;
+}
+
 template <int L, class R>
-__device__ R local_energy(const SweepArgs<R>& a, uint32_t v, V3<R> xv, int lane, uint32_t s0, uint32_t ns,
-                          uint32_t c0, uint32_t nt) {
+__device__ __forceinline__ R local_energy(const SweepArgs<R>& a, const LaneTerm<R>& lt, uint32_t v, V3<R> xv, int lane,
+                          uint32_t s0, uint32_t ns, uint32_t c0, uint32_t nt) {
   auto P = [&](uint32_t i) { return ldp(a.pos, i); };
-  R e = R(0);
-  for (uint32_t i = lane; i < nt; i += L) {
+  R e = lt.live ? cvm::term_energy<R>(lt.rec, xv, a.cr, a.ck, P) : R(0);
+  for (uint32_t i = lane + L; i < nt; i += L) {
     const cvm::TermRecT<R>& t = i < ns ? a.terms[s0 + i] : a.crec[c0 + (i - ns)];
     e += cvm::term_energy<R>(t, xv, a.cr, a.ck, P);
   }
@@ -120,11 +139,11 @@ __device__ R local_energy(const SweepArgs<R>& a, uint32_t v, V3<R> xv, int lane,
 }
 
 template <int L, class R>
-__device__ cvm::Deriv<R> local_deriv(const SweepArgs<R>& a, uint32_t v, V3<R> x, int lane, uint32_t s0,
-                                     uint32_t ns, uint32_t c0, uint32_t nt) {
+__device__ __forceinline__ cvm::Deriv<R> local_deriv(const SweepArgs<R>& a, const LaneTerm<R>& lt, uint32_t v, V3<R> x,
+                                     int lane, uint32_t s0, uint32_t ns, uint32_t c0, uint32_t nt) {
   auto P = [&](uint32_t i) { return ldp(a.pos, i); };
-  cvm::Deriv<R> acc = cvm::deriv_zero<R>();
-  for (uint32_t i = lane; i < nt; i += L) {
+  cvm::Deriv<R> acc = lt.live ? cvm::term_deriv<R>(lt.rec, x, a.cr, a.ck, P) : cvm::deriv_zero<R>();
+  for (uint32_t i = lane + L; i < nt; i += L) {
     const cvm::TermRecT<R>& t = i < ns ? a.terms[s0 + i] : a.crec[c0 + (i - ns)];
     cvm::deriv_add(acc, cvm::term_deriv<R>(t, x, a.cr, a.ck, P));
   }
@@ -147,8 +166,10 @@ template <int L, class R>
 __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uint32_t count) {
   using T4 = typename P4<R>::T;
   const int lane = threadIdx.x & (L - 1);
-  const uint32_t grp = (blockIdx.x * blockDim.x + threadIdx.x) / L;
-  if (grp >= count) return;
+  // Vertices of a colour are in index order, so neighbouring groups share
+  // neighbours: keep consecutive groups on one XCD's L2.
+  const uint32_t grp = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / L;
+  if (grp >= count || *a.stop) return;
   const uint32_t v = a.order[off + grp];
   T4 p4 = a.pos[v];
   if (p4.w > R(0.5)) {  // pinned: no update, zero residual contribution
@@ -158,14 +179,15 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uin
   V3<R> x{p4.x, p4.y, p4.z};
   uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
   uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
-  cvm::Deriv<R> acc = local_deriv<L>(a, v, x, lane, s0, ns, c0, nt);
+  const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt);
+  cvm::Deriv<R> acc = local_deriv<L>(a, lt, v, x, lane, s0, ns, c0, nt);
   T4 tg = a.tgt[v];
   V3<R> target{tg.x, tg.y, tg.z};
   R inertia = tg.w;
   // The accept test compares energies from one evaluation routine, so a
   // candidate equal to x reproduces `old` exactly.
   V3<R> dx = x - target;
-  R old = local_energy<L>(a, v, x, lane, s0, ns, c0, nt) + R(0.5) * inertia * cvm::dot(dx, dx);
+  R old = local_energy<L>(a, lt, v, x, lane, s0, ns, c0, nt) + R(0.5) * inertia * cvm::dot(dx, dx);
   V3<R> delta = cvm::newton_direction<R>(acc, x, target, inertia);
   R gg = cvm::dot(acc.g, acc.g);
   T4 b4 = a.ball[v];
@@ -176,7 +198,7 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uin
   for (int attempt = 0; attempt < 10; ++attempt) {
     V3<R> cand = cvm::truncate_ball<R>(x + delta * scale, center, b4.w, truncated);
     V3<R> dc = cand - target;
-    R e = local_energy<L>(a, v, cand, lane, s0, ns, c0, nt) + R(0.5) * inertia * cvm::dot(dc, dc);
+    R e = local_energy<L>(a, lt, v, cand, lane, s0, ns, c0, nt) + R(0.5) * inertia * cvm::dot(dc, dc);
     if (cvm::finite3(cand) && isfinite(e) && e <= old + R(1e-14)) {
       accepted = cand;
       break;
@@ -187,6 +209,25 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uin
     a.pos[v] = P4<R>::make(accepted.x, accepted.y, accepted.z, R(0));
     a.gsq[v] = (float)gg;
     if (truncated) atomicAdd(a.clipped, 1ull);
+  }
+}
+
+// End of iteration `it` (it = -1: after the truncated guess), on the device
+// so the host need not read the clipped count back every iteration: counts
+// the iteration, accumulates truncations and, when the clipped count reaches
+// the refresh threshold (src/vbd.rs:326-336), raises `stop` so the launches
+// already queued behind it are no-ops until the host has re-detected.
+__global__ void k_iter_end(Scalars* sc, int it, double threshold) {
+  if (sc->stop) return;
+  unsigned long long c = sc->clipped;
+  if (it >= 0) {
+    sc->iters_run += 1;
+    sc->truncated_total += c;
+  }
+  sc->clipped = 0;
+  if (c > 0 && (double)c >= threshold) {
+    sc->stop = 1;
+    sc->stop_it = (unsigned int)(it + 1);  // the next iteration to run after the refresh
   }
 }
 
@@ -201,7 +242,8 @@ __global__ void k_probe(SweepArgs<R> a, uint32_t n, double* out) {
   V3<R> x{p4.x, p4.y, p4.z};
   uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
   uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
-  cvm::Deriv<R> acc = local_deriv<L>(a, v, x, lane, s0, ns, c0, nt);
+  const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt);
+  cvm::Deriv<R> acc = local_deriv<L>(a, lt, v, x, lane, s0, ns, c0, nt);
   if (lane == 0) {
     T4 tg = a.tgt[v];
     V3<R> dx = x - V3<R>{tg.x, tg.y, tg.z};
@@ -441,6 +483,7 @@ SweepArgs<R> sweep_args(cv_gpu& g, const cv_settings* set, const cv_grab* grab) 
   a.crec = PBufs<R>::crec(g);
   a.gsq = g.b.gsq.p;
   a.clipped = &g.d_scal->clipped;
+  a.stop = &g.d_scal->stop;
   a.cr = (R)set->contact.radius;
   a.ck = (R)set->contact.stiffness;
   a.grab.active = grab ? 1 : 0;
@@ -559,6 +602,8 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
   double r = set->contact.radius;
   double rq = std::min(std::max(r + g.h_scal->motion, 2.0 * r), 4.0 * r);
   g.h_scal->clipped = 0;
+  g.h_scal->truncated_total = 0;
+  g.h_scal->stop = g.h_scal->stop_it = g.h_scal->iters_run = 0;
   g.h_scal->err_nonfinite = g.h_scal->err_trust = 0;
   write_scalars(g);
   k_prologue<R><<<blocks(n + d.n_fixed), TPB, 0, st>>>(
@@ -570,54 +615,82 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
   k_truncate_guess<R><<<blocks(n), TPB, 0, st>>>(b.guess64.p, b.centers64.p, b.guess64.p + 3 * (size_t)n,
                                                  d.pinned.p, n, g.d_scal, PBufs<R>::pos(g), &g.d_scal->clipped);
   CV_HIP(hipGetLastError());
-  read_scalars(g);
-  uint64_t clipped = g.h_scal->clipped;
+  const double threshold = set->contact.refresh_fraction * n;
+  k_iter_end<<<1, 1, 0, st>>>(g.d_scal, -1, threshold);
+  CV_HIP(hipGetLastError());
   SweepArgs<R> a = sweep_args<R>(g, set, grab);
-  uint32_t iters = set->iterations;
-  for (uint32_t it = 0; it < iters; ++it) {
-    if (clipped > 0 && (double)clipped >= set->contact.refresh_fraction * n) {
+  const uint32_t iters = set->iterations;
+  // Iterations are queued in batches with no read-back inside a batch: the
+  // first batch is iteration 0 alone (a refresh after it is common), the next
+  // runs to the end.  A pending refresh turns the rest of a batch into no-ops;
+  // the host then re-detects and queues again from the next iteration.
+  uint32_t next = 0, done = 0;
+  struct Launch {
+    size_t ev;
+    uint32_t it;
+  };
+  std::vector<Launch> launches;
+  size_t nev = 0;
+  while (true) {
+    read_scalars(g);
+    uint32_t ran = g.h_scal->iters_run;
+    // Account the iterations that ran since the last read-back.
+    for (const Launch& l : launches) {
+      if (l.it >= ran) continue;
+      out.sweep_launches++;
+      if (g.profile) {
+        float ms = 0;
+        CV_HIP(hipEventElapsedTime(&ms, g.sweep_events[2 * l.ev], g.sweep_events[2 * l.ev + 1]));
+        out.sweep_ms += ms;
+      }
+    }
+    launches.clear();
+    nev = 0;
+    double inc = (double)g.h_scal->incidences;
+    out.sweep_bytes += (ran - done) * (sizeof(R) == 4 ? g.static_bytes32 + 56.0 * inc
+                                                       : g.static_bytes64 + 100.0 * inc);
+    out.sweep_bytes_min += (ran - done) * (g.static_bytes32 + 56.0 * inc);
+    done = ran;
+    if (g.h_scal->stop) {
+      next = g.h_scal->stop_it;
+      if (next >= iters) break;
+      // Refresh before iteration `next` at the current positions.
+      g.h_scal->stop = 0;
+      CV_HIP(hipMemcpyAsync(&g.d_scal->stop, &g.h_scal->stop, sizeof(unsigned int), hipMemcpyHostToDevice, st));
       k_current<R><<<blocks(n), TPB, 0, st>>>(PBufs<R>::pos(g), b.x_old64.p, d.pinned.p, n, g.d_scal, b.x64.p);
       CV_HIP(hipGetLastError());
       detect_and_color<R>(g, s, set, grab, rq, out, false);
       a = sweep_args<R>(g, set, grab);
+    } else if (next >= iters) {
+      break;
     }
-    CV_HIP(hipMemsetAsync(&g.d_scal->clipped, 0, sizeof(unsigned long long), st));
-    size_t nev = 0;
-    for (size_t c = 0; c + 1 < g.color_off.size(); ++c) {
-      uint32_t off = g.color_off[c], cnt = g.color_off[c + 1] - off;
-      if (!cnt) continue;
-      if (g.profile) {
-        if (g.sweep_events.size() < 2 * (nev + 1)) {
-          hipEvent_t e0, e1;
-          CV_HIP(hipEventCreate(&e0));
-          CV_HIP(hipEventCreate(&e1));
-          g.sweep_events.push_back(e0);
-          g.sweep_events.push_back(e1);
+    uint32_t end = next == 0 ? 1 : iters;
+    for (uint32_t it = next; it < end; ++it) {
+      for (size_t c = 0; c + 1 < g.color_off.size(); ++c) {
+        uint32_t off = g.color_off[c], cnt = g.color_off[c + 1] - off;
+        if (!cnt) continue;
+        if (g.profile) {
+          if (g.sweep_events.size() < 2 * (nev + 1)) {
+            hipEvent_t e0, e1;
+            CV_HIP(hipEventCreate(&e0));
+            CV_HIP(hipEventCreate(&e1));
+            g.sweep_events.push_back(e0);
+            g.sweep_events.push_back(e1);
+          }
+          sweep_color<R>(g, a, off, cnt, g.sweep_events[2 * nev], g.sweep_events[2 * nev + 1]);
+        } else {
+          sweep_color<R>(g, a, off, cnt);
         }
-        sweep_color<R>(g, a, off, cnt, g.sweep_events[2 * nev], g.sweep_events[2 * nev + 1]);
-        ++nev;
-      } else {
-        sweep_color<R>(g, a, off, cnt);
+        launches.push_back({nev, it});
+        nev += g.profile ? 1 : 0;
       }
-      out.sweep_launches++;
+      k_iter_end<<<1, 1, 0, st>>>(g.d_scal, (int)it, threshold);
+      CV_HIP(hipGetLastError());
     }
-    // Algorithmic bytes of this sweep: static terms + contact incidences.
-    out.sweep_bytes += sizeof(R) == 4 ? g.static_bytes32 + 56.0 * (double)g.h_scal->incidences
-                                      : g.static_bytes64 + 100.0 * (double)g.h_scal->incidences;
-    out.sweep_bytes_min += g.static_bytes32 + 56.0 * (double)g.h_scal->incidences;
-    CV_HIP(hipGetLastError());
-    unsigned long long hc = 0;
-    CV_HIP(hipMemcpyAsync(&hc, &g.d_scal->clipped, sizeof(hc), hipMemcpyDeviceToHost, st));
-    CV_HIP(hipStreamSynchronize(st));
-    clipped = hc;
-    out.truncated += clipped;
-    out.iterations_run++;
-    for (size_t k = 0; k < nev; ++k) {
-      float ms = 0;
-      CV_HIP(hipEventElapsedTime(&ms, g.sweep_events[2 * k], g.sweep_events[2 * k + 1]));
-      out.sweep_ms += ms;
-    }
+    next = end;
   }
+  out.truncated += g.h_scal->truncated_total;
+  out.iterations_run += g.h_scal->iters_run;
   k_sum_f32<<<1, 1024, 0, st>>>(b.gsq.p, n, &g.d_scal->residual_sum);
   // The trust-region check tolerance of the device path (src/gpu_vbd.rs:436).
   k_finalize<R><<<blocks(n), TPB, 0, st>>>(PBufs<R>::pos(g), b.x_old64.p, b.v_old64.p, b.centers64.p,

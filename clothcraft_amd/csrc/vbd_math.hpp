@@ -560,10 +560,13 @@ CV_HD R term_energy(const TermRecT<F>& t, V3<R> xv, R cr, R ck, PosFn&& P) {
     return bending_e<R, F>(x[0], x[1], x[2], x[3], t.f);
   }
   if (kind == TERM_STITCH) {
+    // Constant indices only (fully unrolled): a record held in registers is
+    // never indexed dynamically.
     int s = rec_slot(t.head), n = rec_count(t.head);
     V3<R> d = mk<R>(0, 0, 0);
-    for (int i = 0; i < n; ++i)
-      if (i != s) d = d + (P(t.ids[i]) - xv) * R(t.f[i]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i < n && i != s) d = d + (P(t.ids[i]) - xv) * R(t.f[i]);
     return R(0.5) * R(t.f[4]) * dot(d, d);
   }
   int s = rec_slot(t.head);
@@ -584,9 +587,11 @@ CV_HD Deriv<R> term_deriv(const TermRecT<F>& t, V3<R> xv, R cr, R ck, PosFn&& P)
   if (kind == TERM_STITCH) {
     int n = rec_count(t.head);
     V3<R> d = mk<R>(0, 0, 0);
-    for (int i = 0; i < n; ++i)
-      if (i != s) d = d + (P(t.ids[i]) - xv) * R(t.f[i]);
-    return stitch_d<R>(d, R(t.f[s]), R(t.f[4]));
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i < n && i != s) d = d + (P(t.ids[i]) - xv) * R(t.f[i]);
+    R ws = s == 0 ? R(t.f[0]) : s == 1 ? R(t.f[1]) : s == 2 ? R(t.f[2]) : R(t.f[3]);
+    return stitch_d<R>(d, ws, R(t.f[4]));
   }
   V3<R> x[4];
   for (int i = 0; i < 4; ++i) x[i] = i == s ? xv : P(t.ids[i]);
