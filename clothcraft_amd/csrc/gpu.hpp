@@ -108,6 +108,7 @@ struct StepBufs {
   DBuf<float4> pos, tgt, ball;        // f32 sweep
   DBuf<double4> pos_d, tgt_d, ball_d;  // f64 sweep
   DBuf<float> gsq;
+  DBuf<double> gtrue;  // per-vertex |g|^2 of the true residual
   DBuf<uint32_t> order;
   // detection
   DBuf<double> fbox, ebox;  // [6F], [6E]

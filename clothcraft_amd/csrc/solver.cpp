@@ -260,6 +260,8 @@ void build_solver(cv_solver& s, const cv_garment* g, const cv_surface* avatar, c
       s.pinned[i] = 1;
       for (int k = 0; k < 3; ++k) s.pin_targets[3 * i + k] = g->pin_targets[3 * i + k];
     }
+  s.n_free = 0;
+  for (uint32_t i = 0; i < n; ++i) s.n_free += s.pinned[i] ? 0u : 1u;
   s.materials.assign(g->materials, g->materials + g->n_materials);
   for (auto& m : s.materials) validate_material(m);
   s.tri_v.assign(g->tri_vertices, g->tri_vertices + 3 * (size_t)g->n_triangles);

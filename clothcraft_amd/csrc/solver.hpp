@@ -34,6 +34,7 @@ struct OgcTopology {  // src/ogc/topology.rs:13-52 over all faces (garment, then
 
 struct cv_solver {
   uint32_t n = 0;
+  uint32_t n_free = 0;                  // vertices without a pin
   std::vector<double> masses;
   std::vector<uint8_t> pinned;
   std::vector<double> pin_targets;

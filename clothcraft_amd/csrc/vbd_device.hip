@@ -217,7 +217,7 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uin
 // the iteration, accumulates truncations and, when the clipped count reaches
 // the refresh threshold (src/vbd.rs:326-336), raises `stop` so the launches
 // already queued behind it are no-ops until the host has re-detected.
-__global__ void k_iter_end(Scalars* sc, int it, double threshold) {
+__global__ void k_iter_end(Scalars* sc, int it, double threshold, double tol, double nfree3) {
   if (sc->stop) return;
   unsigned long long c = sc->clipped;
   if (it >= 0) {
@@ -225,6 +225,12 @@ __global__ void k_iter_end(Scalars* sc, int it, double threshold) {
     sc->truncated_total += c;
   }
   sc->clipped = 0;
+  // Residual-tolerance extension: converged after this sweep ends the step.
+  if (it >= 0 && tol > 0.0 && sqrt(sc->true_residual_sum / nfree3) <= tol) {
+    sc->stop = 2;
+    sc->stop_it = (unsigned int)(it + 1);
+    return;
+  }
   if (c > 0 && (double)c >= threshold) {
     sc->stop = 1;
     sc->stop_it = (unsigned int)(it + 1);  // the next iteration to run after the refresh
@@ -252,6 +258,44 @@ __global__ void k_probe(SweepArgs<R> a, uint32_t n, double* out) {
     out[4 * v + 2] = acc.g.z + dx.z * tg.w;
     out[4 * v + 3] = acc.e + R(0.5) * tg.w * cvm::dot(dx, dx);
   }
+}
+
+// True residual contribution |g_v|^2 of the local objective at the current
+// positions (g = elastic + contact + grab gradient + inertia (x - target)),
+// 0 for pinned vertices: the Newton-style rms of src/solver.rs:229-231.
+template <int L, class R>
+__global__ __launch_bounds__(TPB) void k_true_grad(SweepArgs<R> a, uint32_t n, double* out) {
+  using T4 = typename P4<R>::T;
+  const int lane = threadIdx.x & (L - 1);
+  const uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / L;
+  if (v >= n) return;
+  T4 p4 = a.pos[v];
+  if (p4.w > R(0.5)) {
+    if (lane == 0) out[v] = 0.0;
+    return;
+  }
+  V3<R> x{p4.x, p4.y, p4.z};
+  uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
+  uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
+  const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt);
+  cvm::Deriv<R> acc = local_deriv<L>(a, lt, v, x, lane, s0, ns, c0, nt);
+  if (lane == 0) {
+    T4 tg = a.tgt[v];
+    V3<R> g = acc.g + (x - V3<R>{tg.x, tg.y, tg.z}) * tg.w;
+    out[v] = (double)cvm::dot(g, g);
+  }
+}
+__global__ void k_sum_f64(const double* a, uint32_t n, double* out) {
+  __shared__ double red[1024];
+  double s = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s += a[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
 }
 
 // Step prologue reductions (single block, fixed order => deterministic):
@@ -554,6 +598,18 @@ uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
   return k;
 }
 
+template <class R>
+void true_residual(cv_gpu& g, const SweepArgs<R>& a, uint32_t n) {
+  switch (g.lanes) {
+    case 8: k_true_grad<8, R><<<blocks((uint64_t)n * 8), TPB, 0, g.stream>>>(a, n, g.b.gtrue.p); break;
+    case 16: k_true_grad<16, R><<<blocks((uint64_t)n * 16), TPB, 0, g.stream>>>(a, n, g.b.gtrue.p); break;
+    case 64: k_true_grad<64, R><<<blocks((uint64_t)n * 64), TPB, 0, g.stream>>>(a, n, g.b.gtrue.p); break;
+    default: k_true_grad<32, R><<<blocks((uint64_t)n * 32), TPB, 0, g.stream>>>(a, n, g.b.gtrue.p); break;
+  }
+  k_sum_f64<<<1, 1024, 0, g.stream>>>(g.b.gtrue.p, n, &g.d_scal->true_residual_sum);
+  CV_HIP(hipGetLastError());
+}
+
 struct StepOut {
   uint64_t detections = 0, truncated = 0;
   uint32_t colors = 0, iterations_run = 0;
@@ -561,6 +617,7 @@ struct StepOut {
   double sweep_ms = 0, sweep_bytes = 0, sweep_bytes_min = 0;
   double detection_ms = 0, refit_ms = 0, candidates_ms = 0, narrow_ms = 0;
   uint64_t candidate_pairs = 0, grid_tests = 0;
+  bool true_residual = false;
 };
 
 // Detection + balls (+ colouring) on the positions held in b.x64.
@@ -616,7 +673,11 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
                                                  d.pinned.p, n, g.d_scal, PBufs<R>::pos(g), &g.d_scal->clipped);
   CV_HIP(hipGetLastError());
   const double threshold = set->contact.refresh_fraction * n;
-  k_iter_end<<<1, 1, 0, st>>>(g.d_scal, -1, threshold);
+  const double tol = set->residual_tolerance;
+  const double nfree3 = 3.0 * std::max<uint32_t>(s.n_free, 1);
+  const bool want_true = tol > 0.0 || (set->flags & CV_FLAG_TRUE_RESIDUAL);
+  if (want_true) b.gtrue.reserve(n);
+  k_iter_end<<<1, 1, 0, st>>>(g.d_scal, -1, threshold, tol, nfree3);
   CV_HIP(hipGetLastError());
   SweepArgs<R> a = sweep_args<R>(g, set, grab);
   const uint32_t iters = set->iterations;
@@ -651,6 +712,7 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
                                                        : g.static_bytes64 + 100.0 * inc);
     out.sweep_bytes_min += (ran - done) * (g.static_bytes32 + 56.0 * inc);
     done = ran;
+    if (g.h_scal->stop == 2) break;  // residual tolerance reached
     if (g.h_scal->stop) {
       next = g.h_scal->stop_it;
       if (next >= iters) break;
@@ -684,13 +746,20 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
         launches.push_back({nev, it});
         nev += g.profile ? 1 : 0;
       }
-      k_iter_end<<<1, 1, 0, st>>>(g.d_scal, (int)it, threshold);
+      if (tol > 0.0) true_residual<R>(g, a, n);
+      k_iter_end<<<1, 1, 0, st>>>(g.d_scal, (int)it, threshold, tol, nfree3);
       CV_HIP(hipGetLastError());
     }
     next = end;
   }
   out.truncated += g.h_scal->truncated_total;
   out.iterations_run += g.h_scal->iters_run;
+  if (want_true) {
+    // Post-sweep residual of the final positions (recomputed when the last
+    // batch ended on a refresh or without the tolerance mode).
+    true_residual<R>(g, a, n);
+    out.true_residual = true;
+  }
   k_sum_f32<<<1, 1024, 0, st>>>(b.gsq.p, n, &g.d_scal->residual_sum);
   // The trust-region check tolerance of the device path (src/gpu_vbd.rs:436).
   k_finalize<R><<<blocks(n), TPB, 0, st>>>(PBufs<R>::pos(g), b.x_old64.p, b.v_old64.p, b.centers64.p,
@@ -856,7 +925,8 @@ void fill_metrics(cv_metrics* m, const cv_gpu& g, const cv_solver& s, const cv_s
   m->truncated_vertices = o.truncated;
   m->minimum_distance_m = g.h_scal->min_distance;
   m->residual_rms_n = std::sqrt(g.h_scal->residual_sum / (3.0 * s.n));
-  m->true_residual_rms_n = NAN;
+  m->true_residual_rms_n =
+      o.true_residual ? std::sqrt(g.h_scal->true_residual_sum / (3.0 * std::max<uint32_t>(s.n_free, 1))) : NAN;
   m->colors = o.colors;
   m->iterations_run = o.iterations_run;
   m->sweep_ms = g.profile ? o.sweep_ms : NAN;

@@ -130,6 +130,7 @@ VbdSettings settings_of(const cv_settings* s) {
   o.contact.stiffness = s->contact.stiffness;
   o.contact.relaxation = s->contact.relaxation;
   o.contact.refresh_fraction = s->contact.refresh_fraction;
+  o.residual_tolerance = s->residual_tolerance;
   return o;
 }
 OgcSettings ogc_of(const cv_ogc_settings* s) {
@@ -177,7 +178,7 @@ void fill_metrics(const VbdMetrics& m, cv_metrics* out) {
   out->residual_rms_n = m.residual_rms_n;
   out->true_residual_rms_n = m.true_residual_rms_n;
   out->colors = m.colors;
-  out->iterations_run = (uint32_t)m.iterations;
+  out->iterations_run = (uint32_t)m.iterations_run;
 }
 
 }  // namespace
@@ -267,7 +268,7 @@ int cvo_solver_step(cvo_solver* s, cv_state* st, const cv_settings* settings, co
     GrabT g;
     if (grab) g = grab_of(grab);
     VbdMetrics m = s->s.step(state, vs, grab ? &g : nullptr);
-    if (settings->flags & CV_FLAG_TRUE_RESIDUAL) {
+    if ((settings->flags & CV_FLAG_TRUE_RESIDUAL) && !(vs.residual_tolerance > 0.0)) {
       // Post-sweep residual of the committed positions, against this step's
       // target/inertia and final contact set.
       int n = s->s.garment.n;

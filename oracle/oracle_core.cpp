@@ -512,6 +512,12 @@ VbdMetrics VbdSolver::step(State& state, const VbdSettings& st, const GrabT* gra
     }
     m.truncated_vertices += clipped;
     m.residual_rms_n = std::sqrt(residual / (3.0 * n));
+    m.iterations_run = it + 1;
+    if (st.residual_tolerance > 0.0) {
+      // Extension: true residual after the sweep, against the current contacts.
+      m.true_residual_rms_n = true_residual(x, target, inertia, contacts, st, grab);
+      if (m.true_residual_rms_n <= st.residual_tolerance) break;
+    }
   }
   for (auto& p : x)
     if (!finite(p)) throw std::runtime_error("N|Non-finite VBD result; state unchanged");

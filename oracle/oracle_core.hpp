@@ -851,10 +851,14 @@ struct VbdSettings {  // src/vbd.rs:18-52
   double damping = 1.5;
   V3 gravity{0.0, -9.81, 0.0};
   OgcSettings contact;
+  // Extension with no reference counterpart: > 0 ends the iterations once the
+  // post-sweep (true) residual is <= this value.
+  double residual_tolerance = 0.0;
   void validate() const {
     contact.validate();
     if (!std::isfinite(dt) || !(dt >= 1e-6 && dt <= 0.025) || iterations == 0 || iterations > 200 ||
-        !std::isfinite(damping) || damping < 0.0 || !finite(gravity))
+        !std::isfinite(damping) || damping < 0.0 || !finite(gravity) ||
+        !std::isfinite(residual_tolerance) || residual_tolerance < 0.0)
       throw std::runtime_error("S|Invalid VBD settings");
   }
 };
@@ -901,6 +905,7 @@ struct VbdMetrics {  // src/vbd.rs:61-83
   double residual_rms_n = std::numeric_limits<double>::quiet_NaN();
   double true_residual_rms_n = std::numeric_limits<double>::quiet_NaN();
   uint32_t colors = 0;
+  int iterations_run = 0;
 };
 
 struct Stitch {

@@ -249,3 +249,50 @@ def test_device_resident_run_equals_repeated_steps(gpu):
         gpu.step(sa, a, sc.settings)
     gpu.run(sb, b, sc.settings, 4)
     assert np.array_equal(a.x, b.x) and np.array_equal(a.v, b.v) and a.time == b.time
+
+
+def test_true_residual_and_tolerance_mode_match_oracle(gpu):
+    """Extension (no reference counterpart): the post-sweep residual
+    sqrt(sum_free |g|^2 / 3 n_free) (src/solver.rs:229-231 form) and the
+    residual-tolerance iteration mode agree with the oracle."""
+    sc = scenes.small_sheet_on_sphere(16, 0.2535)
+    s = dict(sc.settings, flags=2)  # CV_FLAG_TRUE_RESIDUAL
+    sg, sr, mg, mr = _one_step(gpu, sc, s)
+    assert np.isfinite(mg["true_residual_rms_n"])
+    assert abs(mg["true_residual_rms_n"] - mr["true_residual_rms_n"]) <= 1e-6 * mr["true_residual_rms_n"] + 1e-9
+    tol = 3.0 * mr["true_residual_rms_n"]
+    st = dict(sc.settings, residual_tolerance=tol, iterations=40)
+    sg, sr, mg, mr = _one_step(gpu, sc, st)
+    print(f"tolerance {tol:.3e}: iterations gpu {mg['iterations_run']} oracle {mr['iterations_run']}")
+    assert mg["iterations_run"] == mr["iterations_run"] < 40
+    assert mg["true_residual_rms_n"] <= tol
+    assert np.abs(sg.x - sr.x).max() / _bbox_diag(sr.x) <= 1e-9
+
+
+@pytest.mark.parametrize("which", ["hanging", "sphere"])
+def test_energy_against_newton_reference(gpu, which):
+    """One device step against one tight f64 Newton step on the same
+    objective (src/solver.rs inertial form equals the VBD objective up to a
+    constant; contact workloads freeze the step's final OGC contacts in the
+    Newton objective).  The gap is the VBD method's (fixed iterations, OGC
+    trust regions) and is printed; the device's objective value must equal
+    the oracle VBD step's."""
+    sc = scenes.hanging_sheet() if which == "hanging" else scenes.small_sheet_on_sphere(24, 0.2535)
+    sg, sr, mg, mr = _one_step(gpu, sc)
+    s = sc.settings
+    ns = O.newton_settings(dt=s["dt"], gravity=s["gravity"], damping=s["damping"], newton_iterations=60,
+                           pcg_iterations=3000, absolute_tolerance=1e-10, relative_tolerance=1e-10)
+    fz = None
+    if sc.avatar is not None:
+        ref = O.Solver(sc.garment, sc.avatar, sc.state)
+        ref.step(sc.state.copy(), s)
+        fz = O.frozen_c(ref.last_contacts(), s["radius"], s["stiffness"], sc.avatar.vertices)
+    nw = sc.state.copy()
+    O.newton_step(sc.garment, nw, ns, None, fz)
+    e_gpu = O.newton_objective(sc.garment, sc.state, sg.x, ns, None, fz)
+    e_ref = O.newton_objective(sc.garment, sc.state, sr.x, ns, None, fz)
+    e_newton = O.newton_objective(sc.garment, sc.state, nw.x, ns, None, fz)
+    gap = abs(e_gpu - e_newton) / abs(e_newton)
+    print(f"{sc.name}: E_vbd {e_gpu:.9f} E_newton {e_newton:.9f} relative gap {gap:.3e}")
+    assert abs(e_gpu - e_ref) <= 1e-9 * abs(e_ref)
+    assert e_gpu >= e_newton - 1e-9 * abs(e_newton)  # Newton is the minimiser of the same objective

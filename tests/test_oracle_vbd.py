@@ -99,3 +99,17 @@ def test_independent_numpy_restatement_agrees_on_a_hanging_sheet():
         ref.step(b, s)
         assert np.abs(a.x - b.x).max() < 1e-12
         assert np.abs(a.v - b.v).max() < 1e-9
+
+
+def test_residual_tolerance_ends_iterations_early():
+    """Extension: with residual_tolerance > 0 the step stops after the first
+    sweep whose true residual is within it; the same step without the
+    tolerance runs all iterations and ends with a smaller residual."""
+    sc = scenes.hanging_sheet()
+    a, b = sc.state.copy(), sc.state.copy()
+    full = O.Solver(sc.garment, None, a).step(a, dict(sc.settings, iterations=30, flags=2))
+    tol = 10.0 * full["true_residual_rms_n"]
+    m = O.Solver(sc.garment, None, b).step(b, dict(sc.settings, iterations=30, residual_tolerance=tol))
+    assert 0 < m["iterations_run"] < 30
+    assert m["true_residual_rms_n"] <= tol
+    assert full["iterations_run"] == 30 and full["true_residual_rms_n"] < m["true_residual_rms_n"]
