@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host CPUs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nx", type=int, default=128, help="quads per side (128 is the workload)")
+    ap.add_argument("--workload", default="sheet_on_sphere", choices=("sheet_on_sphere", "sewn_garment"),
+                    help="sewn_garment: two ~50k-vertex Poisson-disk panels with side and shoulder "
+                         "seams on a capsule body, 40 iterations (not the headline line)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -102,7 +105,11 @@ def main():
 
     from clothcraft_amd import GpuVbd, VbdSolver, scenes  # fails loudly if not built
 
-    sc = scenes.sheet_on_sphere(nx=args.nx)
+    if args.workload == "sewn_garment":
+        from clothcraft_amd import garments
+        sc = garments.sewn_garment_on_body()
+    else:
+        sc = scenes.sheet_on_sphere(nx=args.nx)
     settings = sc.settings
     st = sc.state.copy()
     gpu = GpuVbd(0)
@@ -162,10 +169,15 @@ def main():
         "dtype": "f64" if args.precision == 64 else "f32",
         "data": "synthetic (seeded sheet and icosphere geometry)",
         "config": {
-            "workload": f"sheet on sphere {args.nx}x{args.nx} quads: {sc.garment.n} vertices, "
-                        f"{len(sc.garment.tri_vertices)} triangles, icosphere r 0.25 m "
-                        f"({len(sc.avatar.triangles)} triangles), OGC r 2 mm k 1e5, "
-                        f"{settings['iterations']} iterations, dt 1/60, self-contact on",
+            "workload": (f"sheet on sphere {args.nx}x{args.nx} quads: {sc.garment.n} vertices, "
+                         f"{len(sc.garment.tri_vertices)} triangles, icosphere r 0.25 m "
+                         f"({len(sc.avatar.triangles)} triangles), OGC r 2 mm k 1e5, "
+                         f"{settings['iterations']} iterations, dt 1/60, self-contact on")
+            if args.workload == "sheet_on_sphere" else
+            (f"sewn garment: {sc.garment.n} vertices, {len(sc.garment.tri_vertices)} triangles, "
+             f"{len(sc.garment.stitch_stiffness)} seam stitches, capsule body "
+             f"({len(sc.avatar.triangles)} triangles), OGC r 1 mm k 1e5, "
+             f"{settings['iterations']} iterations, dt 1/60, self-contact on"),
             "steps_timed": f"1..{K} of the drop (warmup on a discarded copy)",
             "iteration_mode": "fixed iterations (residual tolerance 0)",
             "lanes_per_vertex": args.lanes,
@@ -188,7 +200,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": _traffic(),
+            "traffic": _traffic() if (args.workload == "sheet_on_sphere" and args.nx == 128
+                                       and args.lanes == 32 and args.precision == 64) else None,
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_us": 1e6 * avg_launch_s,
             "launches": m["sweep_launches"],

@@ -22,6 +22,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 SOURCES = [
     ("vbd_device.hip", []),
     ("ogc_device.hip", ["-ffp-contract=off"]),
+    ("mlp_device.hip", []),
     ("solver.cpp", ["-ffp-contract=off", "-x", "hip"]),
     ("api.cpp", ["-x", "hip"]),
 ]

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "mlp.hpp"
 #include "solver.hpp"
 
 #define CV_HIP(call)                                                                         \
@@ -138,6 +139,17 @@ struct StepBufs {
   size_t contact_cap = 0;
 };
 
+// Warm-start network state on the device (FP32 master parameters, Adam
+// moments, BF16 operand tiles, gradient scratch, training data).
+struct Mlp {
+  DBuf<float> master, grad, m1, m2, partial, loss_partial, loss;
+  DBuf<uint16_t> fw, bw;
+  DBuf<float> X, T, Xb, Tb;  // training set and the current shuffled copy
+  DBuf<uint32_t> perm;
+  int adam_t = 0;
+  bool ready = false;
+};
+
 struct cv_gpu {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -163,7 +175,35 @@ struct cv_gpu {
   std::vector<hipEvent_t> sweep_events;
   double static_bytes32 = 0, static_bytes64 = 0;  // algorithmic bytes of one sweep, static terms
   bool f64 = true;  // sweep precision (f64 reproduces the f64 reference step)
+  Mlp mlp;
+  DBuf<float> feat, wpred;  // warm-start features [12n] and predictions [3n]
 };
+
+// Runs f, mapping exceptions to a status and message (no exception crosses
+// the C ABI).
+template <class F>
+cv_status guard(std::string* err, F&& f) {
+  try {
+    f();
+    return CV_OK;
+  } catch (const CvError& e) {
+    if (err) *err = e.message;
+    return e.status;
+  } catch (const std::exception& e) {
+    if (err) *err = e.what();
+    return CV_ERR_DEVICE;
+  }
+}
+
+// Warm-start network (mlp_device.hip).
+void mlp_ensure(cv_gpu& g);
+void mlp_set_params(cv_gpu& g, const float* params);
+void mlp_get_params(cv_gpu& g, float* params);
+void mlp_infer_device(cv_gpu& g, const float* dX, uint32_t n, float* dY);
+void mlp_grad_device(cv_gpu& g, const float* dX, const float* dT, uint32_t n, uint32_t row0, uint32_t count,
+                     bool accumulate_loss);
+void mlp_gather(cv_gpu& g, const float* src, const uint32_t* perm, uint32_t n, int width, float* dst);
+void mlp_adam_device(cv_gpu& g, float lr);
 
 // Detection on device positions b.x64 (all n_total vertices) with query
 // radius rq; fills distance, radii, unique contacts and contact records.

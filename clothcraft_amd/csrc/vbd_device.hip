@@ -298,6 +298,63 @@ __global__ void k_sum_f64(const double* a, uint32_t n, double* out) {
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// Warm-start features of each moving vertex (12 floats, dimensionless), with
+// L = |g| dt^2 / (1 + c dt), the free-fall displacement of one step:
+//   0-2  inertial-prediction offset (v dt + g dt^2) / (1 + c dt) / L
+//   3-5  previous velocity v dt / L
+//   6-7  mean Green strain of the incident triangles along warp and weft, x 100
+//   8-10 mean velocity of the one-ring neighbours, v dt / L
+//   11   distance to the nearest contact primitive / query radius (<= 1)
+__global__ void k_warm_features(const double* x, const double* v, const uint32_t* term_off,
+                                const cvm::TermRec64* terms, const uint32_t* nb_off, const uint32_t* nb,
+                                const unsigned long long* dist_bits, uint32_t n, double dt, double damp, double gx,
+                                double gy, double gz, double L, double rq, float* feat) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]}, w[3] = {v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+  double g[3] = {gx, gy, gz};
+  float* f = feat + 12 * (size_t)i;
+  for (int k = 0; k < 3; ++k) {
+    f[k] = (float)((w[k] * dt + g[k] * dt * dt) / damp / L);
+    f[3 + k] = (float)(w[k] * dt / L);
+  }
+  double ew = 0, ev = 0;
+  int tri = 0;
+  for (uint32_t k = term_off[i]; k < term_off[i + 1]; ++k) {
+    const cvm::TermRec64& r = terms[k];
+    if (cvm::rec_kind(r.head) != cvm::TERM_MEMBRANE) continue;
+    double e1[3], e2[3], a[3], b[3];
+    for (int c = 0; c < 3; ++c) {
+      e1[c] = x[3 * r.ids[0] + c] - p[c];
+      e2[c] = x[3 * r.ids[1] + c] - p[c];
+      a[c] = e1[c] * r.f[0] + e2[c] * r.f[1];
+      b[c] = e1[c] * r.f[2] + e2[c] * r.f[3];
+    }
+    ew += 0.5 * (a[0] * a[0] + a[1] * a[1] + a[2] * a[2] - 1.0);
+    ev += 0.5 * (b[0] * b[0] + b[1] * b[1] + b[2] * b[2] - 1.0);
+    ++tri;
+  }
+  f[6] = (float)(tri ? 100.0 * ew / tri : 0.0);
+  f[7] = (float)(tri ? 100.0 * ev / tri : 0.0);
+  double mv[3] = {0, 0, 0};
+  int cnt = 0;
+  for (uint32_t k = nb_off[i]; k < nb_off[i + 1]; ++k) {
+    uint32_t j = nb[k];
+    if (j >= n) continue;
+    for (int c = 0; c < 3; ++c) mv[c] += v[3 * j + c];
+    ++cnt;
+  }
+  for (int c = 0; c < 3; ++c) f[8 + c] = (float)(cnt ? mv[c] / cnt * dt / L : 0.0);
+  double d = __longlong_as_double((long long)dist_bits[i]);
+  f[11] = (float)(fmin(d, rq) / rq);
+}
+// Learned guess x + L * prediction for moving vertices.
+__global__ void k_warm_guess(const float* pred, const double* x, const uint8_t* pinned, uint32_t n, double L,
+                             double* guess) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || pinned[i]) return;
+  for (int k = 0; k < 3; ++k) guess[3 * i + k] = x[3 * i + k] + L * (double)pred[3 * i + k];
+}
 // Step prologue reductions (single block, fixed order => deterministic):
 // origin = centroid (src/gpu_vbd.rs:156), predicted motion max|target - x|
 // (src/vbd.rs:298-302) and max |x|_inf of the moving part (padded radius).
@@ -645,6 +702,25 @@ void detect_and_color(cv_gpu& g, const cv_solver& s, const cv_settings* set, con
   out.narrow_ms += t1 - t0;
 }
 
+// Free-fall displacement of one step, the length unit of the warm-start
+// features and predictions.
+inline double warm_length(const cv_settings* set) {
+  double g = std::sqrt(set->gravity[0] * set->gravity[0] + set->gravity[1] * set->gravity[1] +
+                       set->gravity[2] * set->gravity[2]);
+  return std::max(g * set->dt * set->dt / (1.0 + set->damping * set->dt), 1e-9);
+}
+// Features at the positions in b.x64 / velocities in b.v64, using the last
+// detection's distances.
+void warm_features(cv_gpu& g, const cv_solver& s, const cv_settings* set, double rq) {
+  uint32_t n = s.n;
+  g.feat.reserve(12 * (size_t)n);
+  double dt = set->dt;
+  k_warm_features<<<blocks(n), TPB, 0, g.stream>>>(
+      g.b.x64.p, g.b.v64.p, g.dev.term_off.p, g.dev.terms_d.p, g.dev.nb_off.p, g.dev.nb.p, g.b.dist_bits.p, n, dt,
+      1.0 + set->damping * dt, set->gravity[0], set->gravity[1], set->gravity[2], warm_length(set), rq, g.feat.p);
+  CV_HIP(hipGetLastError());
+}
+
 template <class R>
 void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out) {
   DevSolver& d = g.dev;
@@ -669,6 +745,16 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
       PBufs<R>::tgt(g), b.guess64.p, b.x_old64.p, b.v_old64.p);
   CV_HIP(hipGetLastError());
   detect_and_color<R>(g, s, set, grab, rq, out, true);
+  if (set->flags & CV_FLAG_WARM_START) {
+    // Learned guess in place of the adaptive one; clipped to the balls below
+    // exactly as that guess is.
+    if (!g.mlp.ready) fail(CV_ERR_NOT_BUILT, "warm-start network not set up");
+    warm_features(g, s, set, rq);
+    g.wpred.reserve(3 * (size_t)n);
+    mlp_infer_device(g, g.feat.p, n, g.wpred.p);
+    k_warm_guess<<<blocks(n), TPB, 0, st>>>(g.wpred.p, b.x64.p, d.pinned.p, n, warm_length(set), b.guess64.p);
+    CV_HIP(hipGetLastError());
+  }
   k_truncate_guess<R><<<blocks(n), TPB, 0, st>>>(b.guess64.p, b.centers64.p, b.guess64.p + 3 * (size_t)n,
                                                  d.pinned.p, n, g.d_scal, PBufs<R>::pos(g), &g.d_scal->clipped);
   CV_HIP(hipGetLastError());
@@ -889,19 +975,6 @@ extern thread_local std::string cv_create_error;  // api.cpp
 
 namespace {
 
-template <class F>
-cv_status guard(std::string* err, F&& f) {
-  try {
-    f();
-    return CV_OK;
-  } catch (const CvError& e) {
-    if (err) *err = e.message;
-    return e.status;
-  } catch (const std::exception& e) {
-    if (err) *err = e.what();
-    return CV_ERR_DEVICE;
-  }
-}
 
 void fill_metrics(cv_metrics* m, const cv_gpu& g, const cv_solver& s, const cv_settings* set,
                   const StepOut& o, double step_ms, double gpu_ms, const double* x_new) {
@@ -1247,6 +1320,124 @@ cv_status cv_gpu_step_batch(cv_gpu* g, cv_solver* const* solvers, cv_state* cons
   return CV_OK;
 }
 
+
+// ------------------------------------------------------- learned warm start
+cv_status cv_warm_set_params(cv_gpu* g, const float* params) {
+  if (!g || !params) return CV_ERR_INVALID_ARGUMENT;
+  return guard(&g->last_error, [&] {
+    CV_HIP(hipSetDevice(g->device));
+    for (int i = 0; i < cvmlp::NPARAM; ++i)
+      if (!std::isfinite(params[i])) fail(CV_ERR_INVALID_ARGUMENT, "non-finite warm-start parameter");
+    mlp_set_params(*g, params);
+    CV_HIP(hipStreamSynchronize(g->stream));
+  });
+}
+
+cv_status cv_warm_get_params(cv_gpu* g, float* params) {
+  if (!g || !params) return CV_ERR_INVALID_ARGUMENT;
+  return guard(&g->last_error, [&] {
+    CV_HIP(hipSetDevice(g->device));
+    mlp_get_params(*g, params);
+  });
+}
+
+cv_status cv_warm_infer(cv_gpu* g, const float* features, uint64_t n, float* out) {
+  if (!g || (n && (!features || !out)) || n > 0xFFFFFFF0ull) return CV_ERR_INVALID_ARGUMENT;
+  return guard(&g->last_error, [&] {
+    CV_HIP(hipSetDevice(g->device));
+    if (!g->mlp.ready) fail(CV_ERR_NOT_BUILT, "warm-start network not set up");
+    if (!n) return;
+    DBuf<float> X, Y;
+    X.upload(features, 12 * n, g->stream);
+    Y.reserve(3 * n);
+    mlp_infer_device(*g, X.p, (uint32_t)n, Y.p);
+    CV_HIP(hipMemcpyAsync(out, Y.p, 3 * n * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    CV_HIP(hipStreamSynchronize(g->stream));
+  });
+}
+
+cv_status cv_warm_gradient(cv_gpu* g, const float* features, const float* targets, uint64_t n, float* grad,
+                           float* loss) {
+  if (!g || !features || !targets || !grad || n == 0 || n > 0xFFFFFFF0ull) return CV_ERR_INVALID_ARGUMENT;
+  return guard(&g->last_error, [&] {
+    CV_HIP(hipSetDevice(g->device));
+    if (!g->mlp.ready) fail(CV_ERR_NOT_BUILT, "warm-start network not set up");
+    DBuf<float> X, T;
+    X.upload(features, 12 * n, g->stream);
+    T.upload(targets, 3 * n, g->stream);
+    mlp_grad_device(*g, X.p, T.p, (uint32_t)n, 0, (uint32_t)n, false);
+    CV_HIP(hipMemcpyAsync(grad, g->mlp.grad.p, cvmlp::NPARAM * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    float l = 0;
+    CV_HIP(hipMemcpyAsync(&l, g->mlp.loss.p, sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    CV_HIP(hipStreamSynchronize(g->stream));
+    if (loss) *loss = l / (3.0f * (float)n);
+  });
+}
+
+cv_status cv_warm_train(cv_gpu* g, const float* features, const float* targets, uint64_t n, uint32_t epochs,
+                        uint32_t batch, float lr, uint64_t seed, float* epoch_loss) {
+  if (!g || !features || !targets || n == 0 || n > 0xFFFFFFF0ull || batch == 0 || !(lr > 0.0f))
+    return CV_ERR_INVALID_ARGUMENT;
+  return guard(&g->last_error, [&] {
+    CV_HIP(hipSetDevice(g->device));
+    if (!g->mlp.ready) fail(CV_ERR_NOT_BUILT, "warm-start network not set up");
+    Mlp& m = g->mlp;
+    m.X.upload(features, 12 * n, g->stream);
+    m.T.upload(targets, 3 * n, g->stream);
+    m.Xb.reserve(12 * n);
+    m.Tb.reserve(3 * n);
+    std::vector<uint32_t> perm(n);
+    for (uint64_t i = 0; i < n; ++i) perm[i] = (uint32_t)i;
+    uint64_t state = seed ? seed : 0x9E3779B97F4A7C15ull;
+    auto next = [&] {  // splitmix64
+      uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      return z ^ (z >> 31);
+    };
+    for (uint32_t e = 0; e < epochs; ++e) {
+      for (uint64_t i = n - 1; i > 0; --i) std::swap(perm[i], perm[next() % (i + 1)]);
+      m.perm.upload(perm, g->stream);
+      mlp_gather(*g, m.X.p, m.perm.p, (uint32_t)n, 12, m.Xb.p);
+      mlp_gather(*g, m.T.p, m.perm.p, (uint32_t)n, 3, m.Tb.p);
+      for (uint64_t r = 0; r < n; r += batch) {
+        mlp_grad_device(*g, m.Xb.p, m.Tb.p, (uint32_t)n, (uint32_t)r, batch, r > 0);
+        mlp_adam_device(*g, lr);
+      }
+      if (epoch_loss) {
+        float l = 0;
+        CV_HIP(hipMemcpyAsync(&l, m.loss.p, sizeof(float), hipMemcpyDeviceToHost, g->stream));
+        CV_HIP(hipStreamSynchronize(g->stream));
+        epoch_loss[e] = l / (3.0f * (float)n);
+      }
+    }
+    CV_HIP(hipStreamSynchronize(g->stream));
+  });
+}
+
+cv_status cv_warm_features(cv_gpu* g, cv_solver* s, const cv_state* st, const cv_settings* set, float* out) {
+  if (!g || !s || !out) return CV_ERR_INVALID_ARGUMENT;
+  return guard(&g->last_error, [&] {
+    CV_HIP(hipSetDevice(g->device));
+    validate_settings(set);
+    validate_state(*s, st);
+    ensure_device_solver(*g, *s);
+    ensure_step_buffers(*g, *s);
+    stage_state(*g, *s, st, set);
+    g->pred_valid = false;
+    uint32_t n = s->n;
+    double dt = set->dt, damp = 1.0 + set->damping * dt;
+    k_prologue_reduce<<<1, 1024, 0, g->stream>>>(g->b.x64.p, g->b.v64.p, n, dt, damp, set->gravity[0],
+                                                 set->gravity[1], set->gravity[2], g->d_scal);
+    read_scalars(*g);
+    double r = set->contact.radius;
+    double rq = std::min(std::max(r + g->h_scal->motion, 2.0 * r), 4.0 * r);
+    gpu_detect(*g, *s, rq, set->contact, 0);
+    warm_features(*g, *s, set, rq);
+    CV_HIP(hipMemcpyAsync(out, g->feat.p, 12 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    CV_HIP(hipStreamSynchronize(g->stream));
+  });
+}
 }  // extern "C"
 
 const std::string* cv_gpu_error_string(const cv_gpu* g) { return &g->last_error; }

@@ -67,6 +67,14 @@ def lib():
         L.cv_gpu_step_batch.argtypes = [vp, C.POINTER(vp), C.POINTER(C.POINTER(abi.cv_state)),
                                         C.c_uint32, C.POINTER(abi.cv_settings),
                                         C.POINTER(abi.cv_metrics)]
+        f32p = C.POINTER(C.c_float)
+        L.cv_warm_set_params.argtypes = [vp, f32p]
+        L.cv_warm_get_params.argtypes = [vp, f32p]
+        L.cv_warm_infer.argtypes = [vp, f32p, C.c_uint64, f32p]
+        L.cv_warm_gradient.argtypes = [vp, f32p, f32p, C.c_uint64, f32p, f32p]
+        L.cv_warm_train.argtypes = [vp, f32p, f32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float,
+                                    C.c_uint64, f32p]
+        L.cv_warm_features.argtypes = [vp, vp, C.POINTER(abi.cv_state), C.POINTER(abi.cv_settings), f32p]
         L.cv_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
         L.cv_last_error.restype = C.c_size_t
         L.cv_solver_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
@@ -205,6 +213,11 @@ class GpuVbd:
             self.h = None
 
     @property
+    def warm(self) -> "WarmStart":
+        """The learned warm-start network of this device context."""
+        return WarmStart(self)
+
+    @property
     def check_gradients(self) -> bool:
         return self._check
 
@@ -288,6 +301,65 @@ class GpuVbd:
         if candidates:
             out["vf"] = vf[:int(d.n_vf_candidates)].copy()
             out["ee"] = ee[:int(d.n_ee_candidates)].copy()
+        return out
+
+
+WARM_PARAMS = 35075
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, np.float32)
+    if shape is not None and a.shape != shape:
+        a = a.reshape(shape)
+    return a
+
+
+class WarmStart:
+    """The learned warm start on one GpuVbd (no reference counterpart):
+    parameters, device inference, the device backward pass and training."""
+
+    def __init__(self, gpu: "GpuVbd"):
+        self.gpu = gpu
+
+    def _call(self, rc):
+        if rc != 0:
+            self.gpu._raise(rc)
+
+    def set_params(self, params):
+        p = _f32(params)
+        assert p.size == WARM_PARAMS
+        self._call(lib().cv_warm_set_params(self.gpu.h, abi.ptr(p, C.c_float)))
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(WARM_PARAMS, np.float32)
+        self._call(lib().cv_warm_get_params(self.gpu.h, abi.ptr(out, C.c_float)))
+        return out
+
+    def infer(self, features) -> np.ndarray:
+        X = _f32(features).reshape(-1, 12)
+        out = np.zeros((len(X), 3), np.float32)
+        self._call(lib().cv_warm_infer(self.gpu.h, abi.ptr(X, C.c_float), len(X), abi.ptr(out, C.c_float)))
+        return out
+
+    def gradient(self, features, targets):
+        X, T = _f32(features).reshape(-1, 12), _f32(targets).reshape(-1, 3)
+        g = np.zeros(WARM_PARAMS, np.float32)
+        loss = C.c_float()
+        self._call(lib().cv_warm_gradient(self.gpu.h, abi.ptr(X, C.c_float), abi.ptr(T, C.c_float), len(X),
+                                          abi.ptr(g, C.c_float), C.byref(loss)))
+        return g, float(loss.value)
+
+    def train(self, features, targets, epochs: int, batch: int = 4096, lr: float = 1e-3, seed: int = 1):
+        X, T = _f32(features).reshape(-1, 12), _f32(targets).reshape(-1, 3)
+        losses = np.zeros(epochs, np.float32)
+        self._call(lib().cv_warm_train(self.gpu.h, abi.ptr(X, C.c_float), abi.ptr(T, C.c_float), len(X), epochs,
+                                       batch, lr, seed, abi.ptr(losses, C.c_float)))
+        return losses
+
+    def features(self, solver: "VbdSolver", state, settings) -> np.ndarray:
+        out = np.zeros((solver.n, 12), np.float32)
+        self._call(lib().cv_warm_features(self.gpu.h, solver.h, C.byref(state.to_c()),
+                                          C.byref(_settings(settings)), abi.ptr(out, C.c_float)))
         return out
 
 

@@ -244,6 +244,38 @@ cv_status cv_gpu_detect(cv_gpu* gpu, cv_solver* solver, const double* x, double 
 cv_status cv_gpu_step_batch(cv_gpu* gpu, cv_solver* const* solvers, cv_state* const* states,
                             uint32_t count, const cv_settings* settings, cv_metrics* metrics);
 
+/* ------------------------------------------------- learned warm start */
+/* No reference counterpart.  A per-vertex network (12 -> 128 -> 128 -> 128
+ * -> 3, GELU, BF16 weights and activations, FP32 accumulation on the matrix
+ * cores) predicts each step's displacement; with CV_FLAG_WARM_START in the
+ * settings the step starts from x + L * prediction instead of the adaptive
+ * guess of Prediction::guess (src/vbd/prediction.rs:15-41) and clips it to
+ * the OGC balls as it clips that guess (src/vbd.rs:316-324); iterations,
+ * residuals and energies are unchanged, so the solver itself judges the
+ * network.  L = |g| dt^2 / (1 + c dt).  Features (12 floats per vertex):
+ * inertial-prediction offset / L (3), v dt / L (3), mean Green strain of the
+ * incident triangles along warp and weft x 100 (2), mean one-ring neighbour
+ * velocity v dt / L (3), nearest contact distance / query radius (1).
+ * Parameters: FP32 vector of CV_WARM_PARAMS values, W0 [12 x 128] row-major,
+ * b0, W1 [128 x 128], b1, W2, b2, W3 [128 x 3], b3. */
+#define CV_WARM_PARAMS 35075
+cv_status cv_warm_set_params(cv_gpu* gpu, const float* params);
+cv_status cv_warm_get_params(cv_gpu* gpu, float* params);
+/* Predictions [3n] for features [12n] (host arrays). */
+cv_status cv_warm_infer(cv_gpu* gpu, const float* features, uint64_t n, float* out);
+/* Gradient [CV_WARM_PARAMS] and mean squared error of the n samples as one
+ * batch (the device training kernels' backward pass). */
+cv_status cv_warm_gradient(cv_gpu* gpu, const float* features, const float* targets, uint64_t n,
+                           float* grad, float* loss);
+/* Adam training on the device: `epochs` passes over the n samples in seeded
+ * shuffled order, mini-batches of `batch`; epoch_loss[e] (optional) is the
+ * epoch's mean squared error. */
+cv_status cv_warm_train(cv_gpu* gpu, const float* features, const float* targets, uint64_t n,
+                        uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float* epoch_loss);
+/* The features [12n] the step would feed the network at `state`. */
+cv_status cv_warm_features(cv_gpu* gpu, cv_solver* solver, const cv_state* state,
+                           const cv_settings* settings, float* features);
+
 /* ------------------------------------------------------------ errors */
 /* Copies the last error message of `gpu` (or, for NULL, the calling thread's
  * last create error) into buf; returns the full message length. */

@@ -76,6 +76,7 @@ def lib():
         L.cvo_solver_record_prediction.argtypes = [vp, abi.f64p, C.POINTER(abi.cv_state),
                                                    C.POINTER(abi.cv_settings)]
         L.cvo_solver_clear_prediction.argtypes = [vp]
+        L.cvo_solver_set_next_guess.argtypes = [vp, abi.f64p]
         L.cvo_solver_last_contacts.argtypes = [vp, C.POINTER(cvo_contacts)]
         L.cvo_solver_last_colors.argtypes = [vp, abi.u32p, C.c_uint32]
         L.cvo_solver_last_colors.restype = C.c_uint32
@@ -235,6 +236,11 @@ class Solver:
 
     def clear_prediction(self):
         lib().cvo_solver_clear_prediction(self.h)
+
+    def set_next_guess(self, guess):
+        """The next step starts from `guess` [n, 3] (the warm-start parity hook)."""
+        g = _f(guess)
+        lib().cvo_solver_set_next_guess(self.h, _p(g))
 
     def last_contacts(self, capacity: int = 1 << 20) -> dict:
         return _contacts(lambda out: lib().cvo_solver_last_contacts(self.h, C.byref(out)),

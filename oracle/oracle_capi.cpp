@@ -304,6 +304,10 @@ int cvo_solver_record_prediction(cvo_solver* s, const double* old_velocity, cons
   });
 }
 void cvo_solver_clear_prediction(cvo_solver* s) { s->s.prediction = Prediction{}; }
+// The next step starts from `guess` [3n] instead of Prediction::guess.
+void cvo_solver_set_next_guess(cvo_solver* s, const double* guess) {
+  s->s.next_guess = v3s(guess, s->s.garment.n);
+}
 
 // Last step's detection, colouring and gradients (test introspection).
 void cvo_solver_last_contacts(const cvo_solver* s, cvo_contacts* out) {

@@ -399,7 +399,8 @@ VbdMetrics VbdSolver::step(State& state, const VbdSettings& st, const GrabT* gra
   m.iterations = st.iterations;
   double dt = st.dt;
   std::vector<V3> old_velocity = state.v;
-  std::vector<V3> guess = prediction.guess(state, st);
+  std::vector<V3> guess = next_guess.size() == (size_t)n ? next_guess : prediction.guess(state, st);
+  next_guess.clear();
   double damp = 1.0 + st.damping * dt;
   std::vector<V3> target(n);
   std::vector<double> inertia(n);

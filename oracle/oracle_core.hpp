@@ -919,6 +919,9 @@ std::array<double, 2> extension_percent(const Garment& g, const std::vector<V3>&
 // ------------------------------------------------------------ VbdSolver
 struct VbdSolver {  // src/vbd.rs:89-422
   Prediction prediction;
+  // Test hook: a guess for the next step in place of Prediction::guess (the
+  // learned warm start's parity test installs the device network's guess).
+  std::vector<V3> next_guess;
   Garment garment;
   OgcWorld world;
   std::vector<std::vector<int>> tri_of, hinge_of, stitch_of, graph;

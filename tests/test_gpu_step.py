@@ -296,3 +296,26 @@ def test_energy_against_newton_reference(gpu, which):
     print(f"{sc.name}: E_vbd {e_gpu:.9f} E_newton {e_newton:.9f} relative gap {gap:.3e}")
     assert abs(e_gpu - e_ref) <= 1e-9 * abs(e_ref)
     assert e_gpu >= e_newton - 1e-9 * abs(e_newton)  # Newton is the minimiser of the same objective
+
+
+def test_one_step_parity_sewn_garment_with_seams_and_body(gpu):
+    """Stitch terms, a capsule-union body and two panels: one step and a few
+    more from the oracle's state agree with the oracle."""
+    from clothcraft_amd import garments
+    sc = garments.sewn_garment_on_body(spacing=0.02)
+    st = sc.state.copy()
+    ref, sol = _solvers(sc, st)
+    s = sc.settings
+    for k in range(3):
+        a, b = st.copy(), st.copy()
+        prev_v = st.v.copy()
+        mr = ref.step(a, s)
+        mg = gpu.step(sol, b, s)
+        err = np.abs(a.x - b.x).max() / _bbox_diag(a.x)
+        print(f"{sc.name} step {k}: rel err {err:.3e} contacts {mg['contact_terms']}/{mr['contact_terms']} "
+              f"seam gap {mg['max_seam_gap_m']:.3e}")
+        assert err <= 1e-9
+        assert mg["contact_terms"] == mr["contact_terms"]
+        assert abs(mg["max_seam_gap_m"] - mr["max_seam_gap_m"]) <= 1e-12 + 1e-9 * mr["max_seam_gap_m"]
+        st = a
+        sol.record_prediction(prev_v, st, s)

@@ -1,0 +1,178 @@
+"""The learned warm start (no reference counterpart): the numpy checker is
+pinned by finite differences; the device forward pass, backward pass,
+training, features and the warm-started step are checked against it and
+against the oracle step started from the same guess."""
+import numpy as np
+import pytest
+
+import mlp_ref as M
+import oracle as O
+from clothcraft_amd import scenes
+
+
+def _data(n, seed=0):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, 12)).astype(np.float32)
+    T = np.stack([np.sin(X[:, 0]) + 0.5 * X[:, 3], X[:, 1] * X[:, 2] * 0.3, np.tanh(X[:, 11] - X[:, 6])], 1)
+    return X, T.astype(np.float32)
+
+
+def test_checker_backward_matches_finite_differences():
+    p = M.init_params(3).astype(np.float64)
+    X, T = _data(9)
+    loss, g = M.loss_and_grad(p, X, T, rounding=False)
+    rng = np.random.default_rng(1)
+    idx = np.concatenate([rng.choice(M.NPARAM, 60, replace=False), np.arange(M.NPARAM - 3, M.NPARAM)])
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = 1e-6
+        fd = (M.loss_and_grad(p + e, X, T, False)[0] - M.loss_and_grad(p - e, X, T, False)[0]) / 2e-6
+        assert abs(fd - g[i]) <= 1e-7 + 1e-5 * abs(g[i])
+
+
+def test_bf16_rounding_is_nearest_even():
+    vals = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -7, -2.5, 3.0e-3], np.float32)
+    r = M.bf16(vals)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2 ** -6 and r[3] == 1.0 + 2 ** -7 and r[4] == -2.5
+    assert abs(r[5] - 3.0e-3) <= 3.0e-3 * 2 ** -8
+
+
+@pytest.mark.gpu
+def test_device_inference_matches_checker(gpu):
+    p = M.init_params(5, out_scale=1.0)
+    gpu.warm.set_params(p)
+    assert np.array_equal(gpu.warm.params(), p)
+    X, _ = _data(1000, 2)
+    y = gpu.warm.infer(X)
+    ref, _, _ = M.forward(p, X, rounding=True)
+    exact, _, _ = M.forward(p, X, rounding=False)
+    scale = np.abs(ref).max()
+    print(f"inference: max |gpu - checker(bf16)| {np.abs(y - ref).max():.3e}, "
+          f"vs exact f64 {np.abs(y - exact).max():.3e}, scale {scale:.3e}")
+    assert np.abs(y - ref).max() <= 1e-2 * scale
+    assert np.abs(y - exact).max() <= 5e-2 * scale
+
+
+@pytest.mark.gpu
+def test_device_gradient_matches_checker(gpu):
+    """The backward kernels (matrix-core products for dA and dW, column sums
+    for the biases) against the checker with the same BF16 rounding points."""
+    p = M.init_params(7, out_scale=1.0)
+    gpu.warm.set_params(p)
+    for n in (32, 1000):
+        X, T = _data(n, 4)
+        g, loss = gpu.warm.gradient(X, T)
+        lr, gr = M.loss_and_grad(p, X, T, rounding=True)
+        le, ge = M.loss_and_grad(p, X, T, rounding=False)
+        rel = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+        rel_exact = np.linalg.norm(g - ge) / np.linalg.norm(ge)
+        print(f"n={n}: loss {loss:.6f} / {lr:.6f}, gradient rel err vs checker {rel:.2e}, vs exact {rel_exact:.2e}")
+        assert abs(loss - lr) <= 1e-3 * lr
+        assert rel <= 1e-2
+        assert rel_exact <= 3e-2
+        # every block of parameters is exercised
+        o = 0
+        for name, shape in M.SHAPES:
+            k = int(np.prod(shape))
+            blk = slice(o, o + k)
+            assert np.linalg.norm(g[blk] - gr[blk]) <= 2e-2 * np.linalg.norm(gr[blk]) + 1e-9, name
+            o += k
+
+
+@pytest.mark.gpu
+def test_device_training_fits_a_smooth_map(gpu):
+    p = M.init_params(11)
+    gpu.warm.set_params(p)
+    X, T = _data(20000, 6)
+    losses = gpu.warm.train(X, T, epochs=12, batch=1024, lr=2e-3, seed=3)
+    print("epoch losses", np.round(losses, 5))
+    assert losses[-1] < 0.1 * losses[0]
+    y = gpu.warm.infer(X[:2000])
+    assert np.mean((y - T[:2000]) ** 2) <= 1.5 * losses[-1] + 1e-3
+
+
+def _numpy_features(sc, state, settings, radii, relaxation, rq):
+    g = sc.garment
+    x, v = state.x, state.v
+    dt, c = settings["dt"], settings["damping"]
+    grav = np.array(settings["gravity"])
+    L = np.linalg.norm(grav) * dt * dt / (1 + c * dt)
+    F = np.zeros((g.n, 12))
+    F[:, 0:3] = (v * dt + grav * dt * dt) / (1 + c * dt) / L
+    F[:, 3:6] = v * dt / L
+    tv = g.tri_vertices.astype(np.int64)
+    inv = g.tri_inv_dm.reshape(-1, 2, 2).transpose(0, 2, 1)  # column-major -> [row][col]
+    e1, e2 = x[tv[:, 1]] - x[tv[:, 0]], x[tv[:, 2]] - x[tv[:, 0]]
+    D = np.stack([e1, e2], 2)  # [T,3,2]
+    Fd = D @ inv
+    ew = 0.5 * ((Fd[:, :, 0] ** 2).sum(1) - 1)
+    ev = 0.5 * ((Fd[:, :, 1] ** 2).sum(1) - 1)
+    cnt = np.zeros(g.n)
+    sw, sv = np.zeros(g.n), np.zeros(g.n)
+    for k in range(3):
+        np.add.at(cnt, tv[:, k], 1)
+        np.add.at(sw, tv[:, k], ew)
+        np.add.at(sv, tv[:, k], ev)
+    F[:, 6] = 100 * sw / np.maximum(cnt, 1)
+    F[:, 7] = 100 * sv / np.maximum(cnt, 1)
+    nbrs = [set() for _ in range(g.n)]
+    for a, b, c_ in tv:
+        for p, q in ((a, b), (b, c_), (c_, a)):
+            nbrs[p].add(q)
+            nbrs[q].add(p)
+    for i in range(g.n):
+        nb = sorted(nbrs[i])
+        F[i, 8:11] = v[nb].mean(0) * dt / L
+    F[:, 11] = radii / (relaxation * rq)
+    return F
+
+
+@pytest.mark.gpu
+def test_device_features_match_numpy(gpu):
+    from clothcraft_amd import VbdSolver
+    sc = scenes.small_sheet_on_sphere(16, 0.2535)
+    st = sc.state.copy()
+    rng = np.random.default_rng(0)
+    st.v[:] = rng.normal(scale=0.05, size=st.v.shape)
+    st.x[:] += rng.normal(scale=2e-4, size=st.x.shape)
+    sol = VbdSolver(sc.garment, sc.avatar, st)
+    s = sc.settings
+    f = gpu.warm.features(sol, st, s)
+    dt, c = s["dt"], s["damping"]
+    motion = np.linalg.norm((st.v * dt + np.array(s["gravity"]) * dt * dt) / (1 + c * dt), axis=1).max()
+    rq = min(max(s["radius"] + motion, 2 * s["radius"]), 4 * s["radius"])
+    det = gpu.detect(sol, st.x, rq, candidates=False)
+    ref = _numpy_features(sc, st, s, det["radii"], s["relaxation"], rq)
+    err = np.abs(f - ref).max(0)
+    print("feature max abs err per column", np.array2string(err, precision=2))
+    assert np.all(err <= 1e-5 * (1 + np.abs(ref).max(0)))
+
+
+@pytest.mark.gpu
+def test_warm_started_step_matches_oracle_from_the_same_guess(gpu):
+    """With CV_FLAG_WARM_START the device step starts from x + L * network(x);
+    the oracle step started from that same guess lands at the same place."""
+    from clothcraft_amd import VbdSolver
+    sc = scenes.small_sheet_on_sphere(16, 0.2535)
+    gpu.warm.set_params(M.init_params(13, out_scale=0.3))
+    st = sc.state.copy()
+    st.v[:] = np.array([0.0, -0.1, 0.0])
+    s = dict(sc.settings, flags=8)
+    sol = VbdSolver(sc.garment, sc.avatar, st)
+    feats = gpu.warm.features(sol, st, s)
+    pred = gpu.warm.infer(feats).astype(np.float64)
+    dt, c = s["dt"], s["damping"]
+    L = np.linalg.norm(s["gravity"]) * dt * dt / (1 + c * dt)
+    guess = st.x + L * pred
+    a, b = st.copy(), st.copy()
+    ref = O.Solver(sc.garment, sc.avatar, st)
+    ref.set_next_guess(guess)
+    ref.step(a, dict(s, flags=0))
+    gpu.step(VbdSolver(sc.garment, sc.avatar, st), b, s)
+    err = np.abs(a.x - b.x).max() / np.linalg.norm(a.x.max(0) - a.x.min(0))
+    plain = st.copy()
+    gpu.step(VbdSolver(sc.garment, sc.avatar, st), plain, dict(s, flags=0))
+    print(f"warm step vs oracle rel err {err:.3e}; differs from the plain step by "
+          f"{np.abs(plain.x - b.x).max():.3e} m")
+    assert err <= 1e-9
+    assert np.abs(plain.x - b.x).max() > 1e-7  # the guess did change the step
