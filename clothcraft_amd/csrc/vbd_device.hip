@@ -348,12 +348,15 @@ __global__ void k_warm_features(const double* x, const double* v, const uint32_t
   double d = __longlong_as_double((long long)dist_bits[i]);
   f[11] = (float)(fmin(d, rq) / rq);
 }
-// Learned guess x + L * prediction for moving vertices.
-__global__ void k_warm_guess(const float* pred, const double* x, const uint8_t* pinned, uint32_t n, double L,
-                             double* guess) {
+// Learned guess for moving vertices: the inertial prediction
+// x + (v dt + g dt^2) / (1 + c dt) plus L times the network's correction.
+__global__ void k_warm_guess(const float* pred, const double* x, const double* v, const uint8_t* pinned, uint32_t n,
+                             double dt, double damp, double gx, double gy, double gz, double L, double* guess) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || pinned[i]) return;
-  for (int k = 0; k < 3; ++k) guess[3 * i + k] = x[3 * i + k] + L * (double)pred[3 * i + k];
+  double g[3] = {gx, gy, gz};
+  for (int k = 0; k < 3; ++k)
+    guess[3 * i + k] = x[3 * i + k] + (v[3 * i + k] * dt + g[k] * dt * dt) / damp + L * (double)pred[3 * i + k];
 }
 // Step prologue reductions (single block, fixed order => deterministic):
 // origin = centroid (src/gpu_vbd.rs:156), predicted motion max|target - x|
@@ -752,7 +755,8 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
     warm_features(g, s, set, rq);
     g.wpred.reserve(3 * (size_t)n);
     mlp_infer_device(g, g.feat.p, n, g.wpred.p);
-    k_warm_guess<<<blocks(n), TPB, 0, st>>>(g.wpred.p, b.x64.p, d.pinned.p, n, warm_length(set), b.guess64.p);
+    k_warm_guess<<<blocks(n), TPB, 0, st>>>(g.wpred.p, b.x64.p, b.v64.p, d.pinned.p, n, dt, damp, set->gravity[0],
+                                            set->gravity[1], set->gravity[2], warm_length(set), b.guess64.p);
     CV_HIP(hipGetLastError());
   }
   k_truncate_guess<R><<<blocks(n), TPB, 0, st>>>(b.guess64.p, b.centers64.p, b.guess64.p + 3 * (size_t)n,

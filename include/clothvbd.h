@@ -247,8 +247,9 @@ cv_status cv_gpu_step_batch(cv_gpu* gpu, cv_solver* const* solvers, cv_state* co
 /* ------------------------------------------------- learned warm start */
 /* No reference counterpart.  A per-vertex network (12 -> 128 -> 128 -> 128
  * -> 3, GELU, BF16 weights and activations, FP32 accumulation on the matrix
- * cores) predicts each step's displacement; with CV_FLAG_WARM_START in the
- * settings the step starts from x + L * prediction instead of the adaptive
+ * cores) predicts each step's displacement as a correction to the inertial
+ * prediction; with CV_FLAG_WARM_START in the settings the step starts from
+ * x + (v dt + g dt^2) / (1 + c dt) + L * prediction instead of the adaptive
  * guess of Prediction::guess (src/vbd/prediction.rs:15-41) and clips it to
  * the OGC balls as it clips that guess (src/vbd.rs:316-324); iterations,
  * residuals and energies are unchanged, so the solver itself judges the

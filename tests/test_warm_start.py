@@ -150,8 +150,9 @@ def test_device_features_match_numpy(gpu):
 
 @pytest.mark.gpu
 def test_warm_started_step_matches_oracle_from_the_same_guess(gpu):
-    """With CV_FLAG_WARM_START the device step starts from x + L * network(x);
-    the oracle step started from that same guess lands at the same place."""
+    """With CV_FLAG_WARM_START the device step starts from the inertial
+    prediction plus L * network(features); the oracle step started from that
+    same guess lands at the same place."""
     from clothcraft_amd import VbdSolver
     sc = scenes.small_sheet_on_sphere(16, 0.2535)
     gpu.warm.set_params(M.init_params(13, out_scale=0.3))
@@ -163,7 +164,7 @@ def test_warm_started_step_matches_oracle_from_the_same_guess(gpu):
     pred = gpu.warm.infer(feats).astype(np.float64)
     dt, c = s["dt"], s["damping"]
     L = np.linalg.norm(s["gravity"]) * dt * dt / (1 + c * dt)
-    guess = st.x + L * pred
+    guess = st.x + (st.v * dt + np.array(s["gravity"]) * dt * dt) / (1 + c * dt) + L * pred
     a, b = st.copy(), st.copy()
     ref = O.Solver(sc.garment, sc.avatar, st)
     ref.set_next_guess(guess)
@@ -176,3 +177,58 @@ def test_warm_started_step_matches_oracle_from_the_same_guess(gpu):
           f"{np.abs(plain.x - b.x).max():.3e} m")
     assert err <= 1e-9
     assert np.abs(plain.x - b.x).max() > 1e-7  # the guess did change the step
+
+
+def _objective(sc, st0, x, settings):
+    """The step objective (Newton form, equal to the VBD objective up to a
+    constant) with the step's OGC contacts frozen, from the oracle."""
+    ref = O.Solver(sc.garment, sc.avatar, st0.copy())
+    ref.step(st0.copy(), dict(settings, iterations=1))
+    fz = O.frozen_c(ref.last_contacts(), settings["radius"], settings["stiffness"], sc.avatar.vertices)
+    ns = O.newton_settings(dt=settings["dt"], gravity=settings["gravity"], damping=settings["damping"])
+    return lambda xx: O.newton_objective(sc.garment, st0, xx, ns, None, fz)
+
+
+@pytest.mark.gpu
+def test_trained_warm_start_reaches_the_plain_energy_in_fewer_sweeps(gpu):
+    """Trained on converged steps of two sewn garments, evaluated on a third
+    (held-out seed): after K sweeps the learned start's objective gap to a
+    200-sweep solve is smaller than the plain start's, and the learned start
+    reaches the plain start's K-sweep objective in fewer sweeps."""
+    from clothcraft_amd import VbdSolver, garments, warmstart
+    train = [garments.sewn_garment_on_body(spacing=0.012, seed=s) for s in (2, 5)]
+    Xs, Ys = [], []
+    for i, sc in enumerate(train):
+        for pert in (0.0, 0.05):
+            X, Y, _ = warmstart.generate(gpu, sc, 30, seed=10 + i, perturb=pert, start=5)
+            Xs.append(X)
+            Ys.append(Y)
+    gpu.warm.set_params(M.init_params(0))
+    losses = gpu.warm.train(np.vstack(Xs), np.vstack(Ys), epochs=40, batch=4096, lr=1e-3, seed=1)
+    assert losses[-1] < 0.5 * losses[0]
+    sc = garments.sewn_garment_on_body(spacing=0.012, seed=9)
+    s = sc.settings
+    _, _, states = warmstart.generate(gpu, sc, 3, converge_iterations=1, seed=3, start=20)
+    K = 20
+    gains, gaps = [], []
+    for st0 in states:
+        E = _objective(sc, st0, st0.x, s)
+        ref = st0.copy()
+        gpu.step(VbdSolver(sc.garment, sc.avatar, ref), ref, dict(s, iterations=200))
+        e_ref = E(ref.x)
+
+        def gap(k, flags):
+            a = st0.copy()
+            gpu.step(VbdSolver(sc.garment, sc.avatar, a), a, dict(s, iterations=k, flags=flags))
+            return E(a.x) - e_ref
+
+        plain, warm = gap(K, 0), gap(K, 8)
+        k_eq = next((k for k in (1, 2, 3, 4, 5, 6, 8, 10, 12, 14, 16, 18, 20, 25, 30, 40) if gap(k, 8) <= plain), 60)
+        gains.append(K / k_eq)
+        gaps.append((plain, warm))
+        print(f"K={K}: objective gap plain {plain:.4e} learned {warm:.4e}; learned start reaches the plain "
+              f"{K}-sweep objective in {k_eq} sweeps")
+    print(f"epoch losses {np.round(losses[::10], 5)}; sweeps-to-equal-objective gain per state "
+          f"{np.round(gains, 2)} (mean {np.mean(gains):.2f})")
+    assert np.mean([w for _, w in gaps]) < np.mean([p for p, _ in gaps])
+    assert np.mean(gains) > 1.0
