@@ -177,6 +177,11 @@ struct cv_gpu {
   bool f64 = true;  // sweep precision (f64 reproduces the f64 reference step)
   Mlp mlp;
   DBuf<float> feat, wpred;  // warm-start features [12n] and predictions [3n]
+  // Batched steps: one context per solver of the batch, sharing this
+  // context's device and stream; launch tables of the shared sweeps.
+  std::vector<cv_gpu*> subs;
+  bool owns_stream = true;
+  DBuf<unsigned char> batch_args, batch_slots, batch_ends;
 };
 
 // Runs f, mapping exceptions to a status and message (no exception crosses

@@ -16,6 +16,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 
 #include <hip/hip_ext.h>
 
@@ -161,16 +162,10 @@ __device__ __forceinline__ cvm::Deriv<R> local_deriv(const SweepArgs<R>& a, cons
   return acc;
 }
 
-// One colour of one iteration (src/vbd.rs:337-395).
+// The local solve of vertex v by its group of L lanes (src/vbd.rs:337-395).
 template <int L, class R>
-__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uint32_t count) {
+__device__ __forceinline__ void solve_vertex(const SweepArgs<R>& a, uint32_t v, int lane) {
   using T4 = typename P4<R>::T;
-  const int lane = threadIdx.x & (L - 1);
-  // Vertices of a colour are in index order, so neighbouring groups share
-  // neighbours: keep consecutive groups on one XCD's L2.
-  const uint32_t grp = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / L;
-  if (grp >= count || *a.stop) return;
-  const uint32_t v = a.order[off + grp];
   T4 p4 = a.pos[v];
   if (p4.w > R(0.5)) {  // pinned: no update, zero residual contribution
     if (lane == 0) a.gsq[v] = 0.0f;
@@ -212,6 +207,42 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uin
   }
 }
 
+// One colour of one iteration.
+template <int L, class R>
+__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uint32_t count) {
+  const int lane = threadIdx.x & (L - 1);
+  // Vertices of a colour are in index order, so neighbouring groups share
+  // neighbours: keep consecutive groups on one XCD's L2.
+  const uint32_t grp = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / L;
+  if (grp >= count || *a.stop) return;
+  solve_vertex<L, R>(a, a.order[off + grp], lane);
+}
+
+// One colour index of one iteration of many solvers in a single launch: slot
+// k covers groups [start_k, start_k + count) of solver inst_k's colour
+// starting at `off` in its order.  Each solver's vertices see exactly the
+// work of its own k_sweep launch.
+struct BatchSlot {
+  uint32_t start, off, count, inst;
+};
+template <int L, class R>
+__global__ __launch_bounds__(TPB) void k_sweep_batch(const SweepArgs<R>* args, const BatchSlot* slots,
+                                                     uint32_t nslots, uint32_t total) {
+  const int lane = threadIdx.x & (L - 1);
+  const uint32_t grp = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / L;
+  if (grp >= total) return;
+  uint32_t lo = 0, hi = nslots;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (slots[mid].start <= grp) lo = mid;
+    else hi = mid;
+  }
+  const BatchSlot sl = slots[lo];
+  const SweepArgs<R>& a = args[sl.inst];
+  if (*a.stop) return;
+  solve_vertex<L, R>(a, a.order[sl.off + grp - sl.start], lane);
+}
+
 // End of iteration `it` (it = -1: after the truncated guess), on the device
 // so the host need not read the clipped count back every iteration: counts
 // the iteration, accumulates truncations and, when the clipped count reaches
@@ -234,6 +265,32 @@ __global__ void k_iter_end(Scalars* sc, int it, double threshold, double tol, do
   if (c > 0 && (double)c >= threshold) {
     sc->stop = 1;
     sc->stop_it = (unsigned int)(it + 1);  // the next iteration to run after the refresh
+  }
+}
+
+struct IterEnd {
+  Scalars* sc;
+  int it;
+  double threshold, tol, nfree3;
+};
+__global__ void k_iter_end_batch(const IterEnd* e, uint32_t count) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const IterEnd q = e[i];
+  Scalars* sc = q.sc;
+  if (sc->stop) return;
+  unsigned long long c = sc->clipped;
+  sc->iters_run += 1;
+  sc->truncated_total += c;
+  sc->clipped = 0;
+  if (q.tol > 0.0 && sqrt(sc->true_residual_sum / q.nfree3) <= q.tol) {
+    sc->stop = 2;
+    sc->stop_it = (unsigned int)(q.it + 1);
+    return;
+  }
+  if (c > 0 && (double)c >= q.threshold) {
+    sc->stop = 1;
+    sc->stop_it = (unsigned int)(q.it + 1);
   }
 }
 
@@ -724,72 +781,106 @@ void warm_features(cv_gpu& g, const cv_solver& s, const cv_settings* set, double
   CV_HIP(hipGetLastError());
 }
 
+// One VBD step of one solver as a sequence of phases, so the single-solver
+// step and the batched step (many solvers sharing sweep launches) run the
+// same device work in the same order per solver:
+//   begin()      prologue, detection, colouring, truncated guess;
+//   enqueue()    the sweeps of a range of iterations (or the batch driver's
+//                shared launches) and their end-of-iteration kernels;
+//   after_sync() once the stream has drained: accounting, and the refresh
+//                (re-detection and recolouring) a raised `stop` asks for;
+//   finish()     residuals, trust-region check, commit of the positions.
+// Iterations are queued in batches with no read-back inside a batch: the
+// first batch is iteration 0 alone (a refresh after it is common), the next
+// runs to the end.  A pending refresh turns the rest of a batch into no-ops;
+// the host then re-detects and queues again from the next iteration.
 template <class R>
-void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out) {
-  DevSolver& d = g.dev;
-  StepBufs& b = g.b;
-  hipStream_t st = g.stream;
-  uint32_t n = s.n;
-  double dt = set->dt, damp = 1.0 + set->damping * dt;
-  k_prologue_reduce<<<1, 1024, 0, st>>>(b.x64.p, b.v64.p, n, dt, damp, set->gravity[0], set->gravity[1],
-                                       set->gravity[2], g.d_scal);
-  CV_HIP(hipGetLastError());
-  read_scalars(g);
-  double r = set->contact.radius;
-  double rq = std::min(std::max(r + g.h_scal->motion, 2.0 * r), 4.0 * r);
-  g.h_scal->clipped = 0;
-  g.h_scal->truncated_total = 0;
-  g.h_scal->stop = g.h_scal->stop_it = g.h_scal->iters_run = 0;
-  g.h_scal->err_nonfinite = g.h_scal->err_trust = 0;
-  write_scalars(g);
-  k_prologue<R><<<blocks(n + d.n_fixed), TPB, 0, st>>>(
-      b.x64.p, b.v64.p, d.acc.p, d.masses.p, d.pinned.p, n, d.fixed_pos.p, d.n_fixed, dt, set->damping,
-      set->gravity[0], set->gravity[1], set->gravity[2], g.pred_valid ? 1 : 0, g.d_scal, PBufs<R>::pos(g),
-      PBufs<R>::tgt(g), b.guess64.p, b.x_old64.p, b.v_old64.p);
-  CV_HIP(hipGetLastError());
-  detect_and_color<R>(g, s, set, grab, rq, out, true);
-  if (set->flags & CV_FLAG_WARM_START) {
-    // Learned guess in place of the adaptive one; clipped to the balls below
-    // exactly as that guess is.
-    if (!g.mlp.ready) fail(CV_ERR_NOT_BUILT, "warm-start network not set up");
-    warm_features(g, s, set, rq);
-    g.wpred.reserve(3 * (size_t)n);
-    mlp_infer_device(g, g.feat.p, n, g.wpred.p);
-    k_warm_guess<<<blocks(n), TPB, 0, st>>>(g.wpred.p, b.x64.p, b.v64.p, d.pinned.p, n, dt, damp, set->gravity[0],
-                                            set->gravity[1], set->gravity[2], warm_length(set), b.guess64.p);
-    CV_HIP(hipGetLastError());
-  }
-  k_truncate_guess<R><<<blocks(n), TPB, 0, st>>>(b.guess64.p, b.centers64.p, b.guess64.p + 3 * (size_t)n,
-                                                 d.pinned.p, n, g.d_scal, PBufs<R>::pos(g), &g.d_scal->clipped);
-  CV_HIP(hipGetLastError());
-  const double threshold = set->contact.refresh_fraction * n;
-  const double tol = set->residual_tolerance;
-  const double nfree3 = 3.0 * std::max<uint32_t>(s.n_free, 1);
-  const bool want_true = tol > 0.0 || (set->flags & CV_FLAG_TRUE_RESIDUAL);
-  if (want_true) b.gtrue.reserve(n);
-  k_iter_end<<<1, 1, 0, st>>>(g.d_scal, -1, threshold, tol, nfree3);
-  CV_HIP(hipGetLastError());
-  SweepArgs<R> a = sweep_args<R>(g, set, grab);
-  const uint32_t iters = set->iterations;
-  // Iterations are queued in batches with no read-back inside a batch: the
-  // first batch is iteration 0 alone (a refresh after it is common), the next
-  // runs to the end.  A pending refresh turns the rest of a batch into no-ops;
-  // the host then re-detects and queues again from the next iteration.
-  uint32_t next = 0, done = 0;
+struct StepCtx {
+  cv_gpu& g;
+  cv_solver& s;
+  const cv_settings* set;
+  const cv_grab* grab;
+  StepOut& out;
+  SweepArgs<R> a{};
+  double rq = 0, threshold = 0, tol = 0, nfree3 = 1;
+  bool want_true = false;
+  uint32_t iters = 0, next = 0, done = 0;
+  bool finished = false;
   struct Launch {
     size_t ev;
     uint32_t it;
   };
   std::vector<Launch> launches;
   size_t nev = 0;
-  while (true) {
-    read_scalars(g);
+
+  StepCtx(cv_gpu& g_, cv_solver& s_, const cv_settings* set_, const cv_grab* grab_, StepOut& out_)
+      : g(g_), s(s_), set(set_), grab(grab_), out(out_) {}
+
+  void prologue_reduce() {
+    double dt = set->dt, damp = 1.0 + set->damping * dt;
+    k_prologue_reduce<<<1, 1024, 0, g.stream>>>(g.b.x64.p, g.b.v64.p, s.n, dt, damp, set->gravity[0],
+                                                set->gravity[1], set->gravity[2], g.d_scal);
+    CV_HIP(hipGetLastError());
+    CV_HIP(hipMemcpyAsync(g.h_scal, g.d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, g.stream));
+  }
+  // After prologue_reduce() and a drain of the stream.
+  void begin() {
+    DevSolver& d = g.dev;
+    StepBufs& b = g.b;
+    hipStream_t st = g.stream;
+    uint32_t n = s.n;
+    double dt = set->dt, damp = 1.0 + set->damping * dt;
+    double r = set->contact.radius;
+    rq = std::min(std::max(r + g.h_scal->motion, 2.0 * r), 4.0 * r);
+    g.h_scal->clipped = 0;
+    g.h_scal->truncated_total = 0;
+    g.h_scal->stop = g.h_scal->stop_it = g.h_scal->iters_run = 0;
+    g.h_scal->err_nonfinite = g.h_scal->err_trust = 0;
+    write_scalars(g);
+    k_prologue<R><<<blocks(n + d.n_fixed), TPB, 0, st>>>(
+        b.x64.p, b.v64.p, d.acc.p, d.masses.p, d.pinned.p, n, d.fixed_pos.p, d.n_fixed, dt, set->damping,
+        set->gravity[0], set->gravity[1], set->gravity[2], g.pred_valid ? 1 : 0, g.d_scal, PBufs<R>::pos(g),
+        PBufs<R>::tgt(g), b.guess64.p, b.x_old64.p, b.v_old64.p);
+    CV_HIP(hipGetLastError());
+    detect_and_color<R>(g, s, set, grab, rq, out, true);
+    if (set->flags & CV_FLAG_WARM_START) {
+      // Learned guess in place of the adaptive one; clipped to the balls below
+      // exactly as that guess is.
+      if (!g.mlp.ready) fail(CV_ERR_NOT_BUILT, "warm-start network not set up");
+      warm_features(g, s, set, rq);
+      g.wpred.reserve(3 * (size_t)n);
+      mlp_infer_device(g, g.feat.p, n, g.wpred.p);
+      k_warm_guess<<<blocks(n), TPB, 0, st>>>(g.wpred.p, b.x64.p, b.v64.p, d.pinned.p, n, dt, damp,
+                                              set->gravity[0], set->gravity[1], set->gravity[2], warm_length(set),
+                                              b.guess64.p);
+      CV_HIP(hipGetLastError());
+    }
+    k_truncate_guess<R><<<blocks(n), TPB, 0, st>>>(b.guess64.p, b.centers64.p, b.guess64.p + 3 * (size_t)n,
+                                                   d.pinned.p, n, g.d_scal, PBufs<R>::pos(g), &g.d_scal->clipped);
+    CV_HIP(hipGetLastError());
+    threshold = set->contact.refresh_fraction * n;
+    tol = set->residual_tolerance;
+    nfree3 = 3.0 * std::max<uint32_t>(s.n_free, 1);
+    want_true = tol > 0.0 || (set->flags & CV_FLAG_TRUE_RESIDUAL);
+    if (want_true) b.gtrue.reserve(n);
+    k_iter_end<<<1, 1, 0, st>>>(g.d_scal, -1, threshold, tol, nfree3);
+    CV_HIP(hipGetLastError());
+    a = sweep_args<R>(g, set, grab);
+    iters = set->iterations;
+  }
+  // Queues the scalar read-back that after_sync() consumes.
+  void read_back() {
+    CV_HIP(hipMemcpyAsync(g.h_scal, g.d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, g.stream));
+  }
+  // After read_back() and a drain: accounting; refresh if asked.  Returns the
+  // iteration range [first, end) to queue next, or false when the step's
+  // iterations are over.
+  bool after_sync(uint32_t& first, uint32_t& end) {
     uint32_t ran = g.h_scal->iters_run;
-    // Account the iterations that ran since the last read-back.
     for (const Launch& l : launches) {
       if (l.it >= ran) continue;
       out.sweep_launches++;
-      if (g.profile) {
+      if (g.profile && l.ev != (size_t)-1) {
         float ms = 0;
         CV_HIP(hipEventElapsedTime(&ms, g.sweep_events[2 * l.ev], g.sweep_events[2 * l.ev + 1]));
         out.sweep_ms += ms;
@@ -802,22 +893,30 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
                                                        : g.static_bytes64 + 100.0 * inc);
     out.sweep_bytes_min += (ran - done) * (g.static_bytes32 + 56.0 * inc);
     done = ran;
-    if (g.h_scal->stop == 2) break;  // residual tolerance reached
+    if (g.h_scal->stop == 2) return finished = true, false;  // residual tolerance reached
     if (g.h_scal->stop) {
       next = g.h_scal->stop_it;
-      if (next >= iters) break;
+      if (next >= iters) return finished = true, false;
       // Refresh before iteration `next` at the current positions.
       g.h_scal->stop = 0;
-      CV_HIP(hipMemcpyAsync(&g.d_scal->stop, &g.h_scal->stop, sizeof(unsigned int), hipMemcpyHostToDevice, st));
-      k_current<R><<<blocks(n), TPB, 0, st>>>(PBufs<R>::pos(g), b.x_old64.p, d.pinned.p, n, g.d_scal, b.x64.p);
+      CV_HIP(hipMemcpyAsync(&g.d_scal->stop, &g.h_scal->stop, sizeof(unsigned int), hipMemcpyHostToDevice,
+                            g.stream));
+      k_current<R><<<blocks(s.n), TPB, 0, g.stream>>>(PBufs<R>::pos(g), g.b.x_old64.p, g.dev.pinned.p, s.n,
+                                                      g.d_scal, g.b.x64.p);
       CV_HIP(hipGetLastError());
       detect_and_color<R>(g, s, set, grab, rq, out, false);
       a = sweep_args<R>(g, set, grab);
     } else if (next >= iters) {
-      break;
+      return finished = true, false;
     }
-    uint32_t end = next == 0 ? 1 : iters;
-    for (uint32_t it = next; it < end; ++it) {
+    first = next;
+    end = next == 0 ? 1 : iters;
+    next = end;
+    return true;
+  }
+  // Sweeps of iterations [first, end) on this solver's own launches.
+  void enqueue(uint32_t first, uint32_t end) {
+    for (uint32_t it = first; it < end; ++it) {
       for (size_t c = 0; c + 1 < g.color_off.size(); ++c) {
         uint32_t off = g.color_off[c], cnt = g.color_off[c + 1] - off;
         if (!cnt) continue;
@@ -830,35 +929,61 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
             g.sweep_events.push_back(e1);
           }
           sweep_color<R>(g, a, off, cnt, g.sweep_events[2 * nev], g.sweep_events[2 * nev + 1]);
+          launches.push_back({nev++, it});
         } else {
           sweep_color<R>(g, a, off, cnt);
+          launches.push_back({(size_t)-1, it});
         }
-        launches.push_back({nev, it});
-        nev += g.profile ? 1 : 0;
       }
-      if (tol > 0.0) true_residual<R>(g, a, n);
-      k_iter_end<<<1, 1, 0, st>>>(g.d_scal, (int)it, threshold, tol, nfree3);
-      CV_HIP(hipGetLastError());
+      end_iteration(it);
     }
-    next = end;
   }
-  out.truncated += g.h_scal->truncated_total;
-  out.iterations_run += g.h_scal->iters_run;
-  if (want_true) {
-    // Post-sweep residual of the final positions (recomputed when the last
-    // batch ended on a refresh or without the tolerance mode).
-    true_residual<R>(g, a, n);
-    out.true_residual = true;
+  void end_iteration(uint32_t it) {
+    if (tol > 0.0) true_residual<R>(g, a, s.n);
+    k_iter_end<<<1, 1, 0, g.stream>>>(g.d_scal, (int)it, threshold, tol, nfree3);
+    CV_HIP(hipGetLastError());
   }
-  k_sum_f32<<<1, 1024, 0, st>>>(b.gsq.p, n, &g.d_scal->residual_sum);
-  // The trust-region check tolerance of the device path (src/gpu_vbd.rs:436).
-  k_finalize<R><<<blocks(n), TPB, 0, st>>>(PBufs<R>::pos(g), b.x_old64.p, b.v_old64.p, b.centers64.p,
-                                           b.radii64.p, d.pinned.p, n, dt, set->damping, 1e-8, b.xcur64.p,
-                                           b.v64.p, d.acc.p, g.d_scal);
-  CV_HIP(hipGetLastError());
+  // Residuals and commit kernels; then read_back() and a drain, then check().
+  void finish() {
+    DevSolver& d = g.dev;
+    StepBufs& b = g.b;
+    uint32_t n = s.n;
+    out.truncated += g.h_scal->truncated_total;
+    out.iterations_run += g.h_scal->iters_run;
+    if (want_true) {
+      // Post-sweep residual of the final positions.
+      true_residual<R>(g, a, n);
+      out.true_residual = true;
+    }
+    k_sum_f32<<<1, 1024, 0, g.stream>>>(b.gsq.p, n, &g.d_scal->residual_sum);
+    // The trust-region check tolerance of the device path (src/gpu_vbd.rs:436).
+    k_finalize<R><<<blocks(n), TPB, 0, g.stream>>>(PBufs<R>::pos(g), b.x_old64.p, b.v_old64.p, b.centers64.p,
+                                                   b.radii64.p, d.pinned.p, n, set->dt, set->damping, 1e-8,
+                                                   b.xcur64.p, b.v64.p, d.acc.p, g.d_scal);
+    CV_HIP(hipGetLastError());
+  }
+  void check() const {
+    if (g.h_scal->err_nonfinite) fail(CV_ERR_NONFINITE_RESULT, "Non-finite VBD result; state unchanged");
+    if (g.h_scal->err_trust) fail(CV_ERR_TRUST_REGION, "GPU OGC trust-region check failed; state unchanged");
+  }
+};
+
+template <class R>
+void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out) {
+  StepCtx<R> c(g, s, set, grab, out);
+  c.prologue_reduce();
+  CV_HIP(hipStreamSynchronize(g.stream));
+  c.begin();
+  uint32_t first, end;
+  while (true) {
+    c.read_back();
+    CV_HIP(hipStreamSynchronize(g.stream));
+    if (!c.after_sync(first, end)) break;
+    c.enqueue(first, end);
+  }
+  c.finish();
   read_scalars(g);
-  if (g.h_scal->err_nonfinite) fail(CV_ERR_NONFINITE_RESULT, "Non-finite VBD result; state unchanged");
-  if (g.h_scal->err_trust) fail(CV_ERR_TRUST_REGION, "GPU OGC trust-region check failed; state unchanged");
+  c.check();
 }
 
 void run_step(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out) {
@@ -866,6 +991,130 @@ void run_step(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* gr
     run_step_t<double>(g, s, set, grab, out);
   else
     run_step_t<float>(g, s, set, grab, out);
+}
+
+// Batched step of many solvers (each with its own context g_i): per solver the
+// same device work as run_step_t, with the sweeps of one colour index of one
+// iteration of every solver in a single launch (k_sweep_batch) and the
+// end-of-iteration kernels in another.  Solvers progress through their own
+// iteration indices; a solver waiting for a re-detection sits out.  A solver
+// that fails is dropped with its status; the others continue.
+template <class R>
+void run_batch_t(cv_gpu& P, const std::vector<cv_gpu*>& G, const std::vector<cv_solver*>& S, const cv_settings* set,
+                 std::vector<StepOut>& out, std::vector<cv_status>& status, std::vector<std::string>& err) {
+  size_t N = G.size();
+  hipStream_t st = P.stream;
+  std::vector<std::unique_ptr<StepCtx<R>>> C(N);
+  std::vector<char> live(N, 1);
+  auto drop = [&](size_t i, const CvError& e) {
+    live[i] = 0;
+    status[i] = e.status;
+    err[i] = e.message;
+  };
+  for (size_t i = 0; i < N; ++i) {
+    C[i] = std::make_unique<StepCtx<R>>(*G[i], *S[i], set, nullptr, out[i]);
+    C[i]->prologue_reduce();
+  }
+  CV_HIP(hipStreamSynchronize(st));
+  for (size_t i = 0; i < N; ++i) try {
+      C[i]->begin();
+    } catch (const CvError& e) {
+      drop(i, e);
+    }
+  struct Range {
+    size_t i;
+    uint32_t first, end;
+  };
+  std::vector<SweepArgs<R>> args(N);
+  std::vector<BatchSlot> slots;
+  std::vector<IterEnd> ends;
+  struct Launch {
+    size_t slot0, nslots;
+    uint32_t total;
+  };
+  while (true) {
+    for (size_t i = 0; i < N; ++i)
+      if (live[i] && !C[i]->finished) C[i]->read_back();
+    CV_HIP(hipStreamSynchronize(st));
+    std::vector<Range> rr;
+    for (size_t i = 0; i < N; ++i) {
+      if (!live[i] || C[i]->finished) continue;
+      try {
+        uint32_t f = 0, e = 0;
+        if (C[i]->after_sync(f, e)) rr.push_back({i, f, e});
+      } catch (const CvError& e) {
+        drop(i, e);
+      }
+    }
+    if (rr.empty()) break;
+    for (auto& r : rr) args[r.i] = C[r.i]->a;
+    uint32_t T = 0;
+    for (auto& r : rr) T = std::max(T, r.end - r.first);
+    slots.clear();
+    ends.clear();
+    std::vector<std::vector<Launch>> per_t(T);
+    std::vector<std::pair<size_t, size_t>> end_range(T);
+    for (uint32_t t = 0; t < T; ++t) {
+      size_t maxc = 0;
+      for (auto& r : rr)
+        if (r.first + t < r.end) maxc = std::max(maxc, G[r.i]->color_off.size() - 1);
+      for (size_t c = 0; c < maxc; ++c) {
+        size_t s0 = slots.size();
+        uint32_t total = 0;
+        for (auto& r : rr) {
+          const cv_gpu& gi = *G[r.i];
+          if (r.first + t >= r.end || c + 1 >= gi.color_off.size()) continue;
+          uint32_t off = gi.color_off[c], cnt = gi.color_off[c + 1] - off;
+          if (!cnt) continue;
+          slots.push_back({total, off, cnt, (uint32_t)r.i});
+          total += cnt;
+          C[r.i]->launches.push_back({(size_t)-1, r.first + t});
+        }
+        if (total) per_t[t].push_back({s0, slots.size() - s0, total});
+      }
+      size_t e0 = ends.size();
+      for (auto& r : rr)
+        if (r.first + t < r.end)
+          ends.push_back({G[r.i]->d_scal, (int)(r.first + t), C[r.i]->threshold, C[r.i]->tol, C[r.i]->nfree3});
+      end_range[t] = {e0, ends.size() - e0};
+    }
+    P.batch_args.upload((const unsigned char*)args.data(), N * sizeof(SweepArgs<R>), st);
+    P.batch_slots.upload((const unsigned char*)slots.data(), std::max<size_t>(slots.size(), 1) * sizeof(BatchSlot),
+                         st);
+    P.batch_ends.upload((const unsigned char*)ends.data(), std::max<size_t>(ends.size(), 1) * sizeof(IterEnd), st);
+    const SweepArgs<R>* dargs = (const SweepArgs<R>*)P.batch_args.p;
+    const BatchSlot* dslots = (const BatchSlot*)P.batch_slots.p;
+    const IterEnd* dends = (const IterEnd*)P.batch_ends.p;
+    for (uint32_t t = 0; t < T; ++t) {
+      for (const Launch& l : per_t[t]) {
+        const BatchSlot* sl = dslots + l.slot0;
+        uint32_t ns = (uint32_t)l.nslots;
+        switch (P.lanes) {
+          case 8: k_sweep_batch<8, R><<<blocks((uint64_t)l.total * 8), TPB, 0, st>>>(dargs, sl, ns, l.total); break;
+          case 16: k_sweep_batch<16, R><<<blocks((uint64_t)l.total * 16), TPB, 0, st>>>(dargs, sl, ns, l.total); break;
+          case 64: k_sweep_batch<64, R><<<blocks((uint64_t)l.total * 64), TPB, 0, st>>>(dargs, sl, ns, l.total); break;
+          default: k_sweep_batch<32, R><<<blocks((uint64_t)l.total * 32), TPB, 0, st>>>(dargs, sl, ns, l.total); break;
+        }
+      }
+      for (auto& r : rr)
+        if (r.first + t < r.end && C[r.i]->tol > 0.0) true_residual<R>(*G[r.i], C[r.i]->a, S[r.i]->n);
+      auto [e0, ne] = end_range[t];
+      if (ne) k_iter_end_batch<<<blocks(ne), TPB, 0, st>>>(dends + e0, (uint32_t)ne);
+      CV_HIP(hipGetLastError());
+    }
+  }
+  for (size_t i = 0; i < N; ++i)
+    if (live[i]) {
+      C[i]->finish();
+      C[i]->read_back();
+    }
+  CV_HIP(hipStreamSynchronize(st));
+  for (size_t i = 0; i < N; ++i)
+    if (live[i]) try {
+        C[i]->check();
+      } catch (const CvError& e) {
+        drop(i, e);
+      }
 }
 
 }  // namespace
@@ -1113,6 +1362,8 @@ cv_status cv_gpu_create(int device, cv_gpu** out) {
 void cv_gpu_destroy(cv_gpu* g) {
   if (!g) return;
   (void)hipSetDevice(g->device);
+  for (cv_gpu* c : g->subs) cv_gpu_destroy(c);
+  g->subs.clear();
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   if (g->d_scal) (void)hipFree(g->d_scal);
   if (g->h_scal) (void)hipHostFree(g->h_scal);
@@ -1120,7 +1371,7 @@ void cv_gpu_destroy(cv_gpu* g) {
   if (g->h_v) (void)hipHostFree(g->h_v);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
   if (g->ev1) (void)hipEventDestroy(g->ev1);
-  hipStream_t st = g->stream;
+  hipStream_t st = g->owns_stream ? g->stream : nullptr;
   delete g;
   if (st) (void)hipStreamDestroy(st);
 }
@@ -1190,6 +1441,10 @@ static cv_status run_steps(cv_gpu* g, cv_solver* s, cv_state* st, const cv_setti
     CV_HIP(hipEventRecord(g->ev1, g->stream));
     CV_HIP(hipMemcpyAsync(g->h_x, g->b.xcur64.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     CV_HIP(hipMemcpyAsync(g->h_v, g->b.v64.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    // The acceleration history travels with the solver (Prediction::record,
+    // src/vbd/prediction.rs:43-54), so another context can continue it.
+    s->acc.resize(3 * n);
+    CV_HIP(hipMemcpyAsync(s->acc.data(), g->dev.acc.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     CV_HIP(hipStreamSynchronize(g->stream));
     CV_HIP(hipEventElapsedTime(&gpu_ms, g->ev0, g->ev1));
     // Commit (all checks passed).
@@ -1204,7 +1459,6 @@ static cv_status run_steps(cv_gpu* g, cv_solver* s, cv_state* st, const cv_setti
     s->last_damping = set->damping;
     s->acc_version = next_acc_version();
     g->dev.acc_version = s->acc_version;
-    s->acc.clear();
     double step_ms = now_ms() - t0;
     if (steps > 1) {
       // Run totals (see cv_gpu_run).
@@ -1316,14 +1570,120 @@ cv_status cv_gpu_detect(cv_gpu* g, cv_solver* s, const double* x, double rq, con
 
 cv_status cv_gpu_step_batch(cv_gpu* g, cv_solver* const* solvers, cv_state* const* states, uint32_t count,
                             const cv_settings* settings, cv_metrics* metrics) {
-  if (!g || (!solvers && count)) return CV_ERR_INVALID_ARGUMENT;
-  for (uint32_t i = 0; i < count; ++i) {
-    cv_status r = cv_gpu_step(g, solvers[i], states[i], settings, nullptr, metrics ? &metrics[i] : nullptr);
-    if (r != CV_OK) return r;
-  }
+  if (!g || (count && (!solvers || !states))) return CV_ERR_INVALID_ARGUMENT;
+  if (count == 0) return CV_OK;
+  std::vector<cv_status> status(count, CV_OK);
+  std::vector<std::string> err(count);
+  cv_status outer = guard(&g->last_error, [&] {
+    CV_HIP(hipSetDevice(g->device));
+    validate_settings(settings);
+    if (settings->flags & CV_FLAG_WARM_START)
+      fail(CV_ERR_INVALID_SETTINGS, "the learned warm start is not available in batched steps");
+    for (uint32_t i = 0; i < count; ++i) {
+      if (!solvers[i] || !states[i]) fail(CV_ERR_INVALID_ARGUMENT, "null solver or state in batch");
+      for (uint32_t j = 0; j < i; ++j)
+        if (solvers[j] == solvers[i]) fail(CV_ERR_INVALID_ARGUMENT, "a solver appears twice in one batch");
+    }
+    // One context per batch slot, on this context's device and stream.
+    while (g->subs.size() < count) {
+      cv_gpu* c = new cv_gpu();
+      c->device = g->device;
+      c->stream = g->stream;
+      c->owns_stream = false;
+      g->subs.push_back(c);
+      CV_HIP(hipMalloc((void**)&c->d_scal, sizeof(Scalars)));
+      CV_HIP(hipHostMalloc((void**)&c->h_scal, sizeof(Scalars)));
+      std::memset(c->h_scal, 0, sizeof(Scalars));
+      CV_HIP(hipEventCreate(&c->ev0));
+      CV_HIP(hipEventCreate(&c->ev1));
+    }
+    std::vector<cv_gpu*> G;
+    std::vector<cv_solver*> S;
+    std::vector<uint32_t> idx;
+    std::vector<double> t0(count);
+    for (uint32_t i = 0; i < count; ++i) {
+      cv_gpu& c = *g->subs[i];
+      c.lanes = g->lanes;
+      c.f64 = g->f64;
+      c.profile = false;
+      cv_status r = guard(&err[i], [&] {
+        t0[i] = now_ms();
+        validate_state(*solvers[i], states[i]);
+        ensure_device_solver(c, *solvers[i]);
+        ensure_step_buffers(c, *solvers[i]);
+        stage_state(c, *solvers[i], states[i], settings);
+      });
+      status[i] = r;
+      if (r == CV_OK) {
+        G.push_back(&c);
+        S.push_back(solvers[i]);
+        idx.push_back(i);
+      }
+    }
+    std::vector<StepOut> out(G.size());
+    std::vector<cv_status> st2(G.size(), CV_OK);
+    std::vector<std::string> er2(G.size());
+    if (!G.empty()) {
+      CV_HIP(hipEventRecord(g->ev0, g->stream));
+      if (g->f64)
+        run_batch_t<double>(*g, G, S, settings, out, st2, er2);
+      else
+        run_batch_t<float>(*g, G, S, settings, out, st2, er2);
+      CV_HIP(hipEventRecord(g->ev1, g->stream));
+    }
+    // Commit the solvers whose step succeeded.
+    for (size_t k = 0; k < G.size(); ++k) {
+      uint32_t i = idx[k];
+      if (st2[k] != CV_OK) {
+        status[i] = st2[k];
+        err[i] = er2[k];
+        continue;
+      }
+      cv_gpu& c = *G[k];
+      cv_solver& sv = *S[k];
+      cv_state* stt = states[i];
+      size_t n = sv.n;
+      CV_HIP(hipMemcpyAsync(c.h_x, c.b.xcur64.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      CV_HIP(hipMemcpyAsync(c.h_v, c.b.v64.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      sv.acc.resize(3 * n);
+      CV_HIP(hipMemcpyAsync(sv.acc.data(), c.dev.acc.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    }
+    CV_HIP(hipStreamSynchronize(g->stream));
+    float gpu_ms = 0;
+    if (!G.empty()) CV_HIP(hipEventElapsedTime(&gpu_ms, g->ev0, g->ev1));
+    for (size_t k = 0; k < G.size(); ++k) {
+      uint32_t i = idx[k];
+      if (status[i] != CV_OK) continue;
+      cv_gpu& c = *G[k];
+      cv_solver& sv = *S[k];
+      cv_state* stt = states[i];
+      size_t n = sv.n;
+      std::memcpy(stt->x, c.h_x, 3 * n * sizeof(double));
+      std::memcpy(stt->v, c.h_v, 3 * n * sizeof(double));
+      stt->time += settings->dt;
+      sv.has_last = true;
+      sv.last_x.assign(stt->x, stt->x + 3 * n);
+      sv.last_v.assign(stt->v, stt->v + 3 * n);
+      sv.last_time = stt->time;
+      for (int q = 0; q < 3; ++q) sv.last_gravity[q] = settings->gravity[q];
+      sv.last_damping = settings->damping;
+      sv.acc_version = next_acc_version();
+      c.dev.acc_version = sv.acc_version;
+      if (metrics) {
+        fill_metrics(&metrics[i], c, sv, settings, out[k], now_ms() - t0[i], gpu_ms, stt->x);
+        metrics[i].time_s = stt->time;
+        metrics[i].gpu_compute_ms = gpu_ms;  // the whole batch
+      }
+    }
+  });
+  if (outer != CV_OK) return outer;
+  for (uint32_t i = 0; i < count; ++i)
+    if (status[i] != CV_OK) {
+      g->last_error = "batch slot " + std::to_string(i) + ": " + err[i];
+      return status[i];
+    }
   return CV_OK;
 }
-
 
 // ------------------------------------------------------- learned warm start
 cv_status cv_warm_set_params(cv_gpu* g, const float* params) {

@@ -264,6 +264,21 @@ class GpuVbd:
         state.time = st.time
         return m.as_dict()
 
+    def step_batch(self, solvers, states, settings) -> list:
+        """One step of many independent solvers (cv_gpu_step_batch); states
+        are updated in place; returns per-solver metrics."""
+        n = len(solvers)
+        hs = (C.c_void_p * n)(*[s.h for s in solvers])
+        cs = [st.to_c() for st in states]
+        ps = (C.POINTER(abi.cv_state) * n)(*[C.pointer(c) for c in cs])
+        ms = (abi.cv_metrics * n)()
+        rc = lib().cv_gpu_step_batch(self.h, hs, ps, n, C.byref(_settings(settings)), ms)
+        for st, c in zip(states, cs):
+            st.time = c.time
+        if rc != 0:
+            self._raise(rc)
+        return [m.as_dict() for m in ms]
+
     def probe(self, solver: VbdSolver, state, settings, grab=None) -> np.ndarray:
         out = np.zeros((solver.n, 4))
         g = _grab(grab)

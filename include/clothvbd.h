@@ -237,10 +237,15 @@ cv_status cv_gpu_detect(cv_gpu* gpu, cv_solver* solver, const double* x, double 
                         uint32_t* contact_kind_ids, uint64_t contact_capacity,
                         double* contact_eps, double* radii);
 
-/* Batched uneven scenes (no reference counterpart): steps `count` independent
- * solvers with one call, each with cv_gpu_step semantics (metrics[i] for
- * solver i).  Stops at the first failing solver and returns its status; the
- * solvers before it have stepped.  States are updated in place. */
+/* Batched uneven scenes (no reference counterpart): one step of `count`
+ * independent solvers of any sizes.  Each solver gets exactly the device work
+ * of its own cv_gpu_step (same results, bit for bit); the sweeps of one
+ * colour index of one iteration of all solvers share one launch.  Each solver
+ * commits independently: on failure the status of the first failing solver
+ * is returned (cv_last_error names its slot) and only the failing solvers'
+ * states are left untouched.  metrics[i] (optional) describes solver i, with
+ * gpu_compute_ms the whole batch.  The learned warm start is not available
+ * here. */
 cv_status cv_gpu_step_batch(cv_gpu* gpu, cv_solver* const* solvers, cv_state* const* states,
                             uint32_t count, const cv_settings* settings, cv_metrics* metrics);
 
