@@ -307,3 +307,74 @@ def sewn_garment_on_body(spacing: float = 0.0024, seed: int = 2, width: float = 
     settings = default_settings(dt=1.0 / 60.0, iterations=iterations, radius=radius, stiffness=1e5)
     return Scene(f"sewn_garment_{g.n}v", g, st, body, settings, 240,
                  {"seed": seed, "spacing": spacing, "stitches": len(g.stitch_stiffness)})
+
+
+def grid_panel(width: float, height: float, nx: int, ny: int) -> tuple:
+    """Regular grid rest coordinates and triangles with alternating diagonals
+    (the layout of Pattern::build, src/model.rs:164-191, without its 128-cell
+    limit)."""
+    ii, jj = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1))
+    uv = np.stack([width * ii.ravel() / nx, height * jj.ravel() / ny], 1)
+    tris = []
+    for j in range(ny):
+        for i in range(nx):
+            a = j * (nx + 1) + i
+            b, c = a + 1, a + nx + 1
+            d = c + 1
+            tris += [[a, b, d], [a, d, c]] if (i + j) % 2 == 0 else [[a, b, c], [b, d, c]]
+    return uv, np.array(tris, np.int64)
+
+
+def batched_scenes(count: int = 64, seed: int = 3, v_min: int = 1000, v_max: int = 50000,
+                   iterations: int = 20) -> list:
+    """Independent instances of uneven sizes: vertex counts log-uniform in
+    [v_min, v_max]; each a horizontal sheet 5 cm above an icosphere
+    (r 0.25 m) or capsule (r 0.15 m) collider, or a sewn tube (one Poisson-
+    disk panel wrapped into a cylinder of radius 0.2 m, closed by a seam that
+    starts 4 mm open) around a vertical capsule (r 0.15 m); per-instance StVK
+    mu in [5e3, 5e4] N/m (lambda = 0.4 mu), bending in [5e-6, 1e-4] N m and
+    density in [0.1, 0.4] kg/m^2 (log-uniform except density).  Shared
+    settings: dt 1/60, `iterations` sweeps, OGC radius 2 mm, stiffness 1e5."""
+    from .scenes import icosphere
+    rng = np.random.default_rng(seed)
+    out = []
+    settings = default_settings(dt=1.0 / 60.0, iterations=iterations, radius=0.002, stiffness=1e5)
+    for k in range(count):
+        V = int(round(math.exp(rng.uniform(math.log(v_min), math.log(v_max)))))
+        mu = math.exp(rng.uniform(math.log(5e3), math.log(5e4)))
+        bend = math.exp(rng.uniform(math.log(5e-6), math.log(1e-4)))
+        dens = rng.uniform(0.1, 0.4)
+        mat = Material.stvk(mu, 0.4 * mu, bend, dens)
+        tube = bool(rng.integers(2))
+        if not tube:
+            n = max(2, int(round(math.sqrt(V))) - 1)
+            uv, tri = grid_panel(1.0, 1.0, n, n)
+            g = compile_panels([(uv, tri, 0)], lambda i: mat)
+            g.materials = [mat]
+            if rng.integers(2):
+                body = icosphere(0.25, 4)
+                y = 0.25 + 0.05
+            else:
+                body = capsule((-0.3, 0.0, 0.0), (0.3, 0.0, 0.0), 0.15, 64, 16, 6)
+                y = 0.15 + 0.05
+            X = np.stack([g.uv[:, 0] - 0.5, np.full(g.n, y), g.uv[:, 1] - 0.5], 1)
+            name = f"sheet_{g.n}v"
+        else:
+            R, H = 0.2, 0.6
+            W = 2 * math.pi * R
+            spacing = math.sqrt(0.7 * W * H / V)
+            uv = poisson_disk(W, H, spacing, int(rng.integers(1 << 30)))
+            g = compile_panels([(uv, mesh_panel(uv), 0)], lambda i: mat)
+            g.materials = [mat]
+            gap = 0.004
+            eps = math.asin(0.5 * gap / R)
+            th = eps + (2 * math.pi - 2 * eps) * g.uv[:, 0] / W
+            X = np.stack([R * np.cos(th), 0.5 - g.uv[:, 1], R * np.sin(th)], 1)
+            a = boundary_chain(g.uv, "left", (0.0, H))
+            b = boundary_chain(g.uv, "right", (0.0, H))
+            add_seam(g, a, b, 1e4)
+            body = capsule((0.0, -0.3, 0.0), (0.0, 0.4, 0.0), 0.15, 64, 24, 6)
+            name = f"tube_{g.n}v"
+        out.append(Scene(name, g, State(X, np.zeros_like(X), 0.0), body, dict(settings), 120,
+                         {"mu": mu, "bend": bend, "density": dens}))
+    return out
