@@ -262,11 +262,16 @@ def bench_batch(args):
     K = args.steps
     dev_ms, iters = 0.0, 0
     t0 = time.perf_counter()
+    det = {"detection_updates": 0, "candidate_pairs": 0, "contact_terms": 0, "colors_max": 0}
     for k in range(K):
         ms = gpu.step_batch(sols, states, settings)
         dev_ms += ms[0]["gpu_compute_ms"]
         iters += sum(m["iterations_run"] * sc.garment.n for m, sc in zip(ms, scs))
-        print(f"[bench] batch step {k + 1}/{K}", file=sys.stderr, flush=True)
+        det["detection_updates"] += sum(m["detection_updates"] for m in ms)
+        det["candidate_pairs"] += sum(m["candidate_pairs"] for m in ms)
+        det["contact_terms"] += sum(m["contact_terms"] for m in ms)
+        det["colors_max"] = max(det["colors_max"], max(m["colors"] for m in ms))
+        print(f"[bench] batch step {k + 1}/{K}: {ms[0]['gpu_compute_ms']:.1f} ms", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     n_total = sum(sc.garment.n for sc in scs)
     out = {
@@ -282,6 +287,7 @@ def bench_batch(args):
         "vertex_iters_per_s": iters / (dev_ms / 1e3),
         "steps_per_s_wall_incl_transfers": K / wall,
         "lanes": tuned if tuned else args.lanes,
+        "per_batch_step": {k: (v / K if k != "colors_max" else v) for k, v in det.items()},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -54,7 +54,7 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write(" ".join(cmd) + "\n" + out + err)
             raise RuntimeError(f"hipcc failed on {cmd[-3]}")
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     build_host(verbose)
     return LIB

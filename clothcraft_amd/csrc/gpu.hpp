@@ -4,8 +4,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <condition_variable>
 #include <cstdint>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mlp.hpp"
@@ -150,6 +156,85 @@ struct Mlp {
   bool ready = false;
 };
 
+// Fixed worker threads for the per-solver host phases of a batched step
+// (detection read-backs, host colouring, staging).  run() calls f(i) for
+// every i in [0, count) on the workers and the calling thread and returns
+// when all calls are done; f must not throw.
+class WorkPool {
+ public:
+  explicit WorkPool(unsigned threads) {
+    for (unsigned t = 0; t < threads; ++t) workers_.emplace_back([this] { loop(); });
+  }
+  ~WorkPool() {
+    {
+      std::lock_guard<std::mutex> l(m_);
+      quit_ = true;
+    }
+    cv_.notify_all();
+    for (auto& w : workers_) w.join();
+  }
+  void run(size_t count, const std::function<void(size_t)>& f) {
+    if (count == 0) return;
+    {
+      std::lock_guard<std::mutex> l(m_);
+      f_ = &f;
+      count_ = count;
+      next_.store(0);
+      pending_ = count;
+      ++gen_;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> l(m_);
+    done_.wait(l, [this] { return pending_ == 0 && active_ == 0; });
+    f_ = nullptr;
+  }
+
+ private:
+  // A thread that picked up the function pointer counts as active until it
+  // leaves, so run() cannot return (and the function go out of scope) while a
+  // late worker still holds it.
+  void work() {
+    const std::function<void(size_t)>* f;
+    size_t count, done = 0;
+    {
+      std::lock_guard<std::mutex> l(m_);
+      f = f_;
+      count = count_;
+      if (!f) return;
+      ++active_;
+    }
+    for (size_t i; (i = next_.fetch_add(1)) < count;) {
+      (*f)(i);
+      ++done;
+    }
+    std::lock_guard<std::mutex> l(m_);
+    pending_ -= done;
+    --active_;
+    if (pending_ == 0 && active_ == 0) done_.notify_all();
+  }
+  void loop() {
+    uint64_t seen = 0;
+    while (true) {
+      {
+        std::unique_lock<std::mutex> l(m_);
+        cv_.wait(l, [&] { return quit_ || gen_ != seen; });
+        if (quit_) return;
+        seen = gen_;
+      }
+      work();
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  const std::function<void(size_t)>* f_ = nullptr;
+  size_t count_ = 0, pending_ = 0, active_ = 0;
+  std::atomic<size_t> next_{0};
+  uint64_t gen_ = 0;
+  bool quit_ = false;
+};
+
 struct cv_gpu {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -177,11 +262,14 @@ struct cv_gpu {
   bool f64 = true;  // sweep precision (f64 reproduces the f64 reference step)
   Mlp mlp;
   DBuf<float> feat, wpred;  // warm-start features [12n] and predictions [3n]
-  // Batched steps: one context per solver of the batch, sharing this
-  // context's device and stream; launch tables of the shared sweeps.
+  // Batched steps: one context per solver of the batch, each with its own
+  // stream for its detection and bookkeeping; the shared sweeps run on this
+  // context's stream, ordered against the solvers' streams by events.
+  // Launch tables of the shared sweeps; host worker threads.
   std::vector<cv_gpu*> subs;
-  bool owns_stream = true;
+  hipEvent_t join_ev = nullptr;  // no timing: stream-order handoffs
   DBuf<unsigned char> batch_args, batch_slots, batch_ends;
+  std::unique_ptr<WorkPool> pool;
 };
 
 // Runs f, mapping exceptions to a status and message (no exception crosses

@@ -716,15 +716,19 @@ uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
 }
 
 template <class R>
-void true_residual(cv_gpu& g, const SweepArgs<R>& a, uint32_t n) {
+void true_residual(cv_gpu& g, const SweepArgs<R>& a, uint32_t n, hipStream_t st) {
   switch (g.lanes) {
-    case 8: k_true_grad<8, R><<<blocks((uint64_t)n * 8), TPB, 0, g.stream>>>(a, n, g.b.gtrue.p); break;
-    case 16: k_true_grad<16, R><<<blocks((uint64_t)n * 16), TPB, 0, g.stream>>>(a, n, g.b.gtrue.p); break;
-    case 64: k_true_grad<64, R><<<blocks((uint64_t)n * 64), TPB, 0, g.stream>>>(a, n, g.b.gtrue.p); break;
-    default: k_true_grad<32, R><<<blocks((uint64_t)n * 32), TPB, 0, g.stream>>>(a, n, g.b.gtrue.p); break;
+    case 8: k_true_grad<8, R><<<blocks((uint64_t)n * 8), TPB, 0, st>>>(a, n, g.b.gtrue.p); break;
+    case 16: k_true_grad<16, R><<<blocks((uint64_t)n * 16), TPB, 0, st>>>(a, n, g.b.gtrue.p); break;
+    case 64: k_true_grad<64, R><<<blocks((uint64_t)n * 64), TPB, 0, st>>>(a, n, g.b.gtrue.p); break;
+    default: k_true_grad<32, R><<<blocks((uint64_t)n * 32), TPB, 0, st>>>(a, n, g.b.gtrue.p); break;
   }
-  k_sum_f64<<<1, 1024, 0, g.stream>>>(g.b.gtrue.p, n, &g.d_scal->true_residual_sum);
+  k_sum_f64<<<1, 1024, 0, st>>>(g.b.gtrue.p, n, &g.d_scal->true_residual_sum);
   CV_HIP(hipGetLastError());
+}
+template <class R>
+void true_residual(cv_gpu& g, const SweepArgs<R>& a, uint32_t n) {
+  true_residual<R>(g, a, n, g.stream);
 }
 
 struct StepOut {
@@ -993,12 +997,17 @@ void run_step(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* gr
     run_step_t<float>(g, s, set, grab, out);
 }
 
-// Batched step of many solvers (each with its own context g_i): per solver the
-// same device work as run_step_t, with the sweeps of one colour index of one
-// iteration of every solver in a single launch (k_sweep_batch) and the
-// end-of-iteration kernels in another.  Solvers progress through their own
-// iteration indices; a solver waiting for a re-detection sits out.  A solver
-// that fails is dropped with its status; the others continue.
+// Batched step of many solvers (each with its own context g_i and stream):
+// per solver the same device work as run_step_t, with the sweeps of one
+// colour index of one iteration of every solver in a single launch
+// (k_sweep_batch) and the end-of-iteration kernels in another, both on the
+// parent's stream.  The per-solver phases (prologue, detection, host
+// colouring, refresh, finish) run on host worker threads, each solver on its
+// own stream, so detections of different solvers overlap on the device and
+// their host colourings on the CPU.  Events order the solver streams against
+// the shared launches.  Solvers progress through their own iteration
+// indices; a solver waiting for a re-detection sits out.  A solver that
+// fails is dropped with its status; the others continue.
 template <class R>
 void run_batch_t(cv_gpu& P, const std::vector<cv_gpu*>& G, const std::vector<cv_solver*>& S, const cv_settings* set,
                  std::vector<StepOut>& out, std::vector<cv_status>& status, std::vector<std::string>& err) {
@@ -1011,16 +1020,37 @@ void run_batch_t(cv_gpu& P, const std::vector<cv_gpu*>& G, const std::vector<cv_
     status[i] = e.status;
     err[i] = e.message;
   };
-  for (size_t i = 0; i < N; ++i) {
+  // Solver streams wait for the parent's work so far.
+  auto fork = [&] {
+    CV_HIP(hipEventRecord(P.join_ev, st));
+  };
+  auto wait_parent = [&](size_t i) { CV_HIP(hipStreamWaitEvent(G[i]->stream, P.join_ev, 0)); };
+  // The parent stream waits for a solver stream's work so far.
+  auto join = [&](size_t i) {
+    CV_HIP(hipEventRecord(G[i]->join_ev, G[i]->stream));
+    CV_HIP(hipStreamWaitEvent(st, G[i]->join_ev, 0));
+  };
+  auto on_workers = [&](const std::function<void(size_t)>& f) {
+    P.pool->run(N, [&](size_t i) {
+      if (!live[i]) return;
+      try {
+        CV_HIP(hipSetDevice(P.device));
+        f(i);
+      } catch (const CvError& e) {
+        drop(i, e);
+      } catch (const std::exception& e) {
+        drop(i, CvError{CV_ERR_DEVICE, e.what()});
+      }
+    });
+  };
+  fork();
+  on_workers([&](size_t i) {
+    wait_parent(i);
     C[i] = std::make_unique<StepCtx<R>>(*G[i], *S[i], set, nullptr, out[i]);
     C[i]->prologue_reduce();
-  }
-  CV_HIP(hipStreamSynchronize(st));
-  for (size_t i = 0; i < N; ++i) try {
-      C[i]->begin();
-    } catch (const CvError& e) {
-      drop(i, e);
-    }
+    CV_HIP(hipStreamSynchronize(G[i]->stream));
+    C[i]->begin();
+  });
   struct Range {
     size_t i;
     uint32_t first, end;
@@ -1032,22 +1062,31 @@ void run_batch_t(cv_gpu& P, const std::vector<cv_gpu*>& G, const std::vector<cv_
     size_t slot0, nslots;
     uint32_t total;
   };
+  std::vector<char> ready(N);
+  std::vector<uint32_t> rf(N), re(N);
+  bool swept = false;
   while (true) {
-    for (size_t i = 0; i < N; ++i)
-      if (live[i] && !C[i]->finished) C[i]->read_back();
-    CV_HIP(hipStreamSynchronize(st));
-    std::vector<Range> rr;
-    for (size_t i = 0; i < N; ++i) {
-      if (!live[i] || C[i]->finished) continue;
-      try {
-        uint32_t f = 0, e = 0;
-        if (C[i]->after_sync(f, e)) rr.push_back({i, f, e});
-      } catch (const CvError& e) {
-        drop(i, e);
+    std::fill(ready.begin(), ready.end(), 0);
+    on_workers([&](size_t i) {
+      if (C[i]->finished) return;
+      if (swept) wait_parent(i);
+      C[i]->read_back();
+      CV_HIP(hipStreamSynchronize(G[i]->stream));
+      uint32_t f = 0, e = 0;
+      if (C[i]->after_sync(f, e)) {
+        rf[i] = f;
+        re[i] = e;
+        ready[i] = 1;
       }
-    }
+    });
+    std::vector<Range> rr;
+    for (size_t i = 0; i < N; ++i)
+      if (live[i] && ready[i]) rr.push_back({i, rf[i], re[i]});
     if (rr.empty()) break;
-    for (auto& r : rr) args[r.i] = C[r.i]->a;
+    for (auto& r : rr) {
+      args[r.i] = C[r.i]->a;
+      join(r.i);
+    }
     uint32_t T = 0;
     for (auto& r : rr) T = std::max(T, r.end - r.first);
     slots.clear();
@@ -1097,24 +1136,21 @@ void run_batch_t(cv_gpu& P, const std::vector<cv_gpu*>& G, const std::vector<cv_
         }
       }
       for (auto& r : rr)
-        if (r.first + t < r.end && C[r.i]->tol > 0.0) true_residual<R>(*G[r.i], C[r.i]->a, S[r.i]->n);
+        if (r.first + t < r.end && C[r.i]->tol > 0.0) true_residual<R>(*G[r.i], C[r.i]->a, S[r.i]->n, st);
       auto [e0, ne] = end_range[t];
       if (ne) k_iter_end_batch<<<blocks(ne), TPB, 0, st>>>(dends + e0, (uint32_t)ne);
       CV_HIP(hipGetLastError());
     }
+    fork();
+    swept = true;
   }
-  for (size_t i = 0; i < N; ++i)
-    if (live[i]) {
-      C[i]->finish();
-      C[i]->read_back();
-    }
-  CV_HIP(hipStreamSynchronize(st));
-  for (size_t i = 0; i < N; ++i)
-    if (live[i]) try {
-        C[i]->check();
-      } catch (const CvError& e) {
-        drop(i, e);
-      }
+  on_workers([&](size_t i) {
+    if (swept) wait_parent(i);
+    C[i]->finish();
+    C[i]->read_back();
+    CV_HIP(hipStreamSynchronize(G[i]->stream));
+    C[i]->check();
+  });
 }
 
 }  // namespace
@@ -1296,7 +1332,7 @@ void stage_state(cv_gpu& g, cv_solver& s, const cv_state* st, const cv_settings*
 }
 
 uint64_t next_acc_version() {
-  static uint64_t v = 1ull << 40;
+  static std::atomic<uint64_t> v{1ull << 40};
   return ++v;
 }
 
@@ -1371,7 +1407,8 @@ void cv_gpu_destroy(cv_gpu* g) {
   if (g->h_v) (void)hipHostFree(g->h_v);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
   if (g->ev1) (void)hipEventDestroy(g->ev1);
-  hipStream_t st = g->owns_stream ? g->stream : nullptr;
+  if (g->join_ev) (void)hipEventDestroy(g->join_ev);
+  hipStream_t st = g->stream;
   delete g;
   if (st) (void)hipStreamDestroy(st);
 }
@@ -1584,42 +1621,60 @@ cv_status cv_gpu_step_batch(cv_gpu* g, cv_solver* const* solvers, cv_state* cons
       for (uint32_t j = 0; j < i; ++j)
         if (solvers[j] == solvers[i]) fail(CV_ERR_INVALID_ARGUMENT, "a solver appears twice in one batch");
     }
-    // One context per batch slot, on this context's device and stream.
+    // One context per batch slot, on this context's device, each with its
+    // own stream; host worker threads for the per-solver phases.
+    if (!g->join_ev) CV_HIP(hipEventCreateWithFlags(&g->join_ev, hipEventDisableTiming));
+    if (!g->pool) {
+      unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+      g->pool = std::make_unique<WorkPool>(std::min(8u, hw) - 1);
+    }
     while (g->subs.size() < count) {
       cv_gpu* c = new cv_gpu();
       c->device = g->device;
-      c->stream = g->stream;
-      c->owns_stream = false;
       g->subs.push_back(c);
+      CV_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       CV_HIP(hipMalloc((void**)&c->d_scal, sizeof(Scalars)));
       CV_HIP(hipHostMalloc((void**)&c->h_scal, sizeof(Scalars)));
       std::memset(c->h_scal, 0, sizeof(Scalars));
       CV_HIP(hipEventCreate(&c->ev0));
       CV_HIP(hipEventCreate(&c->ev1));
+      CV_HIP(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
     }
-    std::vector<cv_gpu*> G;
-    std::vector<cv_solver*> S;
-    std::vector<uint32_t> idx;
     std::vector<double> t0(count);
+    auto each_slot = [&](const std::function<void(uint32_t)>& f) {
+      g->pool->run(count, [&](size_t i) {
+        if (status[i] != CV_OK) return;
+        status[i] = guard(&err[i], [&] {
+          CV_HIP(hipSetDevice(g->device));
+          f((uint32_t)i);
+        });
+      });
+    };
+    // Stage every state on its solver's stream (parallel host copies).
     for (uint32_t i = 0; i < count; ++i) {
       cv_gpu& c = *g->subs[i];
       c.lanes = g->lanes;
       c.f64 = g->f64;
       c.profile = false;
-      cv_status r = guard(&err[i], [&] {
-        t0[i] = now_ms();
-        validate_state(*solvers[i], states[i]);
-        ensure_device_solver(c, *solvers[i]);
-        ensure_step_buffers(c, *solvers[i]);
-        stage_state(c, *solvers[i], states[i], settings);
-      });
-      status[i] = r;
-      if (r == CV_OK) {
-        G.push_back(&c);
+    }
+    each_slot([&](uint32_t i) {
+      cv_gpu& c = *g->subs[i];
+      t0[i] = now_ms();
+      validate_state(*solvers[i], states[i]);
+      ensure_device_solver(c, *solvers[i]);
+      ensure_step_buffers(c, *solvers[i]);
+      stage_state(c, *solvers[i], states[i], settings);
+      CV_HIP(hipStreamSynchronize(c.stream));
+    });
+    std::vector<cv_gpu*> G;
+    std::vector<cv_solver*> S;
+    std::vector<uint32_t> idx;
+    for (uint32_t i = 0; i < count; ++i)
+      if (status[i] == CV_OK) {
+        G.push_back(g->subs[i]);
         S.push_back(solvers[i]);
         idx.push_back(i);
       }
-    }
     std::vector<StepOut> out(G.size());
     std::vector<cv_status> st2(G.size(), CV_OK);
     std::vector<std::string> er2(G.size());
@@ -1629,16 +1684,27 @@ cv_status cv_gpu_step_batch(cv_gpu* g, cv_solver* const* solvers, cv_state* cons
         run_batch_t<double>(*g, G, S, settings, out, st2, er2);
       else
         run_batch_t<float>(*g, G, S, settings, out, st2, er2);
+      // Every solver stream's work precedes the end event.
+      for (cv_gpu* c : G) {
+        CV_HIP(hipEventRecord(c->join_ev, c->stream));
+        CV_HIP(hipStreamWaitEvent(g->stream, c->join_ev, 0));
+      }
       CV_HIP(hipEventRecord(g->ev1, g->stream));
+      CV_HIP(hipEventSynchronize(g->ev1));
+    }
+    float gpu_ms = 0;
+    if (!G.empty()) CV_HIP(hipEventElapsedTime(&gpu_ms, g->ev0, g->ev1));
+    std::vector<uint32_t> slot_of(count, UINT32_MAX);
+    for (size_t k = 0; k < G.size(); ++k) {
+      slot_of[idx[k]] = (uint32_t)k;
+      if (st2[k] != CV_OK) {
+        status[idx[k]] = st2[k];
+        err[idx[k]] = er2[k];
+      }
     }
     // Commit the solvers whose step succeeded.
-    for (size_t k = 0; k < G.size(); ++k) {
-      uint32_t i = idx[k];
-      if (st2[k] != CV_OK) {
-        status[i] = st2[k];
-        err[i] = er2[k];
-        continue;
-      }
+    each_slot([&](uint32_t i) {
+      size_t k = slot_of[i];
       cv_gpu& c = *G[k];
       cv_solver& sv = *S[k];
       cv_state* stt = states[i];
@@ -1647,17 +1713,7 @@ cv_status cv_gpu_step_batch(cv_gpu* g, cv_solver* const* solvers, cv_state* cons
       CV_HIP(hipMemcpyAsync(c.h_v, c.b.v64.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
       sv.acc.resize(3 * n);
       CV_HIP(hipMemcpyAsync(sv.acc.data(), c.dev.acc.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    }
-    CV_HIP(hipStreamSynchronize(g->stream));
-    float gpu_ms = 0;
-    if (!G.empty()) CV_HIP(hipEventElapsedTime(&gpu_ms, g->ev0, g->ev1));
-    for (size_t k = 0; k < G.size(); ++k) {
-      uint32_t i = idx[k];
-      if (status[i] != CV_OK) continue;
-      cv_gpu& c = *G[k];
-      cv_solver& sv = *S[k];
-      cv_state* stt = states[i];
-      size_t n = sv.n;
+      CV_HIP(hipStreamSynchronize(c.stream));
       std::memcpy(stt->x, c.h_x, 3 * n * sizeof(double));
       std::memcpy(stt->v, c.h_v, 3 * n * sizeof(double));
       stt->time += settings->dt;
@@ -1674,7 +1730,7 @@ cv_status cv_gpu_step_batch(cv_gpu* g, cv_solver* const* solvers, cv_state* cons
         metrics[i].time_s = stt->time;
         metrics[i].gpu_compute_ms = gpu_ms;  // the whole batch
       }
-    }
+    });
   });
   if (outer != CV_OK) return outer;
   for (uint32_t i = 0; i < count; ++i)
