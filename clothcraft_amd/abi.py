@@ -93,6 +93,7 @@ class cv_metrics(C.Structure):
         ("true_residual_rms_n", C.c_double), ("colors", C.c_uint32), ("iterations_run", C.c_uint32),
         ("sweep_ms", C.c_double), ("sweep_launches", C.c_uint64), ("sweep_bytes", C.c_double),
         ("f64", C.c_uint32), ("sweep_bytes_min", C.c_double), ("steps", C.c_uint32),
+        ("sweep_timed_launches", C.c_uint64), ("sweep_timed_bytes_min", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -5,9 +5,9 @@ its term records and reduce with cross-lane shuffles; the best L depends on
 the terms per vertex (triangles, hinges, stitches, contacts) and on how many
 vertices a colour holds.  The tuner times one step per candidate from scratch
 copies of the given states (the caller's states are not touched), keeps the
-fastest and leaves it set on the GpuVbd.  Timing is device time: the summed
-kernel-stamped sweep events for one solver, the batch's device span for a
-batch.
+fastest and leaves it set on the GpuVbd.  Timing is device time: the sweep
+launches' mean event-stamped duration times their count for one solver, the
+batch's device span for a batch.
 """
 from __future__ import annotations
 
@@ -34,9 +34,11 @@ def tune_lanes(gpu: GpuVbd, scenes, settings, candidates=CANDIDATES, repeats: in
             else:
                 gpu.set_profiling(True)
                 try:
-                    t = gpu.step(sols[0], states[0], settings)["sweep_ms"]
+                    m = gpu.step(sols[0], states[0], settings)
                 finally:
                     gpu.set_profiling(False)
+                # Sampled launches' mean duration times all launches.
+                t = m["sweep_ms"] / max(m["sweep_timed_launches"], 1) * m["sweep_launches"]
             best = min(best, t)
         times[L] = best
     lanes = min(times, key=times.get)

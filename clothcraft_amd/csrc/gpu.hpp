@@ -70,6 +70,7 @@ struct Scalars {
   unsigned int iters_run, pad_;
   unsigned long long n_over;           // edges served by the overflow broad-phase kernel
   unsigned long long cand_export_vf, cand_export_ee;
+  unsigned long long vf_list_n, ee_list_n;  // candidates queued for the narrow phase
   unsigned int err_degenerate, err_overflow, err_nonfinite, err_trust;
 };
 
@@ -125,6 +126,7 @@ struct StepBufs {
   DBuf<double> cell_box_sorted;
   DBuf<uint32_t> bcount, bstart;  // hash-grid buckets [2T+1]
   DBuf<uint32_t> item_len, item_off, item_start, over, pair_owner;  // flattened broad phase
+  DBuf<uint2> pair_list;  // candidates for the narrow phase: VF from the front, EE from the back
   DBuf<unsigned long long> item_key;
   uint64_t pair_cap = 0;
   DBuf<uint32_t> raw_ids;   // [5*cap] kind, ids
@@ -258,6 +260,7 @@ struct cv_gpu {
   // Live sweep-kernel timing (events around every sweep launch).
   bool profile = false;
   std::vector<hipEvent_t> sweep_events;
+  uint64_t timed_counter = 0;  // iterations queued with profiling on
   double static_bytes32 = 0, static_bytes64 = 0;  // algorithmic bytes of one sweep, static terms
   bool f64 = true;  // sweep precision (f64 reproduces the f64 reference step)
   Mlp mlp;

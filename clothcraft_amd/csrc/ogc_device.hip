@@ -239,16 +239,24 @@ struct PairCounts {
   unsigned long long tests = 0, vf_cand = 0, vf_near = 0, ee_cand = 0, ee_near = 0;
 };
 
-// One vertex-face pair that shares a grid cell (src/ogc/world.rs:461-497):
-// padded box test, adjacency, closest point, distance, contact.
-__device__ __forceinline__ void vf_pair(const DetectArgs& a, uint32_t v, bool v_moving, bool v_ex, D3 p,
-                                        const ogm::Box& pb, uint32_t t, const ogm::Box& tb, PairCounts& c) {
+// Detection runs in two passes.  The broad pass takes one (query, grid
+// entry) pair per thread and applies the cheap filters below; the pairs that
+// survive (the candidates of src/ogc/candidates.rs) are appended to a list,
+// and the narrow pass runs the closest-feature geometry over that list with
+// every lane busy.  Per-vertex distance minima and the contact set do not
+// depend on the order in which pairs are examined (contacts are sorted and
+// deduplicated afterwards), so the split leaves results unchanged.
+
+// Vertex-face filter (src/ogc/world.rs:461-470): padded box test, moving
+// check, incidence and seam exclusion.
+__device__ __forceinline__ bool vf_filter(const DetectArgs& a, uint32_t v, bool v_moving, bool v_ex,
+                                          const ogm::Box& pb, uint32_t t, const ogm::Box& tb, PairCounts& c) {
   ++c.tests;
-  if (ogm::box_dist2(pb, tb) > a.pr2) return;
-  if (!v_moving && !a.face_moving[t]) return;
+  if (ogm::box_dist2(pb, tb) > a.pr2) return false;
+  if (!v_moving && !a.face_moving[t]) return false;
   uint32_t t0 = a.faces[3 * t], t1 = a.faces[3 * t + 1], t2 = a.faces[3 * t + 2];
-  if (v == t0 || v == t1 || v == t2) return;
-  if (v_ex && (is_excluded(a, v, t0) || is_excluded(a, v, t1) || is_excluded(a, v, t2))) return;
+  if (v == t0 || v == t1 || v == t2) return false;
+  if (v_ex && (is_excluded(a, v, t0) || is_excluded(a, v, t1) || is_excluded(a, v, t2))) return false;
   ++c.vf_cand;
   if (a.cand_vf) {
     unsigned long long s = atomicAdd(&a.sc->cand_export_vf, 1ull);
@@ -257,6 +265,13 @@ __device__ __forceinline__ void vf_pair(const DetectArgs& a, uint32_t v, bool v_
       a.cand_vf[2 * s + 1] = t;
     }
   }
+  return true;
+}
+// Vertex-face narrow phase (src/ogc/world.rs:471-497): closest point,
+// distance, minima, feasible feature, contact.
+__device__ __forceinline__ void vf_narrow(const DetectArgs& a, uint32_t v, uint32_t t, PairCounts& c) {
+  uint32_t t0 = a.faces[3 * t], t1 = a.faces[3 * t + 1], t2 = a.faces[3 * t + 2];
+  D3 p = ld3(a.x, v);
   D3 q;
   int feat, loc;
   ogm::closest_triangle(p, ld3(a.x, t0), ld3(a.x, t1), ld3(a.x, t2), q, feat, loc);
@@ -280,25 +295,26 @@ __device__ __forceinline__ void vf_pair(const DetectArgs& a, uint32_t v, bool v_
   }
 }
 
-// One edge-edge pair met in the grid cell `cell` (src/ogc/world.rs:498-555).
-// The moving edge e is the first operand; moving/moving pairs are taken once
-// with e < f, and only in the cell holding the lower corner of
-// box(e) ∩ inflated box(f), so each pair is examined once.
-__device__ __forceinline__ void ee_pair(const DetectArgs& a, uint32_t e, const ogm::Box& be, uint32_t ia,
-                                        uint32_t ib, bool ia_ex, bool ib_ex, uint32_t f, const ogm::Box& bfr,
-                                        unsigned long long cell, PairCounts& c) {
+// Edge-edge filter for a pair met in the grid cell `cell`
+// (src/ogc/world.rs:498-512).  The moving edge e is the first operand;
+// moving/moving pairs are taken once with e < f, and only in the cell
+// holding the lower corner of box(e) ∩ inflated box(f), so each pair is
+// examined once.
+__device__ __forceinline__ bool ee_filter(const DetectArgs& a, uint32_t e, const ogm::Box& be, uint32_t ia,
+                                          uint32_t ib, bool ia_ex, bool ib_ex, uint32_t f, const ogm::Box& bfr,
+                                          unsigned long long cell, PairCounts& c) {
   ++c.tests;
-  if (ogm::box_dist2(be, bfr) > a.pr2) return;
-  if (f == e) return;
-  if (a.edge_moving[f] && f < e) return;
+  if (ogm::box_dist2(be, bfr) > a.pr2) return false;
+  if (f == e) return false;
+  if (a.edge_moving[f] && f < e) return false;
   ogm::Box bf = inflate(bfr, a.pr);
   double lx = fmax(be.lo.x, bf.lo.x), ly = fmax(be.lo.y, bf.lo.y), lz = fmax(be.lo.z, bf.lo.z);
-  if (cell_key(cell_of(lx, a.h), cell_of(ly, a.h), cell_of(lz, a.h)) != cell) return;
+  if (cell_key(cell_of(lx, a.h), cell_of(ly, a.h), cell_of(lz, a.h)) != cell) return false;
   uint32_t jc = a.edges[2 * f], jd = a.edges[2 * f + 1];
-  if (ia == jc || ia == jd || ib == jc || ib == jd) return;
+  if (ia == jc || ia == jd || ib == jc || ib == jd) return false;
   if ((ia_ex && (is_excluded(a, ia, jc) || is_excluded(a, ia, jd))) ||
       (ib_ex && (is_excluded(a, ib, jc) || is_excluded(a, ib, jd))))
-    return;
+    return false;
   ++c.ee_cand;
   if (a.cand_ee) {
     unsigned long long s = atomicAdd(&a.sc->cand_export_ee, 1ull);
@@ -307,6 +323,12 @@ __device__ __forceinline__ void ee_pair(const DetectArgs& a, uint32_t e, const o
       a.cand_ee[2 * s + 1] = f;
     }
   }
+  return true;
+}
+// Edge-edge narrow phase (src/ogc/world.rs:513-555).
+__device__ __forceinline__ void ee_narrow(const DetectArgs& a, uint32_t e, uint32_t f, PairCounts& c) {
+  uint32_t ia = a.edges[2 * e], ib = a.edges[2 * e + 1];
+  uint32_t jc = a.edges[2 * f], jd = a.edges[2 * f + 1];
   D3 A = ld3(a.x, ia), B = ld3(a.x, ib), C = ld3(a.x, jc), Dd = ld3(a.x, jd);
   double s, t;
   ogm::closest_segments(A, B, C, Dd, s, t);
@@ -332,6 +354,43 @@ __device__ __forceinline__ void ee_pair(const DetectArgs& a, uint32_t e, const o
     add_contact(a, 1, end_b, ia, ib, 0, 3, 0.0);
   else
     add_contact(a, 3, ia, ib, jc, jd, 4, 1e-3 * a.rest_len2[e] * a.rest_len2[f]);
+}
+
+// Block-local staging of the candidate list.  Candidates are appended to
+// LDS (one LDS atomic per wavefront and kind) and written out in bulk with
+// one global atomic per block and kind when a buffer nears full and at the
+// end: same-address global atomics per wavefront serialise across the whole
+// device.  VF entries grow from the front of the global list, EE entries
+// from the back (the two never meet: the list holds one slot per broad-phase
+// pair, and each pair yields at most one candidate).
+constexpr int kListBuf = 1024;
+struct ListStage {
+  uint2 vf[kListBuf], ee[kListBuf];
+  uint32_t n_vf, n_ee;
+  unsigned long long base_vf, base_ee;
+};
+__device__ __forceinline__ void stage_pair(uint2* buf, uint32_t* n, bool take, uint32_t q, uint32_t p) {
+  uint64_t mask = __ballot(take);
+  if (!mask) return;
+  uint32_t lane = threadIdx.x & 63;
+  uint32_t leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(n, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (take) buf[base + __popcll(mask & ((1ull << lane) - 1))] = make_uint2(q, p);
+}
+// Called by every thread of the block, after a barrier.
+__device__ __forceinline__ void flush_stage(ListStage& st, const DetectArgs& a, uint2* list, uint64_t W) {
+  if (threadIdx.x == 0) {
+    st.base_vf = st.n_vf ? atomicAdd(&a.sc->vf_list_n, (unsigned long long)st.n_vf) : 0;
+    st.base_ee = st.n_ee ? atomicAdd(&a.sc->ee_list_n, (unsigned long long)st.n_ee) : 0;
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < st.n_vf; k += blockDim.x) list[st.base_vf + k] = st.vf[k];
+  for (uint32_t k = threadIdx.x; k < st.n_ee; k += blockDim.x) list[W - 1 - (st.base_ee + k)] = st.ee[k];
+  __syncthreads();
+  if (threadIdx.x == 0) st.n_vf = st.n_ee = 0;
+  __syncthreads();
 }
 
 __device__ __forceinline__ void add_counts(const DetectArgs& a, const PairCounts& c) {
@@ -402,7 +461,8 @@ __global__ void k_item_owner(const uint32_t* off, uint32_t NI, uint32_t* owner, 
   }
   for (uint32_t o = o0; o < o1; ++o) owner[o] = i;
 }
-// One (query item, bucket entry) pair per thread, grid-stride.
+// Broad pass: one (query item, bucket entry) pair per thread, grid-stride;
+// the candidates go to the narrow-phase list.
 // The pair range is split into 8 contiguous chunks, one per XCD (blocks b
 // with b % 8 == x run on XCD x), and each XCD's blocks stride through their
 // chunk: neighbouring slots read the same buckets from one L2, and every XCD
@@ -410,29 +470,66 @@ __global__ void k_item_owner(const uint32_t* off, uint32_t NI, uint32_t* owner, 
 // multiple of 8.
 __global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, const uint32_t* owner, const uint32_t* off,
                                                const uint32_t* start, const unsigned long long* ikey,
-                                               uint32_t NI, uint64_t cap) {
+                                               uint32_t NI, uint64_t cap, uint2* list) {
+  __shared__ ListStage stage;
   uint64_t W = off[NI];
   if (W > cap) W = 0;  // capacity flagged by k_item_owner: nothing is examined, the host retries
+  if (threadIdx.x == 0) stage.n_vf = stage.n_ee = 0;
+  __syncthreads();
   PairCounts cnt;
   const uint32_t X = 8, x = blockIdx.x % X, nbx = gridDim.x / X;
   const uint64_t lo = W * x / X, hi = W * (x + 1) / X, step = (uint64_t)nbx * blockDim.x;
-  for (uint64_t j = lo + (uint64_t)(blockIdx.x / X) * blockDim.x + threadIdx.x; j < hi; j += step) {
-    uint32_t item = owner[j];
-    uint32_t it = start[item] + (uint32_t)(j - off[item]);
-    unsigned long long key = ikey[item];
-    const double* bx = a.sbox + 6 * (size_t)it;
-    ogm::Box eb{{bx[0], bx[1], bx[2]}, {bx[3], bx[4], bx[5]}};
-    uint32_t prim = a.sprim[it];
-    if (a.skey[it] != key) continue;  // another cell hashed to the same bucket
-    if (item < a.n_total) {
-      uint32_t v = item;
-      D3 p = ld3(a.x, v);
-      vf_pair(a, v, v < a.n, a.ex_off[v] != a.ex_off[v + 1], p, ogm::box_of(p), prim, eb, cnt);
+  // Block-uniform trip count (the loop holds barriers).
+  for (uint64_t j0 = lo + (uint64_t)(blockIdx.x / X) * blockDim.x; j0 < hi; j0 += step) {
+    uint64_t j = j0 + threadIdx.x;
+    bool vf = false, ee = false;
+    uint32_t q = 0, prim = 0;
+    if (j < hi) {
+      uint32_t item = owner[j];
+      uint32_t it = start[item] + (uint32_t)(j - off[item]);
+      unsigned long long key = ikey[item];
+      prim = a.sprim[it];
+      if (a.skey[it] == key) {  // else another cell hashed to the same bucket
+        const double* bx = a.sbox + 6 * (size_t)it;
+        ogm::Box eb{{bx[0], bx[1], bx[2]}, {bx[3], bx[4], bx[5]}};
+        if (item < a.n_total) {
+          q = item;
+          D3 p = ld3(a.x, q);
+          vf = vf_filter(a, q, q < a.n, a.ex_off[q] != a.ex_off[q + 1], ogm::box_of(p), prim, eb, cnt);
+        } else {
+          q = (item - a.n_total) / kEdgeCells;
+          uint32_t ia = a.edges[2 * q], ib = a.edges[2 * q + 1];
+          ee = ee_filter(a, q, ldbox(a.ebox, q), ia, ib, a.ex_off[ia] != a.ex_off[ia + 1],
+                         a.ex_off[ib] != a.ex_off[ib + 1], prim, eb, key, cnt);
+        }
+      }
+    }
+    stage_pair(stage.vf, &stage.n_vf, vf, q, prim);
+    stage_pair(stage.ee, &stage.n_ee, ee, q, prim);
+    __syncthreads();
+    bool full = stage.n_vf > kListBuf - TPB || stage.n_ee > kListBuf - TPB;
+    __syncthreads();  // every thread has read the counts before they change again
+    if (full) flush_stage(stage, a, list, W);
+  }
+  flush_stage(stage, a, list, W);
+  add_counts(a, cnt);
+}
+// Narrow pass over the candidate list, VF and EE in separate grid-stride
+// ranges of one launch (no host read-back of the counts).
+__global__ __launch_bounds__(TPB) void k_narrow(DetectArgs a, const uint32_t* off, uint32_t NI, uint64_t cap,
+                                                const uint2* list) {
+  uint64_t W = off[NI];
+  if (W > cap) W = 0;
+  uint64_t nvf = W ? a.sc->vf_list_n : 0, nee = W ? a.sc->ee_list_n : 0;
+  PairCounts cnt;
+  uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nvf + nee; j += step) {
+    if (j < nvf) {
+      uint2 e = list[j];
+      vf_narrow(a, e.x, e.y, cnt);
     } else {
-      uint32_t e = (item - a.n_total) / kEdgeCells;
-      uint32_t ia = a.edges[2 * e], ib = a.edges[2 * e + 1];
-      ee_pair(a, e, ldbox(a.ebox, e), ia, ib, a.ex_off[ia] != a.ex_off[ia + 1], a.ex_off[ib] != a.ex_off[ib + 1],
-              prim, eb, key, cnt);
+      uint2 e = list[W - 1 - (j - nvf)];
+      ee_narrow(a, e.x, e.y, cnt);
     }
   }
   add_counts(a, cnt);
@@ -463,7 +560,8 @@ __global__ __launch_bounds__(TPB) void k_ee_overflow(DetectArgs a, const uint32_
             if (a.skey[it] != key) continue;
             const double* bx = a.sbox + 6 * (size_t)it;
             ogm::Box bfr{{bx[0], bx[1], bx[2]}, {bx[3], bx[4], bx[5]}};
-            ee_pair(a, e, be, ia, ib, ia_ex, ib_ex, a.sprim[it], bfr, key, cnt);
+            uint32_t f = a.sprim[it];
+            if (ee_filter(a, e, be, ia, ib, ia_ex, ib_ex, f, bfr, key, cnt)) ee_narrow(a, e, f, cnt);
           }
         }
   }
@@ -653,6 +751,7 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   hs.vf_count = hs.ee_count = hs.vf_cand = hs.ee_cand = hs.grid_tests = 0;
   hs.raw_contacts = hs.contacts = hs.active = hs.incidences = 0;
   hs.cand_export_vf = hs.cand_export_ee = 0;
+  hs.vf_list_n = hs.ee_list_n = 0;
   hs.err_degenerate = hs.err_overflow = 0;
   hs.rq = rq;
   hs.padded_radius2 = pr2;
@@ -728,10 +827,13 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   // flagged on the device and the detection is run again with more room.
   if (b.pair_cap == 0) b.pair_cap = std::max<uint64_t>(4ull * grid_entries + (1u << 20), 1u << 22);
   b.pair_owner.reserve(b.pair_cap);
+  b.pair_list.reserve(b.pair_cap);
   k_item_owner<<<blocks(NI), TPB, 0, st>>>(b.item_off.p, NI, b.pair_owner.p, b.pair_cap, g.d_scal);
   CV_HIP(hipGetLastError());
   k_pairs<<<8 * 256, TPB, 0, st>>>(
-      a, b.pair_owner.p, b.item_off.p, b.item_start.p, b.item_key.p, NI, b.pair_cap);
+      a, b.pair_owner.p, b.item_off.p, b.item_start.p, b.item_key.p, NI, b.pair_cap, b.pair_list.p);
+  CV_HIP(hipGetLastError());
+  k_narrow<<<8 * 256, TPB, 0, st>>>(a, b.item_off.p, NI, b.pair_cap, b.pair_list.p);
   CV_HIP(hipGetLastError());
   constexpr int G = 16;
   k_ee_overflow<G><<<64, TPB, 0, st>>>(a, b.over.p, &g.d_scal->n_over);

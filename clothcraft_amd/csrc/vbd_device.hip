@@ -734,8 +734,8 @@ void true_residual(cv_gpu& g, const SweepArgs<R>& a, uint32_t n) {
 struct StepOut {
   uint64_t detections = 0, truncated = 0;
   uint32_t colors = 0, iterations_run = 0;
-  uint64_t sweep_launches = 0;
-  double sweep_ms = 0, sweep_bytes = 0, sweep_bytes_min = 0;
+  uint64_t sweep_launches = 0, sweep_timed_launches = 0;
+  double sweep_ms = 0, sweep_bytes = 0, sweep_bytes_min = 0, sweep_timed_bytes_min = 0;
   double detection_ms = 0, refit_ms = 0, candidates_ms = 0, narrow_ms = 0;
   uint64_t candidate_pairs = 0, grid_tests = 0;
   bool true_residual = false;
@@ -815,7 +815,9 @@ struct StepCtx {
     uint32_t it;
   };
   std::vector<Launch> launches;
+  std::vector<uint32_t> timed_its;  // iterations whose launches carry events
   size_t nev = 0;
+  static constexpr uint64_t kTimedStride = 4;
 
   StepCtx(cv_gpu& g_, cv_solver& s_, const cv_settings* set_, const cv_grab* grab_, StepOut& out_)
       : g(g_), s(s_), set(set_), grab(grab_), out(out_) {}
@@ -881,6 +883,8 @@ struct StepCtx {
   // iterations are over.
   bool after_sync(uint32_t& first, uint32_t& end) {
     uint32_t ran = g.h_scal->iters_run;
+    double inc = (double)g.h_scal->incidences;
+    double it_bytes32 = g.static_bytes32 + 56.0 * inc;
     for (const Launch& l : launches) {
       if (l.it >= ran) continue;
       out.sweep_launches++;
@@ -888,14 +892,16 @@ struct StepCtx {
         float ms = 0;
         CV_HIP(hipEventElapsedTime(&ms, g.sweep_events[2 * l.ev], g.sweep_events[2 * l.ev + 1]));
         out.sweep_ms += ms;
+        out.sweep_timed_launches++;
       }
     }
+    for (uint32_t it : timed_its)
+      if (it < ran) out.sweep_timed_bytes_min += it_bytes32;
     launches.clear();
+    timed_its.clear();
     nev = 0;
-    double inc = (double)g.h_scal->incidences;
-    out.sweep_bytes += (ran - done) * (sizeof(R) == 4 ? g.static_bytes32 + 56.0 * inc
-                                                       : g.static_bytes64 + 100.0 * inc);
-    out.sweep_bytes_min += (ran - done) * (g.static_bytes32 + 56.0 * inc);
+    out.sweep_bytes += (ran - done) * (sizeof(R) == 4 ? it_bytes32 : g.static_bytes64 + 100.0 * inc);
+    out.sweep_bytes_min += (ran - done) * it_bytes32;
     done = ran;
     if (g.h_scal->stop == 2) return finished = true, false;  // residual tolerance reached
     if (g.h_scal->stop) {
@@ -921,10 +927,15 @@ struct StepCtx {
   // Sweeps of iterations [first, end) on this solver's own launches.
   void enqueue(uint32_t first, uint32_t end) {
     for (uint32_t it = first; it < end; ++it) {
+      // With profiling on, every launch of every kTimedStride-th iteration is
+      // stamped with begin/end events (stamping all of them costs more host
+      // time per launch than the kernel runs).
+      bool timed = g.profile && (g.timed_counter++ % kTimedStride == 0);
+      if (timed) timed_its.push_back(it);
       for (size_t c = 0; c + 1 < g.color_off.size(); ++c) {
         uint32_t off = g.color_off[c], cnt = g.color_off[c + 1] - off;
         if (!cnt) continue;
-        if (g.profile) {
+        if (timed) {
           if (g.sweep_events.size() < 2 * (nev + 1)) {
             hipEvent_t e0, e1;
             CV_HIP(hipEventCreate(&e0));
@@ -1292,6 +1303,8 @@ void fill_metrics(cv_metrics* m, const cv_gpu& g, const cv_solver& s, const cv_s
   m->colors = o.colors;
   m->iterations_run = o.iterations_run;
   m->sweep_ms = g.profile ? o.sweep_ms : NAN;
+  m->sweep_timed_launches = o.sweep_timed_launches;
+  m->sweep_timed_bytes_min = o.sweep_timed_bytes_min;
   m->f64 = g.f64 ? 1 : 0;
   m->sweep_launches = o.sweep_launches;
   m->sweep_bytes = o.sweep_bytes;
@@ -1472,6 +1485,8 @@ static cv_status run_steps(cv_gpu* g, cv_solver* s, cv_state* st, const cv_setti
       tot.iterations_run += o.iterations_run;
       tot.sweep_launches += o.sweep_launches;
       tot.sweep_ms += o.sweep_ms;
+      tot.sweep_timed_launches += o.sweep_timed_launches;
+      tot.sweep_timed_bytes_min += o.sweep_timed_bytes_min;
       tot.sweep_bytes += o.sweep_bytes;
       tot.sweep_bytes_min += o.sweep_bytes_min;
     }
@@ -1504,6 +1519,8 @@ static cv_status run_steps(cv_gpu* g, cv_solver* s, cv_state* st, const cv_setti
       o.iterations_run = tot.iterations_run;
       o.sweep_launches = tot.sweep_launches;
       o.sweep_ms = tot.sweep_ms;
+      o.sweep_timed_launches = tot.sweep_timed_launches;
+      o.sweep_timed_bytes_min = tot.sweep_timed_bytes_min;
       o.sweep_bytes = tot.sweep_bytes;
       o.sweep_bytes_min = tot.sweep_bytes_min;
     }

@@ -160,7 +160,8 @@ typedef struct cv_metrics {
   double true_residual_rms_n; /* sqrt(sum_free |g|^2 / 3 n_free) after the last sweep, or NaN */
   uint32_t colors;            /* colour count of the last colouring                    */
   uint32_t iterations_run;    /* sweeps actually run (< iterations with a tolerance)   */
-  double sweep_ms;            /* summed device time of the sweep kernels (profiling on), else NaN */
+  double sweep_ms;            /* profiling on: summed device time of the sweep launches
+                                 stamped with events (sweep_timed_launches of them), else NaN */
   uint64_t sweep_launches;    /* sweep kernel launches in the step                     */
   double sweep_bytes;         /* algorithmic bytes those launches move in the sweep's precision */
   uint32_t f64;               /* 1 when the sweep ran in f64, 0 for f32                */
@@ -168,6 +169,9 @@ typedef struct cv_metrics {
                                  64 B per free vertex + 72 per triangle, 52 per hinge,
                                  68 per stitch, 56 per contact incidence             */
   uint32_t steps;             /* steps these metrics cover (1 for cv_gpu_step)         */
+  uint64_t sweep_timed_launches;  /* launches in sweep_ms: every launch of every 4th
+                                     iteration queued while profiling is on           */
+  double sweep_timed_bytes_min;   /* sweep_bytes_min of those launches                */
 } cv_metrics;
 
 /* ------------------------------------------------------------- solver */
@@ -293,7 +297,9 @@ size_t cv_solver_last_error(const cv_solver* solver, char* buf, size_t n);
  * round-off) or 32 (faster; strain resolution limits parity on stiff fabrics). */
 cv_status cv_gpu_set_precision(cv_gpu* gpu, int bits);
 
-/* Events around every sweep launch; metrics.sweep_ms then reports their sum. */
+/* Begin/end events around the sweep launches of every 4th iteration;
+ * metrics.sweep_ms reports their summed duration, sweep_timed_launches their
+ * count and sweep_timed_bytes_min their algorithmic bytes. */
 cv_status cv_gpu_set_profiling(cv_gpu* gpu, int enabled);
 
 /* Lanes per vertex of the sweep kernel (8, 16, 32 or 64); the autotuner sets it. */
