@@ -71,6 +71,7 @@ struct Scalars {
   unsigned long long n_over;           // edges served by the overflow broad-phase kernel
   unsigned long long cand_export_vf, cand_export_ee;
   unsigned long long vf_list_n, ee_list_n;  // candidates queued for the narrow phase
+  unsigned long long grid_entries;          // hash-grid registrations of the last detection
   unsigned int err_degenerate, err_overflow, err_nonfinite, err_trust;
 };
 
@@ -129,6 +130,7 @@ struct StepBufs {
   DBuf<uint2> pair_list;  // candidates for the narrow phase: VF from the front, EE from the back
   DBuf<unsigned long long> item_key;
   uint64_t pair_cap = 0;
+  uint64_t grid_cap = 0, grid_last = 0;  // registration capacity; last detection's count
   DBuf<uint32_t> raw_ids;   // [5*cap] kind, ids
   DBuf<uint32_t> raw_count;
   DBuf<double> raw_eps;
@@ -251,7 +253,10 @@ struct cv_gpu {
   std::string last_error;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // host-side contact incidence for the greedy colouring
-  std::vector<uint32_t> h_cids, h_ccount;
+  // Pinned copy of the contact list for the host colouring: kind and ids
+  // (5 per contact) then counts (1 per contact), capacity h_cont_cap contacts.
+  uint32_t* h_cont = nullptr;
+  size_t h_cont_cap = 0;
   std::vector<uint32_t> cnb_off, cnb;
   std::vector<uint32_t> color_of, order, color_off;
   // Sweep launch shape (lanes per vertex), chosen by the autotuner.

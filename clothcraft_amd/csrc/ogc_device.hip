@@ -102,20 +102,29 @@ __global__ void k_reg_count(const double* fbox, const double* ebox, uint32_t F, 
   count[i] = (uint32_t)(c > kMaxCells ? 1 : c);  // over the cap: one sentinel entry, see k_reg_fill
 }
 // Owner primitive of every entry (entries of primitive i are off[i]..off[i+1]).
-__global__ void k_reg_owner(const uint32_t* off, uint32_t N, uint32_t* owner) {
+// Entries past the registration capacity raise the capacity flag (the host
+// grows the buffers and detects again); thread 0 records the entry count.
+__global__ void k_reg_owner(const uint32_t* off, uint32_t N, uint32_t* owner, uint64_t cap, Scalars* sc) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) sc->grid_entries = off[N];
   if (i >= N) return;
+  if (off[N] > cap) {
+    if (i == 0) atomicOr(&sc->err_overflow, 8u);
+    return;
+  }
   for (uint32_t o = off[i]; o < off[i + 1]; ++o) owner[o] = i;
 }
 // One thread per entry: its cell key, primitive id and rank in its bucket
 // (one returning atomic per thread, all in flight together).  Faces use
 // buckets [0, T), edges [T, 2T).
+// The grid covers the capacity; the entry count M = off[N] is read here.
 __global__ void k_reg_fill(const double* fbox, const double* ebox, uint32_t F, double pr, double h,
-                           const uint32_t* off, const uint32_t* owner, uint32_t M, int log2t,
+                           const uint32_t* off, uint32_t N, const uint32_t* owner, uint64_t cap, int log2t,
                            unsigned long long* keys, uint32_t* prims, uint32_t* ranks, uint32_t* bcount,
                            Scalars* sc) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= M) return;
+  uint32_t M = off[N];
+  if (M > cap || j >= M) return;
   uint32_t i = owner[j];
   uint32_t table = i < F ? 0u : (1u << log2t), prim = i < F ? i : i - F;
   long long r[6];
@@ -137,11 +146,12 @@ __global__ void k_reg_fill(const double* fbox, const double* ebox, uint32_t F, d
 // Bucketed entries carry the primitive's (unpadded) box, so a query scans
 // contiguous memory and tests boxes without a dependent gather.
 __global__ void k_reg_scatter(const unsigned long long* keys, const uint32_t* prims, const uint32_t* ranks,
-                              uint32_t M, uint32_t face_entries, int log2t, const uint32_t* bstart,
-                              const double* fbox, const double* ebox, unsigned long long* skey,
-                              uint32_t* sprim, double* sbox) {
+                              const uint32_t* off, uint32_t F, uint32_t N, uint64_t cap, int log2t,
+                              const uint32_t* bstart, const double* fbox, const double* ebox,
+                              unsigned long long* skey, uint32_t* sprim, double* sbox) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= M) return;
+  uint32_t M = off[N], face_entries = off[F];
+  if (M > cap || j >= M) return;
   bool face = j < face_entries;
   uint32_t b = (face ? 0u : (1u << log2t)) + bucket_of(keys[j], log2t);
   uint32_t pos = bstart[b] + ranks[j];
@@ -605,13 +615,15 @@ __global__ void k_unique_write(const uint32_t* raw, const uint32_t* raw_count, c
   ceps[dst] = raw_eps[src];
 }
 __global__ void k_store_u64(unsigned long long* dst, unsigned long long v) { *dst = v; }
+__global__ void k_store_count(const uint32_t* src, unsigned long long* dst) { *dst = *src; }
 
 // Active contacts, incidence counts and incidence keys (v << 32 | contact).
+// Over the unique contacts (count in sc->contacts; the grid covers the raw count).
 __global__ void k_contact_post(const double* x, const uint32_t* cids, const uint32_t* ccount,
-                               const double* ceps, uint64_t m, uint32_t n, double r, double k,
-                               uint32_t* vcount, unsigned long long* inc_key, Scalars* sc) {
+                               const double* ceps, uint32_t n, double r, double k, uint32_t* vcount,
+                               unsigned long long* inc_key, Scalars* sc) {
   uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (c >= m) return;
+  if (c >= sc->contacts) return;
   int kind = (int)cids[5 * c];
   D3 p[4];
   for (int q = 0; q < 4; ++q) p[q] = ld3(x, cids[5 * c + 1 + q]);
@@ -680,8 +692,10 @@ void read_scalars(cv_gpu& g) {
 
 // Register face and edge boxes (inflated by pr) in the hash grid: entries
 // bucketed by cell-key hash, faces in buckets [0, T) and edges in [T, 2T).
-// Returns log2(T).
-int build_grid(cv_gpu& g, uint32_t F, uint32_t E, double pr, double h, uint32_t& entries) {
+// Buffers are sized from the previous detection's entry count (with room to
+// grow), so no read-back is needed here; an overflow is flagged on the
+// device and the caller detects again with more room.  Returns log2(T).
+int build_grid(cv_gpu& g, uint32_t F, uint32_t E, double pr, double h) {
   StepBufs& b = g.b;
   hipStream_t st = g.stream;
   uint32_t N = F + E;
@@ -694,38 +708,39 @@ int build_grid(cv_gpu& g, uint32_t F, uint32_t E, double pr, double h, uint32_t&
     return rocprim::exclusive_scan(t, bytes, b.cell_count.p, b.cell_off.p, 0u, (size_t)N + 1,
                                    rocprim::plus<uint32_t>(), st);
   });
-  uint32_t tot[2] = {0, 0};  // entries of faces, all entries
-  CV_HIP(hipMemcpyAsync(&tot[0], b.cell_off.p + F, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  CV_HIP(hipMemcpyAsync(&tot[1], b.cell_off.p + N, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  CV_HIP(hipStreamSynchronize(st));
-  uint32_t M = tot[1];
-  entries = M;
-  if (M > (1u << 30)) fail(CV_ERR_CAPACITY, "OGC grid registration exceeds 2^30 cells");
+  if (b.grid_cap == 0) {
+    // First detection of this context: read the exact count once.
+    uint32_t M0 = 0;
+    CV_HIP(hipMemcpyAsync(&M0, b.cell_off.p + N, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    CV_HIP(hipStreamSynchronize(st));
+    b.grid_last = M0;
+    b.grid_cap = b.grid_last + b.grid_last / 2 + 4096;
+  }
+  uint64_t M = std::max<uint64_t>(b.grid_last, 1), cap = b.grid_cap;
+  if (cap > (1u << 30)) fail(CV_ERR_CAPACITY, "OGC grid registration exceeds 2^30 cells");
   int log2t = 10;
-  while (log2t < 28 && (1u << log2t) < M / 4) ++log2t;
+  while (log2t < 28 && (1ull << log2t) < M / 4) ++log2t;
   size_t T2 = (size_t)2 << log2t;
-  b.cell_key.reserve(std::max<uint32_t>(M, 1));
-  b.cell_val.reserve(std::max<uint32_t>(M, 1));
-  b.cell_rank.reserve(std::max<uint32_t>(M, 1));
-  b.cell_key_sorted.reserve(std::max<uint32_t>(M, 1));
-  b.cell_val_sorted.reserve(std::max<uint32_t>(M, 1));
-  b.cell_owner.reserve(std::max<uint32_t>(M, 1));
-  b.cell_box_sorted.reserve(6 * (size_t)std::max<uint32_t>(M, 1));
+  b.cell_key.reserve(cap);
+  b.cell_val.reserve(cap);
+  b.cell_rank.reserve(cap);
+  b.cell_key_sorted.reserve(cap);
+  b.cell_val_sorted.reserve(cap);
+  b.cell_owner.reserve(cap);
+  b.cell_box_sorted.reserve(6 * cap);
   b.bcount.reserve(T2 + 1);
   b.bstart.reserve(T2 + 1);
   CV_HIP(hipMemsetAsync(b.bcount.p, 0, (T2 + 1) * sizeof(uint32_t), st));
-  k_reg_owner<<<blocks(N), TPB, 0, st>>>(b.cell_off.p, N, b.cell_owner.p);
-  if (M)
-    k_reg_fill<<<blocks(M), TPB, 0, st>>>(b.fbox.p, b.ebox.p, F, pr, h, b.cell_off.p, b.cell_owner.p, M, log2t,
-                                          b.cell_key.p, b.cell_val.p, b.cell_rank.p, b.bcount.p, g.d_scal);
+  k_reg_owner<<<blocks(N), TPB, 0, st>>>(b.cell_off.p, N, b.cell_owner.p, cap, g.d_scal);
+  k_reg_fill<<<blocks(cap), TPB, 0, st>>>(b.fbox.p, b.ebox.p, F, pr, h, b.cell_off.p, N, b.cell_owner.p, cap,
+                                          log2t, b.cell_key.p, b.cell_val.p, b.cell_rank.p, b.bcount.p, g.d_scal);
   CV_HIP(hipGetLastError());
   with_temp(g, [&](void* t, size_t& bytes) {
     return rocprim::exclusive_scan(t, bytes, b.bcount.p, b.bstart.p, 0u, T2 + 1, rocprim::plus<uint32_t>(),
                                    st);
   });
-  if (M)
-    k_reg_scatter<<<blocks(M), TPB, 0, st>>>(b.cell_key.p, b.cell_val.p, b.cell_rank.p, M, tot[0], log2t,
-                                             b.bstart.p, b.fbox.p, b.ebox.p, b.cell_key_sorted.p,
+  k_reg_scatter<<<blocks(cap), TPB, 0, st>>>(b.cell_key.p, b.cell_val.p, b.cell_rank.p, b.cell_off.p, F, N, cap,
+                                             log2t, b.bstart.p, b.fbox.p, b.ebox.p, b.cell_key_sorted.p,
                                              b.cell_val_sorted.p, b.cell_box_sorted.p);
   CV_HIP(hipGetLastError());
   return log2t;
@@ -801,8 +816,7 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   a.sc = g.d_scal;
   // Faces and edges registered with the padded inflation; points look up
   // their own cell, edges the cells of their own box.
-  uint32_t grid_entries = 0;
-  a.log2t = build_grid(g, d.F, d.E, pr_reg, h, grid_entries);
+  a.log2t = build_grid(g, d.F, d.E, pr_reg, h);
   a.skey = b.cell_key_sorted.p;
   a.sprim = b.cell_val_sorted.p;
   a.sbox = b.cell_box_sorted.p;
@@ -825,7 +839,7 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   });
   // Pair-slot capacity: grown from the last detection's need; an overflow is
   // flagged on the device and the detection is run again with more room.
-  if (b.pair_cap == 0) b.pair_cap = std::max<uint64_t>(4ull * grid_entries + (1u << 20), 1u << 22);
+  if (b.pair_cap == 0) b.pair_cap = std::max<uint64_t>(4ull * b.grid_cap + (1u << 20), 1u << 22);
   b.pair_owner.reserve(b.pair_cap);
   b.pair_list.reserve(b.pair_cap);
   k_item_owner<<<blocks(NI), TPB, 0, st>>>(b.item_off.p, NI, b.pair_owner.p, b.pair_cap, g.d_scal);
@@ -839,12 +853,17 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   k_ee_overflow<G><<<64, TPB, 0, st>>>(a, b.over.p, &g.d_scal->n_over);
   CV_HIP(hipGetLastError());
   read_scalars(g);
-  if (g.h_scal->err_overflow & 4u) {
-    // More (query, entry) pairs than slots: grow and detect again.
-    uint32_t need = 0;
-    CV_HIP(hipMemcpyAsync(&need, b.item_off.p + NI, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    CV_HIP(hipStreamSynchronize(st));
-    b.pair_cap = (uint64_t)need + need / 2 + 1024;
+  b.grid_last = g.h_scal->grid_entries;
+  if (g.h_scal->err_overflow & 12u) {
+    // More grid registrations (8) or (query, entry) pairs (4) than room:
+    // grow and detect again.
+    if (g.h_scal->err_overflow & 8u) b.grid_cap = b.grid_last + b.grid_last / 2 + 4096;
+    if (g.h_scal->err_overflow & 4u) {
+      uint32_t need = 0;
+      CV_HIP(hipMemcpyAsync(&need, b.item_off.p + NI, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      CV_HIP(hipStreamSynchronize(st));
+      b.pair_cap = (uint64_t)need + need / 2 + 1024;
+    }
     gpu_detect(g, s, rq, ogc, cand_cap);
     return;
   }
@@ -859,7 +878,6 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   b.perm_tmp.reserve(m);
   b.uniq_flag.reserve(m + 1);
   b.uniq_pos.reserve(m + 1);
-  uint64_t unique = 0;
   if (m) {
     k_contact_keys<<<blocks(m), TPB, 0, st>>>(b.raw_ids.p, m, b.key_hi.p, b.key_lo.p, b.perm.p);
     with_temp(g, [&](void* t, size_t& bytes) {  // by low key
@@ -878,29 +896,37 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
       return rocprim::exclusive_scan(t, bytes, b.uniq_flag.p, b.uniq_pos.p, 0u, (size_t)m + 1,
                                      rocprim::plus<uint32_t>(), st);
     });
-    uint32_t u32 = 0;
-    CV_HIP(hipMemcpyAsync(&u32, b.uniq_pos.p + m, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    CV_HIP(hipStreamSynchronize(st));
-    unique = u32;
     k_unique_write<<<blocks(m), TPB, 0, st>>>(b.raw_ids.p, b.raw_count.p, b.raw_eps.p, b.perm.p,
                                               b.uniq_flag.p, b.uniq_pos.p, m, b.c_ids.p, b.c_count.p,
                                               b.c_eps.p);
+    // The unique count lives in the device scalars like every other counter
+    // (no read-back here; the next scalar read-back carries it).
+    k_store_count<<<1, 1, 0, st>>>(b.uniq_pos.p + m, &g.d_scal->contacts);
+    // The contact list for the host colouring, copied with room for all m
+    // raw contacts so the copy needs no count first.
+    if (g.h_cont_cap < m) {
+      if (g.h_cont) CV_HIP(hipHostFree(g.h_cont));
+      g.h_cont = nullptr;
+      size_t cap = m + m / 2 + 1024;
+      CV_HIP(hipHostMalloc((void**)&g.h_cont, 6 * cap * sizeof(uint32_t)));
+      g.h_cont_cap = cap;
+    }
+    CV_HIP(hipMemcpyAsync(g.h_cont, b.c_ids.p, 5 * m * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    CV_HIP(hipMemcpyAsync(g.h_cont + 5 * g.h_cont_cap, b.c_count.p, m * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          st));
+  } else {
+    k_store_u64<<<1, 1, 0, st>>>(&g.d_scal->contacts, 0ull);
   }
-  // The unique count lives in the device scalars like every other counter,
-  // so later scalar read-backs keep it (passed by value: no host staging).
-  k_store_u64<<<1, 1, 0, st>>>(&g.d_scal->contacts, unique);
   CV_HIP(hipGetLastError());
-  g.h_scal->contacts = unique;
   // Incidence, active count, per-vertex contact records.
   b.vcount.reserve(d.n + 1);
   b.coff.reserve(d.n + 1);
   CV_HIP(hipMemsetAsync(b.vcount.p, 0, sizeof(uint32_t) * (d.n + 1), st));
-  if (unique) {
-    b.inc_key.reserve(4 * unique);
-    b.inc_key_sorted.reserve(4 * unique);
-    k_contact_post<<<blocks(unique), TPB, 0, st>>>(b.x64.p, b.c_ids.p, b.c_count.p, b.c_eps.p, unique,
-                                                   d.n, ogc.radius, ogc.stiffness, b.vcount.p,
-                                                   b.inc_key.p, g.d_scal);
+  if (m) {
+    b.inc_key.reserve(4 * m);
+    b.inc_key_sorted.reserve(4 * m);
+    k_contact_post<<<blocks(m), TPB, 0, st>>>(b.x64.p, b.c_ids.p, b.c_count.p, b.c_eps.p, d.n, ogc.radius,
+                                              ogc.stiffness, b.vcount.p, b.inc_key.p, g.d_scal);
   }
   with_temp(g, [&](void* t, size_t& bytes) {
     return rocprim::exclusive_scan(t, bytes, b.vcount.p, b.coff.p, 0u, (size_t)d.n + 1,
@@ -909,7 +935,6 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   b.radii64.reserve(d.n);
   k_radii<<<1, 1024, 0, st>>>(b.dist_bits.p, d.n, ogc.relaxation, rq, b.radii64.p, g.d_scal);
   read_scalars(g);
-  g.h_scal->contacts = unique;
   uint64_t ninc = g.h_scal->incidences;
   b.crec.reserve(std::max<uint64_t>(ninc, 1));
   b.crec_d.reserve(std::max<uint64_t>(ninc, 1));

@@ -682,33 +682,27 @@ void sweep_color(cv_gpu& g, const SweepArgs<R>& a, uint32_t off, uint32_t count,
   }
 }
 
-// Host greedy colouring of the current contact set.
+// Host greedy colouring of the current contact set (the detection left the
+// contact list in g.h_cont and its count in the scalars it read back).
 uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
   uint64_t m = g.h_scal->contacts;
-  g.h_cids.resize(5 * m);
-  g.h_ccount.resize(m);
-  if (m) {
-    CV_HIP(hipMemcpyAsync(g.h_cids.data(), g.b.c_ids.p, 5 * m * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                          g.stream));
-    CV_HIP(hipMemcpyAsync(g.h_ccount.data(), g.b.c_count.p, m * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                          g.stream));
-    CV_HIP(hipStreamSynchronize(g.stream));
-  }
+  const uint32_t* cids = g.h_cont;
+  const uint32_t* ccount = g.h_cont + 5 * g.h_cont_cap;
   uint32_t n = s.n;
   g.cnb_off.assign(n + 1, 0);
   for (uint64_t c = 0; c < m; ++c)
-    for (uint32_t q = 0; q < g.h_ccount[c]; ++q) {
-      uint32_t v = g.h_cids[5 * c + 1 + q];
-      if (v < n) g.cnb_off[v + 1] += g.h_ccount[c];
+    for (uint32_t q = 0; q < ccount[c]; ++q) {
+      uint32_t v = cids[5 * c + 1 + q];
+      if (v < n) g.cnb_off[v + 1] += ccount[c];
     }
   for (uint32_t v = 0; v < n; ++v) g.cnb_off[v + 1] += g.cnb_off[v];
   g.cnb.resize(g.cnb_off[n]);
   std::vector<uint32_t> cur(g.cnb_off.begin(), g.cnb_off.end() - 1);
   for (uint64_t c = 0; c < m; ++c)
-    for (uint32_t q = 0; q < g.h_ccount[c]; ++q) {
-      uint32_t v = g.h_cids[5 * c + 1 + q];
+    for (uint32_t q = 0; q < ccount[c]; ++q) {
+      uint32_t v = cids[5 * c + 1 + q];
       if (v < n)
-        for (uint32_t r = 0; r < g.h_ccount[c]; ++r) g.cnb[cur[v]++] = g.h_cids[5 * c + 1 + r];
+        for (uint32_t r = 0; r < ccount[c]; ++r) g.cnb[cur[v]++] = cids[5 * c + 1 + r];
     }
   uint32_t k = greedy_colors(s, g.cnb_off.data(), g.cnb.data(), grab, g.color_of, g.order, g.color_off);
   g.b.order.upload(g.order, g.stream);
@@ -1418,6 +1412,7 @@ void cv_gpu_destroy(cv_gpu* g) {
   if (g->h_scal) (void)hipHostFree(g->h_scal);
   if (g->h_x) (void)hipHostFree(g->h_x);
   if (g->h_v) (void)hipHostFree(g->h_v);
+  if (g->h_cont) (void)hipHostFree(g->h_cont);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
   if (g->ev1) (void)hipEventDestroy(g->ev1);
   if (g->join_ev) (void)hipEventDestroy(g->join_ev);
