@@ -42,10 +42,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
 
 
-def _traffic():
+def _traffic(kernel):
     """HBM bytes per sweep launch from the committed rocprofv3 counter runs
     (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "k_sweep_sheet_on_sphere_128_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_sheet_on_sphere_128_traffic.json")
     try:
         with open(path) as f:
             return json.load(f).get("bytes_per_launch")
@@ -76,7 +76,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", type=int, default=64, choices=(32, 64),
                     help="sweep arithmetic (64: f64 default; 32: f32 opt-in)")
-    ap.add_argument("--lanes", type=int, default=32, choices=(8, 16, 32, 64))
+    ap.add_argument("--lanes", type=int, default=32, choices=(8, 16, 32, 64),
+                    help="lanes per vertex of the sweep kernel")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host CPUs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -86,7 +87,8 @@ def main():
                     help="sewn_garment: two ~50k-vertex Poisson-disk panels with side and shoulder "
                          "seams on a capsule body, 40 iterations; batch64: 64 independent sheets and "
                          "sewn tubes of 1k-50k vertices in batched steps (not the headline line)")
-    ap.add_argument("--autotune", action="store_true", help="pick the lanes per vertex by timing")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batch64: use --lanes instead of timing the lanes per vertex first")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -164,6 +166,7 @@ def main():
     bytes_per_launch = m["sweep_timed_bytes_min"] / timed
     achieved = bytes_per_launch / avg_launch_s / 1e9
     sweep_s = avg_launch_s * launches
+    kernel = "k_sweep"
 
     out = {
         "metric": "steps/s + vertex-iters/s at fixed residual; warm-start vs plain iters-to-tol",
@@ -204,14 +207,15 @@ def main():
         "first_contact_by_step": None if first_contact is None else first_contact + 10,
         "sweep_share_of_step": sweep_s / dev_s,
         "roofline": {
-            "kernel": "k_sweep",
+            "kernel": kernel,
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": _traffic() if (args.workload == "sheet_on_sphere" and args.nx == 128
-                                       and args.lanes == 32 and args.precision == 64) else None,
+            "traffic": _traffic(kernel) if (args.workload == "sheet_on_sphere" and args.nx == 128
+                                             and args.precision == 64
+                                             and args.lanes == 32) else None,
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_us": 1e6 * avg_launch_s,
             "launches": m["sweep_launches"],
@@ -252,7 +256,7 @@ def bench_batch(args):
     gpu = GpuVbd(0)
     gpu.set_precision(args.precision)
     tuned = None
-    if args.autotune:
+    if not args.no_autotune:
         tuned = autotune.tune_lanes(gpu, scs, settings, repeats=1)
     else:
         gpu.set_lanes(args.lanes)

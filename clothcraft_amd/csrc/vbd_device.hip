@@ -139,21 +139,34 @@ __device__ __forceinline__ R local_energy(const SweepArgs<R>& a, const LaneTerm<
   return group_sum<L>(e);
 }
 
-template <int L, class R>
+// Derivatives at x, with acc.e replaced by the local energy at x as
+// local_energy computes it (same routine, same per-lane order and the same
+// reduction), so it serves as the line search's reference energy without a
+// second pass over the terms.  E = false leaves acc.e undefined.
+template <int L, class R, bool E = true>
 __device__ __forceinline__ cvm::Deriv<R> local_deriv(const SweepArgs<R>& a, const LaneTerm<R>& lt, uint32_t v, V3<R> x,
                                      int lane, uint32_t s0, uint32_t ns, uint32_t c0, uint32_t nt) {
   auto P = [&](uint32_t i) { return ldp(a.pos, i); };
-  cvm::Deriv<R> acc = lt.live ? cvm::term_deriv<R>(lt.rec, x, a.cr, a.ck, P) : cvm::deriv_zero<R>();
+  cvm::Deriv<R> acc = cvm::deriv_zero<R>();
+  R e = R(0);
+  if (lt.live) {
+    acc = cvm::term_deriv<R>(lt.rec, x, a.cr, a.ck, P);
+    if (E) e = cvm::term_energy<R>(lt.rec, x, a.cr, a.ck, P);
+  }
   for (uint32_t i = lane + L; i < nt; i += L) {
     const cvm::TermRecT<R>& t = i < ns ? a.terms[s0 + i] : a.crec[c0 + (i - ns)];
     cvm::deriv_add(acc, cvm::term_deriv<R>(t, x, a.cr, a.ck, P));
+    if (E) e += cvm::term_energy<R>(t, x, a.cr, a.ck, P);
   }
   if (lane == 0) {
     V3<R> d;
     R wv;
-    if (grab_d(a.grab, v, x, a.pos, d, wv)) cvm::deriv_add(acc, cvm::stitch_d<R>(d, wv, a.grab.k));
+    if (grab_d(a.grab, v, x, a.pos, d, wv)) {
+      cvm::deriv_add(acc, cvm::stitch_d<R>(d, wv, a.grab.k));
+      if (E) e += R(0.5) * a.grab.k * cvm::dot(d, d);
+    }
   }
-  acc.e = group_sum<L>(acc.e);
+  if (E) acc.e = group_sum<L>(e);
   acc.g.x = group_sum<L>(acc.g.x);
   acc.g.y = group_sum<L>(acc.g.y);
   acc.g.z = group_sum<L>(acc.g.z);
@@ -182,7 +195,7 @@ __device__ __forceinline__ void solve_vertex(const SweepArgs<R>& a, uint32_t v, 
   // The accept test compares energies from one evaluation routine, so a
   // candidate equal to x reproduces `old` exactly.
   V3<R> dx = x - target;
-  R old = local_energy<L>(a, lt, v, x, lane, s0, ns, c0, nt) + R(0.5) * inertia * cvm::dot(dx, dx);
+  R old = acc.e + R(0.5) * inertia * cvm::dot(dx, dx);
   V3<R> delta = cvm::newton_direction<R>(acc, x, target, inertia);
   R gg = cvm::dot(acc.g, acc.g);
   T4 b4 = a.ball[v];
@@ -335,7 +348,7 @@ __global__ __launch_bounds__(TPB) void k_true_grad(SweepArgs<R> a, uint32_t n, d
   uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
   uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
   const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt);
-  cvm::Deriv<R> acc = local_deriv<L>(a, lt, v, x, lane, s0, ns, c0, nt);
+  cvm::Deriv<R> acc = local_deriv<L, R, false>(a, lt, v, x, lane, s0, ns, c0, nt);
   if (lane == 0) {
     T4 tg = a.tgt[v];
     V3<R> g = acc.g + (x - V3<R>{tg.x, tg.y, tg.z}) * tg.w;

@@ -273,6 +273,14 @@ CV_HD void deriv_add(Deriv<R>& a, const Deriv<R>& b) {
 // src/energy.rs:10-15 with derivatives of the smooth tension/compression split.
 template <class R>
 CV_HD void axial3(R s, R k, R c, R delta, R& e, R& e1, R& e2) {
+  if (c == R(1) && delta > R(0)) {
+    // Pure tension law: the smoothed-compression part is multiplied by
+    // 1 - c = 0 and finite (r >= delta > 0), so these are the same values.
+    e = R(0.5) * k * (c * s * s);
+    e1 = k * (c * s);
+    e2 = k * c;
+    return;
+  }
   R r = sqrt(s * s + delta * delta);
   R p = (s + r - delta) * R(0.5);
   R p1 = R(0.5) * (R(1) + s / r);
@@ -283,6 +291,7 @@ CV_HD void axial3(R s, R k, R c, R delta, R& e, R& e1, R& e2) {
 }
 template <class R>
 CV_HD R axial_e(R s, R k, R c, R delta) {
+  if (c == R(1) && delta > R(0)) return R(0.5) * k * (c * s * s);  // as in axial3
   R r = sqrt(s * s + delta * delta);
   R p = (s + r - delta) * R(0.5);
   return R(0.5) * k * (c * s * s + (R(1) - c) * p * p);
@@ -340,10 +349,11 @@ CV_HD R bending_e(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, const F* f) {
   return R(0.5) * R(f[0]) * dot(d, d);
 }
 // d^2 nhat_k / dn_i dn_j contracted with y: T_ij = (3 (y.n) n_i n_j - y_i n_j - y_j n_i - (y.n) d_ij) / l^2
+// (il = 1 / l).
 template <class R>
-CV_HD M3<R> unit_normal_curv(V3<R> nh, V3<R> y, R l) {
+CV_HD M3<R> unit_normal_curv(V3<R> nh, V3<R> y, R il) {
   R yn = dot(y, nh);
-  R il2 = R(1) / (l * l);
+  R il2 = il * il;
   R n[3] = {nh.x, nh.y, nh.z}, yy[3] = {y.x, y.y, y.z};
   M3<R> t;
   for (int i = 0; i < 3; ++i)
@@ -358,7 +368,9 @@ CV_HD Deriv<R> bending_d(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, int slot, const
   V3<R> n1 = cross(e, u);
   V3<R> n2 = cross(w, e);
   R l1 = sqrt(dot(n1, n1)), l2 = sqrt(dot(n2, n2));
-  V3<R> a = n1 * (R(1) / l1), b = n2 * (R(1) / l2);
+  // One division per normal; the Jacobian and curvature reuse the reciprocals.
+  R il1 = R(1) / l1, il2 = R(1) / l2;
+  V3<R> a = n1 * il1, b = n2 * il2;
   V3<R> D = a - b;
   // dn1 = a1 x delta, dn2 = a2 x delta for the moving slot.
   V3<R> a1, a2;
@@ -389,7 +401,7 @@ CV_HD Deriv<R> bending_d(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, int slot, const
         s1 += p1 * A1.m[k][j];
         s2 += p2 * A2.m[k][j];
       }
-      J.m[i][j] = s1 / l1 - s2 / l2;
+      J.m[i][j] = s1 * il1 - s2 * il2;
     }
   Deriv<R> d;
   d.e = R(0.5) * kappa * dot(D, D);
@@ -398,8 +410,8 @@ CV_HD Deriv<R> bending_d(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, int slot, const
   for (int j = 0; j < 3; ++j) g[j] = kappa * (J.m[0][j] * Dn[0] + J.m[1][j] * Dn[1] + J.m[2][j] * Dn[2]);
   d.g = mk<R>(g[0], g[1], g[2]);
   M3<R> JJ = mul_tn(J, J);
-  M3<R> T1 = unit_normal_curv(a, D, l1);
-  M3<R> T2 = unit_normal_curv(b, D, l2);
+  M3<R> T1 = unit_normal_curv(a, D, il1);
+  M3<R> T2 = unit_normal_curv(b, D, il2);
   M3<R> C1 = mul_tn(A1, mul(T1, A1));
   M3<R> C2 = mul_tn(A2, mul(T2, A2));
   int k = 0;

@@ -319,3 +319,21 @@ def test_one_step_parity_sewn_garment_with_seams_and_body(gpu):
         assert abs(mg["max_seam_gap_m"] - mr["max_seam_gap_m"]) <= 1e-12 + 1e-9 * mr["max_seam_gap_m"]
         st = a
         sol.record_prediction(prev_v, st, s)
+
+
+
+@pytest.mark.parametrize("lanes", [8, 16, 64])
+def test_sweep_lane_shapes_match_oracle(gpu, lanes):
+    """Every lanes-per-vertex shape of the sweep (32 is the default the other
+    tests use) gives the oracle's step on a contact scene and on the hanging
+    sheet."""
+    gpu.set_lanes(lanes)
+    try:
+        for sc, tol in ((scenes.small_sheet_on_sphere(24, 0.2535), 1e-5), (scenes.hanging_sheet(), 1e-9)):
+            sg, sr, mg, mr = _one_step(gpu, sc)
+            err = np.abs(sg.x - sr.x).max() / _bbox_diag(sr.x)
+            print(f"{sc.name} lanes {lanes} rel err {err:.3e}")
+            assert mg["detection_updates"] == mr["detection_updates"]
+            assert err <= tol
+    finally:
+        gpu.set_lanes(32)
