@@ -35,13 +35,16 @@ struct DBuf {
   ~DBuf() {
     if (p) (void)hipFree(p);
   }
+  // Grows by at least half the current capacity, so sizes that creep up
+  // step by step (contact and pair counts) reallocate rarely: hipFree waits
+  // for the device.
   void reserve(size_t count) {
     if (count <= n) return;
     if (p) CV_HIP(hipFree(p));
     p = nullptr;
-    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    CV_HIP(hipMalloc(&p, bytes));
-    n = count;
+    size_t cap = std::max(count, n + n / 2);
+    CV_HIP(hipMalloc(&p, std::max<size_t>(cap, 1) * sizeof(T)));
+    n = cap;
   }
   void upload(const T* src, size_t count, hipStream_t s) {
     reserve(count);
@@ -128,6 +131,7 @@ struct StepBufs {
   DBuf<uint32_t> bcount, bstart;  // hash-grid buckets [2T+1]
   DBuf<uint32_t> item_len, item_off, item_start, over, pair_owner;  // flattened broad phase
   DBuf<uint2> pair_list;  // candidates for the narrow phase: VF from the front, EE from the back
+  DBuf<uint32_t> scan_sums;  // tile totals of the two-level scan
   DBuf<unsigned long long> item_key;
   uint64_t pair_cap = 0;
   uint64_t grid_cap = 0, grid_last = 0;  // registration capacity; last detection's count

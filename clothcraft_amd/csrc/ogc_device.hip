@@ -676,6 +676,53 @@ __global__ void k_radii(const unsigned long long* bits, uint32_t n, double relax
 
 inline unsigned blocks(uint64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
 
+// Exclusive scan of uint32 counts in tiles of 4096 (1024 threads x 4): one
+// launch up to 4096 elements, three (tiles, scan of tile sums, add) up to
+// 4096^2.  rocPRIM's scan queries device properties on every call, host time
+// that the detection's chain of short kernels would wait for.
+constexpr uint32_t kScanTile = 4096;
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t* in, uint32_t* out, uint32_t count,
+                                                     uint32_t* sums) {
+  __shared__ uint32_t wsum[16];
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4;
+  uint32_t v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = base + k < count ? in[base + k] : 0u;
+  const uint32_t s = v[0] + v[1] + v[2] + v[3];
+  const uint32_t incl = wave_incl_scan(s);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  if (w == 0) {
+    uint32_t x = lane < 16 ? wsum[lane] : 0u;
+    x = wave_incl_scan(x);
+    if (lane < 16) wsum[lane] = x;
+  }
+  __syncthreads();
+  uint32_t e = incl - s + (w ? wsum[w - 1] : 0u);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < count) out[base + k] = e;
+    e += v[k];
+  }
+  if (sums && threadIdx.x == 1023) sums[blockIdx.x] = wsum[15];
+}
+__global__ __launch_bounds__(1024) void k_scan_add(uint32_t* out, uint32_t count, const uint32_t* sums) {
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4, add = sums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k < count) out[base + k] += add;
+}
+
 template <class F>
 void with_temp(cv_gpu& g, F&& f) {
   size_t bytes = 0;
@@ -683,6 +730,25 @@ void with_temp(cv_gpu& g, F&& f) {
   g.b.temp.reserve(bytes + 256);
   bytes = g.b.temp.n;
   CV_HIP(f((void*)g.b.temp.p, bytes));
+}
+
+void scan_u32(cv_gpu& g, const uint32_t* in, uint32_t* out, size_t count) {
+  hipStream_t st = g.stream;
+  if (count == 0) return;
+  if (count <= kScanTile) {
+    k_scan_tiles<<<1, 1024, 0, st>>>(in, out, (uint32_t)count, nullptr);
+  } else if (count <= (size_t)kScanTile * kScanTile) {
+    uint32_t tiles = (uint32_t)((count + kScanTile - 1) / kScanTile);
+    g.b.scan_sums.reserve(tiles);
+    k_scan_tiles<<<tiles, 1024, 0, st>>>(in, out, (uint32_t)count, g.b.scan_sums.p);
+    k_scan_tiles<<<1, 1024, 0, st>>>(g.b.scan_sums.p, g.b.scan_sums.p, tiles, nullptr);
+    k_scan_add<<<tiles, 1024, 0, st>>>(out, (uint32_t)count, g.b.scan_sums.p);
+  } else {
+    with_temp(g, [&](void* t, size_t& bytes) {
+      return rocprim::exclusive_scan(t, bytes, in, out, 0u, count, rocprim::plus<uint32_t>(), st);
+    });
+  }
+  CV_HIP(hipGetLastError());
 }
 
 void read_scalars(cv_gpu& g) {
@@ -704,10 +770,7 @@ int build_grid(cv_gpu& g, uint32_t F, uint32_t E, double pr, double h) {
   k_reg_count<<<blocks(N), TPB, 0, st>>>(b.fbox.p, b.ebox.p, F, E, pr, h, b.cell_count.p);
   CV_HIP(hipGetLastError());
   CV_HIP(hipMemsetAsync(b.cell_count.p + N, 0, sizeof(uint32_t), st));
-  with_temp(g, [&](void* t, size_t& bytes) {
-    return rocprim::exclusive_scan(t, bytes, b.cell_count.p, b.cell_off.p, 0u, (size_t)N + 1,
-                                   rocprim::plus<uint32_t>(), st);
-  });
+  scan_u32(g, b.cell_count.p, b.cell_off.p, (size_t)N + 1);
   if (b.grid_cap == 0) {
     // First detection of this context: read the exact count once.
     uint32_t M0 = 0;
@@ -735,10 +798,7 @@ int build_grid(cv_gpu& g, uint32_t F, uint32_t E, double pr, double h) {
   k_reg_fill<<<blocks(cap), TPB, 0, st>>>(b.fbox.p, b.ebox.p, F, pr, h, b.cell_off.p, N, b.cell_owner.p, cap,
                                           log2t, b.cell_key.p, b.cell_val.p, b.cell_rank.p, b.bcount.p, g.d_scal);
   CV_HIP(hipGetLastError());
-  with_temp(g, [&](void* t, size_t& bytes) {
-    return rocprim::exclusive_scan(t, bytes, b.bcount.p, b.bstart.p, 0u, T2 + 1, rocprim::plus<uint32_t>(),
-                                   st);
-  });
+  scan_u32(g, b.bcount.p, b.bstart.p, T2 + 1);
   k_reg_scatter<<<blocks(cap), TPB, 0, st>>>(b.cell_key.p, b.cell_val.p, b.cell_rank.p, b.cell_off.p, F, N, cap,
                                              log2t, b.bstart.p, b.fbox.p, b.ebox.p, b.cell_key_sorted.p,
                                              b.cell_val_sorted.p, b.cell_box_sorted.p);
@@ -833,10 +893,7 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   k_items<<<blocks(d.n_total + d.E), TPB, 0, st>>>(a, b.item_len.p, b.item_start.p, b.item_key.p, b.over.p,
                                                    &g.d_scal->n_over);
   CV_HIP(hipGetLastError());
-  with_temp(g, [&](void* t, size_t& bytes) {
-    return rocprim::exclusive_scan(t, bytes, b.item_len.p, b.item_off.p, 0u, (size_t)NI + 1,
-                                   rocprim::plus<uint32_t>(), st);
-  });
+  scan_u32(g, b.item_len.p, b.item_off.p, (size_t)NI + 1);
   // Pair-slot capacity: grown from the last detection's need; an overflow is
   // flagged on the device and the detection is run again with more room.
   if (b.pair_cap == 0) b.pair_cap = std::max<uint64_t>(4ull * b.grid_cap + (1u << 20), 1u << 22);
@@ -892,10 +949,7 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
     // b.cell_key now holds hi keys in final order; perm the permutation.
     k_unique_flags<<<blocks(m), TPB, 0, st>>>(b.cell_key.p, b.key_lo.p, b.perm.p, m, b.uniq_flag.p);
     CV_HIP(hipMemsetAsync(b.uniq_flag.p + m, 0, sizeof(uint32_t), st));
-    with_temp(g, [&](void* t, size_t& bytes) {
-      return rocprim::exclusive_scan(t, bytes, b.uniq_flag.p, b.uniq_pos.p, 0u, (size_t)m + 1,
-                                     rocprim::plus<uint32_t>(), st);
-    });
+    scan_u32(g, b.uniq_flag.p, b.uniq_pos.p, (size_t)m + 1);
     k_unique_write<<<blocks(m), TPB, 0, st>>>(b.raw_ids.p, b.raw_count.p, b.raw_eps.p, b.perm.p,
                                               b.uniq_flag.p, b.uniq_pos.p, m, b.c_ids.p, b.c_count.p,
                                               b.c_eps.p);
@@ -928,10 +982,7 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
     k_contact_post<<<blocks(m), TPB, 0, st>>>(b.x64.p, b.c_ids.p, b.c_count.p, b.c_eps.p, d.n, ogc.radius,
                                               ogc.stiffness, b.vcount.p, b.inc_key.p, g.d_scal);
   }
-  with_temp(g, [&](void* t, size_t& bytes) {
-    return rocprim::exclusive_scan(t, bytes, b.vcount.p, b.coff.p, 0u, (size_t)d.n + 1,
-                                   rocprim::plus<uint32_t>(), st);
-  });
+  scan_u32(g, b.vcount.p, b.coff.p, (size_t)d.n + 1);
   b.radii64.reserve(d.n);
   k_radii<<<1, 1024, 0, st>>>(b.dist_bits.p, d.n, ogc.relaxation, rq, b.radii64.p, g.d_scal);
   read_scalars(g);

@@ -823,7 +823,8 @@ struct StepCtx {
   };
   std::vector<Launch> launches;
   std::vector<uint32_t> timed_its;  // iterations whose launches carry events
-  size_t nev = 0;
+  std::vector<size_t> pending_ev;   // completed timed launches not yet read
+  size_t nev = 0;                   // event pairs used in this step
   static constexpr uint64_t kTimedStride = 4;
 
   StepCtx(cv_gpu& g_, cv_solver& s_, const cv_settings* set_, const cv_grab* grab_, StepOut& out_)
@@ -896,9 +897,7 @@ struct StepCtx {
       if (l.it >= ran) continue;
       out.sweep_launches++;
       if (g.profile && l.ev != (size_t)-1) {
-        float ms = 0;
-        CV_HIP(hipEventElapsedTime(&ms, g.sweep_events[2 * l.ev], g.sweep_events[2 * l.ev + 1]));
-        out.sweep_ms += ms;
+        pending_ev.push_back(l.ev);  // read by drain_timing() once more work is queued
         out.sweep_timed_launches++;
       }
     }
@@ -906,7 +905,6 @@ struct StepCtx {
       if (it < ran) out.sweep_timed_bytes_min += it_bytes32;
     launches.clear();
     timed_its.clear();
-    nev = 0;
     out.sweep_bytes += (ran - done) * (sizeof(R) == 4 ? it_bytes32 : g.static_bytes64 + 100.0 * inc);
     out.sweep_bytes_min += (ran - done) * it_bytes32;
     done = ran;
@@ -960,6 +958,16 @@ struct StepCtx {
       end_iteration(it);
     }
   }
+  // Event times of completed timed launches; called after the next work is
+  // queued so the host's event queries overlap the device's work.
+  void drain_timing() {
+    for (size_t ev : pending_ev) {
+      float ms = 0;
+      CV_HIP(hipEventElapsedTime(&ms, g.sweep_events[2 * ev], g.sweep_events[2 * ev + 1]));
+      out.sweep_ms += ms;
+    }
+    pending_ev.clear();
+  }
   void end_iteration(uint32_t it) {
     if (tol > 0.0) true_residual<R>(g, a, s.n);
     k_iter_end<<<1, 1, 0, g.stream>>>(g.d_scal, (int)it, threshold, tol, nfree3);
@@ -1002,8 +1010,10 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
     CV_HIP(hipStreamSynchronize(g.stream));
     if (!c.after_sync(first, end)) break;
     c.enqueue(first, end);
+    c.drain_timing();
   }
   c.finish();
+  c.drain_timing();
   read_scalars(g);
   c.check();
 }
