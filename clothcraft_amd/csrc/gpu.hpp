@@ -263,6 +263,11 @@ struct cv_gpu {
   size_t h_cont_cap = 0;
   std::vector<uint32_t> cnb_off, cnb;
   std::vector<uint32_t> color_of, order, color_off;
+  // Inputs of the last colouring, to reuse it when they repeat.
+  bool color_valid = false;
+  uint64_t color_generation = 0, color_m = 0;
+  uint32_t color_key[4] = {0, 0, 0, 0};
+  std::vector<uint32_t> color_cids;  // contact kinds/ids then counts
   // Sweep launch shape (lanes per vertex), chosen by the autotuner.
   int lanes = 32;
   bool pred_valid = false;

@@ -702,6 +702,19 @@ uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
   const uint32_t* cids = g.h_cont;
   const uint32_t* ccount = g.h_cont + 5 * g.h_cont_cap;
   uint32_t n = s.n;
+  // The colouring is a function of the solver, the contact list and the grab
+  // vertices: when all three equal the last colouring's, it is reused (the
+  // order already sits on the device).  Contact-free steps always reuse it.
+  uint32_t key[4] = {grab ? 1u : 0u, grab ? grab->vertices[0] : 0u, grab ? grab->vertices[1] : 0u,
+                     grab ? grab->vertices[2] : 0u};
+  if (g.color_valid && g.color_generation == s.generation && g.color_m == m &&
+      std::memcmp(g.color_key, key, sizeof key) == 0 &&
+      std::equal(cids, cids + 5 * m, g.color_cids.begin()) &&
+      std::equal(ccount, ccount + m, g.color_cids.begin() + 5 * m))
+    return (uint32_t)(g.color_off.size() - 1);
+  g.color_valid = false;
+  g.color_cids.assign(cids, cids + 5 * m);
+  g.color_cids.insert(g.color_cids.end(), ccount, ccount + m);
   g.cnb_off.assign(n + 1, 0);
   for (uint64_t c = 0; c < m; ++c)
     for (uint32_t q = 0; q < ccount[c]; ++q) {
@@ -719,6 +732,10 @@ uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
     }
   uint32_t k = greedy_colors(s, g.cnb_off.data(), g.cnb.data(), grab, g.color_of, g.order, g.color_off);
   g.b.order.upload(g.order, g.stream);
+  g.color_valid = true;
+  g.color_generation = s.generation;
+  g.color_m = m;
+  std::memcpy(g.color_key, key, sizeof key);
   return k;
 }
 

@@ -74,6 +74,9 @@ def test_f64_kernel_hessians_match_finite_differences(sheet):
     (1, [[0.1, 0.004, 0.002], [-1.0, 0.0, 0.0], [1.0, 0.0, 0.0]], 3),
     (2, [[0.1, 0.1, 0.004], [-1.0, -1.0, 0.0], [1.0, -1.0, 0.0], [0.0, 1.0, 0.0]], 4),
     (3, [[-1.0, 0.01, 0.004], [1.0, 0.0, 0.004], [0.0, -1.0, 0.0], [0.02, 1.0, 0.0]], 4),
+    # quadratic part of the barrier (d between r/2 and r), point-point and point-face
+    (0, [[0.1, 0.101, 0.007], [0.1, 0.1, 0.0]], 2),
+    (2, [[0.1, 0.1, -0.007], [-1.0, -1.0, 0.0], [1.0, -1.0, 0.0], [0.0, 1.0, 0.0]], 4),
 ])
 def test_contact_kernel_matches_oracle_block(kind, pts, count):
     x = np.zeros((4, 3))
