@@ -147,6 +147,8 @@ def main():
         done += k
         if first_contact is None and m["contact_terms"] > 0:
             first_contact = done - k
+        if done % 50 == 0:
+            print(f"[bench] {done}/{args.steps} steps", file=sys.stderr, flush=True)
     wall_s = time.perf_counter() - t0
     gpu.set_profiling(False)
     if not np.all(np.isfinite(st.x)):
