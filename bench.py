@@ -83,7 +83,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nx", type=int, default=128, help="quads per side (128 is the workload)")
     ap.add_argument("--workload", default="sheet_on_sphere",
-                    choices=("sheet_on_sphere", "sewn_garment", "batch64"),
+                    choices=("sheet_on_sphere", "sewn_garment", "batch64", "warm_infer"),
                     help="sewn_garment: two ~50k-vertex Poisson-disk panels with side and shoulder "
                          "seams on a capsule body, 40 iterations; batch64: 64 independent sheets and "
                          "sewn tubes of 1k-50k vertices in batched steps (not the headline line)")
@@ -112,6 +112,8 @@ def main():
 
     if args.workload == "batch64":
         return bench_batch(args)
+    if args.workload == "warm_infer":
+        return bench_warm_infer(args)
 
     if args.workload == "sewn_garment":
         from clothcraft_amd import garments
@@ -248,6 +250,31 @@ def main():
             "vertex_iters_per_s": n_free * settings["iterations"] / per_step,
         }
     print(json.dumps(out), flush=True)
+
+
+def bench_warm_infer(args):
+    """Warm-start network inference (BF16 MFMA) on 4 Mi feature rows per
+    call.  The line's rate includes the host transfers of cv_warm_infer; the
+    kernel's own duration comes from a kernel trace of this command."""
+    from clothcraft_amd import GpuVbd, warmstart
+    n = 1 << 22
+    gpu = GpuVbd(0)
+    gpu.warm.set_params(warmstart.init_params(0))
+    X = np.random.default_rng(0).normal(size=(n, 12)).astype(np.float32)
+    for _ in range(max(args.warmup, 1)):
+        gpu.warm.infer(X)
+    K = args.steps
+    t0 = time.perf_counter()
+    for k in range(K):
+        gpu.warm.infer(X)
+    wall = time.perf_counter() - t0
+    flop = 69376.0 * n  # 2 (12*128 + 128*128 + 128*128 + 128*3) per row
+    print(json.dumps({
+        "metric": "warm-start inference rows/s (host transfers included)", "value": K * n / wall,
+        "unit": "rows/s", "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": 1e3 * wall / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic normal features", "config": {"workload": f"MLP 12-128-128-128-3 GELU, {n} rows per call"},
+        "algorithmic_flop_per_call": flop}), flush=True)
 
 
 def bench_batch(args):

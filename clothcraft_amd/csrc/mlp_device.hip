@@ -31,12 +31,13 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f4;
 
 constexpr int ROW = 136;  // LDS row stride (bf16 elements)
 
-// Round-to-nearest-even FP32 -> BF16 (bit pattern), as the CPU checker does.
+// Round-to-nearest-even FP32 -> BF16 (bit pattern), as the CPU checker does:
+// one v_cvt_pk_bf16_f32 on gfx950.
 __device__ __forceinline__ uint16_t bf16_bits(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)(u >> 16 | ((u & 0xFFFFu) ? 0x40u : 0u));
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+  __bf16 h = (__bf16)f;
+  uint16_t u;
+  __builtin_memcpy(&u, &h, 2);
+  return u;
 }
 __device__ __forceinline__ float bf16_float(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
@@ -48,9 +49,32 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU z Phi(z) with Phi(z) = (1 + erf(z / sqrt 2)) / 2, and its derivative
+// Phi(z) + z phi(z).  erf is evaluated with Abramowitz & Stegun 7.1.25
+// (|error| <= 2.5e-5, two orders below the BF16 rounding (2^-9 relative) the
+// activations go through): erf(x) = 1 - (a1 t + a2 t^2 + a3 t^3) exp(-x^2),
+// t = 1 / (1 + p x), x >= 0; exp(-x^2) = exp(-z^2 / 2) is shared with phi(z).
+// The inference kernel is bound by this VALU work, not by the matrix cores.
+struct GeluParts {
+  float cdf, pdf;  // Phi(z), phi(z)
+};
+__device__ __forceinline__ GeluParts gelu_parts(float z) {
+  float x = fabsf(z) * 0.70710678118654752f;
+  float t = __builtin_amdgcn_rcpf(fmaf(0.47047f, x, 1.0f));  // v_rcp_f32, 1 ulp
+  float poly = fmaf(fmaf(0.7478556f, t, -0.0958798f), t, 0.3480242f) * t;
+  // exp(-z^2 / 2) = 2^(-z^2 log2(e) / 2) with the bare v_exp_f32 (results
+  // below 2^-126 flush to 0, where the GELU terms vanish anyway).
+  float e = __builtin_amdgcn_exp2f(z * z * -0.72134752044448170f);
+  float erfx = fmaf(-poly, e, 1.0f);
+  GeluParts g;
+  g.cdf = fmaf(0.5f, copysignf(erfx, z), 0.5f);
+  g.pdf = 0.39894228040143268f * e;
+  return g;
+}
+__device__ __forceinline__ float gelu(float z) { return z * gelu_parts(z).cdf; }
 __device__ __forceinline__ float gelu_d(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+  GeluParts g = gelu_parts(z);
+  return g.cdf + z * g.pdf;
 }
 
 __device__ __forceinline__ f4 mfma(bf16x8 a, bf16x8 b, f4 c) {
@@ -144,22 +168,73 @@ __device__ __forceinline__ void load_inputs(const float* X, uint32_t n, uint32_t
   }
 }
 
-// Inference: 4 wavefronts per block, each a 16-row tile at a time.
-__global__ __launch_bounds__(256) void k_mlp_infer(Weights W, const float* X, uint32_t n, float* Y) {
-  __shared__ __attribute__((aligned(16))) uint16_t buf[4][4][16 * ROW];
-  __shared__ float ybuf[4][16 * OUT];
-  int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  uint16_t* act[4] = {buf[wave][0], buf[wave][1], buf[wave][2], buf[wave][3]};
-  uint16_t* none[3] = {nullptr, nullptr, nullptr};
-  uint32_t tiles = (n + 15) / 16, step = gridDim.x * 4;
-  // Uniform trip count for the block; a wave past the last tile idles.
-  for (uint32_t b = blockIdx.x * 4; b < tiles; b += step) {
-    uint32_t t = b + wave;
-    load_inputs(X, t < tiles ? n : 0, t * 16, act[0], lane);
+// Inference.  The forward weight tiles (77.8 KB of pre-swizzled BF16) and
+// biases are staged in LDS once per block; each of the block's 8 wavefronts
+// then runs 16-row tiles on its own (no block barriers), with the output
+// column tiles of a layer accumulating independently (up to 8 MFMA chains
+// in flight) and two wavefronts per SIMD, so one wavefront's GELU work
+// overlaps the other's matrix-core work.  The LDS footprint holds one block
+// per CU; the grid strides over the row tiles.
+constexpr int IW = 8;  // wavefronts per inference block
+__device__ __forceinline__ bf16x8 lds_w(const uint16_t* w, int tile, int lane) {
+  return *(const bf16x8*)(w + (tile * 64 + lane) * 8);
+}
+// One 16-row tile through a hidden layer: out = gelu(in W + b), KC K-chunks
+// of 32, 8 column tiles of 16.
+template <int KC>
+__device__ __forceinline__ void hidden_layer(const uint16_t* in, const uint16_t* w, const float* bias, uint16_t* out,
+                                             int lane) {
+  f4 acc[8];
+#pragma unroll
+  for (int nt = 0; nt < 8; ++nt) acc[nt] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kc = 0; kc < KC; ++kc) {
+    bf16x8 a = lds_a(in, kc, lane);
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt) acc[nt] = mfma(a, lds_w(w, kc * 8 + nt, lane), acc[nt]);
+  }
+#pragma unroll
+  for (int nt = 0; nt < 8; ++nt) {
+    int c = nt * 16 + lane % 16;
+    float bc = bias[c];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[(4 * (lane / 16) + j) * ROW + c] = bf16_bits(gelu(acc[nt][j] + bc));
+  }
+}
+__global__ __launch_bounds__(64 * IW) void k_mlp_infer(Weights W, const float* X, uint32_t n, float* Y) {
+  __shared__ __attribute__((aligned(16))) uint16_t wl[FW_TOTAL];
+  __shared__ __attribute__((aligned(16))) uint16_t buf[IW][2][16 * ROW];
+  __shared__ float ybuf[IW][16 * OUT];
+  __shared__ float bl[3 * H + OUT];
+  {
+    const uint4* src = (const uint4*)W.f[0];  // the layers' tiles are contiguous from layer 0
+    uint4* dst = (uint4*)wl;
+    for (uint32_t i = threadIdx.x; i < FW_TOTAL / 8; i += blockDim.x) dst[i] = src[i];
+    for (uint32_t i = threadIdx.x; i < 3 * H + OUT; i += blockDim.x) bl[i] = W.bias[i / H < 3 ? i / H : 3][i % H];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  uint16_t* a = buf[wave][0];
+  uint16_t* b = buf[wave][1];
+  const uint32_t tiles = (n + 15) / 16;
+  for (uint32_t t = blockIdx.x * IW + wave; t < tiles; t += gridDim.x * IW) {
+    load_inputs(X, n, t * 16, a, lane);
     wave_sync();
-    forward_tile<false>(W, act, none, ybuf[wave], lane);
+    hidden_layer<1>(a, wl + FW_OFS[0], bl, b, lane);
     wave_sync();
-    if (lane < 16 * OUT && t < tiles) {
+    hidden_layer<4>(b, wl + FW_OFS[1], bl + H, a, lane);
+    wave_sync();
+    hidden_layer<4>(a, wl + FW_OFS[2], bl + 2 * H, b, lane);
+    wave_sync();
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc) acc = mfma(lds_a(b, kc, lane), lds_w(wl + FW_OFS[3], kc, lane), acc);
+    int c = lane % 16;
+    if (c < OUT)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) ybuf[wave][(4 * (lane / 16) + j) * OUT + c] = acc[j] + bl[3 * H + c];
+    wave_sync();
+    if (lane < 16 * OUT) {
       uint32_t s = t * 16 + lane / OUT;
       if (s < n) Y[(size_t)s * OUT + lane % OUT] = ybuf[wave][lane];
     }
@@ -401,8 +476,9 @@ void mlp_infer_device(cv_gpu& g, const float* dX, uint32_t n, float* dY) {
   if (!g.mlp.ready) fail(CV_ERR_NOT_BUILT, "warm-start network not set up");
   if (n == 0) return;
   unsigned tiles = (n + 15) / 16;
-  unsigned blocks = std::min<unsigned>((tiles + 3) / 4, 2048);
-  k_mlp_infer<<<blocks, 256, 0, g.stream>>>(weights_of(g), dX, n, dY);
+  // One block per CU fits in LDS; enough blocks to cover every CU.
+  unsigned blocks = std::min<unsigned>((tiles + IW - 1) / IW, 1024);
+  k_mlp_infer<<<blocks, 64 * IW, 0, g.stream>>>(weights_of(g), dX, n, dY);
   CV_HIP(hipGetLastError());
 }
 

@@ -18,6 +18,27 @@ import numpy as np
 from .vbd import GpuVbd, VbdSolver
 
 
+# Layer shapes in the order of the FP32 parameter vector (mlp.hpp):
+# W0 [12 x 128] (input-major), b0, W1 [128 x 128], b1, W2, b2, W3 [128 x 3], b3.
+SHAPES = [("W0", (12, 128)), ("b0", (128,)), ("W1", (128, 128)), ("b1", (128,)), ("W2", (128, 128)),
+          ("b2", (128,)), ("W3", (128, 3)), ("b3", (3,))]
+
+
+def init_params(seed: int = 0, out_scale: float = 0.1) -> np.ndarray:
+    """Fresh network parameters: He-normal weights, zero biases, the output
+    layer scaled by out_scale so an untrained network starts close to the
+    plain inertial guess."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for name, shape in SHAPES:
+        if name.startswith("W"):
+            w = rng.normal(0.0, np.sqrt(2.0 / shape[0]), size=shape)
+            parts.append(w * out_scale if name == "W3" else w)
+        else:
+            parts.append(np.zeros(shape))
+    return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
+
+
 def step_length(settings) -> float:
     g = np.linalg.norm(settings["gravity"])
     return max(g * settings["dt"] ** 2 / (1.0 + settings["damping"] * settings["dt"]), 1e-9)
