@@ -493,6 +493,8 @@ void validate_settings(const cv_settings* s) {  // src/vbd.rs:37-51, src/ogc/wor
     fail(CV_ERR_INVALID_SETTINGS, "Invalid VBD settings");
   if (!std::isfinite(s->residual_tolerance) || s->residual_tolerance < 0.0)
     fail(CV_ERR_INVALID_SETTINGS, "Invalid VBD settings");
+  if (s->flags & ~(uint32_t)(CV_FLAG_DIAGNOSTICS_OFF | CV_FLAG_TRUE_RESIDUAL | CV_FLAG_WARM_START))
+    fail(CV_ERR_INVALID_SETTINGS, "Invalid VBD settings");
 }
 
 void validate_state(const cv_solver& s, const cv_state* st) {  // src/vbd.rs:257-264

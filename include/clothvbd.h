@@ -134,7 +134,7 @@ typedef struct cv_settings {
 enum {
   CV_FLAG_DIAGNOSTICS_OFF = 1u << 0, /* skip max_seam_gap / extension_percent metrics      */
   CV_FLAG_TRUE_RESIDUAL = 1u << 1,   /* compute the post-sweep residual every iteration     */
-  CV_FLAG_FAST_COLORING = 1u << 2,   /* device coloring (not the reference greedy order)    */
+  /* 1u << 2 is reserved (rejected) */
   CV_FLAG_WARM_START = 1u << 3       /* learned initial guess in place of the adaptive one  */
 };
 

@@ -226,6 +226,18 @@ def test_invalid_grab_is_transactional(gpu):
     assert np.array_equal(st.x, old.x) and st.time == old.time
 
 
+def test_unknown_settings_flags_are_rejected_transactionally(gpu):
+    from clothcraft_amd import VbdError, VbdSolver
+    sc = scenes.free_panel(2, 2)
+    st = sc.state.copy()
+    sol = VbdSolver(sc.garment, None, st)
+    old = st.copy()
+    with pytest.raises(VbdError) as e:
+        gpu.step(sol, st, dict(sc.settings, flags=1 << 2))
+    assert e.value.message == "Invalid VBD settings"
+    assert np.array_equal(st.x, old.x) and st.time == old.time
+
+
 def test_grab_step_matches_oracle(gpu):
     sc = scenes.free_panel(6, 4)
     st = sc.state.copy()
