@@ -155,6 +155,17 @@ def main():
     gpu.set_profiling(False)
     if not np.all(np.isfinite(st.x)):
         raise RuntimeError("non-finite state after the timed steps")
+    # The reference's call pattern: one GpuVbd::step per frame, state in and
+    # out through host memory and the metrics (diagnostics included) each
+    # step; wall time per call, continuing from the final state on a copy.
+    api_st = st.copy()
+    api_sol = VbdSolver(sc.garment, sc.avatar, api_st)
+    api_ms = []
+    for _ in range(20):
+        t1 = time.perf_counter()
+        gpu.step(api_sol, api_st, settings)
+        api_ms.append(1e3 * (time.perf_counter() - t1))
+    api_ms = np.array(api_ms[2:])
     m = dict(m, **tot)
 
     K = args.steps
@@ -202,6 +213,8 @@ def main():
         "vertex_iters_per_s": vi / dev_s,
         "vertex_iters_per_s_sweep_only": vi / sweep_s if sweep_s > 0 else None,
         "steps_per_s_wall_incl_transfers": K / wall_s,
+        "step_api_wall_ms": {"median": float(np.median(api_ms)), "p95": float(np.percentile(api_ms, 95)),
+                             "calls": int(len(api_ms)), "warmup_calls": 2},
         "residual_rms_n_last_step": m["residual_rms_n"],
         "contacts_last_step": m["contact_terms"],
         "colors_last_step": m["colors"],

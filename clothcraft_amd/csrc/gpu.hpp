@@ -75,6 +75,8 @@ struct Scalars {
   unsigned long long cand_export_vf, cand_export_ee;
   unsigned long long vf_list_n, ee_list_n;  // candidates queued for the narrow phase
   unsigned long long grid_entries;          // hash-grid registrations of the last detection
+  double diag_p95, diag_max;                // extension percent of the committed positions
+  unsigned long long diag_gap_bits;         // max seam gap (bit pattern of a non-negative double)
   unsigned int err_degenerate, err_overflow, err_nonfinite, err_trust;
 };
 
@@ -107,6 +109,11 @@ struct DevSolver {
   DBuf<double> fixed_pos;  // avatar f64
   double fixed_abs_max = 0;
   double cell = 0;         // grid cell size for detection
+  // garment diagnostics: triangles, inverse rest shapes, grain cos/sin,
+  // areas, seam probes
+  uint32_t T = 0, P = 0;
+  DBuf<uint32_t> dg_tri, dg_pr_off, dg_pr_v;
+  DBuf<double> dg_inv, dg_cs, dg_area, dg_pr_w;
   // prediction
   DBuf<double> acc;
   uint64_t acc_version = 0;
@@ -132,6 +139,8 @@ struct StepBufs {
   DBuf<uint32_t> item_len, item_off, item_start, over, pair_owner;  // flattened broad phase
   DBuf<uint2> pair_list;  // candidates for the narrow phase: VF from the front, EE from the back
   DBuf<uint32_t> scan_sums;  // tile totals of the two-level scan
+  DBuf<unsigned long long> ext_key, ext_key_sorted;  // diagnostics: per-triangle extension
+  DBuf<double> ext_area, ext_area_sorted, ext_prefix;
   DBuf<unsigned long long> item_key;
   uint64_t pair_cap = 0;
   uint64_t grid_cap = 0, grid_last = 0;  // registration capacity; last detection's count
@@ -318,6 +327,11 @@ void mlp_adam_device(cv_gpu& g, float lr);
 // Detection on device positions b.x64 (all n_total vertices) with query
 // radius rq; fills distance, radii, unique contacts and contact records.
 // Candidate pairs are also exported when cand_cap > 0.
+// Garment diagnostics of the positions dx (moving vertices, f64) on the
+// device into the scalars: max seam gap and the area-weighted 95th percentile
+// and maximum of the positive axial extension.
+void gpu_diagnostics(cv_gpu& g, const double* dx);
+
 void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings& ogc,
                 size_t cand_cap);
 void ensure_device_solver(cv_gpu& g, const cv_solver& s);

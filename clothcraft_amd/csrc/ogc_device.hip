@@ -806,6 +806,49 @@ int build_grid(cv_gpu& g, uint32_t F, uint32_t E, double pr, double h) {
   return log2t;
 }
 
+// Garment diagnostics (src/garment/diagnostics.rs:7-28, src/garment/template.rs:536-538),
+// compiled without FMA contraction so each triangle's value rounds as the
+// host formula does.  Positive axial extension of every triangle in %, as a
+// key whose order is the value's (non-negative doubles order like their bits).
+__global__ void k_ext_values(const double* x, const uint32_t* tri, const double* inv, const double* cs,
+                             const double* area, uint32_t T, unsigned long long* key, double* w) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  uint32_t a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+  double co = cs[2 * t], sn = cs[2 * t + 1];
+  double ww = 0.0, uu = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double d0 = x[3 * b + k] - x[3 * a + k], d1 = x[3 * c + k] - x[3 * a + k];
+    double f0 = d0 * inv[4 * t] + d1 * inv[4 * t + 1], f1 = d0 * inv[4 * t + 2] + d1 * inv[4 * t + 3];
+    double wk = f0 * co + f1 * sn, uk = -f0 * sn + f1 * co;
+    ww += wk * wk;
+    uu += uk * uk;
+  }
+  double v = 100.0 * fmax(fmax(sqrt(ww), sqrt(uu)) - 1.0, 0.0);
+  key[t] = (unsigned long long)__double_as_longlong(v);
+  w[t] = area[t];
+}
+// The first sorted triangle whose cumulative area reaches 95 % of the total,
+// and the largest value.
+__global__ void k_ext_pick(const unsigned long long* key, const double* prefix, uint32_t T, Scalars* sc) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= T) return;
+  double thr = 0.95 * prefix[T - 1];
+  if (prefix[i] >= thr && (i == 0 || prefix[i - 1] < thr)) sc->diag_p95 = __longlong_as_double((long long)key[i]);
+  if (i == T - 1) sc->diag_max = __longlong_as_double((long long)key[i]);
+}
+__global__ void k_seam_gap(const double* x, const uint32_t* off, const uint32_t* v, const double* w, uint32_t P,
+                           Scalars* sc) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  double g[3] = {0.0, 0.0, 0.0};
+  for (uint32_t k = off[p]; k < off[p + 1]; ++k)
+    for (int c = 0; c < 3; ++c) g[c] += x[3 * v[k] + c] * w[k];
+  double d = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  atomicMax(&sc->diag_gap_bits, (unsigned long long)__double_as_longlong(d));
+}
+
 }  // namespace
 
 void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings& ogc, size_t cand_cap) {
@@ -1002,4 +1045,34 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
     CV_HIP(hipGetLastError());
   }
   if (g.h_scal->err_degenerate) fail(CV_ERR_DEGENERATE_FACE, "OGC degenerate face; step rejected");
+}
+
+void gpu_diagnostics(cv_gpu& g, const double* dx) {
+  DevSolver& d = g.dev;
+  StepBufs& b = g.b;
+  hipStream_t st = g.stream;
+  k_store_u64<<<1, 1, 0, st>>>(&g.d_scal->diag_gap_bits, 0ull);
+  k_store_u64<<<1, 1, 0, st>>>((unsigned long long*)&g.d_scal->diag_p95, 0ull);
+  k_store_u64<<<1, 1, 0, st>>>((unsigned long long*)&g.d_scal->diag_max, 0ull);
+  if (d.P) k_seam_gap<<<blocks(d.P), TPB, 0, st>>>(dx, d.dg_pr_off.p, d.dg_pr_v.p, d.dg_pr_w.p, d.P, g.d_scal);
+  uint32_t T = d.T;
+  if (T) {
+    b.ext_key.reserve(T);
+    b.ext_key_sorted.reserve(T);
+    b.ext_area.reserve(T);
+    b.ext_area_sorted.reserve(T);
+    b.ext_prefix.reserve(T);
+    k_ext_values<<<blocks(T), TPB, 0, st>>>(dx, d.dg_tri.p, d.dg_inv.p, d.dg_cs.p, d.dg_area.p, T, b.ext_key.p,
+                                            b.ext_area.p);
+    with_temp(g, [&](void* t, size_t& bytes) {
+      return rocprim::radix_sort_pairs(t, bytes, b.ext_key.p, b.ext_key_sorted.p, b.ext_area.p,
+                                       b.ext_area_sorted.p, (size_t)T, 0, 64, st);
+    });
+    with_temp(g, [&](void* t, size_t& bytes) {
+      return rocprim::inclusive_scan(t, bytes, b.ext_area_sorted.p, b.ext_prefix.p, (size_t)T,
+                                     rocprim::plus<double>(), st);
+    });
+    k_ext_pick<<<blocks(T), TPB, 0, st>>>(b.ext_key_sorted.p, b.ext_prefix.p, T, g.d_scal);
+  }
+  CV_HIP(hipGetLastError());
 }

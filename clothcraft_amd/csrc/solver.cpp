@@ -565,44 +565,3 @@ uint32_t greedy_colors(const cv_solver& s, const uint32_t* cnb_off, const uint32
   return ncolors;
 }
 
-void garment_diagnostics(const cv_solver& s, const double* x, double& seam_gap, double& p95,
-                         double& pmax) {
-  seam_gap = 0.0;
-  for (uint32_t p = 0; p + 1 < s.pr_off.size(); ++p) {
-    double g[3] = {0, 0, 0};
-    for (uint32_t k = s.pr_off[p]; k < s.pr_off[p + 1]; ++k)
-      for (int c = 0; c < 3; ++c) g[c] += x[3 * s.pr_v[k] + c] * s.pr_w[k];
-    seam_gap = std::max(seam_gap, std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]));
-  }
-  size_t T = s.tri_area.size();
-  std::vector<std::pair<double, double>> vals(T);
-  for (size_t t = 0; t < T; ++t) {
-    const uint32_t* tv = &s.tri_v[3 * t];
-    const double* inv = &s.tri_inv_dm[4 * t];
-    const cv_material& m = s.materials[s.tri_mat[t]];
-    double sn = std::sin(m.grain_degrees * (M_PI / 180.0)), cs = std::cos(m.grain_degrees * (M_PI / 180.0));
-    double w[3], u[3];
-    for (int c = 0; c < 3; ++c) {
-      double d0 = x[3 * tv[1] + c] - x[3 * tv[0] + c], d1 = x[3 * tv[2] + c] - x[3 * tv[0] + c];
-      double f0 = d0 * inv[0] + d1 * inv[1], f1 = d0 * inv[2] + d1 * inv[3];
-      w[c] = f0 * cs + f1 * sn;
-      u[c] = -f0 * sn + f1 * cs;
-    }
-    double sw = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-    double su = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-    vals[t] = {100.0 * std::max(std::max(sw, su) - 1.0, 0.0), s.tri_area[t]};
-  }
-  std::sort(vals.begin(), vals.end(), [](auto& a, auto& b) { return a.first < b.first; });
-  double total = 0.0;
-  for (auto& v : vals) total += v.second;
-  double area = 0.0;
-  p95 = 0.0;
-  for (auto& v : vals) {
-    area += v.second;
-    if (area >= 0.95 * total) {
-      p95 = v.first;
-      break;
-    }
-  }
-  pmax = vals.empty() ? 0.0 : vals.back().first;
-}

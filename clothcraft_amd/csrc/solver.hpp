@@ -87,8 +87,5 @@ uint32_t greedy_colors(const cv_solver& s, const uint32_t* cnb_off, const uint32
                        const cv_grab* grab, std::vector<uint32_t>& color_of,
                        std::vector<uint32_t>& order, std::vector<uint32_t>& color_off);
 
-// Garment diagnostics (src/garment/diagnostics.rs:7-28, template.rs:536-538).
-void garment_diagnostics(const cv_solver& s, const double* x, double& seam_gap, double& p95,
-                         double& pmax);
 
 [[noreturn]] void fail(cv_status st, const std::string& msg);

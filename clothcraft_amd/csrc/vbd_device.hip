@@ -763,6 +763,7 @@ struct StepOut {
   double detection_ms = 0, refit_ms = 0, candidates_ms = 0, narrow_ms = 0;
   uint64_t candidate_pairs = 0, grid_tests = 0;
   bool true_residual = false;
+  bool diagnostics = false;  // garment diagnostics computed after the commit
 };
 
 // Detection + balls (+ colouring) on the positions held in b.x64.
@@ -832,6 +833,7 @@ struct StepCtx {
   SweepArgs<R> a{};
   double rq = 0, threshold = 0, tol = 0, nfree3 = 1;
   bool want_true = false;
+  bool diag = true;  // garment diagnostics after the commit (the last step of a run)
   uint32_t iters = 0, next = 0, done = 0;
   bool finished = false;
   struct Launch {
@@ -1008,6 +1010,10 @@ struct StepCtx {
                                                    b.radii64.p, d.pinned.p, n, set->dt, set->damping, 1e-8,
                                                    b.xcur64.p, b.v64.p, d.acc.p, g.d_scal);
     CV_HIP(hipGetLastError());
+    if (diag && !(set->flags & CV_FLAG_DIAGNOSTICS_OFF)) {
+      gpu_diagnostics(g, b.xcur64.p);
+      out.diagnostics = true;
+    }
   }
   void check() const {
     if (g.h_scal->err_nonfinite) fail(CV_ERR_NONFINITE_RESULT, "Non-finite VBD result; state unchanged");
@@ -1016,8 +1022,9 @@ struct StepCtx {
 };
 
 template <class R>
-void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out) {
+void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out, bool diag) {
   StepCtx<R> c(g, s, set, grab, out);
+  c.diag = diag;
   c.prologue_reduce();
   CV_HIP(hipStreamSynchronize(g.stream));
   c.begin();
@@ -1035,11 +1042,11 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
   c.check();
 }
 
-void run_step(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out) {
+void run_step(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out, bool diag) {
   if (g.f64)
-    run_step_t<double>(g, s, set, grab, out);
+    run_step_t<double>(g, s, set, grab, out, diag);
   else
-    run_step_t<float>(g, s, set, grab, out);
+    run_step_t<float>(g, s, set, grab, out, diag);
 }
 
 // Batched step of many solvers (each with its own context g_i and stream):
@@ -1237,6 +1244,27 @@ void ensure_device_solver(cv_gpu& g, const cv_solver& s) {
   d.fixed_abs_max = 0;
   for (double v : s.avatar) d.fixed_abs_max = std::max(d.fixed_abs_max, std::fabs(v));
   d.cell = s.face_extent_median;
+  {
+    // Garment diagnostics inputs (grain rotation per triangle as the host
+    // formula computes it).
+    d.T = (uint32_t)s.tri_area.size();
+    d.P = s.pr_off.empty() ? 0u : (uint32_t)(s.pr_off.size() - 1);
+    std::vector<double> cs(2 * (size_t)std::max<uint32_t>(d.T, 1), 0.0);
+    for (uint32_t t = 0; t < d.T; ++t) {
+      const cv_material& m = s.materials[s.tri_mat[t]];
+      cs[2 * t] = std::cos(m.grain_degrees * (M_PI / 180.0));
+      cs[2 * t + 1] = std::sin(m.grain_degrees * (M_PI / 180.0));
+    }
+    auto nz_u = [](std::vector<uint32_t> v) { if (v.empty()) v.push_back(0); return v; };
+    auto nz_d = [](std::vector<double> v) { if (v.empty()) v.push_back(0); return v; };
+    d.dg_tri.upload(nz_u(s.tri_v), st);
+    d.dg_inv.upload(nz_d(s.tri_inv_dm), st);
+    d.dg_cs.upload(cs, st);
+    d.dg_area.upload(nz_d(s.tri_area), st);
+    d.dg_pr_off.upload(nz_u(s.pr_off), st);
+    d.dg_pr_v.upload(nz_u(s.pr_v), st);
+    d.dg_pr_w.upload(nz_d(s.pr_w), st);
+  }
   // Algorithmic bytes of one vertex-iteration (each term's data and the
   // neighbour positions read once, position written once):
   //   f32: 64 fixed + 72 per triangle, 52 per hinge, 68 per stitch;
@@ -1344,12 +1372,11 @@ void fill_metrics(cv_metrics* m, const cv_gpu& g, const cv_solver& s, const cv_s
   m->sweep_bytes = o.sweep_bytes;
   m->sweep_bytes_min = o.sweep_bytes_min;
   m->steps = 1;
-  if (!(set->flags & CV_FLAG_DIAGNOSTICS_OFF)) {
-    double gap, p95, pmax;
-    garment_diagnostics(s, x_new, gap, p95, pmax);
-    m->max_seam_gap_m = gap;
-    m->extension_p95_percent = p95;
-    m->extension_max_percent = pmax;
+  if (o.diagnostics) {
+    // Computed on the device from the committed positions (gpu_diagnostics).
+    m->max_seam_gap_m = __builtin_bit_cast(double, (uint64_t)g.h_scal->diag_gap_bits);
+    m->extension_p95_percent = g.h_scal->diag_p95;
+    m->extension_max_percent = g.h_scal->diag_max;
   } else {
     m->max_seam_gap_m = m->extension_p95_percent = m->extension_max_percent = NAN;
   }
@@ -1514,7 +1541,7 @@ static cv_status run_steps(cv_gpu* g, cv_solver* s, cv_state* st, const cv_setti
       } else {
         CV_HIP(hipEventRecord(g->ev0, g->stream));
       }
-      run_step(*g, *s, set, grab, o);
+      run_step(*g, *s, set, grab, o, k + 1 == steps);
       tot.detections += o.detections;
       tot.truncated += o.truncated;
       tot.iterations_run += o.iterations_run;

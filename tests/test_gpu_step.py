@@ -71,6 +71,11 @@ def test_one_step_parity_hanging_sheet(gpu, bits, tol):
     assert mg["colors"] == mr["colors"]
     assert mg["detection_updates"] == mr["detection_updates"]
     assert err <= tol
+    # Extension in percent points: the f32 step's position error (a few 1e-6
+    # of the diagonal, i.e. ~1e-5 of an edge) moves it by ~1e-4 points.
+    dtol = 1e-6 if bits == 64 else 1e-3
+    for key in ("extension_p95_percent", "extension_max_percent"):
+        assert abs(mg[key] - mr[key]) <= dtol * (1.0 + abs(mr[key])), key
 
 
 def test_one_step_parity_sheet_on_sphere_with_contacts(gpu):
@@ -329,6 +334,9 @@ def test_one_step_parity_sewn_garment_with_seams_and_body(gpu):
         assert err <= 1e-9
         assert mg["contact_terms"] == mr["contact_terms"]
         assert abs(mg["max_seam_gap_m"] - mr["max_seam_gap_m"]) <= 1e-12 + 1e-9 * mr["max_seam_gap_m"]
+        # device diagnostics of the committed positions (src/garment/diagnostics.rs:7-28)
+        for key in ("extension_p95_percent", "extension_max_percent"):
+            assert abs(mg[key] - mr[key]) <= 1e-9 + 1e-9 * abs(mr[key]), key
         st = a
         sol.record_prediction(prev_v, st, s)
 
