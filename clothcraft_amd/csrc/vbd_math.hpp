@@ -348,19 +348,6 @@ CV_HD R bending_e(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, const F* f) {
   V3<R> d = n1 * (R(1) / l1) - n2 * (R(1) / l2);
   return R(0.5) * R(f[0]) * dot(d, d);
 }
-// d^2 nhat_k / dn_i dn_j contracted with y: T_ij = (3 (y.n) n_i n_j - y_i n_j - y_j n_i - (y.n) d_ij) / l^2
-// (il = 1 / l).
-template <class R>
-CV_HD M3<R> unit_normal_curv(V3<R> nh, V3<R> y, R il) {
-  R yn = dot(y, nh);
-  R il2 = il * il;
-  R n[3] = {nh.x, nh.y, nh.z}, yy[3] = {y.x, y.y, y.z};
-  M3<R> t;
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j)
-      t.m[i][j] = (R(3) * yn * n[i] * n[j] - yy[i] * n[j] - yy[j] * n[i] - (i == j ? yn : R(0))) * il2;
-  return t;
-}
 template <class R, class F>
 CV_HD Deriv<R> bending_d(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, int slot, const F* f) {
   R kappa = R(f[0]);
@@ -388,35 +375,46 @@ CV_HD Deriv<R> bending_d(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, int slot, const
     a1 = z;
     a2 = -e;
   }
-  M3<R> A1 = skew(a1), A2 = skew(a2);
-  // J = P1 A1 / l1 - P2 A2 / l2
-  M3<R> J;
-  R an[3] = {a.x, a.y, a.z}, bn[3] = {b.x, b.y, b.z};
+  // J = P1 [a1]x / l1 - P2 [a2]x / l2 with P = I - n n^T.  Since
+  // n^T [a1]x = (n x a1)^T, P1 [a1]x = [a1]x - a c1^T with c1 = a x a1.
+  V3<R> c1 = cross(a, a1), c2 = cross(b, a2);
+  R ax[3] = {a.x, a.y, a.z}, bx[3] = {b.x, b.y, b.z};
+  R c1x[3] = {c1.x, c1.y, c1.z}, c2x[3] = {c2.x, c2.y, c2.z};
+  R s1[3][3] = {{R(0), -a1.z, a1.y}, {a1.z, R(0), -a1.x}, {-a1.y, a1.x, R(0)}};
+  R s2[3][3] = {{R(0), -a2.z, a2.y}, {a2.z, R(0), -a2.x}, {-a2.y, a2.x, R(0)}};
+  R J[3][3];
+#pragma unroll
   for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) {
-      R s1 = R(0), s2 = R(0);
-      for (int k = 0; k < 3; ++k) {
-        R p1 = (i == k ? R(1) : R(0)) - an[i] * an[k];
-        R p2 = (i == k ? R(1) : R(0)) - bn[i] * bn[k];
-        s1 += p1 * A1.m[k][j];
-        s2 += p2 * A2.m[k][j];
-      }
-      J.m[i][j] = s1 * il1 - s2 * il2;
-    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) J[i][j] = (s1[i][j] - ax[i] * c1x[j]) * il1 - (s2[i][j] - bx[i] * c2x[j]) * il2;
   Deriv<R> d;
   d.e = R(0.5) * kappa * dot(D, D);
   R Dn[3] = {D.x, D.y, D.z};
   R g[3];
-  for (int j = 0; j < 3; ++j) g[j] = kappa * (J.m[0][j] * Dn[0] + J.m[1][j] * Dn[1] + J.m[2][j] * Dn[2]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) g[j] = kappa * (J[0][j] * Dn[0] + J[1][j] * Dn[1] + J[2][j] * Dn[2]);
   d.g = mk<R>(g[0], g[1], g[2]);
-  M3<R> JJ = mul_tn(J, J);
-  M3<R> T1 = unit_normal_curv(a, D, il1);
-  M3<R> T2 = unit_normal_curv(b, D, il2);
-  M3<R> C1 = mul_tn(A1, mul(T1, A1));
-  M3<R> C2 = mul_tn(A2, mul(T2, A2));
+  // Curvature part [a]x^T T [a]x for T = (3 yn n n^T - y n^T - n y^T - yn I) / l^2,
+  // using [a]x^T u = u x a:  = (3 yn c c^T - q c^T - c q^T - yn (|a|^2 I - a a^T)) / l^2
+  // with c = n x a (c1, c2 above), q = D x a, yn = D . n.
+  V3<R> q1 = cross(D, a1), q2 = cross(D, a2);
+  R q1x[3] = {q1.x, q1.y, q1.z}, q2x[3] = {q2.x, q2.y, q2.z};
+  R a1x[3] = {a1.x, a1.y, a1.z}, a2x[3] = {a2.x, a2.y, a2.z};
+  R yn1 = dot(D, a), yn2 = dot(D, b);
+  R n1a = dot(a1, a1), n2a = dot(a2, a2);
+  R k1 = il1 * il1, k2 = il2 * il2;
   int k = 0;
+#pragma unroll
   for (int i = 0; i < 3; ++i)
-    for (int j = i; j < 3; ++j, ++k) d.h.a[k] = kappa * (JJ.m[i][j] + C1.m[i][j] - C2.m[i][j]);
+#pragma unroll
+    for (int j = i; j < 3; ++j, ++k) {
+      R jj = J[0][i] * J[0][j] + J[1][i] * J[1][j] + J[2][i] * J[2][j];
+      R e1 = R(3) * yn1 * c1x[i] * c1x[j] - q1x[i] * c1x[j] - c1x[i] * q1x[j] -
+             yn1 * ((i == j ? n1a : R(0)) - a1x[i] * a1x[j]);
+      R e2 = R(3) * yn2 * c2x[i] * c2x[j] - q2x[i] * c2x[j] - c2x[i] * q2x[j] -
+             yn2 * ((i == j ? n2a : R(0)) - a2x[i] * a2x[j]);
+      d.h.a[k] = kappa * (jj + e1 * k1 - e2 * k2);
+    }
   return d;
 }
 
