@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -1704,8 +1705,15 @@ cv_status cv_gpu_step_batch(cv_gpu* g, cv_solver* const* solvers, cv_state* cons
     // own stream; host worker threads for the per-solver phases.
     if (!g->join_ev) CV_HIP(hipEventCreateWithFlags(&g->join_ev, hipEventDisableTiming));
     if (!g->pool) {
+      // Host threads for the per-solver phases: CLOTHVBD_HOST_THREADS, else
+      // min(8, hardware threads); the calling thread is one of them.
       unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-      g->pool = std::make_unique<WorkPool>(std::min(8u, hw) - 1);
+      unsigned want = std::min(8u, hw);
+      if (const char* e = std::getenv("CLOTHVBD_HOST_THREADS")) {
+        long v = std::strtol(e, nullptr, 10);
+        if (v >= 1 && v <= 256) want = (unsigned)v;
+      }
+      g->pool = std::make_unique<WorkPool>(want - 1);
     }
     while (g->subs.size() < count) {
       cv_gpu* c = new cv_gpu();

@@ -249,7 +249,9 @@ cv_status cv_gpu_detect(cv_gpu* gpu, cv_solver* solver, const double* x, double 
  * is returned (cv_last_error names its slot) and only the failing solvers'
  * states are left untouched.  metrics[i] (optional) describes solver i, with
  * gpu_compute_ms the whole batch.  The learned warm start is not available
- * here. */
+ * here.  Per-solver host phases (detection read-backs, colouring, staging)
+ * run on min(8, hardware threads) worker threads, or CLOTHVBD_HOST_THREADS
+ * if set, read when the first batch runs on this context. */
 cv_status cv_gpu_step_batch(cv_gpu* gpu, cv_solver* const* solvers, cv_state* const* states,
                             uint32_t count, const cv_settings* settings, cv_metrics* metrics);
 
