@@ -50,3 +50,29 @@ def test_batch_failure_leaves_only_the_failing_state_unchanged(gpu):
     assert not np.array_equal(states[0].x, before[0].x)
     assert np.array_equal(states[1].x, before[1].x, equal_nan=True)
     assert not np.array_equal(states[2].x, before[2].x)
+
+
+def test_autotuner_picks_the_fastest_lanes_and_keeps_results(gpu):
+    """The lanes autotuner times every candidate shape on scratch copies
+    (callers' states untouched), leaves the fastest set, and every shape gives
+    the same batched step as 32 lanes to round-off (the lane-tree reduction
+    order depends on the lane count)."""
+    from clothcraft_amd import GpuVbd, VbdSolver, autotune
+    scs = _instances()[:3]
+    settings = dict(scenes.default_settings(dt=1 / 60, iterations=8, radius=0.002, stiffness=1e5))
+    before = [sc.state.x.copy() for sc in scs]
+    res = autotune.tune_lanes(gpu, scs, settings, repeats=1)
+    try:
+        assert res["lanes"] in autotune.CANDIDATES and set(res["ms"]) == set(autotune.CANDIDATES)
+        assert res["ms"][res["lanes"]] == min(res["ms"].values())
+        assert all(np.array_equal(b, sc.state.x) for b, sc in zip(before, scs))
+        ref = GpuVbd(0)
+        a = [sc.state.copy() for sc in scs]
+        b = [sc.state.copy() for sc in scs]
+        ref.step_batch([VbdSolver(sc.garment, sc.avatar, st) for sc, st in zip(scs, a)], a, settings)
+        gpu.step_batch([VbdSolver(sc.garment, sc.avatar, st) for sc, st in zip(scs, b)], b, settings)
+        for x, y in zip(a, b):
+            diag = float(np.linalg.norm(x.x.max(axis=0) - x.x.min(axis=0)))
+            assert np.abs(x.x - y.x).max() / diag <= 1e-9
+    finally:
+        gpu.set_lanes(32)
