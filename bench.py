@@ -87,6 +87,8 @@ def main():
                     help="sewn_garment: two ~50k-vertex Poisson-disk panels with side and shoulder "
                          "seams on a capsule body, 40 iterations; batch64: 64 independent sheets and "
                          "sewn tubes of 1k-50k vertices in batched steps (not the headline line)")
+    ap.add_argument("--fast-coloring", action="store_true",
+                    help="smallest-last colouring (fewer colours; not the reference's colouring order)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batch64: use --lanes instead of timing the lanes per vertex first")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -121,6 +123,9 @@ def main():
     else:
         sc = scenes.sheet_on_sphere(nx=args.nx)
     settings = sc.settings
+    if args.fast_coloring:
+        from clothcraft_amd.abi import CV_FLAG_FAST_COLORING
+        settings = dict(settings, flags=settings.get("flags", 0) | CV_FLAG_FAST_COLORING)
     st = sc.state.copy()
     gpu = GpuVbd(0)
     gpu.set_precision(args.precision)
@@ -208,6 +213,8 @@ def main():
              f"{settings['iterations']} iterations, dt 1/60, self-contact on"),
             "steps_timed": f"1..{K} of the drop (warmup on a discarded copy)",
             "iteration_mode": "fixed iterations (residual tolerance 0)",
+            "coloring": ("smallest-last order (CV_FLAG_FAST_COLORING)" if args.fast_coloring
+                         else "reference vertex-order greedy"),
             "lanes_per_vertex": args.lanes,
         },
         "vertex_iters_per_s": vi / dev_s,

@@ -28,6 +28,7 @@ STATUS_NAMES = {
 
 CV_FLAG_DIAGNOSTICS_OFF = 1 << 0
 CV_FLAG_TRUE_RESIDUAL = 1 << 1
+CV_FLAG_FAST_COLORING = 1 << 2
 CV_FLAG_WARM_START = 1 << 3
 
 

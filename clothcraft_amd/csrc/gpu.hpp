@@ -262,6 +262,7 @@ struct cv_gpu {
   Scalars* h_scal = nullptr;  // pinned host mirror
   double* h_x = nullptr;      // pinned staging [3n]
   double* h_v = nullptr;
+  double* h_acc = nullptr;    // pinned staging of the acceleration history [3n]
   size_t h_cap = 0;
   std::string last_error;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -271,7 +272,8 @@ struct cv_gpu {
   uint32_t* h_cont = nullptr;
   size_t h_cont_cap = 0;
   std::vector<uint32_t> cnb_off, cnb;
-  std::vector<uint32_t> color_of, order, color_off;
+  std::vector<uint32_t> order, color_off;
+  ColoringState coloring;  // incremental greedy colouring across detections
   // Inputs of the last colouring, to reuse it when they repeat.
   bool color_valid = false;
   uint64_t color_generation = 0, color_m = 0;

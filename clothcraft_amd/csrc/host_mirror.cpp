@@ -60,9 +60,46 @@ uint32_t cvh_solver_edges(const cv_solver* s, uint32_t* out, uint32_t cap) {
   return E;
 }
 
+}  // extern "C"
+
+// Per-vertex contact neighbourhoods (CSR) from (kind, ids[4], count) rows.
+static void contact_csr(uint32_t n, const uint32_t* kind_ids, const uint32_t* counts, uint64_t m,
+                        std::vector<uint32_t>& off, std::vector<uint32_t>& nb) {
+  off.assign(n + 1, 0);
+  for (uint64_t c = 0; c < m; ++c)
+    for (uint32_t q = 0; q < counts[c]; ++q) {
+      uint32_t v = kind_ids[5 * c + 1 + q];
+      if (v < n) off[v + 1] += counts[c];
+    }
+  for (uint32_t v = 0; v < n; ++v) off[v + 1] += off[v];
+  nb.resize(off[n]);
+  std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+  for (uint64_t c = 0; c < m; ++c)
+    for (uint32_t q = 0; q < counts[c]; ++q) {
+      uint32_t v = kind_ids[5 * c + 1 + q];
+      if (v < n)
+        for (uint32_t r = 0; r < counts[c]; ++r) nb[cur[v]++] = kind_ids[5 * c + 1 + r];
+    }
+}
+
+extern "C" {
+
+// Incremental colouring state for tests of update_colors.
+void* cvh_coloring_new() { return new ColoringState(); }
+void cvh_coloring_free(void* p) { delete (ColoringState*)p; }
+uint32_t cvh_coloring_update(const cv_solver* s, void* state, const uint32_t* kind_ids, const uint32_t* counts,
+                             uint64_t m, const cv_grab* grab, int smallest_last, uint32_t* color_of) {
+  std::vector<uint32_t> off, nb, order, coff;
+  contact_csr(s->n, kind_ids, counts, m, off, nb);
+  ColoringState& cs = *(ColoringState*)state;
+  uint32_t k = update_colors(*s, cs, off.data(), nb.data(), grab, smallest_last != 0, order, coff);
+  std::memcpy(color_of, cs.color_of.data(), s->n * sizeof(uint32_t));
+  return k;
+}
+
 // Greedy colouring with contact cliques given as (kind, ids[4], count) rows.
 uint32_t cvh_greedy_colors(const cv_solver* s, const uint32_t* kind_ids, const uint32_t* counts,
-                           uint64_t m, const cv_grab* grab, uint32_t* color_of) {
+                           uint64_t m, const cv_grab* grab, uint32_t* color_of, int smallest_last) {
   uint32_t n = s->n;
   std::vector<uint32_t> off(n + 1, 0), nb;
   for (uint64_t c = 0; c < m; ++c)
@@ -80,7 +117,7 @@ uint32_t cvh_greedy_colors(const cv_solver* s, const uint32_t* kind_ids, const u
         for (uint32_t r = 0; r < counts[c]; ++r) nb[cur[v]++] = kind_ids[5 * c + 1 + r];
     }
   std::vector<uint32_t> col, order, coff;
-  uint32_t k = greedy_colors(*s, off.data(), nb.data(), grab, col, order, coff);
+  uint32_t k = greedy_colors(*s, off.data(), nb.data(), grab, col, order, coff, smallest_last != 0);
   std::memcpy(color_of, col.data(), n * sizeof(uint32_t));
   return k;
 }

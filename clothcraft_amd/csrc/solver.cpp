@@ -1,9 +1,10 @@
 // solver.cpp — cv_solver construction (VbdSolver::new, src/vbd.rs:101-192),
-// step-argument validation, greedy colouring and garment diagnostics.
+// step-argument validation and greedy colouring.
 #include "solver.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <set>
 #include <unordered_map>
@@ -243,6 +244,38 @@ void validate_start(const OgcTopology& t, const std::vector<double>& x) {
     fail(CV_ERR_INITIAL_TOUCHING, "OGC initial touching/coplanar features: separation must be positive");
 }
 
+// Smallest-last vertex order of a graph (CSR, symmetric): repeatedly remove
+// a vertex of least remaining degree (ties: smallest index), then reverse the
+// removal sequence.  Greedy colouring in this order needs at most one colour
+// more than the graph's degeneracy, and fewer colours than index order on
+// irregular meshes.
+std::vector<uint32_t> smallest_last_order(uint32_t n, const std::vector<uint32_t>& off,
+                                          const std::vector<uint32_t>& adj) {
+  std::vector<uint32_t> deg(n);
+  std::set<std::pair<uint32_t, uint32_t>> queue;
+  for (uint32_t v = 0; v < n; ++v) {
+    deg[v] = off[v + 1] - off[v];
+    queue.insert({deg[v], v});
+  }
+  std::vector<char> removed(n, 0);
+  std::vector<uint32_t> seq;
+  seq.reserve(n);
+  while (!queue.empty()) {
+    uint32_t v = queue.begin()->second;
+    queue.erase(queue.begin());
+    removed[v] = 1;
+    seq.push_back(v);
+    for (uint32_t k = off[v]; k < off[v + 1]; ++k) {
+      uint32_t u = adj[k];
+      if (u >= n || removed[u]) continue;
+      queue.erase({deg[u], u});
+      queue.insert({--deg[u], u});
+    }
+  }
+  std::reverse(seq.begin(), seq.end());
+  return seq;
+}
+
 }  // namespace
 
 // VbdSolver::new (src/vbd.rs:101-192).
@@ -410,6 +443,7 @@ void build_solver(cv_solver& s, const cv_garment* g, const cv_surface* avatar, c
   for (uint32_t i = 0; i < n; ++i) s.graph_off[i + 1] = s.graph_off[i] + (uint32_t)graph[i].size();
   s.graph.clear();
   for (uint32_t i = 0; i < n; ++i) s.graph.insert(s.graph.end(), graph[i].begin(), graph[i].end());
+  s.sl_order = smallest_last_order(n, s.graph_off, s.graph);
 
   s.term_off.assign(n + 1, 0);
   s.terms64.clear();
@@ -493,7 +527,8 @@ void validate_settings(const cv_settings* s) {  // src/vbd.rs:37-51, src/ogc/wor
     fail(CV_ERR_INVALID_SETTINGS, "Invalid VBD settings");
   if (!std::isfinite(s->residual_tolerance) || s->residual_tolerance < 0.0)
     fail(CV_ERR_INVALID_SETTINGS, "Invalid VBD settings");
-  if (s->flags & ~(uint32_t)(CV_FLAG_DIAGNOSTICS_OFF | CV_FLAG_TRUE_RESIDUAL | CV_FLAG_WARM_START))
+  if (s->flags &
+      ~(uint32_t)(CV_FLAG_DIAGNOSTICS_OFF | CV_FLAG_TRUE_RESIDUAL | CV_FLAG_FAST_COLORING | CV_FLAG_WARM_START))
     fail(CV_ERR_INVALID_SETTINGS, "Invalid VBD settings");
 }
 
@@ -529,7 +564,7 @@ bool prediction_valid(const cv_solver& s, const cv_state* st, const cv_settings*
 
 uint32_t greedy_colors(const cv_solver& s, const uint32_t* cnb_off, const uint32_t* cnb,
                        const cv_grab* grab, std::vector<uint32_t>& color_of,
-                       std::vector<uint32_t>& order, std::vector<uint32_t>& color_off) {
+                       std::vector<uint32_t>& order, std::vector<uint32_t>& color_off, bool smallest_last) {
   uint32_t n = s.n;
   const uint32_t UNSET = 0xFFFFFFFFu;
   color_of.assign(n, UNSET);
@@ -539,7 +574,8 @@ uint32_t greedy_colors(const cv_solver& s, const uint32_t* cnb_off, const uint32
   auto in_grab = [&](uint32_t v) {
     return grab && (grab->vertices[0] == v || grab->vertices[1] == v || grab->vertices[2] == v);
   };
-  for (uint32_t v = 0; v < n; ++v) {
+  for (uint32_t idx = 0; idx < n; ++idx) {
+    const uint32_t v = smallest_last ? s.sl_order[idx] : idx;
     auto mark = [&](uint32_t i) {
       if (i < n && color_of[i] != UNSET) seen[color_of[i]] = v;
     };
@@ -565,3 +601,80 @@ uint32_t greedy_colors(const cv_solver& s, const uint32_t* cnb_off, const uint32
   return ncolors;
 }
 
+
+uint32_t update_colors(const cv_solver& s, ColoringState& cs, const uint32_t* cnb_off, const uint32_t* cnb,
+                       const cv_grab* grab, bool smallest_last, std::vector<uint32_t>& order,
+                       std::vector<uint32_t>& color_off) {
+  const uint32_t n = s.n;
+  uint32_t key[4] = {grab ? 1u : 0u, grab ? grab->vertices[0] : 0u, grab ? grab->vertices[1] : 0u,
+                     grab ? grab->vertices[2] : 0u};
+  auto remember = [&] {
+    cs.valid = true;
+    cs.smallest_last = smallest_last;
+    cs.generation = s.generation;
+    std::memcpy(cs.grab_key, key, sizeof key);
+    cs.cnb_off.assign(cnb_off, cnb_off + n + 1);
+    cs.cnb.assign(cnb, cnb + cnb_off[n]);
+  };
+  if (!cs.valid || cs.generation != s.generation || cs.smallest_last != smallest_last) {
+    uint32_t k = greedy_colors(s, cnb_off, cnb, grab, cs.color_of, order, color_off, smallest_last);
+    cs.pos.assign(n, 0);
+    for (uint32_t i = 0; i < n; ++i) cs.pos[smallest_last ? s.sl_order[i] : i] = i;
+    remember();
+    return k;
+  }
+  // Dirty: vertices whose contact neighbour list changed (lists are in the
+  // canonical contact order, so an unchanged neighbourhood gives an equal
+  // list), and the grab vertices when the grab changed.
+  cs.dirty.assign(n, 0);
+  for (uint32_t v = 0; v < n; ++v) {
+    uint32_t a0 = cs.cnb_off[v], a1 = cs.cnb_off[v + 1], b0 = cnb_off[v], b1 = cnb_off[v + 1];
+    if (a1 - a0 != b1 - b0 || !std::equal(cnb + b0, cnb + b1, cs.cnb.begin() + a0)) cs.dirty[v] = 1;
+  }
+  if (std::memcmp(key, cs.grab_key, sizeof key) != 0)
+    for (int k = 1; k < 4; ++k) {
+      if (cs.grab_key[0] && cs.grab_key[k] < n) cs.dirty[cs.grab_key[k]] = 1;
+      if (key[0] && key[k] < n) cs.dirty[key[k]] = 1;
+    }
+  auto in_grab = [&](uint32_t v) {
+    return grab && (grab->vertices[0] == v || grab->vertices[1] == v || grab->vertices[2] == v);
+  };
+  const uint32_t UNSET = 0xFFFFFFFFu;
+  cs.scratch_seen.assign(n + 2, UNSET);
+  std::vector<uint32_t>& seen = cs.scratch_seen;
+  for (uint32_t idx = 0; idx < n; ++idx) {
+    const uint32_t v = smallest_last ? s.sl_order[idx] : idx;
+    if (!cs.dirty[v]) continue;
+    // Colour from the neighbours that come earlier in the order (the ones a
+    // from-scratch pass has coloured by now).
+    auto mark = [&](uint32_t i) {
+      if (i < n && cs.pos[i] < idx) seen[cs.color_of[i]] = v;
+    };
+    for (uint32_t k = s.graph_off[v]; k < s.graph_off[v + 1]; ++k) mark(s.graph[k]);
+    for (uint32_t k = cnb_off[v]; k < cnb_off[v + 1]; ++k) mark(cnb[k]);
+    if (in_grab(v))
+      for (int k = 0; k < 3; ++k) mark(grab->vertices[k]);
+    uint32_t c = 0;
+    while (seen[c] == v) ++c;
+    if (c == cs.color_of[v]) continue;
+    cs.color_of[v] = c;
+    // Later neighbours may now choose differently.
+    auto touch = [&](uint32_t i) {
+      if (i < n && cs.pos[i] > idx) cs.dirty[i] = 1;
+    };
+    for (uint32_t k = s.graph_off[v]; k < s.graph_off[v + 1]; ++k) touch(s.graph[k]);
+    for (uint32_t k = cnb_off[v]; k < cnb_off[v + 1]; ++k) touch(cnb[k]);
+    if (in_grab(v))
+      for (int k = 0; k < 3; ++k) touch(grab->vertices[k]);
+  }
+  remember();
+  uint32_t ncolors = 0;
+  for (uint32_t v = 0; v < n; ++v) ncolors = std::max(ncolors, cs.color_of[v] + 1);
+  color_off.assign(ncolors + 1, 0);
+  for (uint32_t v = 0; v < n; ++v) color_off[cs.color_of[v] + 1]++;
+  for (uint32_t c = 0; c < ncolors; ++c) color_off[c + 1] += color_off[c];
+  order.assign(n, 0);
+  std::vector<uint32_t> cur(color_off.begin(), color_off.end() - 1);
+  for (uint32_t v = 0; v < n; ++v) order[cur[cs.color_of[v]]++] = v;
+  return ncolors;
+}

@@ -53,6 +53,7 @@ struct cv_solver {
   std::vector<double> avatar;           // [3*n_fixed] fixed positions
   OgcTopology top;
   std::vector<uint32_t> graph_off, graph;  // static conflict graph (sorted)
+  std::vector<uint32_t> sl_order;          // smallest-last order of that graph (CV_FLAG_FAST_COLORING)
   std::vector<uint32_t> term_off;          // [n+1] static TermRec per vertex
   std::vector<cvm::TermRec> terms;      // f32 coefficients (f32 sweep)
   std::vector<cvm::TermRec64> terms64;  // f64 coefficients (f64 sweep)
@@ -80,12 +81,33 @@ void validate_state(const cv_solver& s, const cv_state* st);
 void validate_grab(const cv_solver& s, const cv_grab* g);
 bool prediction_valid(const cv_solver& s, const cv_state* st, const cv_settings* set);
 
-// Greedy colouring in vertex order (src/vbd/coloring.rs:5-43).  `contact_*`
-// is the per-vertex contact incidence: for moving vertex v, the vertices of
-// each incident contact.  Returns colour of each vertex and the colour count.
+// Greedy colouring in vertex order (src/vbd/coloring.rs:5-43), or with
+// smallest_last in the smallest-last order of the static conflict graph
+// (s.sl_order: fewer colours, a different Gauss-Seidel order than the
+// reference).  cnb_off/cnb is the per-vertex contact incidence: for moving
+// vertex v, the vertices of each incident contact.  Returns colour of each
+// vertex and the colour count; `order` lists vertices colour by colour, in
+// index order within a colour.
 uint32_t greedy_colors(const cv_solver& s, const uint32_t* cnb_off, const uint32_t* cnb,
                        const cv_grab* grab, std::vector<uint32_t>& color_of,
-                       std::vector<uint32_t>& order, std::vector<uint32_t>& color_off);
+                       std::vector<uint32_t>& order, std::vector<uint32_t>& color_off,
+                       bool smallest_last = false);
 
+
+// Greedy colouring kept across detections of one solver: a vertex is
+// recoloured only when its contact neighbourhood or grab membership changed
+// or an earlier vertex (in colouring order) among its neighbours changed
+// colour, so the result equals greedy_colors from scratch while the work
+// follows the change.  A new solver generation or order mode starts over.
+struct ColoringState {
+  bool valid = false, smallest_last = false;
+  uint64_t generation = 0;
+  uint32_t grab_key[4] = {0, 0, 0, 0};
+  std::vector<uint32_t> color_of, cnb_off, cnb, pos, scratch_seen;
+  std::vector<uint8_t> dirty;
+};
+uint32_t update_colors(const cv_solver& s, ColoringState& cs, const uint32_t* cnb_off, const uint32_t* cnb,
+                       const cv_grab* grab, bool smallest_last, std::vector<uint32_t>& order,
+                       std::vector<uint32_t>& color_off);
 
 [[noreturn]] void fail(cv_status st, const std::string& msg);

@@ -698,7 +698,7 @@ void sweep_color(cv_gpu& g, const SweepArgs<R>& a, uint32_t off, uint32_t count,
 
 // Host greedy colouring of the current contact set (the detection left the
 // contact list in g.h_cont and its count in the scalars it read back).
-uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
+uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab, bool smallest_last) {
   uint64_t m = g.h_scal->contacts;
   const uint32_t* cids = g.h_cont;
   const uint32_t* ccount = g.h_cont + 5 * g.h_cont_cap;
@@ -706,8 +706,8 @@ uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
   // The colouring is a function of the solver, the contact list and the grab
   // vertices: when all three equal the last colouring's, it is reused (the
   // order already sits on the device).  Contact-free steps always reuse it.
-  uint32_t key[4] = {grab ? 1u : 0u, grab ? grab->vertices[0] : 0u, grab ? grab->vertices[1] : 0u,
-                     grab ? grab->vertices[2] : 0u};
+  uint32_t key[4] = {(grab ? 1u : 0u) | (smallest_last ? 2u : 0u), grab ? grab->vertices[0] : 0u,
+                     grab ? grab->vertices[1] : 0u, grab ? grab->vertices[2] : 0u};
   if (g.color_valid && g.color_generation == s.generation && g.color_m == m &&
       std::memcmp(g.color_key, key, sizeof key) == 0 &&
       std::equal(cids, cids + 5 * m, g.color_cids.begin()) &&
@@ -731,7 +731,8 @@ uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
       if (v < n)
         for (uint32_t r = 0; r < ccount[c]; ++r) g.cnb[cur[v]++] = cids[5 * c + 1 + r];
     }
-  uint32_t k = greedy_colors(s, g.cnb_off.data(), g.cnb.data(), grab, g.color_of, g.order, g.color_off);
+  uint32_t k = update_colors(s, g.coloring, g.cnb_off.data(), g.cnb.data(), grab, smallest_last, g.order,
+                             g.color_off);
   g.b.order.upload(g.order, g.stream);
   g.color_valid = true;
   g.color_generation = s.generation;
@@ -787,7 +788,7 @@ void detect_and_color(cv_gpu& g, const cv_solver& s, const cv_settings* set, con
                                               g.b.centers64.p, g.b.guess64.p + 3 * (size_t)n);
   CV_HIP(hipGetLastError());
   double t1 = now_ms();
-  out.colors = color_on_host(g, s, grab);
+  out.colors = color_on_host(g, s, grab, (set->flags & CV_FLAG_FAST_COLORING) != 0);
   out.detection_ms += now_ms() - t0;
   out.narrow_ms += t1 - t0;
 }
@@ -1327,8 +1328,10 @@ void ensure_step_buffers(cv_gpu& g, const cv_solver& s) {
   if (g.h_cap < 3 * n) {
     if (g.h_x) CV_HIP(hipHostFree(g.h_x));
     if (g.h_v) CV_HIP(hipHostFree(g.h_v));
+    if (g.h_acc) CV_HIP(hipHostFree(g.h_acc));
     CV_HIP(hipHostMalloc((void**)&g.h_x, 3 * n * sizeof(double)));
     CV_HIP(hipHostMalloc((void**)&g.h_v, 3 * n * sizeof(double)));
+    CV_HIP(hipHostMalloc((void**)&g.h_acc, 3 * n * sizeof(double)));
     g.h_cap = 3 * n;
   }
 }
@@ -1480,6 +1483,7 @@ void cv_gpu_destroy(cv_gpu* g) {
   if (g->h_scal) (void)hipHostFree(g->h_scal);
   if (g->h_x) (void)hipHostFree(g->h_x);
   if (g->h_v) (void)hipHostFree(g->h_v);
+  if (g->h_acc) (void)hipHostFree(g->h_acc);
   if (g->h_cont) (void)hipHostFree(g->h_cont);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
   if (g->ev1) (void)hipEventDestroy(g->ev1);
@@ -1558,11 +1562,11 @@ static cv_status run_steps(cv_gpu* g, cv_solver* s, cv_state* st, const cv_setti
     CV_HIP(hipMemcpyAsync(g->h_v, g->b.v64.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     // The acceleration history travels with the solver (Prediction::record,
     // src/vbd/prediction.rs:43-54), so another context can continue it.
-    s->acc.resize(3 * n);
-    CV_HIP(hipMemcpyAsync(s->acc.data(), g->dev.acc.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    CV_HIP(hipMemcpyAsync(g->h_acc, g->dev.acc.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     CV_HIP(hipStreamSynchronize(g->stream));
     CV_HIP(hipEventElapsedTime(&gpu_ms, g->ev0, g->ev1));
     // Commit (all checks passed).
+    s->acc.assign(g->h_acc, g->h_acc + 3 * n);
     std::memcpy(st->x, g->h_x, 3 * n * sizeof(double));
     std::memcpy(st->v, g->h_v, 3 * n * sizeof(double));
     st->time += steps * set->dt;
@@ -1798,9 +1802,9 @@ cv_status cv_gpu_step_batch(cv_gpu* g, cv_solver* const* solvers, cv_state* cons
       size_t n = sv.n;
       CV_HIP(hipMemcpyAsync(c.h_x, c.b.xcur64.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
       CV_HIP(hipMemcpyAsync(c.h_v, c.b.v64.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-      sv.acc.resize(3 * n);
-      CV_HIP(hipMemcpyAsync(sv.acc.data(), c.dev.acc.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      CV_HIP(hipMemcpyAsync(c.h_acc, c.dev.acc.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
       CV_HIP(hipStreamSynchronize(c.stream));
+      sv.acc.assign(c.h_acc, c.h_acc + 3 * n);
       std::memcpy(stt->x, c.h_x, 3 * n * sizeof(double));
       std::memcpy(stt->v, c.h_v, 3 * n * sizeof(double));
       stt->time += settings->dt;

@@ -36,8 +36,13 @@ def lib():
         L.cvh_solver_edges.argtypes = [vp, abi.u32p, C.c_uint32]
         L.cvh_solver_edges.restype = C.c_uint32
         L.cvh_greedy_colors.argtypes = [vp, abi.u32p, abi.u32p, C.c_uint64, C.POINTER(abi.cv_grab),
-                                        abi.u32p]
+                                        abi.u32p, C.c_int]
         L.cvh_greedy_colors.restype = C.c_uint32
+        L.cvh_coloring_new.restype = C.c_void_p
+        L.cvh_coloring_free.argtypes = [C.c_void_p]
+        L.cvh_coloring_update.argtypes = [vp, C.c_void_p, abi.u32p, abi.u32p, C.c_uint64, C.POINTER(abi.cv_grab),
+                                          C.c_int, abi.u32p]
+        L.cvh_coloring_update.restype = C.c_uint32
         for name in ("cvh_term_deriv_f64", "cvh_term_deriv_f32"):
             getattr(L, name).argtypes = [C.c_void_p, abi.f64p, abi.f64p, C.c_double, C.c_double,
                                          abi.f64p]
@@ -100,14 +105,38 @@ class HostSolver:
         k = lib().cvh_solver_edges(self.h, abi.ptr(out, C.c_uint32), 1 << 21)
         return out[:k].copy()
 
-    def greedy_colors(self, kind_ids, counts, grab=None):
+    def greedy_colors(self, kind_ids, counts, grab=None, smallest_last=False):
         ki = np.ascontiguousarray(kind_ids, np.uint32).reshape(-1, 5)
         cn = np.ascontiguousarray(counts, np.uint32)
         out = np.zeros(self.n, np.uint32)
         from .vbd import _grab
         g = _grab(grab)
         k = lib().cvh_greedy_colors(self.h, abi.ptr(ki, C.c_uint32), abi.ptr(cn, C.c_uint32), len(cn),
-                                    C.byref(g) if g else None, abi.ptr(out, C.c_uint32))
+                                    C.byref(g) if g else None, abi.ptr(out, C.c_uint32), int(smallest_last))
+        return out, int(k)
+
+
+class IncrementalColoring:
+    """update_colors (the step's colouring kept across detections) for tests."""
+
+    def __init__(self, solver: "HostSolver"):
+        self.solver = solver
+        self.h = lib().cvh_coloring_new()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().cvh_coloring_free(self.h)
+            self.h = None
+
+    def update(self, kind_ids, counts, grab=None, smallest_last=False):
+        ki = np.ascontiguousarray(kind_ids, np.uint32).reshape(-1, 5)
+        cn = np.ascontiguousarray(counts, np.uint32)
+        out = np.zeros(self.solver.n, np.uint32)
+        from .vbd import _grab
+        g = _grab(grab)
+        k = lib().cvh_coloring_update(self.solver.h, self.h, abi.ptr(ki, C.c_uint32), abi.ptr(cn, C.c_uint32),
+                                      len(cn), C.byref(g) if g else None, int(smallest_last),
+                                      abi.ptr(out, C.c_uint32))
         return out, int(k)
 
 

@@ -134,7 +134,11 @@ typedef struct cv_settings {
 enum {
   CV_FLAG_DIAGNOSTICS_OFF = 1u << 0, /* skip max_seam_gap / extension_percent metrics      */
   CV_FLAG_TRUE_RESIDUAL = 1u << 1,   /* compute the post-sweep residual every iteration     */
-  /* 1u << 2 is reserved (rejected) */
+  CV_FLAG_FAST_COLORING = 1u << 2,   /* greedy colouring in smallest-last order of the static
+                                        conflict graph: fewer colours and launches, but a
+                                        different Gauss-Seidel order than the reference's
+                                        vertex-order colouring (src/vbd/coloring.rs:5-43),
+                                        so steps are not the reference's steps            */
   CV_FLAG_WARM_START = 1u << 3       /* learned initial guess in place of the adaptive one  */
 };
 

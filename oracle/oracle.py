@@ -83,7 +83,8 @@ def lib():
         L.cvo_solver_detect_and_color.argtypes = [vp, abi.f64p, C.c_double,
                                                   C.POINTER(abi.cv_ogc_settings),
                                                   C.POINTER(abi.cv_grab), abi.u32p, abi.u32p,
-                                                  C.POINTER(cvo_contacts), C.c_char_p, C.c_size_t]
+                                                  C.POINTER(cvo_contacts), C.c_char_p, C.c_size_t,
+                                                  C.c_int]
         L.cvo_solver_local_gradient.argtypes = [vp, abi.f64p, abi.f64p, abi.f64p,
                                                 C.POINTER(abi.cv_settings), C.POINTER(abi.cv_grab),
                                                 C.c_int, abi.f64p, C.c_char_p, C.c_size_t]
@@ -251,8 +252,9 @@ class Solver:
         k = lib().cvo_solver_last_colors(self.h, abi.ptr(out, C.c_uint32), self.n)
         return out, int(k)
 
-    def detect_and_color(self, x_moving, rq, ogc=None, grab=None):
-        """OgcWorld::detect at x plus colors(); the set becomes the one
+    def detect_and_color(self, x_moving, rq, ogc=None, grab=None, smallest_last=False):
+        """OgcWorld::detect at x plus colors() (in smallest-last order of the
+        static graph with smallest_last); the set becomes the one
         local_gradient(use_last_contacts=True) evaluates."""
         x = _f(x_moving)
         cols = np.zeros(self.n, np.uint32)
@@ -265,7 +267,7 @@ class Solver:
             rc = lib().cvo_solver_detect_and_color(self.h, _p(x), rq, C.byref(ogc),
                                                    C.byref(g) if g else None,
                                                    abi.ptr(cols, C.c_uint32), C.byref(k),
-                                                   C.byref(out), err, 512)
+                                                   C.byref(out), err, 512, int(smallest_last))
             if rc != 0:
                 raise OracleError(rc, err.value.decode())
 

@@ -131,6 +131,7 @@ VbdSettings settings_of(const cv_settings* s) {
   o.contact.relaxation = s->contact.relaxation;
   o.contact.refresh_fraction = s->contact.refresh_fraction;
   o.residual_tolerance = s->residual_tolerance;
+  o.smallest_last = (s->flags & CV_FLAG_FAST_COLORING) != 0;
   return o;
 }
 OgcSettings ogc_of(const cv_ogc_settings* s) {
@@ -325,13 +326,15 @@ uint32_t cvo_solver_last_colors(const cvo_solver* s, uint32_t* color_of_vertex, 
 int cvo_solver_detect_and_color(cvo_solver* s, const double* x_moving, double rq,
                                 const cv_ogc_settings* settings, const cv_grab* grab,
                                 uint32_t* color_of_vertex, uint32_t* n_colors, cvo_contacts* out,
-                                char* err, size_t errn) {
+                                char* err, size_t errn, int smallest_last_order) {
   return guarded(err, errn, [&] {
     auto x = s->s.world.positions(v3s(x_moving, s->s.garment.n));
     ContactSet c = s->s.world.detect(x, rq, ogc_of(settings));
     GrabT g;
     if (grab) g = grab_of(grab);
-    auto cols = colors(s->s.graph, c, grab ? &g : nullptr);
+    std::vector<int> sl;
+    if (smallest_last_order) sl = smallest_last(s->s.graph);
+    auto cols = colors(s->s.graph, c, grab ? &g : nullptr, smallest_last_order ? &sl : nullptr);
     for (size_t ci = 0; ci < cols.size(); ++ci)
       for (int v : cols[ci]) color_of_vertex[v] = (uint32_t)ci;
     *n_colors = (uint32_t)cols.size();

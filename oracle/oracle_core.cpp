@@ -427,7 +427,10 @@ VbdMetrics VbdSolver::step(State& state, const VbdSettings& st, const GrabT* gra
     m.profile.aabb_rejections += p.aabb_rejections;
   };
   addp(contacts.profile);
-  auto cols = colors(graph, contacts, grab);
+  std::vector<int> sl;
+  if (st.smallest_last) sl = smallest_last(graph);
+  const std::vector<int>* order = st.smallest_last ? &sl : nullptr;
+  auto cols = colors(graph, contacts, grab, order);
   uint64_t clipped = 0;
   for (int i = 0; i < n; ++i) {
     if (!garment.pins[i]) {
@@ -447,7 +450,7 @@ VbdMetrics VbdSolver::step(State& state, const VbdSettings& st, const GrabT* gra
     if (clipped > 0 && (double)clipped >= st.contact.refresh_fraction * n) {
       double t = now_ms();
       contacts = world.detect(x, rq, st.contact);
-      cols = colors(graph, contacts, grab);
+      cols = colors(graph, contacts, grab, order);
       m.detection_ms += now_ms() - t;
       m.detection_updates += 1;
       m.broad_phase_rebuilds += contacts.broad_phase_rebuilt ? 1 : 0;

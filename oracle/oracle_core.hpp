@@ -817,14 +817,18 @@ struct GrabT {
   double stiffness;
   bool contains(int v) const { return vertices[0] == v || vertices[1] == v || vertices[2] == v; }
 };
+// `order` (optional): the vertex sequence to colour in; the reference colours
+// in index order.  The fast-colouring extension passes smallest_last(graph).
 inline std::vector<std::vector<int>> colors(const std::vector<std::vector<int>>& graph,
-                                            const ContactSet& contact, const GrabT* grab) {
+                                            const ContactSet& contact, const GrabT* grab,
+                                            const std::vector<int>* order = nullptr) {
   int n = (int)graph.size();
   const int UNSET = -1;
   std::vector<int> assigned(n, UNSET);
   std::vector<std::vector<int>> out;
   std::vector<int> seen(n + 1, -1);
-  for (int v = 0; v < n; ++v) {
+  for (int idx = 0; idx < n; ++idx) {
+    const int v = order ? (*order)[idx] : idx;
     auto mark = [&](int i) {
       if (i < n && assigned[i] != UNSET) seen[assigned[i]] = v;
     };
@@ -844,6 +848,37 @@ inline std::vector<std::vector<int>> colors(const std::vector<std::vector<int>>&
   return out;
 }
 
+// Smallest-last order (extension, no reference counterpart): remove a vertex
+// of least remaining degree, ties to the smaller index, until none is left;
+// colour in the reverse removal order.  Restated here with bucket lists.
+inline std::vector<int> smallest_last(const std::vector<std::vector<int>>& graph) {
+  int n = (int)graph.size();
+  std::vector<int> deg(n);
+  int maxd = 0;
+  for (int v = 0; v < n; ++v) {
+    deg[v] = (int)graph[v].size();
+    maxd = std::max(maxd, deg[v]);
+  }
+  std::vector<std::set<int>> bucket(maxd + 1);
+  for (int v = 0; v < n; ++v) bucket[deg[v]].insert(v);
+  std::vector<char> gone(n, 0);
+  std::vector<int> removal;
+  for (int step = 0; step < n; ++step) {
+    int d = 0;
+    while (bucket[d].empty()) ++d;
+    int v = *bucket[d].begin();
+    bucket[d].erase(bucket[d].begin());
+    gone[v] = 1;
+    removal.push_back(v);
+    for (int u : graph[v])
+      if (u < n && !gone[u]) {
+        bucket[deg[u]].erase(u);
+        bucket[--deg[u]].insert(u);
+      }
+  }
+  return std::vector<int>(removal.rbegin(), removal.rend());
+}
+
 // ------------------------------------------------------------ prediction
 struct VbdSettings {  // src/vbd.rs:18-52
   double dt = 1.0 / 240.0;
@@ -854,6 +889,8 @@ struct VbdSettings {  // src/vbd.rs:18-52
   // Extension with no reference counterpart: > 0 ends the iterations once the
   // post-sweep (true) residual is <= this value.
   double residual_tolerance = 0.0;
+  // Extension: colour in smallest-last order of the static graph.
+  bool smallest_last = false;
   void validate() const {
     contact.validate();
     if (!std::isfinite(dt) || !(dt >= 1e-6 && dt <= 0.025) || iterations == 0 || iterations > 200 ||

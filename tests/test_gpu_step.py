@@ -238,7 +238,7 @@ def test_unknown_settings_flags_are_rejected_transactionally(gpu):
     sol = VbdSolver(sc.garment, None, st)
     old = st.copy()
     with pytest.raises(VbdError) as e:
-        gpu.step(sol, st, dict(sc.settings, flags=1 << 2))
+        gpu.step(sol, st, dict(sc.settings, flags=1 << 4))
     assert e.value.message == "Invalid VBD settings"
     assert np.array_equal(st.x, old.x) and st.time == old.time
 
@@ -357,3 +357,18 @@ def test_sweep_lane_shapes_match_oracle(gpu, lanes):
             assert err <= tol
     finally:
         gpu.set_lanes(32)
+
+
+def test_fast_colouring_step_matches_the_oracle_in_the_same_mode(gpu):
+    """CV_FLAG_FAST_COLORING (smallest-last order of the static graph): the
+    device step equals the oracle's step in the same colouring mode, with no
+    more colours than the reference's vertex-order colouring."""
+    from clothcraft_amd.abi import CV_FLAG_FAST_COLORING
+    for sc, tol in ((scenes.hanging_sheet(), 1e-9), (scenes.small_sheet_on_sphere(24, 0.2535), 1e-5)):
+        s = dict(sc.settings, flags=CV_FLAG_FAST_COLORING)
+        sg, sr, mg, mr = _one_step(gpu, sc, settings=s)
+        _, _, mplain, _ = _one_step(gpu, sc)
+        err = np.abs(sg.x - sr.x).max() / _bbox_diag(sr.x)
+        print(f"{sc.name} fast colouring: rel err {err:.3e} colours {mg['colors']} (vertex order {mplain['colors']})")
+        assert mg["colors"] == mr["colors"] <= mplain["colors"]
+        assert err <= tol
