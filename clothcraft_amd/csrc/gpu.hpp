@@ -139,6 +139,7 @@ struct StepBufs {
   DBuf<uint32_t> item_len, item_off, item_start, over, pair_owner;  // flattened broad phase
   DBuf<uint2> pair_list;  // candidates for the narrow phase: VF from the front, EE from the back
   DBuf<uint32_t> scan_sums;  // tile totals of the two-level scan
+  DBuf<double> red_parts;    // per-block partials of the prologue reduction
   DBuf<unsigned long long> ext_key, ext_key_sorted;  // diagnostics: per-triangle extension
   DBuf<double> ext_area, ext_area_sorted, ext_prefix;
   DBuf<unsigned long long> item_key;

@@ -459,17 +459,41 @@ __global__ void k_items(DetectArgs a, uint32_t* len, uint32_t* start, unsigned l
     for (; c < kEdgeCells; ++c) len[base + c] = 0;
   }
 }
-// Owner item of every flattened pair slot; slots past `cap` raise the
-// capacity flag (the host grows the buffer and detects again).
-__global__ void k_item_owner(const uint32_t* off, uint32_t NI, uint32_t* owner, uint64_t cap, Scalars* sc) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= NI) return;
-  uint32_t o0 = off[i], o1 = off[i + 1];
-  if (o1 > cap) {
-    atomicOr(&sc->err_overflow, 4u);
+// Owner item of every flattened pair slot.  Each wave takes 64 consecutive
+// items and writes their contiguous slot range together, one slot per lane
+// (an item's slots can number in the hundreds, a thread per item would write
+// them serially); a slot's item is found by binary search over the wave's 65
+// offsets staged in LDS.  A range ending past `cap` raises the capacity flag
+// (the host grows the buffer and detects again; the broad pass examines
+// nothing meanwhile).
+__global__ __launch_bounds__(TPB) void k_item_owner(const uint32_t* off, uint32_t NI, uint32_t* owner, uint64_t cap,
+                                                    Scalars* sc) {
+  constexpr uint32_t WPB = TPB / 64;
+  __shared__ uint32_t so[WPB][65];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t base = (blockIdx.x * WPB + w) * 64;
+  const bool live = base < NI;
+  const uint32_t cnt = live ? min(64u, NI - base) : 0;
+  uint32_t* o = so[w];
+  if (live) {
+    if (lane < cnt) o[lane] = off[base + lane];
+    if (lane == 0) o[cnt] = off[base + cnt];
+  }
+  __syncthreads();
+  if (!live) return;
+  const uint32_t lo = o[0], hi = o[cnt];
+  if (hi > cap) {
+    if (lane == 0) atomicOr(&sc->err_overflow, 4u);
     return;
   }
-  for (uint32_t o = o0; o < o1; ++o) owner[o] = i;
+  for (uint32_t s = lo + lane; s < hi; s += 64) {
+    uint32_t a = 0, b = cnt;  // o[a] <= s < o[b]
+    while (b - a > 1) {
+      uint32_t m = (a + b) >> 1;
+      if (o[m] <= s) a = m; else b = m;
+    }
+    owner[s] = base + a;
+  }
 }
 // Broad pass: one (query item, bucket entry) pair per thread, grid-stride;
 // the candidates go to the narrow-phase list.
@@ -656,22 +680,23 @@ __global__ void k_contact_records(const unsigned long long* inc_sorted, uint64_t
   rec[i] = r;
 }
 // Radii and minimum distance over moving vertices (single block, fixed order).
-__global__ void k_radii(const unsigned long long* bits, uint32_t n, double relaxation, double rq,
-                        double* radii, Scalars* sc) {
-  __shared__ double red[1024];
-  double m = rq;
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    double d = __longlong_as_double((long long)bits[i]);
-    radii[i] = relaxation * d;
-    m = fmin(m, d);
+// Radii relaxation * min(d, rq) per moving vertex and the minimum distance
+// (order-independent: an atomic minimum on the bit patterns of non-negative
+// doubles; sc->min_distance starts at rq).
+__global__ void k_radii(const unsigned long long* bits, uint32_t n, double relaxation, double* radii,
+                        Scalars* sc) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long m = ~0ull;
+  if (i < n) {
+    unsigned long long b = bits[i];
+    radii[i] = relaxation * __longlong_as_double((long long)b);
+    m = b;
   }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long t = __shfl_xor(m, o, 64);
+    m = t < m ? t : m;
   }
-  if (threadIdx.x == 0) sc->min_distance = red[0];
+  if ((threadIdx.x & 63) == 0 && m != ~0ull) atomicMin((unsigned long long*)&sc->min_distance, m);
 }
 
 inline unsigned blocks(uint64_t n) { return (unsigned)((n + TPB - 1) / TPB); }
@@ -1027,7 +1052,9 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   }
   scan_u32(g, b.vcount.p, b.coff.p, (size_t)d.n + 1);
   b.radii64.reserve(d.n);
-  k_radii<<<1, 1024, 0, st>>>(b.dist_bits.p, d.n, ogc.relaxation, rq, b.radii64.p, g.d_scal);
+  k_store_u64<<<1, 1, 0, st>>>((unsigned long long*)&g.d_scal->min_distance,
+                               (unsigned long long)__builtin_bit_cast(uint64_t, rq));
+  k_radii<<<blocks(d.n), TPB, 0, st>>>(b.dist_bits.p, d.n, ogc.relaxation, b.radii64.p, g.d_scal);
   read_scalars(g);
   uint64_t ninc = g.h_scal->incidences;
   b.crec.reserve(std::max<uint64_t>(ninc, 1));

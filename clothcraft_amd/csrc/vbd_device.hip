@@ -432,11 +432,18 @@ __global__ void k_warm_guess(const float* pred, const double* x, const double* v
 // Step prologue reductions (single block, fixed order => deterministic):
 // origin = centroid (src/gpu_vbd.rs:156), predicted motion max|target - x|
 // (src/vbd.rs:298-302) and max |x|_inf of the moving part (padded radius).
-__global__ void k_prologue_reduce(const double* x, const double* v, uint32_t n, double dt, double damp,
-                                  double gx, double gy, double gz, Scalars* sc) {
-  __shared__ double sx[1024], sy[1024], sz[1024], sm[1024], sa[1024];
+// Centroid (the origin of the sweep's relative positions), largest inertial
+// motion |target - x| (for rq, src/vbd.rs:298-304) and largest |x|_inf (the
+// detection's padding scale), in two deterministic passes: fixed contiguous
+// chunks per block, then the chunks' partials in block order.
+constexpr uint32_t kPrologueChunk = 2048;
+__global__ __launch_bounds__(256) void k_prologue_partial(const double* x, const double* v, uint32_t n, double dt,
+                                                          double damp, double gx, double gy, double gz,
+                                                          double* part) {
+  __shared__ double red[5][256];
   double ax = 0, ay = 0, az = 0, mo = 0, mabs = 0;
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+  uint32_t i0 = blockIdx.x * kPrologueChunk, i1 = min(n, i0 + kPrologueChunk);
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     double px = x[3 * i], py = x[3 * i + 1], pz = x[3 * i + 2];
     ax += px;
     ay += py;
@@ -448,29 +455,36 @@ __global__ void k_prologue_reduce(const double* x, const double* v, uint32_t n, 
     mo = fmax(mo, sqrt(dx * dx + dy * dy + dz * dz));
     mabs = fmax(mabs, fmax(fabs(px), fmax(fabs(py), fabs(pz))));
   }
-  sx[threadIdx.x] = ax;
-  sy[threadIdx.x] = ay;
-  sz[threadIdx.x] = az;
-  sm[threadIdx.x] = mo;
-  sa[threadIdx.x] = mabs;
+  red[0][threadIdx.x] = ax;
+  red[1][threadIdx.x] = ay;
+  red[2][threadIdx.x] = az;
+  red[3][threadIdx.x] = mo;
+  red[4][threadIdx.x] = mabs;
   __syncthreads();
   for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) {
-      sx[threadIdx.x] += sx[threadIdx.x + s];
-      sy[threadIdx.x] += sy[threadIdx.x + s];
-      sz[threadIdx.x] += sz[threadIdx.x + s];
-      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
-      sa[threadIdx.x] = fmax(sa[threadIdx.x], sa[threadIdx.x + s]);
+      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+      for (int k = 3; k < 5; ++k) red[k][threadIdx.x] = fmax(red[k][threadIdx.x], red[k][threadIdx.x + s]);
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    sc->origin[0] = sx[0] / n;
-    sc->origin[1] = sy[0] / n;
-    sc->origin[2] = sz[0] / n;
-    sc->motion = sm[0];
-    sc->scale = sa[0];
+  if (threadIdx.x < 5) part[5 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+__global__ void k_prologue_final(const double* part, uint32_t nparts, uint32_t n, Scalars* sc) {
+  if (threadIdx.x != 0) return;
+  double ax = 0, ay = 0, az = 0, mo = 0, mabs = 0;
+  for (uint32_t b = 0; b < nparts; ++b) {
+    ax += part[5 * b];
+    ay += part[5 * b + 1];
+    az += part[5 * b + 2];
+    mo = fmax(mo, part[5 * b + 3]);
+    mabs = fmax(mabs, part[5 * b + 4]);
   }
+  sc->origin[0] = ax / n;
+  sc->origin[1] = ay / n;
+  sc->origin[2] = az / n;
+  sc->motion = mo;
+  sc->scale = mabs;
 }
 
 // Targets, inertia, adaptive guess (src/vbd.rs:283-297, src/vbd/prediction.rs:15-41).
@@ -741,6 +755,16 @@ uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab, bool 
   return k;
 }
 
+// Origin, motion and scale of the positions in b.x64 into the scalars.
+void prologue_reduce(cv_gpu& g, uint32_t n, double dt, double damp, const double* gravity) {
+  uint32_t parts = std::max<uint32_t>(1, (n + kPrologueChunk - 1) / kPrologueChunk);
+  g.b.red_parts.reserve(5 * (size_t)parts);
+  k_prologue_partial<<<parts, 256, 0, g.stream>>>(g.b.x64.p, g.b.v64.p, n, dt, damp, gravity[0], gravity[1],
+                                                  gravity[2], g.b.red_parts.p);
+  k_prologue_final<<<1, 64, 0, g.stream>>>(g.b.red_parts.p, parts, n, g.d_scal);
+  CV_HIP(hipGetLastError());
+}
+
 template <class R>
 void true_residual(cv_gpu& g, const SweepArgs<R>& a, uint32_t n, hipStream_t st) {
   switch (g.lanes) {
@@ -853,8 +877,7 @@ struct StepCtx {
 
   void prologue_reduce() {
     double dt = set->dt, damp = 1.0 + set->damping * dt;
-    k_prologue_reduce<<<1, 1024, 0, g.stream>>>(g.b.x64.p, g.b.v64.p, s.n, dt, damp, set->gravity[0],
-                                                set->gravity[1], set->gravity[2], g.d_scal);
+    ::prologue_reduce(g, s.n, dt, damp, set->gravity);
     CV_HIP(hipGetLastError());
     CV_HIP(hipMemcpyAsync(g.h_scal, g.d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, g.stream));
   }
@@ -1625,8 +1648,7 @@ cv_status cv_gpu_probe(cv_gpu* g, cv_solver* s, const cv_state* st, const cv_set
     size_t n = s->n;
     hipStream_t stream = g->stream;
     double dt = set->dt, damp = 1.0 + set->damping * dt;
-    k_prologue_reduce<<<1, 1024, 0, stream>>>(g->b.x64.p, g->b.v64.p, (uint32_t)n, dt, damp,
-                                              set->gravity[0], set->gravity[1], set->gravity[2], g->d_scal);
+    prologue_reduce(*g, (uint32_t)n, dt, damp, set->gravity);
     read_scalars(*g);
     if (g->f64)
       probe_t<double>(*g, *s, st, set, grab, out);
@@ -1938,8 +1960,7 @@ cv_status cv_warm_features(cv_gpu* g, cv_solver* s, const cv_state* st, const cv
     g->pred_valid = false;
     uint32_t n = s->n;
     double dt = set->dt, damp = 1.0 + set->damping * dt;
-    k_prologue_reduce<<<1, 1024, 0, g->stream>>>(g->b.x64.p, g->b.v64.p, n, dt, damp, set->gravity[0],
-                                                 set->gravity[1], set->gravity[2], g->d_scal);
+    prologue_reduce(*g, n, dt, damp, set->gravity);
     read_scalars(*g);
     double r = set->contact.radius;
     double rq = std::min(std::max(r + g->h_scal->motion, 2.0 * r), 4.0 * r);
