@@ -88,7 +88,7 @@ def main():
                          "seams on a capsule body, 40 iterations; batch64: 64 independent sheets and "
                          "sewn tubes of 1k-50k vertices in batched steps (not the headline line)")
     ap.add_argument("--fast-coloring", action="store_true",
-                    help="smallest-last colouring (fewer colours; not the reference's colouring order)")
+                    help="fast colouring on the device (fewer colours; not the reference's colouring order)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batch64: use --lanes instead of timing the lanes per vertex first")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -213,8 +213,8 @@ def main():
              f"{settings['iterations']} iterations, dt 1/60, self-contact on"),
             "steps_timed": f"1..{K} of the drop (warmup on a discarded copy)",
             "iteration_mode": "fixed iterations (residual tolerance 0)",
-            "coloring": ("smallest-last order (CV_FLAG_FAST_COLORING)" if args.fast_coloring
-                         else "reference vertex-order greedy"),
+            "coloring": ("fast: static smallest-last, device repair (CV_FLAG_FAST_COLORING)"
+                         if args.fast_coloring else "reference vertex-order greedy"),
             "lanes_per_vertex": args.lanes,
         },
         "vertex_iters_per_s": vi / dev_s,

@@ -23,6 +23,7 @@ SOURCES = [
     ("vbd_device.hip", []),
     ("ogc_device.hip", ["-ffp-contract=off"]),
     ("mlp_device.hip", []),
+    ("color_device.hip", []),
     ("solver.cpp", ["-ffp-contract=off", "-x", "hip"]),
     ("api.cpp", ["-x", "hip"]),
 ]

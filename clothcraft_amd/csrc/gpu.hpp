@@ -114,6 +114,13 @@ struct DevSolver {
   uint32_t T = 0, P = 0;
   DBuf<uint32_t> dg_tri, dg_pr_off, dg_pr_v;
   DBuf<double> dg_inv, dg_cs, dg_area, dg_pr_w;
+  // fast colouring: static conflict graph, its smallest-last colouring (per
+  // vertex and as a colour-grouped order with host offsets) and the colouring
+  // priority; built on the first fast colouring of this solver
+  bool fc_ready = false;
+  DBuf<uint32_t> graph_off, graph, fc_col0, fc_order0;
+  DBuf<unsigned long long> fc_prio;  // colouring priority (static colour, then hash)
+  std::vector<uint32_t> fc_off0;
   // prediction
   DBuf<double> acc;
   uint64_t acc_version = 0;
@@ -129,6 +136,10 @@ struct StepBufs {
   DBuf<float> gsq;
   DBuf<double> gtrue;  // per-vertex |g|^2 of the true residual
   DBuf<uint32_t> order;
+  // fast colouring: contact-neighbour CSR, round states, pending-round
+  // flags, colour histogram, sort keys
+  DBuf<uint32_t> fc_cnt, fc_off, fc_cur, fc_nb, fc_state, fc_pend, fc_hist, fc_key, fc_key_sorted,
+      fc_val;
   // detection
   DBuf<double> fbox, ebox;  // [6F], [6E]
   DBuf<uint32_t> cell_count, cell_off;
@@ -270,6 +281,10 @@ struct cv_gpu {
   // host-side contact incidence for the greedy colouring
   // Pinned copy of the contact list for the host colouring: kind and ids
   // (5 per contact) then counts (1 per contact), capacity h_cont_cap contacts.
+  uint32_t* h_fc = nullptr;     // pinned read-back of the fast colouring's histogram
+  bool host_contacts = true;    // detection copies the contact list to h_cont
+  bool h_cont_valid = false;    // h_cont holds the last detection's contacts
+  bool detected = false;        // a detection of the current device solver ran
   uint32_t* h_cont = nullptr;
   size_t h_cont_cap = 0;
   std::vector<uint32_t> cnb_off, cnb;
@@ -337,5 +352,11 @@ void gpu_diagnostics(cv_gpu& g, const double* dx);
 
 void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings& ogc,
                 size_t cand_cap);
+// Exclusive scan of count uint32 values on the context's stream.
+void device_scan_u32(cv_gpu& g, const uint32_t* in, uint32_t* out, size_t count);
+// Fast colouring (CV_FLAG_FAST_COLORING) of the last detection's contacts and
+// the grab, on the device: the colour-grouped vertex order into b.order and
+// the colour offsets into color_off; returns the colour count.
+uint32_t gpu_fast_colors(cv_gpu& g, const cv_solver& s, const cv_grab* grab, std::vector<uint32_t>& color_off);
 void ensure_device_solver(cv_gpu& g, const cv_solver& s);
 void ensure_step_buffers(cv_gpu& g, const cv_solver& s);

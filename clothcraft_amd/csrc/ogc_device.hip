@@ -876,6 +876,8 @@ __global__ void k_seam_gap(const double* x, const uint32_t* off, const uint32_t*
 
 }  // namespace
 
+void device_scan_u32(cv_gpu& g, const uint32_t* in, uint32_t* out, size_t count) { scan_u32(g, in, out, count); }
+
 void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings& ogc, size_t cand_cap) {
   DevSolver& d = g.dev;
   StepBufs& b = g.b;
@@ -1026,19 +1028,24 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
     k_store_count<<<1, 1, 0, st>>>(b.uniq_pos.p + m, &g.d_scal->contacts);
     // The contact list for the host colouring, copied with room for all m
     // raw contacts so the copy needs no count first.
-    if (g.h_cont_cap < m) {
+    g.h_cont_valid = g.host_contacts;
+    if (g.host_contacts && g.h_cont_cap < m) {
       if (g.h_cont) CV_HIP(hipHostFree(g.h_cont));
       g.h_cont = nullptr;
       size_t cap = m + m / 2 + 1024;
       CV_HIP(hipHostMalloc((void**)&g.h_cont, 6 * cap * sizeof(uint32_t)));
       g.h_cont_cap = cap;
     }
-    CV_HIP(hipMemcpyAsync(g.h_cont, b.c_ids.p, 5 * m * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    CV_HIP(hipMemcpyAsync(g.h_cont + 5 * g.h_cont_cap, b.c_count.p, m * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                          st));
+    if (g.host_contacts) {
+      CV_HIP(hipMemcpyAsync(g.h_cont, b.c_ids.p, 5 * m * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      CV_HIP(hipMemcpyAsync(g.h_cont + 5 * g.h_cont_cap, b.c_count.p, m * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, st));
+    }
   } else {
+    g.h_cont_valid = true;
     k_store_u64<<<1, 1, 0, st>>>(&g.d_scal->contacts, 0ull);
   }
+  g.detected = true;
   CV_HIP(hipGetLastError());
   // Incidence, active count, per-vertex contact records.
   b.vcount.reserve(d.n + 1);

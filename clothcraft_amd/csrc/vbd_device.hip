@@ -712,7 +712,7 @@ void sweep_color(cv_gpu& g, const SweepArgs<R>& a, uint32_t off, uint32_t count,
 
 // Host greedy colouring of the current contact set (the detection left the
 // contact list in g.h_cont and its count in the scalars it read back).
-uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab, bool smallest_last) {
+uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
   uint64_t m = g.h_scal->contacts;
   const uint32_t* cids = g.h_cont;
   const uint32_t* ccount = g.h_cont + 5 * g.h_cont_cap;
@@ -720,7 +720,7 @@ uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab, bool 
   // The colouring is a function of the solver, the contact list and the grab
   // vertices: when all three equal the last colouring's, it is reused (the
   // order already sits on the device).  Contact-free steps always reuse it.
-  uint32_t key[4] = {(grab ? 1u : 0u) | (smallest_last ? 2u : 0u), grab ? grab->vertices[0] : 0u,
+  uint32_t key[4] = {grab ? 1u : 0u, grab ? grab->vertices[0] : 0u,
                      grab ? grab->vertices[1] : 0u, grab ? grab->vertices[2] : 0u};
   if (g.color_valid && g.color_generation == s.generation && g.color_m == m &&
       std::memcmp(g.color_key, key, sizeof key) == 0 &&
@@ -745,9 +745,28 @@ uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab, bool 
       if (v < n)
         for (uint32_t r = 0; r < ccount[c]; ++r) g.cnb[cur[v]++] = cids[5 * c + 1 + r];
     }
-  uint32_t k = update_colors(s, g.coloring, g.cnb_off.data(), g.cnb.data(), grab, smallest_last, g.order,
+  uint32_t k = update_colors(s, g.coloring, g.cnb_off.data(), g.cnb.data(), grab, false, g.order,
                              g.color_off);
   g.b.order.upload(g.order, g.stream);
+  g.color_valid = true;
+  g.color_generation = s.generation;
+  g.color_m = m;
+  std::memcpy(g.color_key, key, sizeof key);
+  return k;
+}
+
+// The fast colouring (on the device, gpu_fast_colors) of the last detection;
+// reused while the detections find no contacts and the grab is unchanged.
+uint32_t color_on_device(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
+  const uint64_t m = g.h_scal->contacts;
+  uint32_t key[4] = {(grab ? 1u : 0u) | 4u, grab ? grab->vertices[0] : 0u, grab ? grab->vertices[1] : 0u,
+                     grab ? grab->vertices[2] : 0u};
+  if (g.color_valid && g.color_generation == s.generation && g.color_m == 0 && m == 0 &&
+      std::memcmp(g.color_key, key, sizeof key) == 0)
+    return (uint32_t)(g.color_off.size() - 1);
+  g.color_valid = false;
+  g.color_cids.clear();
+  uint32_t k = gpu_fast_colors(g, s, grab, g.color_off);
   g.color_valid = true;
   g.color_generation = s.generation;
   g.color_m = m;
@@ -797,6 +816,12 @@ template <class R>
 void detect_and_color(cv_gpu& g, const cv_solver& s, const cv_settings* set, const cv_grab* grab, double rq,
                       StepOut& out, bool first) {
   double t0 = now_ms();
+  const bool fast = (set->flags & CV_FLAG_FAST_COLORING) != 0;
+  struct Restore {
+    cv_gpu& g;
+    ~Restore() { g.host_contacts = true; }
+  } restore{g};
+  g.host_contacts = !fast;  // the fast colouring reads the contacts on the device
   gpu_detect(g, s, rq, set->contact, 0);
   out.detections++;
   out.candidate_pairs += g.h_scal->vf_cand + g.h_scal->ee_cand;
@@ -812,7 +837,7 @@ void detect_and_color(cv_gpu& g, const cv_solver& s, const cv_settings* set, con
                                               g.b.centers64.p, g.b.guess64.p + 3 * (size_t)n);
   CV_HIP(hipGetLastError());
   double t1 = now_ms();
-  out.colors = color_on_host(g, s, grab, (set->flags & CV_FLAG_FAST_COLORING) != 0);
+  out.colors = fast ? color_on_device(g, s, grab) : color_on_host(g, s, grab);
   out.detection_ms += now_ms() - t0;
   out.narrow_ms += t1 - t0;
 }
@@ -1237,6 +1262,8 @@ void ensure_device_solver(cv_gpu& g, const cv_solver& s) {
   if (d.generation == s.generation) return;
   hipStream_t st = g.stream;
   const OgcTopology& t = s.top;
+  d.fc_ready = false;
+  g.detected = false;
   d.n = s.n;
   d.n_total = t.n_total;
   d.n_fixed = t.n_total - s.n;
@@ -1508,6 +1535,7 @@ void cv_gpu_destroy(cv_gpu* g) {
   if (g->h_v) (void)hipHostFree(g->h_v);
   if (g->h_acc) (void)hipHostFree(g->h_acc);
   if (g->h_cont) (void)hipHostFree(g->h_cont);
+  if (g->h_fc) (void)hipHostFree(g->h_fc);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
   if (g->ev1) (void)hipEventDestroy(g->ev1);
   if (g->join_ev) (void)hipEventDestroy(g->join_ev);
@@ -1654,6 +1682,25 @@ cv_status cv_gpu_probe(cv_gpu* g, cv_solver* s, const cv_state* st, const cv_set
       probe_t<double>(*g, *s, st, set, grab, out);
     else
       probe_t<float>(*g, *s, st, set, grab, out);
+  });
+}
+
+cv_status cv_gpu_colors(cv_gpu* g, cv_solver* s, const cv_grab* grab, uint32_t flags, uint32_t* color_of_vertex,
+                        uint32_t n, uint32_t* n_colors) {
+  if (!g || !s || !color_of_vertex || n != s->n) return CV_ERR_INVALID_ARGUMENT;
+  return guard(&g->last_error, [&] {
+    CV_HIP(hipSetDevice(g->device));
+    const bool fast = (flags & CV_FLAG_FAST_COLORING) != 0;
+    if (g->dev.generation != s->generation || !g->detected || (!fast && !g->h_cont_valid))
+      fail(CV_ERR_INVALID_ARGUMENT, "cv_gpu_colors needs a cv_gpu_detect of this solver first");
+    validate_grab(*s, grab);
+    uint32_t k = fast ? color_on_device(*g, *s, grab) : color_on_host(*g, *s, grab);
+    std::vector<uint32_t> order(n);
+    CV_HIP(hipMemcpyAsync(order.data(), g->b.order.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+    CV_HIP(hipStreamSynchronize(g->stream));
+    for (uint32_t c = 0; c < k; ++c)
+      for (uint32_t i = g->color_off[c]; i < g->color_off[c + 1]; ++i) color_of_vertex[order[i]] = c;
+    if (n_colors) *n_colors = k;
   });
 }
 

@@ -64,6 +64,8 @@ def lib():
         L.cv_gpu_detect.argtypes = [vp, vp, abi.f64p, C.c_double, C.POINTER(abi.cv_ogc_settings),
                                     C.POINTER(abi.cv_detection), abi.u32p, C.c_uint64, abi.u32p,
                                     C.c_uint64, abi.u32p, C.c_uint64, abi.f64p, abi.f64p]
+        L.cv_gpu_colors.argtypes = [vp, vp, C.POINTER(abi.cv_grab), C.c_uint32, abi.u32p, C.c_uint32,
+                                    abi.u32p]
         L.cv_gpu_step_batch.argtypes = [vp, C.POINTER(vp), C.POINTER(C.POINTER(abi.cv_state)),
                                         C.c_uint32, C.POINTER(abi.cv_settings),
                                         C.POINTER(abi.cv_metrics)]
@@ -317,6 +319,20 @@ class GpuVbd:
             out["vf"] = vf[:int(d.n_vf_candidates)].copy()
             out["ee"] = ee[:int(d.n_ee_candidates)].copy()
         return out
+
+    def colors(self, solver: VbdSolver, grab=None, fast: bool = False):
+        """Colouring of the last detect()'s contacts (and the grab) as a step
+        colours them: the reference's greedy in vertex order, or the fast
+        colouring on the device.  Returns (colour of each vertex, count)."""
+        out = np.zeros(solver.n, np.uint32)
+        k = C.c_uint32()
+        g = _grab(grab)
+        rc = lib().cv_gpu_colors(self.h, solver.h, C.byref(g) if g else None,
+                                 abi.CV_FLAG_FAST_COLORING if fast else 0, abi.ptr(out, C.c_uint32),
+                                 solver.n, C.byref(k))
+        if rc != 0:
+            self._raise(rc)
+        return out, int(k.value)
 
 
 WARM_PARAMS = 35075

@@ -134,11 +134,15 @@ typedef struct cv_settings {
 enum {
   CV_FLAG_DIAGNOSTICS_OFF = 1u << 0, /* skip max_seam_gap / extension_percent metrics      */
   CV_FLAG_TRUE_RESIDUAL = 1u << 1,   /* compute the post-sweep residual every iteration     */
-  CV_FLAG_FAST_COLORING = 1u << 2,   /* greedy colouring in smallest-last order of the static
-                                        conflict graph: fewer colours and launches, but a
-                                        different Gauss-Seidel order than the reference's
-                                        vertex-order colouring (src/vbd/coloring.rs:5-43),
-                                        so steps are not the reference's steps            */
+  CV_FLAG_FAST_COLORING = 1u << 2,   /* colouring on the device: the static conflict graph
+                                        coloured once in smallest-last order, and after
+                                        each detection the conflicts that contacts and the
+                                        grab add repaired by Jones-Plassmann rounds (hash
+                                        priority).  Fewer colours and launches and no host
+                                        round trip, but a different Gauss-Seidel order
+                                        than the reference's vertex-order colouring
+                                        (src/vbd/coloring.rs:5-43), so steps are not the
+                                        reference's steps                                 */
   CV_FLAG_WARM_START = 1u << 3       /* learned initial guess in place of the adaptive one  */
 };
 
@@ -244,6 +248,14 @@ cv_status cv_gpu_detect(cv_gpu* gpu, cv_solver* solver, const double* x, double 
                         uint32_t* ee_pairs, uint64_t ee_capacity,
                         uint32_t* contact_kind_ids, uint64_t contact_capacity,
                         double* contact_eps, double* radii);
+
+/* Colouring of the contact set that the last cv_gpu_detect of this solver
+ * found, with the grab (NULL for none), exactly as a step colours it: the
+ * reference's greedy in vertex order (src/vbd/coloring.rs:5-43), or with
+ * CV_FLAG_FAST_COLORING in `flags` the fast colouring on the device.  Writes
+ * the colour of each moving vertex; returns the colour count in *n_colors. */
+cv_status cv_gpu_colors(cv_gpu* gpu, cv_solver* solver, const cv_grab* grab, uint32_t flags,
+                        uint32_t* color_of_vertex, uint32_t n, uint32_t* n_colors);
 
 /* Batched uneven scenes (no reference counterpart): one step of `count`
  * independent solvers of any sizes.  Each solver gets exactly the device work

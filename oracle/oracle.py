@@ -252,10 +252,11 @@ class Solver:
         k = lib().cvo_solver_last_colors(self.h, abi.ptr(out, C.c_uint32), self.n)
         return out, int(k)
 
-    def detect_and_color(self, x_moving, rq, ogc=None, grab=None, smallest_last=False):
+    def detect_and_color(self, x_moving, rq, ogc=None, grab=None, smallest_last=False, fast=False):
         """OgcWorld::detect at x plus colors() (in smallest-last order of the
-        static graph with smallest_last); the set becomes the one
-        local_gradient(use_last_contacts=True) evaluates."""
+        static graph with smallest_last; the fast colouring, fast_colors, with
+        fast); the set becomes the one local_gradient(use_last_contacts=True)
+        evaluates."""
         x = _f(x_moving)
         cols = np.zeros(self.n, np.uint32)
         k = C.c_uint32()
@@ -267,7 +268,8 @@ class Solver:
             rc = lib().cvo_solver_detect_and_color(self.h, _p(x), rq, C.byref(ogc),
                                                    C.byref(g) if g else None,
                                                    abi.ptr(cols, C.c_uint32), C.byref(k),
-                                                   C.byref(out), err, 512, int(smallest_last))
+                                                   C.byref(out), err, 512,
+                                                   2 if fast else int(smallest_last))
             if rc != 0:
                 raise OracleError(rc, err.value.decode())
 
@@ -596,11 +598,13 @@ class Prediction:
         lib().cvo_prediction_record(self.h, _p(ov), C.byref(state.to_c()), C.byref(settings_c(settings)))
 
 
-def colors(graph, kind_ids, counts, grab=None):
-    """coloring::colors (src/vbd/coloring.rs:5-43) on an explicit graph."""
+def colors(graph, kind_ids, counts, grab=None, mode=0):
+    """coloring::colors (src/vbd/coloring.rs:5-43) on an explicit graph
+    (mode 0); mode 1 the same greedy in smallest-last order, mode 2 the fast
+    colouring (fast_colors in oracle_core.hpp)."""
     L = lib()
     L.cvo_colors.argtypes = [C.c_uint32, abi.u32p, abi.u32p, abi.u32p, abi.u32p, C.c_uint64,
-                             C.POINTER(abi.cv_grab), abi.u32p]
+                             C.POINTER(abi.cv_grab), abi.u32p, C.c_int]
     L.cvo_colors.restype = C.c_uint32
     n = len(graph)
     off = np.zeros(n + 1, np.uint32)
@@ -612,7 +616,8 @@ def colors(graph, kind_ids, counts, grab=None):
     out = np.zeros(n, np.uint32)
     g = grab_c(grab)
     k = L.cvo_colors(n, abi.ptr(off, C.c_uint32), abi.ptr(nb, C.c_uint32), abi.ptr(ki, C.c_uint32),
-                     abi.ptr(cn, C.c_uint32), len(cn), C.byref(g) if g else None, abi.ptr(out, C.c_uint32))
+                     abi.ptr(cn, C.c_uint32), len(cn), C.byref(g) if g else None, abi.ptr(out, C.c_uint32),
+                     mode)
     groups = [[] for _ in range(k)]
     for v in range(n):
         groups[int(out[v])].append(v)

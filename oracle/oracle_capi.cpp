@@ -131,7 +131,7 @@ VbdSettings settings_of(const cv_settings* s) {
   o.contact.relaxation = s->contact.relaxation;
   o.contact.refresh_fraction = s->contact.refresh_fraction;
   o.residual_tolerance = s->residual_tolerance;
-  o.smallest_last = (s->flags & CV_FLAG_FAST_COLORING) != 0;
+  o.fast_coloring = (s->flags & CV_FLAG_FAST_COLORING) != 0;
   return o;
 }
 OgcSettings ogc_of(const cv_ogc_settings* s) {
@@ -320,21 +320,28 @@ uint32_t cvo_solver_last_colors(const cvo_solver* s, uint32_t* color_of_vertex, 
       if ((uint32_t)v < n) color_of_vertex[v] = (uint32_t)c;
   return (uint32_t)s->s.last_colors.size();
 }
+// Colourings of a conflict graph, contact set and grab: 0 the reference's
+// greedy in index order (src/vbd/coloring.rs:5-43), 1 the same greedy in
+// smallest-last order of the static graph, 2 the fast colouring (fast_colors).
+std::vector<std::vector<int>> color_mode(const std::vector<std::vector<int>>& graph, const ContactSet& c,
+                                         const GrabT* g, int mode) {
+  if (mode == 0) return colors(graph, c, g);
+  std::vector<int> sl = smallest_last(graph);
+  return mode == 1 ? colors(graph, c, g, &sl) : fast_colors(graph, c, g, sl);
+}
 // Colouring of an arbitrary contact set at positions x: runs detect on x with
-// rq, then the greedy colouring (src/vbd/coloring.rs) — used to compare the
-// product's colouring with the reference order.
+// rq, then the colouring of `mode` (color_mode) — used to compare the
+// product's colourings with the oracle's.
 int cvo_solver_detect_and_color(cvo_solver* s, const double* x_moving, double rq,
                                 const cv_ogc_settings* settings, const cv_grab* grab,
                                 uint32_t* color_of_vertex, uint32_t* n_colors, cvo_contacts* out,
-                                char* err, size_t errn, int smallest_last_order) {
+                                char* err, size_t errn, int mode) {
   return guarded(err, errn, [&] {
     auto x = s->s.world.positions(v3s(x_moving, s->s.garment.n));
     ContactSet c = s->s.world.detect(x, rq, ogc_of(settings));
     GrabT g;
     if (grab) g = grab_of(grab);
-    std::vector<int> sl;
-    if (smallest_last_order) sl = smallest_last(s->s.graph);
-    auto cols = colors(s->s.graph, c, grab ? &g : nullptr, smallest_last_order ? &sl : nullptr);
+    auto cols = color_mode(s->s.graph, c, grab ? &g : nullptr, mode);
     for (size_t ci = 0; ci < cols.size(); ++ci)
       for (int v : cols[ci]) color_of_vertex[v] = (uint32_t)ci;
     *n_colors = (uint32_t)cols.size();
@@ -785,10 +792,12 @@ void cvo_prediction_record(cvo_prediction* p, const double* old_v, const cv_stat
 }
 }
 
-// Greedy colouring (src/vbd/coloring.rs:5-43) on an explicit conflict graph
-// (CSR) and contact rows (kind, ids[4], count), with an optional grab.
+// Colouring of `mode` (color_mode; 0 is the reference's greedy,
+// src/vbd/coloring.rs:5-43) on an explicit conflict graph (CSR) and contact
+// rows (kind, ids[4], count), with an optional grab.
 extern "C" uint32_t cvo_colors(uint32_t n, const uint32_t* g_off, const uint32_t* g_nb, const uint32_t* kind_ids,
-                               const uint32_t* counts, uint64_t m, const cv_grab* grab, uint32_t* color_of) {
+                               const uint32_t* counts, uint64_t m, const cv_grab* grab, uint32_t* color_of,
+                               int mode) {
   std::vector<std::vector<int>> graph(n);
   for (uint32_t v = 0; v < n; ++v)
     for (uint32_t k = g_off[v]; k < g_off[v + 1]; ++k) graph[v].push_back((int)g_nb[k]);
@@ -806,7 +815,7 @@ extern "C" uint32_t cvo_colors(uint32_t n, const uint32_t* g_off, const uint32_t
   }
   GrabT g;
   if (grab) g = grab_of(grab);
-  auto cols = colors(graph, c, grab ? &g : nullptr);
+  auto cols = color_mode(graph, c, grab ? &g : nullptr, mode);
   for (size_t k = 0; k < cols.size(); ++k)
     for (int v : cols[k]) color_of[v] = (uint32_t)k;
   return (uint32_t)cols.size();

@@ -428,9 +428,11 @@ VbdMetrics VbdSolver::step(State& state, const VbdSettings& st, const GrabT* gra
   };
   addp(contacts.profile);
   std::vector<int> sl;
-  if (st.smallest_last) sl = smallest_last(graph);
-  const std::vector<int>* order = st.smallest_last ? &sl : nullptr;
-  auto cols = colors(graph, contacts, grab, order);
+  if (st.fast_coloring) sl = smallest_last(graph);
+  auto step_colors = [&] {
+    return st.fast_coloring ? fast_colors(graph, contacts, grab, sl) : colors(graph, contacts, grab);
+  };
+  auto cols = step_colors();
   uint64_t clipped = 0;
   for (int i = 0; i < n; ++i) {
     if (!garment.pins[i]) {
@@ -450,7 +452,7 @@ VbdMetrics VbdSolver::step(State& state, const VbdSettings& st, const GrabT* gra
     if (clipped > 0 && (double)clipped >= st.contact.refresh_fraction * n) {
       double t = now_ms();
       contacts = world.detect(x, rq, st.contact);
-      cols = colors(graph, contacts, grab, order);
+      cols = step_colors();
       m.detection_ms += now_ms() - t;
       m.detection_updates += 1;
       m.broad_phase_rebuilds += contacts.broad_phase_rebuilt ? 1 : 0;

@@ -879,6 +879,48 @@ inline std::vector<int> smallest_last(const std::vector<std::vector<int>>& graph
   return std::vector<int>(removal.rbegin(), removal.rend());
 }
 
+// Fast colouring (extension, no reference counterpart; the product computes
+// it on the device).  The static conflict graph is coloured once, greedily in
+// smallest-last order (colours col0).  Each detection then colours the graph
+// with its contacts and grab greedily in the order of the static colour
+// classes: col0 ascending, ties by decreasing priority.  Without contacts this
+// reproduces col0 exactly (each vertex of class c sees colours 0..c-1 on its
+// earlier neighbours), and a greedy pass in the class order of a colouring
+// needs no more colours than that colouring has, so contacts are absorbed by
+// the whole graph rather than by new colours.  It is the result of
+// Jones-Plassmann rounds with that priority (a vertex whose uncoloured
+// neighbours all come later takes the smallest free colour), restated
+// sequentially.  Priority: the lowbias32 integer hash of the vertex index (a
+// bijection on 32 bits, so the order is total).
+inline uint32_t repair_priority(uint32_t v) {
+  v ^= v >> 16;
+  v *= 0x7feb352du;
+  v ^= v >> 15;
+  v *= 0x846ca68bu;
+  v ^= v >> 16;
+  return v;
+}
+inline std::vector<std::vector<int>> fast_colors(const std::vector<std::vector<int>>& graph,
+                                                 const ContactSet& contact, const GrabT* grab,
+                                                 const std::vector<int>& sl) {
+  const int n = (int)graph.size();
+  std::vector<int> col0(n, -1), seen(n + 1, -1);
+  for (int v : sl) {  // static colouring
+    for (int u : graph[v])
+      if (u < n && col0[u] >= 0) seen[col0[u]] = v;
+    int c = 0;
+    while (seen[c] == v) ++c;
+    col0[v] = c;
+  }
+  std::vector<int> order(n);
+  for (int v = 0; v < n; ++v) order[v] = v;
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    if (col0[a] != col0[b]) return col0[a] < col0[b];
+    return repair_priority(a) > repair_priority(b);
+  });
+  return colors(graph, contact, grab, &order);
+}
+
 // ------------------------------------------------------------ prediction
 struct VbdSettings {  // src/vbd.rs:18-52
   double dt = 1.0 / 240.0;
@@ -889,8 +931,8 @@ struct VbdSettings {  // src/vbd.rs:18-52
   // Extension with no reference counterpart: > 0 ends the iterations once the
   // post-sweep (true) residual is <= this value.
   double residual_tolerance = 0.0;
-  // Extension: colour in smallest-last order of the static graph.
-  bool smallest_last = false;
+  // Extension: the fast colouring (fast_colors) in place of the reference's.
+  bool fast_coloring = false;
   void validate() const {
     contact.validate();
     if (!std::isfinite(dt) || !(dt >= 1e-6 && dt <= 0.025) || iterations == 0 || iterations > 200 ||

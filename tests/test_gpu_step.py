@@ -231,6 +231,41 @@ def test_invalid_grab_is_transactional(gpu):
     assert np.array_equal(st.x, old.x) and st.time == old.time
 
 
+def test_colourings_on_the_device_equal_the_oracle(gpu):
+    """The colouring a step uses, on the contacts of a device detection: the
+    reference's vertex-order greedy (host) and the fast colouring (device
+    Jones-Plassmann repair of the static smallest-last colouring) equal the
+    oracle's, class for class, with and without a grab, on the draped sheet
+    and the sewn garment; the fast colouring is proper."""
+    from clothcraft_amd import OgcSettings, VbdSolver, garments
+    sc, x = _contact_scene()
+    cases = [(sc, scenes.State(np.ascontiguousarray(x), np.zeros_like(x)), x, 0.008)]
+    g2 = garments.sewn_garment_on_body(spacing=0.012)
+    # detection at perturbed positions (self-contacts); solvers built at rest
+    xg = g2.state.x + np.random.default_rng(5).normal(0.0, 0.004, g2.state.x.shape)
+    cases.append((g2, g2.state, xg, 0.008))
+    grab = dict(vertices=(3, 40, 77), weights=(0.2, 0.3, 0.5), target=(0, 0, 0), stiffness=10.0)
+    for scn, st, xs, rq in cases:
+        sol = VbdSolver(scn.garment, scn.avatar, st)
+        ref = O.Solver(scn.garment, scn.avatar, st)
+        det = gpu.detect(sol, xs, rq, OgcSettings(radius=0.002), candidates=False)
+        assert len(det["kind_ids"]) > 0
+        n = scn.garment.n
+        for gr in (None, grab):
+            for fast in (False, True):
+                exp, kexp, c = ref.detect_and_color(xs, rq, O.ogc_c(radius=0.002), gr, fast=fast)
+                assert np.array_equal(c["kind_ids"], det["kind_ids"])
+                got, k = gpu.colors(sol, gr, fast=fast)
+                print(f"n {n} contacts {len(det['kind_ids'])} grab {gr is not None} fast {fast}: {k} colours")
+                assert k == kexp and np.array_equal(got, exp)
+                if fast:
+                    for row, cnt in zip(c["kind_ids"], c["counts"]):
+                        vs = [int(v) for v in row[1:1 + cnt] if v < n]
+                        assert len({int(got[v]) for v in vs}) == len(vs)
+                    for t in scn.garment.tri_vertices:
+                        assert len({int(got[v]) for v in t}) == 3
+
+
 def test_unknown_settings_flags_are_rejected_transactionally(gpu):
     from clothcraft_amd import VbdError, VbdSolver
     sc = scenes.free_panel(2, 2)
@@ -360,9 +395,10 @@ def test_sweep_lane_shapes_match_oracle(gpu, lanes):
 
 
 def test_fast_colouring_step_matches_the_oracle_in_the_same_mode(gpu):
-    """CV_FLAG_FAST_COLORING (smallest-last order of the static graph): the
-    device step equals the oracle's step in the same colouring mode, with no
-    more colours than the reference's vertex-order colouring."""
+    """CV_FLAG_FAST_COLORING (static smallest-last colouring repaired on the
+    device after each detection): the device step equals the oracle's step in
+    the same colouring mode, with no more colours than the reference's
+    vertex-order colouring."""
     from clothcraft_amd.abi import CV_FLAG_FAST_COLORING
     for sc, tol in ((scenes.hanging_sheet(), 1e-9), (scenes.small_sheet_on_sphere(24, 0.2535), 1e-5)):
         s = dict(sc.settings, flags=CV_FLAG_FAST_COLORING)

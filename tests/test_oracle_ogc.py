@@ -361,6 +361,101 @@ def _oracle_colors(graph, contacts, grab):
     return O.colors(graph, ki, counts, grab)
 
 
+def _lowbias32(v):
+    v &= 0xFFFFFFFF
+    v ^= v >> 16
+    v = (v * 0x7FEB352D) & 0xFFFFFFFF
+    v ^= v >> 15
+    v = (v * 0x846CA68B) & 0xFFFFFFFF
+    v ^= v >> 16
+    return v
+
+
+def _jones_plassmann(graph, rows, counts, grab, col0):
+    """The fast colouring run as the device runs it: synchronous rounds, each
+    colouring the vertices whose uncoloured neighbours all come later in the
+    priority order (static colour ascending, then hash descending), from the
+    colours held at the round's start."""
+    n = len(graph)
+    nbrs = [set(r) for r in graph]
+    for row, k in zip(rows, counts):
+        vs = [int(v) for v in row[1:1 + k] if v < n]
+        for a in vs:
+            nbrs[a].update(b for b in vs if b != a)
+    if grab:
+        gv = [v for v in grab["vertices"] if v < n]
+        for a in gv:
+            nbrs[a].update(b for b in gv if b != a)
+    key = [(-col0[v], _lowbias32(v)) for v in range(n)]
+    col = [None] * n
+    rounds = 0
+    while any(c is None for c in col):
+        rounds += 1
+        start = list(col)
+        for v in range(n):
+            if start[v] is not None:
+                continue
+            if any(start[u] is None and key[u] > key[v] for u in nbrs[v]):
+                continue
+            used = {start[u] for u in nbrs[v] if start[u] is not None}
+            c = 0
+            while c in used:
+                c += 1
+            col[v] = c
+    return col, rounds
+
+
+def test_fast_colouring_equals_jones_plassmann_rounds_and_is_proper():
+    """Extension (CV_FLAG_FAST_COLORING): the oracle's sequential statement
+    (greedy in the order of the static smallest-last colour classes, ties by
+    decreasing hash) equals synchronous Jones-Plassmann rounds simulated
+    independently here, on random graphs with random contact cliques and a
+    grab; the colouring is proper, and without contacts it is the
+    smallest-last greedy colouring itself."""
+    rng = np.random.default_rng(11)
+    for trial in range(12):
+        n = int(rng.integers(20, 90))
+        graph = [set() for _ in range(n)]
+        for _ in range(int(rng.integers(n, 3 * n))):
+            a, b = (int(v) for v in rng.integers(0, n, 2))
+            if a != b:
+                graph[a].add(b)
+                graph[b].add(a)
+        graph = [sorted(r) for r in graph]
+        m = int(rng.integers(1, 4 * n))
+        rows = np.zeros((m, 5), np.uint32)
+        counts = np.zeros(m, np.uint32)
+        for i in range(m):
+            k = int(rng.integers(2, 5))
+            rows[i, 0] = k - 2 if k < 4 else int(rng.integers(2, 4))
+            rows[i, 1:1 + k] = rng.choice(n + 6, k, replace=False)  # ids >= n: avatar
+            counts[i] = k
+        grab = dict(vertices=tuple(int(v) for v in rng.choice(n, 3, replace=False)), weights=(0.2, 0.3, 0.5),
+                    target=(0, 0, 0), stiffness=1.0) if trial % 2 else None
+        none = np.zeros((0, 5), np.uint32)
+        sl = O.colors(graph, none, np.zeros(0, np.uint32), None, mode=1)
+        col0 = [0] * n
+        for c, vs in enumerate(sl):
+            for v in vs:
+                col0[v] = c
+        assert O.colors(graph, none, np.zeros(0, np.uint32), None, mode=2) == sl
+        jp0, _ = _jones_plassmann(graph, none, [], None, col0)
+        assert jp0 == col0
+        got = O.colors(graph, rows, counts, grab, mode=2)
+        jp, rounds = _jones_plassmann(graph, rows, counts, grab, col0)
+        exp = [[v for v in range(n) if jp[v] == c] for c in range(max(jp) + 1)]
+        assert got == exp, trial
+        assert rounds >= len(sl)
+        col = jp
+        for v in range(n):
+            assert all(col[u] != col[v] for u in graph[v])
+        for row, k in zip(rows, counts):
+            vs = [int(v) for v in row[1:1 + k] if v < n]
+            assert len({col[v] for v in vs}) == len(vs)
+        if grab:
+            assert len({col[v] for v in grab["vertices"]}) == 3
+
+
 # ------------------------------------------------------------- prediction
 def test_adaptive_guess_preserves_damped_fall_and_supported_rest_and_invalidates_edits():
     """src/vbd/prediction.rs:60-87."""
