@@ -84,7 +84,8 @@ bool prediction_valid(const cv_solver& s, const cv_state* st, const cv_settings*
 // Greedy colouring in vertex order (src/vbd/coloring.rs:5-43), or with
 // smallest_last in the smallest-last order of the static conflict graph
 // (s.sl_order: fewer colours, a different Gauss-Seidel order than the
-// reference).  cnb_off/cnb is the per-vertex contact incidence: for moving
+// reference; without contacts these are the static classes of the device's
+// fast colouring).  cnb_off/cnb is the per-vertex contact incidence: for moving
 // vertex v, the vertices of each incident contact.  Returns colour of each
 // vertex and the colour count; `order` lists vertices colour by colour, in
 // index order within a colour.

@@ -140,34 +140,28 @@ __device__ __forceinline__ R local_energy(const SweepArgs<R>& a, const LaneTerm<
   return group_sum<L>(e);
 }
 
-// Derivatives at x, with acc.e replaced by the local energy at x as
-// local_energy computes it (same routine, same per-lane order and the same
-// reduction), so it serves as the line search's reference energy without a
-// second pass over the terms.  E = false leaves acc.e undefined.
+// Derivatives and the local energy at x, the energy summed from the terms'
+// own values in the per-lane order and reduction local_energy uses, so it
+// serves as the line search's reference energy without a second pass over
+// the terms.  It can differ from local_energy at x in the last bits (the two
+// routines may contract differently); the accept slack of 1e-14 covers that.
+// E = false leaves acc.e undefined.
 template <int L, class R, bool E = true>
 __device__ __forceinline__ cvm::Deriv<R> local_deriv(const SweepArgs<R>& a, const LaneTerm<R>& lt, uint32_t v, V3<R> x,
                                      int lane, uint32_t s0, uint32_t ns, uint32_t c0, uint32_t nt) {
   auto P = [&](uint32_t i) { return ldp(a.pos, i); };
   cvm::Deriv<R> acc = cvm::deriv_zero<R>();
-  R e = R(0);
-  if (lt.live) {
-    acc = cvm::term_deriv<R>(lt.rec, x, a.cr, a.ck, P);
-    if (E) e = cvm::term_energy<R>(lt.rec, x, a.cr, a.ck, P);
-  }
+  if (lt.live) acc = cvm::term_deriv<R>(lt.rec, x, a.cr, a.ck, P);
   for (uint32_t i = lane + L; i < nt; i += L) {
     const cvm::TermRecT<R>& t = i < ns ? a.terms[s0 + i] : a.crec[c0 + (i - ns)];
     cvm::deriv_add(acc, cvm::term_deriv<R>(t, x, a.cr, a.ck, P));
-    if (E) e += cvm::term_energy<R>(t, x, a.cr, a.ck, P);
   }
   if (lane == 0) {
     V3<R> d;
     R wv;
-    if (grab_d(a.grab, v, x, a.pos, d, wv)) {
-      cvm::deriv_add(acc, cvm::stitch_d<R>(d, wv, a.grab.k));
-      if (E) e += R(0.5) * a.grab.k * cvm::dot(d, d);
-    }
+    if (grab_d(a.grab, v, x, a.pos, d, wv)) cvm::deriv_add(acc, cvm::stitch_d<R>(d, wv, a.grab.k));
   }
-  if (E) acc.e = group_sum<L>(e);
+  if (E) acc.e = group_sum<L>(acc.e);
   acc.g.x = group_sum<L>(acc.g.x);
   acc.g.y = group_sum<L>(acc.g.y);
   acc.g.z = group_sum<L>(acc.g.z);
