@@ -82,10 +82,35 @@ __device__ __forceinline__ auto ldp(const T4* pos, uint32_t i) {
   return V3<R>{q.x, q.y, q.z};
 }
 
+// Value of v in another lane of the same 16-lane row, by a DPP move (no LDS
+// round trip); CTRL is the DPP control (quad permute, row mirrors).  All
+// lanes of the row are active wherever a group of >= 8 lanes reduces.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  int lo = (int)(uint32_t)b, hi = (int)(uint32_t)(b >> 32);
+  lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  const int b = __builtin_bit_cast(int, v);
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, CTRL, 0xF, 0xF, false));
+}
+// Sum over a group of L >= 8 aligned lanes, the same bits in every lane.
+// Levels pair lanes whose partial sums cover the same terms: distance 1 and 2
+// in a quad (quad permutes [1,0,3,2] and [2,3,0,1]), the mirrored quads of a
+// half-row, the mirrored half-rows of a row, then lane shuffles across rows.
 template <int L, class R>
 __device__ __forceinline__ R group_sum(R v) {
+  static_assert(L >= 8 && L <= 64 && (L & (L - 1)) == 0, "group of 8, 16, 32 or 64 lanes");
+  v += dpp_move<0xB1>(v);  // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);  // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  if (L >= 16) v += dpp_move<0x140>(v);  // row_mirror
 #pragma unroll
-  for (int m = L / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, L);
+  for (int m = 16; m < L; m <<= 1) v += __shfl_xor(v, m, L);
   return v;
 }
 
