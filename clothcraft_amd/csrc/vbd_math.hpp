@@ -9,8 +9,9 @@
 //   bending   — src/energy.rs:31-46, closed form with the exact second
 //               derivative of the unit normals (not the Gauss-Newton part only);
 //   stitch    — src/vbd.rs:217-227 (seams) and :237-246 (grab);
-//   contacts  — src/ogc/potential.rs:8-65 via a 3-variable second-order jet,
-//               the same forward-mode construction as src/math.rs:33-122.
+//   contacts  — src/ogc/potential.rs:8-65, closed-form derivatives of the
+//               four distance kinds and the EE mollifier (the reference
+//               differentiates them with Jet<3>, src/math.rs:33-122).
 // Positions are relative to a per-step origin (the garment centroid,
 // src/gpu_vbd.rs:154-156) so f32 differences stay accurate.
 #pragma once
@@ -98,6 +99,15 @@ CV_HD void s3_add_diag(S3<R>& s, R c) {
   s.a[3] += c;
   s.a[5] += c;
 }
+template <class R>
+CV_HD void s3_add_sym(S3<R>& s, V3<R> x, V3<R> y, R c) {  // s += c (x y^T + y x^T)
+  s.a[0] += c * (R(2) * x.x * y.x);
+  s.a[1] += c * (x.x * y.y + y.x * x.y);
+  s.a[2] += c * (x.x * y.z + y.x * x.z);
+  s.a[3] += c * (R(2) * x.y * y.y);
+  s.a[4] += c * (x.y * y.z + y.y * x.z);
+  s.a[5] += c * (R(2) * x.z * y.z);
+}
 
 // Full 3x3 row-major for the bending Hessian algebra.
 template <class R>
@@ -131,120 +141,6 @@ CV_HD M3<R> mul_tn(const M3<R>& a, const M3<R>& b) {  // a^T b
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) r.m[i][j] = a.m[0][i] * b.m[0][j] + a.m[1][i] * b.m[1][j] + a.m[2][i] * b.m[2][j];
   return r;
-}
-
-// -------------------------------------------------------------- 3-var jet
-// Second-order forward AD over the 3 coordinates of one vertex; symmetric
-// Hessian storage.  Same product/chain rules as src/math.rs:44-106.
-template <class R>
-struct J3 {
-  R v;
-  R g[3];
-  R h[6];
-};
-template <class R>
-CV_HD J3<R> jc(R v) {
-  J3<R> j;
-  j.v = v;
-  for (int i = 0; i < 3; ++i) j.g[i] = R(0);
-  for (int i = 0; i < 6; ++i) j.h[i] = R(0);
-  return j;
-}
-template <class R>
-CV_HD J3<R> jvar(R v, int i) {
-  J3<R> j = jc<R>(v);
-  j.g[i] = R(1);
-  return j;
-}
-#define CV_HIDX(i, j) ((i) == 0 ? (j) : ((i) == 1 ? 2 + (j) : 5))  // valid for i <= j
-template <class R>
-CV_HD J3<R> operator+(const J3<R>& a, const J3<R>& b) {
-  J3<R> r;
-  r.v = a.v + b.v;
-  for (int i = 0; i < 3; ++i) r.g[i] = a.g[i] + b.g[i];
-  for (int i = 0; i < 6; ++i) r.h[i] = a.h[i] + b.h[i];
-  return r;
-}
-template <class R>
-CV_HD J3<R> operator-(const J3<R>& a, const J3<R>& b) {
-  J3<R> r;
-  r.v = a.v - b.v;
-  for (int i = 0; i < 3; ++i) r.g[i] = a.g[i] - b.g[i];
-  for (int i = 0; i < 6; ++i) r.h[i] = a.h[i] - b.h[i];
-  return r;
-}
-template <class R>
-CV_HD J3<R> operator-(const J3<R>& a) {
-  J3<R> r;
-  r.v = -a.v;
-  for (int i = 0; i < 3; ++i) r.g[i] = -a.g[i];
-  for (int i = 0; i < 6; ++i) r.h[i] = -a.h[i];
-  return r;
-}
-template <class R>
-CV_HD J3<R> operator*(const J3<R>& a, const J3<R>& b) {
-  J3<R> r;
-  r.v = a.v * b.v;
-  for (int i = 0; i < 3; ++i) r.g[i] = a.g[i] * b.v + a.v * b.g[i];
-  int k = 0;
-  for (int i = 0; i < 3; ++i)
-    for (int j = i; j < 3; ++j, ++k)
-      r.h[k] = a.h[k] * b.v + a.v * b.h[k] + a.g[i] * b.g[j] + b.g[i] * a.g[j];
-  return r;
-}
-template <class R>
-CV_HD J3<R> operator*(const J3<R>& a, R s) {
-  J3<R> r;
-  r.v = a.v * s;
-  for (int i = 0; i < 3; ++i) r.g[i] = a.g[i] * s;
-  for (int i = 0; i < 6; ++i) r.h[i] = a.h[i] * s;
-  return r;
-}
-template <class R>
-CV_HD J3<R> chain(const J3<R>& a, R value, R first, R second) {
-  J3<R> r;
-  r.v = value;
-  for (int i = 0; i < 3; ++i) r.g[i] = first * a.g[i];
-  int k = 0;
-  for (int i = 0; i < 3; ++i)
-    for (int j = i; j < 3; ++j, ++k) r.h[k] = first * a.h[k] + second * a.g[i] * a.g[j];
-  return r;
-}
-template <class R>
-CV_HD J3<R> operator/(const J3<R>& a, const J3<R>& b) {
-  R bv = b.v;
-  return a * chain(b, R(1) / bv, R(-1) / (bv * bv), R(2) / (bv * bv * bv));
-}
-template <class R>
-CV_HD J3<R> jsqrt(const J3<R>& a) {
-  R r = sqrt(a.v);
-  return chain(a, r, R(0.5) / r, R(-0.25) / (r * r * r));
-}
-template <class R>
-CV_HD J3<R> jlog(const J3<R>& a) {
-  return chain(a, R(log(a.v)), R(1) / a.v, R(-1) / (a.v * a.v));
-}
-template <class R>
-struct JV {  // jet 3-vector
-  J3<R> c[3];
-};
-template <class R>
-CV_HD JV<R> jsub(const JV<R>& a, const JV<R>& b) {
-  return {{a.c[0] - b.c[0], a.c[1] - b.c[1], a.c[2] - b.c[2]}};
-}
-template <class R>
-CV_HD J3<R> jdot(const JV<R>& a, const JV<R>& b) {
-  return a.c[0] * b.c[0] + a.c[1] * b.c[1] + a.c[2] * b.c[2];
-}
-template <class R>
-CV_HD JV<R> jcross(const JV<R>& a, const JV<R>& b) {
-  return {{a.c[1] * b.c[2] - a.c[2] * b.c[1], a.c[2] * b.c[0] - a.c[0] * b.c[2],
-           a.c[0] * b.c[1] - a.c[1] * b.c[0]}};
-}
-template <class R>
-CV_HD JV<R> jpoint(V3<R> p, bool variable) {
-  if (variable) return {{jvar<R>(p.x, 0), jvar<R>(p.y, 1), jvar<R>(p.z, 2)}};
-  return {{jc<R>(p.x), jc<R>(p.y), jc<R>(p.z)}};
 }
 
 // Output of one term for vertex v: energy, gradient, Hessian (symmetric).
@@ -435,18 +331,7 @@ CV_HD Deriv<R> stitch_d(V3<R> d, R wv, R k) {
 // Kinds follow src/ogc/potential.rs:171-177.
 enum { KIND_PP = 0, KIND_PE = 1, KIND_PF = 2, KIND_EE = 3 };
 
-// src/ogc/potential.rs:8-22 on a jet or a plain value.
-template <class R>
-CV_HD J3<R> activation_j(const J3<R>& d, R r, R k) {
-  if (d.v >= r) return jc<R>(R(0));
-  if (d.v <= R(0)) return jc<R>(R(INFINITY));
-  R tau = R(0.5) * r;
-  if (d.v >= tau) {
-    J3<R> g = jc<R>(r) - d;
-    return (jc<R>(R(0.5) * k) * g) * g;
-  }
-  return jc<R>(R(0.5) * k * tau * tau) - jc<R>(k * tau * tau) * jlog(d / jc<R>(tau));
-}
+// src/ogc/potential.rs:8-22.
 template <class R>
 CV_HD R activation_v(R d, R r, R k) {
   if (d >= r) return R(0);
@@ -458,39 +343,7 @@ CV_HD R activation_v(R d, R r, R k) {
   }
   return R(0.5) * k * tau * tau - k * tau * tau * R(log(d / tau));
 }
-// Contact::energy (src/ogc/potential.rs:186-212) with vertex `slot` as the variable.
-template <class R>
-CV_HD J3<R> contact_j(int kind, const V3<R>* p, int slot, R eps, R r, R k) {
-  JV<R> q[4];
-  for (int i = 0; i < 4; ++i) q[i] = jpoint(p[i], i == slot);
-  J3<R> d;
-  if (kind == KIND_PP) {
-    JV<R> v = jsub(q[0], q[1]);
-    d = jsqrt(jdot(v, v));
-  } else if (kind == KIND_PE) {
-    JV<R> u = jsub(q[2], q[1]), w = jsub(q[0], q[1]);
-    J3<R> s = jdot(w, u) / jdot(u, u);
-    JV<R> dd = {{w.c[0] - s * u.c[0], w.c[1] - s * u.c[1], w.c[2] - s * u.c[2]}};
-    d = jsqrt(jdot(dd, dd));
-  } else if (kind == KIND_PF) {
-    JV<R> n = jcross(jsub(q[2], q[1]), jsub(q[3], q[1]));
-    J3<R> h = jdot(jsub(q[0], q[1]), n) / jsqrt(jdot(n, n));
-    d = h.v < R(0) ? -h : h;
-  } else {
-    JV<R> n = jcross(jsub(q[1], q[0]), jsub(q[3], q[2]));
-    J3<R> nn = jdot(n, n);
-    if (nn.v < R(1e-30)) return jc<R>(R(0));
-    J3<R> h = jdot(jsub(q[0], q[2]), n) / jsqrt(nn);
-    d = h.v < R(0) ? -h : h;
-  }
-  J3<R> e = activation_j(d, r, k);
-  if (kind == KIND_EE) {
-    JV<R> n = jcross(jsub(q[1], q[0]), jsub(q[3], q[2]));
-    J3<R> qq = jdot(n, n) * (R(1) / eps);
-    if (qq.v < R(1)) e = e * qq * (jc<R>(R(2)) - qq);
-  }
-  return e;
-}
+// Contact::energy (src/ogc/potential.rs:186-212).
 template <class R>
 CV_HD R contact_e(int kind, const V3<R>* p, R eps, R r, R k) {
   R d;
@@ -521,14 +374,146 @@ CV_HD R contact_e(int kind, const V3<R>* p, R eps, R r, R k) {
   }
   return e;
 }
+// First and second derivatives of the activation at d in (0, r).
+template <class R>
+CV_HD void activation_d(R d, R r, R k, R& e1, R& e2) {
+  R tau = R(0.5) * r;
+  if (d >= tau) {
+    e1 = -k * (r - d);
+    e2 = k;
+  } else {
+    R kt = k * tau * tau;
+    e1 = -kt / d;
+    e2 = kt / (d * d);
+  }
+}
+// Unsigned plane distance d = |r . n| / |n| and its derivatives in one
+// vertex x, where that vertex moves r by c dx and the normal by a x dx.
+// With m = |n|, n^ = n / m, h = r . n^, rho = r - h n^ (the in-plane part of
+// r) and c_a = n^ x a:
+//   grad h = c n^ + (rho x a) / m
+//   hess h = -(c / m)(n^ c_a^T + c_a n^T) - (c_a w^T + w c_a^T) / m^2
+//            - (h / m^2)(|a|^2 I - a a^T - c_a c_a^T),   w = rho x a,
+// from dn^ = (I - n^ n^T) dn / m and dm = c_a . dx; d = sign(h) h.
+template <class R>
+CV_HD void plane_dist_d(V3<R> r, V3<R> n, R nn, V3<R> a, R c, R& d, V3<R>& g, S3<R>& H) {
+  R m = sqrt(nn);
+  R h = dot(r, n) / m;  // as contact_e
+  R sg = h < R(0) ? R(-1) : R(1);
+  d = h < R(0) ? -h : h;
+  R im = R(1) / m;
+  V3<R> nh = n * im;
+  V3<R> ca = cross(nh, a);
+  V3<R> w = cross(r - nh * h, a);
+  g = (nh * c + w * im) * sg;
+  H = s3_zero<R>();
+  R im2 = im * im;
+  s3_add_sym(H, nh, ca, -sg * c * im);
+  s3_add_sym(H, ca, w, -sg * im2);
+  R q = -sg * h * im2;
+  s3_add_diag(H, q * dot(a, a));
+  s3_add_outer(H, a, -q);
+  s3_add_outer(H, ca, -q);
+}
+// Contact energy of vertex `slot` with its gradient and Hessian in closed
+// form: E = phi(d) (times the EE mollifier M(q) = q (2 - q) while q < 1), so
+// grad E = phi' grad d (+ phi M' grad q) and hess E = phi'' grad d grad d^T +
+// phi' hess d (+ the mollifier's product-rule terms), with the distance
+// derivatives of each kind:
+//   PP  d = |p0 - p1|:             grad = +-u^, hess = (I - u^ u^T) / d;
+//   PE  distance of p0 to the line p1 p2, t = w - s u (w = p0 - p1,
+//       u = p2 - p1, s = w.u / u.u), t^ = t / d; for the point
+//       grad = t^, hess = (I - u u^T/u.u - t^ t^T) / d; for an edge end with
+//       weight b (1 - s at p1, s at p2) grad = -b t^ and
+//       hess = b^2 (I - u u^T/u.u - t^ t^T) / d + g (t^ u^T + u t^T) / u.u
+//              - d t^ t^T / u.u, g = -(1 - s) at p1 and s at p2;
+//   PF, EE  plane distances (plane_dist_d) with (a, c) of the moving vertex;
+//   EE  q = |n|^2 / eps: grad q = 2 (n x a) / eps, hess q = 2 (|a|^2 I - a a^T) / eps.
+// Values follow contact_e's operations, so the energy is the one the line
+// search evaluates.  Degenerate cases as the energy: zero beyond r and for
+// parallel edges, infinite energy (no derivatives) at d <= 0.
 template <class R>
 CV_HD Deriv<R> contact_d(int kind, const V3<R>* p, int slot, R eps, R r, R k) {
-  J3<R> j = contact_j(kind, p, slot, eps, r, k);
-  Deriv<R> d;
-  d.e = j.v;
-  d.g = mk<R>(j.g[0], j.g[1], j.g[2]);
-  for (int i = 0; i < 6; ++i) d.h.a[i] = j.h[i];
-  return d;
+  Deriv<R> o = deriv_zero<R>();
+  R d;
+  V3<R> gd;
+  S3<R> hd = s3_zero<R>();
+  V3<R> n = mk<R>(0, 0, 0), a = mk<R>(0, 0, 0);
+  R nn = R(0);
+  if (kind == KIND_PP) {
+    V3<R> v = p[0] - p[1];
+    d = sqrt(dot(v, v));
+    R id = R(1) / d;
+    V3<R> uh = v * id;
+    gd = slot == 0 ? uh : -uh;
+    s3_add_diag(hd, id);
+    s3_add_outer(hd, uh, -id);
+  } else if (kind == KIND_PE) {
+    V3<R> u = p[2] - p[1], w = p[0] - p[1];
+    R uu = dot(u, u);
+    R s = dot(w, u) / uu;
+    V3<R> t = w - u * s;
+    d = sqrt(dot(t, t));
+    R id = R(1) / d;
+    V3<R> th = t * id;
+    R b = slot == 0 ? R(1) : (slot == 1 ? R(1) - s : s);
+    gd = slot == 0 ? th : th * (-b);
+    R bb = b * b * id;
+    s3_add_diag(hd, bb);
+    s3_add_outer(hd, u, -bb / uu);
+    s3_add_outer(hd, th, -bb);
+    if (slot != 0) {
+      R gam = slot == 1 ? s - R(1) : s;
+      s3_add_sym(hd, th, u, gam / uu);
+      s3_add_outer(hd, th, -d / uu);
+    }
+  } else if (kind == KIND_PF) {
+    n = cross(p[2] - p[1], p[3] - p[1]);
+    nn = dot(n, n);
+    R c = slot == 0 ? R(1) : (slot == 1 ? R(-1) : R(0));
+    a = slot == 1 ? p[3] - p[2] : (slot == 2 ? p[1] - p[3] : (slot == 3 ? p[2] - p[1] : a));
+    plane_dist_d(p[0] - p[1], n, nn, a, c, d, gd, hd);
+  } else {
+    n = cross(p[1] - p[0], p[3] - p[2]);
+    nn = dot(n, n);
+    if (nn < R(1e-30)) return o;
+    R c = slot == 0 ? R(1) : (slot == 2 ? R(-1) : R(0));
+    a = slot == 0 ? p[3] - p[2] : (slot == 1 ? p[2] - p[3] : (slot == 2 ? p[0] - p[1] : p[1] - p[0]));
+    plane_dist_d(p[0] - p[2], n, nn, a, c, d, gd, hd);
+  }
+  R phi = activation_v(d, r, k);
+  R q = kind == KIND_EE ? nn * (R(1) / eps) : R(1);
+  bool moll = kind == KIND_EE && q < R(1);
+  o.e = moll ? phi * q * (R(2) - q) : phi;
+  if (d >= r) return o;
+  if (d <= R(0)) {
+    if (moll) {  // infinite energy times the mollifier's derivatives
+      R nan = R(NAN);
+      o.g = mk<R>(nan, nan, nan);
+      for (int i = 0; i < 6; ++i) o.h.a[i] = nan;
+    }
+    return o;
+  }
+  R e1, e2;
+  activation_d(d, r, k, e1, e2);
+  if (!moll) {
+    o.g = gd * e1;
+    s3_add_outer(o.h, gd, e2);
+    for (int i = 0; i < 6; ++i) o.h.a[i] += e1 * hd.a[i];
+    return o;
+  }
+  R M = q * (R(2) - q), M1 = R(2) - R(2) * q;
+  R te = R(2) / eps;
+  V3<R> gq = cross(n, a) * te;
+  o.g = gd * (e1 * M) + gq * (phi * M1);
+  s3_add_outer(o.h, gd, e2 * M);
+  for (int i = 0; i < 6; ++i) o.h.a[i] += (e1 * M) * hd.a[i];
+  s3_add_sym(o.h, gd, gq, e1 * M1);
+  s3_add_outer(o.h, gq, R(-2) * phi);
+  R pm = phi * M1 * te;
+  s3_add_diag(o.h, pm * dot(a, a));
+  s3_add_outer(o.h, a, -pm);
+  return o;
 }
 
 // ------------------------------------------------------- term records

@@ -77,6 +77,13 @@ def test_f64_kernel_hessians_match_finite_differences(sheet):
     # quadratic part of the barrier (d between r/2 and r), point-point and point-face
     (0, [[0.1, 0.101, 0.007], [0.1, 0.1, 0.0]], 2),
     (2, [[0.1, 0.1, -0.007], [-1.0, -1.0, 0.0], [1.0, -1.0, 0.0], [0.0, 1.0, 0.0]], 4),
+    (1, [[0.3, -0.006, 0.004], [-1.0, 0.0, 0.0], [1.0, 0.001, 0.0]], 3),
+    # nearly parallel edges: the EE mollifier q (2 - q) is active (q < 1), log and quadratic ranges
+    (3, [[-1.0, 0.0, 0.004], [1.0, 0.0, 0.004], [-1.0, -0.01, 0.0], [1.0, 0.01, 0.0]], 4),
+    (3, [[-1.0, 0.003, -0.007], [1.0, 0.0, -0.0068], [-1.0, -0.01, 0.0], [1.0, 0.012, 0.0005]], 4),
+    # tilted face and edge, every vertex of the contact moving
+    (2, [[0.0614, 0.2, 0.060799], [-0.7, -0.4, 0.2], [0.9, -0.5, -0.1], [0.1, 0.8, 0.05]], 4),
+    (3, [[-0.4, 0.3, 0.1], [0.6, -0.2, 0.104], [0.1, -0.6, 0.098], [-0.05, 0.7, 0.1]], 4),
 ])
 def test_contact_kernel_matches_oracle_block(kind, pts, count):
     x = np.zeros((4, 3))
