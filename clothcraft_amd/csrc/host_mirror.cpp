@@ -15,7 +15,7 @@ namespace {
 template <class R>
 struct Pos {
   const double* x;
-  cvm::V3<R> operator()(uint32_t i) const { return {(R)x[3 * i], (R)x[3 * i + 1], (R)x[3 * i + 2]}; }
+  cvm::V3<R> operator()(int, uint32_t i) const { return {(R)x[3 * i], (R)x[3 * i + 1], (R)x[3 * i + 2]}; }
 };
 template <class R>
 void deriv_out(const cvm::Deriv<R>& d, double* out) {
@@ -195,7 +195,7 @@ namespace {
 template <class R>
 struct PosArr {
   const R* p;  // [4 * n] xyz w
-  cvm::V3<R> operator()(uint32_t i) const { return {p[4 * i], p[4 * i + 1], p[4 * i + 2]}; }
+  cvm::V3<R> operator()(int, uint32_t i) const { return {p[4 * i], p[4 * i + 1], p[4 * i + 2]}; }
 };
 template <class R>
 void sweep_color(uint32_t count, const uint32_t* order, R* pos, const R* tgt, const R* ball,
@@ -223,7 +223,7 @@ void sweep_color(uint32_t count, const uint32_t* order, R* pos, const R* tgt, co
     cvm::V3<R> target{tgt[4 * v], tgt[4 * v + 1], tgt[4 * v + 2]};
     R inertia = tgt[4 * v + 3];
     cvm::V3<R> dx = x - target;
-    R old = energy(x) + R(0.5) * inertia * cvm::dot(dx, dx);
+    R old = acc.e + R(0.5) * inertia * cvm::dot(dx, dx);  // the terms' own values, as the device
     cvm::V3<R> delta = cvm::newton_direction<R>(acc, x, target, inertia);
     gsq[v] = cvm::dot(acc.g, acc.g);
     cvm::V3<R> center{ball[4 * v], ball[4 * v + 1], ball[4 * v + 2]};

@@ -130,13 +130,15 @@ __device__ __forceinline__ bool grab_d(const GrabArgs<R>& g, uint32_t v, V3<R> x
   return true;
 }
 
-// The first term record of each lane is loaded once per vertex solve and kept
-// in registers for the derivative and every line-search energy; only the
-// neighbour positions are re-read (they are L1/L2 hits).  Holding the
-// positions in registers too exceeds the f64 register budget and spills.
+// The first term record of each lane and the positions of its four vertices
+// are loaded once per vertex solve and kept in registers for the derivative
+// and every line-search energy (the neighbours do not move during the solve;
+// the vertex's own slot is replaced by the candidate).  Unused ids of a
+// record are 0, a valid vertex.  Records past the first L are re-read.
 template <class R>
 struct LaneTerm {
   cvm::TermRecT<R> rec;
+  V3<R> q[4];
   bool live;
 };
 template <class R>
@@ -144,15 +146,20 @@ __device__ __forceinline__ LaneTerm<R> load_lane_term(const SweepArgs<R>& a, int
                                                       uint32_t c0, uint32_t nt) {
   LaneTerm<R> t;
   t.live = (uint32_t)lane < nt;
-  if (t.live) t.rec = (uint32_t)lane < ns ? a.terms[s0 + lane] : a.crec[c0 + (lane - ns)];
+  if (t.live) {
+    t.rec = (uint32_t)lane < ns ? a.terms[s0 + lane] : a.crec[c0 + (lane - ns)];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t.q[k] = ldp(a.pos, t.rec.ids[k]);
+  }
   return t;
 }
 
 template <int L, class R>
 __device__ __forceinline__ R local_energy(const SweepArgs<R>& a, const LaneTerm<R>& lt, uint32_t v, V3<R> xv, int lane,
                           uint32_t s0, uint32_t ns, uint32_t c0, uint32_t nt) {
-  auto P = [&](uint32_t i) { return ldp(a.pos, i); };
-  R e = lt.live ? cvm::term_energy<R>(lt.rec, xv, a.cr, a.ck, P) : R(0);
+  auto P = [&](int, uint32_t i) { return ldp(a.pos, i); };
+  auto Q = [&](int k, uint32_t) { return lt.q[k]; };
+  R e = lt.live ? cvm::term_energy<R>(lt.rec, xv, a.cr, a.ck, Q) : R(0);
   for (uint32_t i = lane + L; i < nt; i += L) {
     const cvm::TermRecT<R>& t = i < ns ? a.terms[s0 + i] : a.crec[c0 + (i - ns)];
     e += cvm::term_energy<R>(t, xv, a.cr, a.ck, P);
@@ -174,9 +181,10 @@ __device__ __forceinline__ R local_energy(const SweepArgs<R>& a, const LaneTerm<
 template <int L, class R, bool E = true>
 __device__ __forceinline__ cvm::Deriv<R> local_deriv(const SweepArgs<R>& a, const LaneTerm<R>& lt, uint32_t v, V3<R> x,
                                      int lane, uint32_t s0, uint32_t ns, uint32_t c0, uint32_t nt) {
-  auto P = [&](uint32_t i) { return ldp(a.pos, i); };
+  auto P = [&](int, uint32_t i) { return ldp(a.pos, i); };
+  auto Q = [&](int k, uint32_t) { return lt.q[k]; };
   cvm::Deriv<R> acc = cvm::deriv_zero<R>();
-  if (lt.live) acc = cvm::term_deriv<R>(lt.rec, x, a.cr, a.ck, P);
+  if (lt.live) acc = cvm::term_deriv<R>(lt.rec, x, a.cr, a.ck, Q);
   for (uint32_t i = lane + L; i < nt; i += L) {
     const cvm::TermRecT<R>& t = i < ns ? a.terms[s0 + i] : a.crec[c0 + (i - ns)];
     cvm::deriv_add(acc, cvm::term_deriv<R>(t, x, a.cr, a.ck, P));
@@ -199,17 +207,19 @@ __device__ __forceinline__ cvm::Deriv<R> local_deriv(const SweepArgs<R>& a, cons
 template <int L, class R>
 __device__ __forceinline__ void solve_vertex(const SweepArgs<R>& a, uint32_t v, int lane) {
   using T4 = typename P4<R>::T;
-  T4 p4 = a.pos[v];
+  // Everything indexed by v is loaded at once, ahead of the term records.
+  const T4 p4 = a.pos[v];
+  const uint32_t s0 = a.term_off[v], s1 = a.term_off[v + 1], c0 = a.coff[v], c1 = a.coff[v + 1];
+  const T4 tg = a.tgt[v];
+  const T4 b4 = a.ball[v];
   if (p4.w > R(0.5)) {  // pinned: no update, zero residual contribution
     if (lane == 0) a.gsq[v] = 0.0f;
     return;
   }
   V3<R> x{p4.x, p4.y, p4.z};
-  uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
-  uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
+  const uint32_t ns = s1 - s0, nt = ns + (c1 - c0);
   const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt);
   cvm::Deriv<R> acc = local_deriv<L>(a, lt, v, x, lane, s0, ns, c0, nt);
-  T4 tg = a.tgt[v];
   V3<R> target{tg.x, tg.y, tg.z};
   R inertia = tg.w;
   // The accept test compares energies from one evaluation routine, so a
@@ -218,7 +228,6 @@ __device__ __forceinline__ void solve_vertex(const SweepArgs<R>& a, uint32_t v, 
   R old = acc.e + R(0.5) * inertia * cvm::dot(dx, dx);
   V3<R> delta = cvm::newton_direction<R>(acc, x, target, inertia);
   R gg = cvm::dot(acc.g, acc.g);
-  T4 b4 = a.ball[v];
   V3<R> center{b4.x, b4.y, b4.z};
   bool truncated = false;
   V3<R> accepted = x;
@@ -247,8 +256,12 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uin
   // Vertices of a colour are in index order, so neighbouring groups share
   // neighbours: keep consecutive groups on one XCD's L2.
   const uint32_t grp = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / L;
-  if (grp >= count || *a.stop) return;
-  solve_vertex<L, R>(a, a.order[off + grp], lane);
+  if (grp >= count) return;
+  // The vertex id and the stop flag are loaded together (one memory round
+  // trip instead of two before the solve starts).
+  const uint32_t v = a.order[off + grp];
+  if (*a.stop) return;
+  solve_vertex<L, R>(a, v, lane);
 }
 
 // One colour index of one iteration of many solvers in a single launch: slot

@@ -543,15 +543,18 @@ CV_HD uint32_t rec_head(uint32_t kind, int slot, int count, int ckind) {
   return kind | ((uint32_t)slot << 4) | ((uint32_t)count << 8) | ((uint32_t)ckind << 12);
 }
 
-// Position lookup abstraction: P(i) returns the relative position of vertex i.
+// Position lookup abstraction: P(k, i) returns the relative position of vertex
+// i = t.ids[k], the record's k-th vertex (k is a compile-time constant at
+// every call, so a caller may serve it from registers).
 template <class R, class F, class PosFn>
 CV_HD R term_energy(const TermRecT<F>& t, V3<R> xv, R cr, R ck, PosFn&& P) {
   uint32_t kind = rec_kind(t.head);
-  if (kind == TERM_MEMBRANE) return membrane_e<R, F>(xv, P(t.ids[0]), P(t.ids[1]), t.f);
+  if (kind == TERM_MEMBRANE) return membrane_e<R, F>(xv, P(0, t.ids[0]), P(1, t.ids[1]), t.f);
   if (kind == TERM_HINGE) {
     int s = rec_slot(t.head);
     V3<R> x[4];
-    for (int i = 0; i < 4; ++i) x[i] = i == s ? xv : P(t.ids[i]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x[i] = i == s ? xv : P(i, t.ids[i]);
     return bending_e<R, F>(x[0], x[1], x[2], x[3], t.f);
   }
   if (kind == TERM_STITCH) {
@@ -561,22 +564,24 @@ CV_HD R term_energy(const TermRecT<F>& t, V3<R> xv, R cr, R ck, PosFn&& P) {
     V3<R> d = mk<R>(0, 0, 0);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      if (i < n && i != s) d = d + (P(t.ids[i]) - xv) * R(t.f[i]);
+      if (i < n && i != s) d = d + (P(i, t.ids[i]) - xv) * R(t.f[i]);
     return R(0.5) * R(t.f[4]) * dot(d, d);
   }
   int s = rec_slot(t.head);
   V3<R> x[4];
-  for (int i = 0; i < 4; ++i) x[i] = i == s ? xv : P(t.ids[i]);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) x[i] = i == s ? xv : P(i, t.ids[i]);
   return contact_e<R>(rec_ckind(t.head), x, R(t.f[0]), cr, ck);
 }
 template <class R, class F, class PosFn>
 CV_HD Deriv<R> term_deriv(const TermRecT<F>& t, V3<R> xv, R cr, R ck, PosFn&& P) {
   uint32_t kind = rec_kind(t.head);
-  if (kind == TERM_MEMBRANE) return membrane_d<R, F>(xv, P(t.ids[0]), P(t.ids[1]), t.f);
+  if (kind == TERM_MEMBRANE) return membrane_d<R, F>(xv, P(0, t.ids[0]), P(1, t.ids[1]), t.f);
   int s = rec_slot(t.head);
   if (kind == TERM_HINGE) {
     V3<R> x[4];
-    for (int i = 0; i < 4; ++i) x[i] = i == s ? xv : P(t.ids[i]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x[i] = i == s ? xv : P(i, t.ids[i]);
     return bending_d<R, F>(x[0], x[1], x[2], x[3], s, t.f);
   }
   if (kind == TERM_STITCH) {
@@ -584,12 +589,13 @@ CV_HD Deriv<R> term_deriv(const TermRecT<F>& t, V3<R> xv, R cr, R ck, PosFn&& P)
     V3<R> d = mk<R>(0, 0, 0);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      if (i < n && i != s) d = d + (P(t.ids[i]) - xv) * R(t.f[i]);
+      if (i < n && i != s) d = d + (P(i, t.ids[i]) - xv) * R(t.f[i]);
     R ws = s == 0 ? R(t.f[0]) : s == 1 ? R(t.f[1]) : s == 2 ? R(t.f[2]) : R(t.f[3]);
     return stitch_d<R>(d, ws, R(t.f[4]));
   }
   V3<R> x[4];
-  for (int i = 0; i < 4; ++i) x[i] = i == s ? xv : P(t.ids[i]);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) x[i] = i == s ? xv : P(i, t.ids[i]);
   return contact_d<R>(rec_ckind(t.head), x, s, R(t.f[0]), cr, ck);
 }
 
