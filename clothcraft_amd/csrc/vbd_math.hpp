@@ -346,7 +346,7 @@ CV_HD R activation_v(R d, R r, R k) {
 // Contact::energy (src/ogc/potential.rs:186-212).
 template <class R>
 CV_HD R contact_e(int kind, const V3<R>* p, R eps, R r, R k) {
-  R d;
+  R d, nn = R(0);
   if (kind == KIND_PP) {
     V3<R> v = p[0] - p[1];
     d = sqrt(dot(v, v));
@@ -355,21 +355,20 @@ CV_HD R contact_e(int kind, const V3<R>* p, R eps, R r, R k) {
     R s = dot(w, u) / dot(u, u);
     V3<R> dd = w - u * s;
     d = sqrt(dot(dd, dd));
-  } else if (kind == KIND_PF) {
-    V3<R> n = cross(p[2] - p[1], p[3] - p[1]);
-    R h = dot(p[0] - p[1], n) / sqrt(dot(n, n));
-    d = h < R(0) ? -h : h;
   } else {
-    V3<R> n = cross(p[1] - p[0], p[3] - p[2]);
-    R nn = dot(n, n);
-    if (nn < R(1e-30)) return R(0);
-    R h = dot(p[0] - p[2], n) / sqrt(nn);
+    // PF: n = (p2 - p1) x (p3 - p1), r = p0 - p1; EE: n = (p1 - p0) x (p3 - p2),
+    // r = p0 - p2 (one code path for both: a wavefront holding both kinds
+    // runs it once).
+    const bool pf = kind == KIND_PF;
+    V3<R> n = cross(pf ? p[2] - p[1] : p[1] - p[0], pf ? p[3] - p[1] : p[3] - p[2]);
+    nn = dot(n, n);
+    if (!pf && nn < R(1e-30)) return R(0);
+    R h = dot(pf ? p[0] - p[1] : p[0] - p[2], n) / sqrt(nn);
     d = h < R(0) ? -h : h;
   }
   R e = activation_v(d, r, k);
   if (kind == KIND_EE) {
-    V3<R> n = cross(p[1] - p[0], p[3] - p[2]);
-    R qq = dot(n, n) * (R(1) / eps);
+    R qq = nn * (R(1) / eps);
     if (qq < R(1)) e = e * qq * (R(2) - qq);
   }
   return e;
@@ -467,19 +466,20 @@ CV_HD Deriv<R> contact_d(int kind, const V3<R>* p, int slot, R eps, R r, R k) {
       s3_add_sym(hd, th, u, gam / uu);
       s3_add_outer(hd, th, -d / uu);
     }
-  } else if (kind == KIND_PF) {
-    n = cross(p[2] - p[1], p[3] - p[1]);
+  } else {  // PF and EE share one path (see contact_e)
+    const bool pf = kind == KIND_PF;
+    n = cross(pf ? p[2] - p[1] : p[1] - p[0], pf ? p[3] - p[1] : p[3] - p[2]);
     nn = dot(n, n);
-    R c = slot == 0 ? R(1) : (slot == 1 ? R(-1) : R(0));
-    a = slot == 1 ? p[3] - p[2] : (slot == 2 ? p[1] - p[3] : (slot == 3 ? p[2] - p[1] : a));
-    plane_dist_d(p[0] - p[1], n, nn, a, c, d, gd, hd);
-  } else {
-    n = cross(p[1] - p[0], p[3] - p[2]);
-    nn = dot(n, n);
-    if (nn < R(1e-30)) return o;
-    R c = slot == 0 ? R(1) : (slot == 2 ? R(-1) : R(0));
-    a = slot == 0 ? p[3] - p[2] : (slot == 1 ? p[2] - p[3] : (slot == 2 ? p[0] - p[1] : p[1] - p[0]));
-    plane_dist_d(p[0] - p[2], n, nn, a, c, d, gd, hd);
+    if (!pf && nn < R(1e-30)) return o;
+    R c;
+    if (pf) {
+      c = slot == 0 ? R(1) : (slot == 1 ? R(-1) : R(0));
+      a = slot == 1 ? p[3] - p[2] : (slot == 2 ? p[1] - p[3] : (slot == 3 ? p[2] - p[1] : a));
+    } else {
+      c = slot == 0 ? R(1) : (slot == 2 ? R(-1) : R(0));
+      a = slot == 0 ? p[3] - p[2] : (slot == 1 ? p[2] - p[3] : (slot == 2 ? p[0] - p[1] : p[1] - p[0]));
+    }
+    plane_dist_d(pf ? p[0] - p[1] : p[0] - p[2], n, nn, a, c, d, gd, hd);
   }
   R phi = activation_v(d, r, k);
   R q = kind == KIND_EE ? nn * (R(1) / eps) : R(1);
