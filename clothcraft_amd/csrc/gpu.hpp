@@ -136,6 +136,8 @@ struct StepBufs {
   DBuf<float> gsq;
   DBuf<double> gtrue;  // per-vertex |g|^2 of the true residual
   DBuf<uint32_t> order;
+  DBuf<uint4> vinfo;  // colour-ordered (vertex, term offset, term count, contact offset)
+  DBuf<uint32_t> vnc;  // colour-ordered contact-record counts
   // fast colouring: contact-neighbour CSR, round states, pending-round
   // flags, colour histogram, sort keys
   DBuf<uint32_t> fc_cnt, fc_off, fc_cur, fc_nb, fc_state, fc_pend, fc_hist, fc_key, fc_key_sorted,

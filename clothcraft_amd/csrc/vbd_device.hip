@@ -61,6 +61,8 @@ template <class R>
 struct SweepArgs {
   using T4 = typename P4<R>::T;
   const uint32_t* order;
+  const uint4* vinfo;  // per colour-ordered slot: vertex, s0, ns, c0 (k_vertex_info)
+  const uint32_t* vnc;  // and its contact-record count
   T4* pos;
   const T4* tgt;
   const T4* ball;
@@ -203,22 +205,23 @@ __device__ __forceinline__ cvm::Deriv<R> local_deriv(const SweepArgs<R>& a, cons
   return acc;
 }
 
-// The local solve of vertex v by its group of L lanes (src/vbd.rs:337-395).
+// The local solve of vertex vi.x by its group of L lanes (src/vbd.rs:337-395);
+// vi = (vertex, first term record, term count, first contact record), nc the
+// contact-record count.
 template <int L, class R>
-__device__ __forceinline__ void solve_vertex(const SweepArgs<R>& a, uint32_t v, int lane) {
+__device__ __forceinline__ void solve_vertex(const SweepArgs<R>& a, uint4 vi, uint32_t nc, int lane) {
   using T4 = typename P4<R>::T;
-  // Everything indexed by v is loaded at once, ahead of the term records.
+  const uint32_t v = vi.x, s0 = vi.y, ns = vi.z, c0 = vi.w, nt = ns + nc;
+  // The vertex's data and the lane's first term record are loaded at once.
   const T4 p4 = a.pos[v];
-  const uint32_t s0 = a.term_off[v], s1 = a.term_off[v + 1], c0 = a.coff[v], c1 = a.coff[v + 1];
   const T4 tg = a.tgt[v];
   const T4 b4 = a.ball[v];
+  const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt);
   if (p4.w > R(0.5)) {  // pinned: no update, zero residual contribution
     if (lane == 0) a.gsq[v] = 0.0f;
     return;
   }
   V3<R> x{p4.x, p4.y, p4.z};
-  const uint32_t ns = s1 - s0, nt = ns + (c1 - c0);
-  const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt);
   cvm::Deriv<R> acc = local_deriv<L>(a, lt, v, x, lane, s0, ns, c0, nt);
   V3<R> target{tg.x, tg.y, tg.z};
   R inertia = tg.w;
@@ -257,11 +260,12 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uin
   // neighbours: keep consecutive groups on one XCD's L2.
   const uint32_t grp = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / L;
   if (grp >= count) return;
-  // The vertex id and the stop flag are loaded together (one memory round
-  // trip instead of two before the solve starts).
-  const uint32_t v = a.order[off + grp];
+  // The vertex's slot data and the stop flag are loaded together (one
+  // memory round trip instead of two before the solve starts).
+  const uint4 vi = a.vinfo[off + grp];
+  const uint32_t nc = a.vnc[off + grp];
   if (*a.stop) return;
-  solve_vertex<L, R>(a, v, lane);
+  solve_vertex<L, R>(a, vi, nc, lane);
 }
 
 // One colour index of one iteration of many solvers in a single launch: slot
@@ -286,7 +290,8 @@ __global__ __launch_bounds__(TPB) void k_sweep_batch(const SweepArgs<R>* args, c
   const BatchSlot sl = slots[lo];
   const SweepArgs<R>& a = args[sl.inst];
   if (*a.stop) return;
-  solve_vertex<L, R>(a, a.order[sl.off + grp - sl.start], lane);
+  const uint32_t slot = sl.off + grp - sl.start;
+  solve_vertex<L, R>(a, a.vinfo[slot], a.vnc[slot], lane);
 }
 
 // End of iteration `it` (it = -1: after the truncated guess), on the device
@@ -695,6 +700,8 @@ template <class R>
 SweepArgs<R> sweep_args(cv_gpu& g, const cv_settings* set, const cv_grab* grab) {
   SweepArgs<R> a{};
   a.order = g.b.order.p;
+  a.vinfo = g.b.vinfo.p;
+  a.vnc = g.b.vnc.p;
   a.pos = PBufs<R>::pos(g);
   a.tgt = PBufs<R>::tgt(g);
   a.ball = PBufs<R>::ball(g);
@@ -787,6 +794,25 @@ uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
   return k;
 }
 
+// Per colour-ordered slot i: the vertex order[i], its first term record and
+// term count, its first contact record and contact-record count, so a sweep
+// group loads them in one step and its term records in the next.
+__global__ void k_vertex_info(const uint32_t* order, const uint32_t* term_off, const uint32_t* coff, uint32_t n,
+                              uint4* vinfo, uint32_t* vnc) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t v = order[i];
+  vinfo[i] = make_uint4(v, term_off[v], term_off[v + 1] - term_off[v], coff[v]);
+  vnc[i] = coff[v + 1] - coff[v];
+}
+void vertex_info(cv_gpu& g, uint32_t n) {
+  g.b.vinfo.reserve(n);
+  g.b.vnc.reserve(n);
+  k_vertex_info<<<blocks(n), TPB, 0, g.stream>>>(g.b.order.p, g.dev.term_off.p, g.b.coff.p, n, g.b.vinfo.p,
+                                                 g.b.vnc.p);
+  CV_HIP(hipGetLastError());
+}
+
 // The fast colouring (on the device, gpu_fast_colors) of the last detection;
 // reused while the detections find no contacts and the grab is unchanged.
 uint32_t color_on_device(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
@@ -870,6 +896,7 @@ void detect_and_color(cv_gpu& g, const cv_solver& s, const cv_settings* set, con
   CV_HIP(hipGetLastError());
   double t1 = now_ms();
   out.colors = fast ? color_on_device(g, s, grab) : color_on_host(g, s, grab);
+  vertex_info(g, n);  // the contact offsets changed, and perhaps the order
   out.detection_ms += now_ms() - t0;
   out.narrow_ms += t1 - t0;
 }
