@@ -70,7 +70,8 @@ struct Scalars {
   unsigned long long clipped;
   unsigned long long truncated_total;  // clipped vertices summed over the step's iterations
   unsigned int stop, stop_it;          // refresh needed after iteration stop_it (device decides)
-  unsigned int iters_run, pad_;
+  unsigned int iters_run;
+  unsigned int sweep_blocks;           // blocks of the iteration's last sweep launch that have finished
   unsigned long long n_over;           // edges served by the overflow broad-phase kernel
   unsigned long long cand_export_vf, cand_export_ee;
   unsigned long long vf_list_n, ee_list_n;  // candidates queued for the narrow phase

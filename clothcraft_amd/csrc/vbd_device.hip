@@ -252,20 +252,72 @@ __device__ __forceinline__ void solve_vertex(const SweepArgs<R>& a, uint4 vi, ui
   }
 }
 
+// End of iteration `it` (it = -1: after the truncated guess), on the device
+// so the host need not read the clipped count back every iteration: counts
+// the iteration, accumulates truncations (c, the iteration's clipped count)
+// and, when the clipped count reaches the refresh threshold
+// (src/vbd.rs:326-336), raises `stop` so the launches already queued behind
+// it are no-ops until the host has re-detected; with the residual tolerance,
+// a converged sweep ends the step.
+__device__ __forceinline__ void iter_end(Scalars* sc, unsigned long long c, int it, double threshold, double tol,
+                                         double nfree3) {
+  if (sc->stop) return;
+  if (it >= 0) {
+    sc->iters_run += 1;
+    sc->truncated_total += c;
+  }
+  sc->clipped = 0;
+  // Residual-tolerance extension: converged after this sweep ends the step.
+  if (it >= 0 && tol > 0.0 && sqrt(sc->true_residual_sum / nfree3) <= tol) {
+    sc->stop = 2;
+    sc->stop_it = (unsigned int)(it + 1);
+    return;
+  }
+  if (c > 0 && (double)c >= threshold) {
+    sc->stop = 1;
+    sc->stop_it = (unsigned int)(it + 1);  // the next iteration to run after the refresh
+  }
+}
+__global__ void k_iter_end(Scalars* sc, int it, double threshold, double tol, double nfree3) {
+  if (it < 0) sc->sweep_blocks = 0;
+  iter_end(sc, sc->clipped, it, threshold, tol, nfree3);
+}
+// The iteration's end folded into its last sweep launch: the last block to
+// finish (device-scope counter) runs iter_end.  Only without the residual
+// tolerance, which needs a residual pass between the sweep and the decision.
+struct IterTail {
+  Scalars* sc;  // null: no tail
+  int it;
+  double threshold;
+};
+__device__ __forceinline__ void iter_tail(const IterTail& t) {
+  __syncthreads();  // the block's clipped-count atomics are done
+  if (threadIdx.x != 0) return;
+  __threadfence();
+  if (atomicAdd(&t.sc->sweep_blocks, 1u) != gridDim.x - 1) return;
+  __threadfence();
+  t.sc->sweep_blocks = 0;
+  // Read through to the device-coherent value (this CU's L1 may hold a stale
+  // line of the scalars).
+  unsigned long long c = __hip_atomic_load(&t.sc->clipped, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  iter_end(t.sc, c, t.it, t.threshold, 0.0, 1.0);
+}
+
 // One colour of one iteration.
 template <int L, class R>
-__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uint32_t count) {
+__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uint32_t count, IterTail tail) {
   const int lane = threadIdx.x & (L - 1);
   // Vertices of a colour are in index order, so neighbouring groups share
   // neighbours: keep consecutive groups on one XCD's L2.
   const uint32_t grp = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / L;
-  if (grp >= count) return;
-  // The vertex's slot data and the stop flag are loaded together (one
-  // memory round trip instead of two before the solve starts).
-  const uint4 vi = a.vinfo[off + grp];
-  const uint32_t nc = a.vnc[off + grp];
-  if (*a.stop) return;
-  solve_vertex<L, R>(a, vi, nc, lane);
+  if (grp < count) {
+    // The vertex's slot data and the stop flag are loaded together (one
+    // memory round trip instead of two before the solve starts).
+    const uint4 vi = a.vinfo[off + grp];
+    const uint32_t nc = a.vnc[off + grp];
+    if (!*a.stop) solve_vertex<L, R>(a, vi, nc, lane);
+  }
+  if (tail.sc) iter_tail(tail);  // block-uniform
 }
 
 // One colour index of one iteration of many solvers in a single launch: slot
@@ -292,31 +344,6 @@ __global__ __launch_bounds__(TPB) void k_sweep_batch(const SweepArgs<R>* args, c
   if (*a.stop) return;
   const uint32_t slot = sl.off + grp - sl.start;
   solve_vertex<L, R>(a, a.vinfo[slot], a.vnc[slot], lane);
-}
-
-// End of iteration `it` (it = -1: after the truncated guess), on the device
-// so the host need not read the clipped count back every iteration: counts
-// the iteration, accumulates truncations and, when the clipped count reaches
-// the refresh threshold (src/vbd.rs:326-336), raises `stop` so the launches
-// already queued behind it are no-ops until the host has re-detected.
-__global__ void k_iter_end(Scalars* sc, int it, double threshold, double tol, double nfree3) {
-  if (sc->stop) return;
-  unsigned long long c = sc->clipped;
-  if (it >= 0) {
-    sc->iters_run += 1;
-    sc->truncated_total += c;
-  }
-  sc->clipped = 0;
-  // Residual-tolerance extension: converged after this sweep ends the step.
-  if (it >= 0 && tol > 0.0 && sqrt(sc->true_residual_sum / nfree3) <= tol) {
-    sc->stop = 2;
-    sc->stop_it = (unsigned int)(it + 1);
-    return;
-  }
-  if (c > 0 && (double)c >= threshold) {
-    sc->stop = 1;
-    sc->stop_it = (unsigned int)(it + 1);  // the next iteration to run after the refresh
-  }
 }
 
 struct IterEnd {
@@ -730,22 +757,22 @@ SweepArgs<R> sweep_args(cv_gpu& g, const cv_settings* set, const cv_grab* grab) 
 // stamps at the kernel's own begin and end (hipExtLaunchKernelGGL), so the
 // summed time is kernel execution only, as a kernel trace reports it.
 template <int L, class R>
-void launch_sweep(const SweepArgs<R>& a, uint32_t off, uint32_t count, hipStream_t st, hipEvent_t e0,
-                  hipEvent_t e1) {
+void launch_sweep(const SweepArgs<R>& a, uint32_t off, uint32_t count, const IterTail& tail, hipStream_t st,
+                  hipEvent_t e0, hipEvent_t e1) {
   dim3 grid(blocks((uint64_t)count * L)), block(TPB);
   if (e0)
-    hipExtLaunchKernelGGL(k_sweep<L, R>, grid, block, 0, st, e0, e1, 0, a, off, count);
+    hipExtLaunchKernelGGL(k_sweep<L, R>, grid, block, 0, st, e0, e1, 0, a, off, count, tail);
   else
-    k_sweep<L, R><<<grid, block, 0, st>>>(a, off, count);
+    k_sweep<L, R><<<grid, block, 0, st>>>(a, off, count, tail);
 }
 template <class R>
-void sweep_color(cv_gpu& g, const SweepArgs<R>& a, uint32_t off, uint32_t count, hipEvent_t e0 = nullptr,
-                 hipEvent_t e1 = nullptr) {
+void sweep_color(cv_gpu& g, const SweepArgs<R>& a, uint32_t off, uint32_t count, const IterTail& tail = IterTail{},
+                 hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   switch (g.lanes) {
-    case 8: launch_sweep<8>(a, off, count, g.stream, e0, e1); break;
-    case 16: launch_sweep<16>(a, off, count, g.stream, e0, e1); break;
-    case 64: launch_sweep<64>(a, off, count, g.stream, e0, e1); break;
-    default: launch_sweep<32>(a, off, count, g.stream, e0, e1); break;
+    case 8: launch_sweep<8>(a, off, count, tail, g.stream, e0, e1); break;
+    case 16: launch_sweep<16>(a, off, count, tail, g.stream, e0, e1); break;
+    case 64: launch_sweep<64>(a, off, count, tail, g.stream, e0, e1); break;
+    default: launch_sweep<32>(a, off, count, tail, g.stream, e0, e1); break;
   }
 }
 
@@ -1065,9 +1092,17 @@ struct StepCtx {
       // time per launch than the kernel runs).
       bool timed = g.profile && (g.timed_counter++ % kTimedStride == 0);
       if (timed) timed_its.push_back(it);
+      // Without the residual tolerance the iteration's end rides on its last
+      // (non-empty) colour's launch.
+      size_t last = 0;
+      for (size_t c = 0; c + 1 < g.color_off.size(); ++c)
+        if (g.color_off[c + 1] > g.color_off[c]) last = c;
+      const bool fold = tol <= 0.0 && g.color_off.size() > 1 && g.color_off.back() > 0;
       for (size_t c = 0; c + 1 < g.color_off.size(); ++c) {
         uint32_t off = g.color_off[c], cnt = g.color_off[c + 1] - off;
         if (!cnt) continue;
+        IterTail tail{};
+        if (fold && c == last) tail = IterTail{g.d_scal, (int)it, threshold};
         if (timed) {
           if (g.sweep_events.size() < 2 * (nev + 1)) {
             hipEvent_t e0, e1;
@@ -1076,14 +1111,14 @@ struct StepCtx {
             g.sweep_events.push_back(e0);
             g.sweep_events.push_back(e1);
           }
-          sweep_color<R>(g, a, off, cnt, g.sweep_events[2 * nev], g.sweep_events[2 * nev + 1]);
+          sweep_color<R>(g, a, off, cnt, tail, g.sweep_events[2 * nev], g.sweep_events[2 * nev + 1]);
           launches.push_back({nev++, it});
         } else {
-          sweep_color<R>(g, a, off, cnt);
+          sweep_color<R>(g, a, off, cnt, tail);
           launches.push_back({(size_t)-1, it});
         }
       }
-      end_iteration(it);
+      if (!fold) end_iteration(it);
     }
   }
   // Event times of completed timed launches; called after the next work is
