@@ -1165,12 +1165,19 @@ struct StepCtx {
   }
 };
 
+// carry_in: the previous step of a device-resident run already queued this
+// step's start state and prologue reduction and read them back with its own
+// final scalars; carry_out: queue them for the next step the same way (one
+// host round trip fewer per step).
 template <class R>
-void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out, bool diag) {
+void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out, bool diag,
+                bool carry_in, bool carry_out) {
   StepCtx<R> c(g, s, set, grab, out);
   c.diag = diag;
-  c.prologue_reduce();
-  CV_HIP(hipStreamSynchronize(g.stream));
+  if (!carry_in) {
+    c.prologue_reduce();
+    CV_HIP(hipStreamSynchronize(g.stream));
+  }
   c.begin();
   uint32_t first, end;
   while (true) {
@@ -1182,15 +1189,24 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
   }
   c.finish();
   c.drain_timing();
+  if (carry_out) {
+    // The commit is the next step's state: positions into x64 (velocities
+    // are in v64), then its origin, motion and scale.
+    CV_HIP(hipMemcpyAsync(g.b.x64.p, g.b.xcur64.p, 3 * (size_t)s.n * sizeof(double), hipMemcpyDeviceToDevice,
+                          g.stream));
+    double dt = set->dt;
+    ::prologue_reduce(g, s.n, dt, 1.0 + set->damping * dt, set->gravity);
+  }
   read_scalars(g);
   c.check();
 }
 
-void run_step(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out, bool diag) {
+void run_step(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* grab, StepOut& out, bool diag,
+              bool carry_in = false, bool carry_out = false) {
   if (g.f64)
-    run_step_t<double>(g, s, set, grab, out, diag);
+    run_step_t<double>(g, s, set, grab, out, diag, carry_in, carry_out);
   else
-    run_step_t<float>(g, s, set, grab, out, diag);
+    run_step_t<float>(g, s, set, grab, out, diag, carry_in, carry_out);
 }
 
 // Batched step of many solvers (each with its own context g_i and stream):
@@ -1682,16 +1698,15 @@ static cv_status run_steps(cv_gpu* g, cv_solver* s, cv_state* st, const cv_setti
     for (uint32_t k = 0; k < steps; ++k) {
       o = StepOut{};
       if (k > 0) {
-        // The previous step's commit is the new state: positions into x64,
-        // velocities already in v64, acceleration history on the device.
-        CV_HIP(hipMemcpyAsync(g->b.x64.p, g->b.xcur64.p, 3 * n * sizeof(double), hipMemcpyDeviceToDevice,
-                              g->stream));
+        // The previous step's commit is the new state (the previous step
+        // queued it into x64 with this step's prologue reduction);
+        // velocities are in v64, the acceleration history on the device.
         g->pred_valid = true;
         t0 = now_ms();
       } else {
         CV_HIP(hipEventRecord(g->ev0, g->stream));
       }
-      run_step(*g, *s, set, grab, o, k + 1 == steps);
+      run_step(*g, *s, set, grab, o, k + 1 == steps, k > 0, k + 1 < steps);
       tot.detections += o.detections;
       tot.truncated += o.truncated;
       tot.iterations_run += o.iterations_run;
