@@ -48,7 +48,10 @@ __device__ __forceinline__ void stbox(double* b, uint32_t i, const ogm::Box& v) 
   b[6 * i + 4] = v.hi.y;
   b[6 * i + 5] = v.hi.z;
 }
-__device__ __forceinline__ long long cell_of(double v, double h) { return (long long)floor(v / h); }
+// Grid cell of a coordinate, from the inverse cell size (a multiply instead of
+// an f64 division).  Any monotone map is a valid grid: registration and
+// queries use this one function.
+__device__ __forceinline__ long long cell_of(double v, double inv_h) { return (long long)floor(v * inv_h); }
 __device__ __forceinline__ unsigned long long cell_key(long long i, long long j, long long k) {
   return ((unsigned long long)(i & 0x1FFFFF) << 42) | ((unsigned long long)(j & 0x1FFFFF) << 21) |
          (unsigned long long)(k & 0x1FFFFF);
@@ -74,13 +77,13 @@ __global__ void k_boxes(const double* x, const uint32_t* faces, const uint8_t* f
   }
 }
 
-__device__ __forceinline__ void cell_range(const ogm::Box& b, double h, long long r[6]) {
-  r[0] = cell_of(b.lo.x, h);
-  r[1] = cell_of(b.lo.y, h);
-  r[2] = cell_of(b.lo.z, h);
-  r[3] = cell_of(b.hi.x, h);
-  r[4] = cell_of(b.hi.y, h);
-  r[5] = cell_of(b.hi.z, h);
+__device__ __forceinline__ void cell_range(const ogm::Box& b, double inv_h, long long r[6]) {
+  r[0] = cell_of(b.lo.x, inv_h);
+  r[1] = cell_of(b.lo.y, inv_h);
+  r[2] = cell_of(b.lo.z, inv_h);
+  r[3] = cell_of(b.hi.x, inv_h);
+  r[4] = cell_of(b.hi.y, inv_h);
+  r[5] = cell_of(b.hi.z, inv_h);
 }
 
 __device__ __forceinline__ uint32_t bucket_of(unsigned long long key, int log2t) {
@@ -93,11 +96,11 @@ __device__ __forceinline__ ogm::Box prim_box(const double* fbox, const double* e
 // Cells covered by each primitive box inflated by pr: faces 0..F-1, then
 // edges F..F+E-1.
 __global__ void k_reg_count(const double* fbox, const double* ebox, uint32_t F, uint32_t E, double pr,
-                            double h, uint32_t* count) {
+                            double inv_h, uint32_t* count) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= F + E) return;
   long long r[6];
-  cell_range(inflate(prim_box(fbox, ebox, F, i), pr), h, r);
+  cell_range(inflate(prim_box(fbox, ebox, F, i), pr), inv_h, r);
   long long c = (r[3] - r[0] + 1) * (r[4] - r[1] + 1) * (r[5] - r[2] + 1);
   count[i] = (uint32_t)(c > kMaxCells ? 1 : c);  // over the cap: one sentinel entry, see k_reg_fill
 }
@@ -118,7 +121,7 @@ __global__ void k_reg_owner(const uint32_t* off, uint32_t N, uint32_t* owner, ui
 // (one returning atomic per thread, all in flight together).  Faces use
 // buckets [0, T), edges [T, 2T).
 // The grid covers the capacity; the entry count M = off[N] is read here.
-__global__ void k_reg_fill(const double* fbox, const double* ebox, uint32_t F, double pr, double h,
+__global__ void k_reg_fill(const double* fbox, const double* ebox, uint32_t F, double pr, double inv_h,
                            const uint32_t* off, uint32_t N, const uint32_t* owner, uint64_t cap, int log2t,
                            unsigned long long* keys, uint32_t* prims, uint32_t* ranks, uint32_t* bcount,
                            Scalars* sc) {
@@ -128,15 +131,16 @@ __global__ void k_reg_fill(const double* fbox, const double* ebox, uint32_t F, d
   uint32_t i = owner[j];
   uint32_t table = i < F ? 0u : (1u << log2t), prim = i < F ? i : i - F;
   long long r[6];
-  cell_range(inflate(prim_box(fbox, ebox, F, i), pr), h, r);
+  cell_range(inflate(prim_box(fbox, ebox, F, i), pr), inv_h, r);
   long long nx = r[3] - r[0] + 1, ny = r[4] - r[1] + 1, nz = r[5] - r[2] + 1;
   unsigned long long key;
   if (nx * ny * nz > kMaxCells) {  // over the per-primitive cap: report, never drop silently
     key = ~0ull;                    // sentinel key never matches a cell (cell keys use 63 bits)
     atomicOr(&sc->err_overflow, 2u);
   } else {
-    long long k = (long long)(j - off[i]);
-    long long z = k % nz, y = (k / nz) % ny, x = k / (nz * ny);
+    // 32-bit index arithmetic: k < nx ny nz <= kMaxCells.
+    const uint32_t k = j - off[i], uz = (uint32_t)nz, uy = (uint32_t)ny;
+    const uint32_t kz = k / uz, z = k - kz * uz, x = kz / uy, y = kz - x * uy;
     key = cell_key(r[0] + x, r[1] + y, r[2] + z);
   }
   keys[j] = key;
@@ -179,7 +183,7 @@ struct DetectArgs {
   const double* rest_len2;
   const double* fbox;
   const double* ebox;
-  double rq, pr, pr2, h;
+  double rq, pr, pr2, inv_h;
   const unsigned long long* skey;  // bucketed entries: cell key, primitive id and box
   const uint32_t* sprim;
   const double* sbox;
@@ -319,7 +323,7 @@ __device__ __forceinline__ bool ee_filter(const DetectArgs& a, uint32_t e, const
   if (a.edge_moving[f] && f < e) return false;
   ogm::Box bf = inflate(bfr, a.pr);
   double lx = fmax(be.lo.x, bf.lo.x), ly = fmax(be.lo.y, bf.lo.y), lz = fmax(be.lo.z, bf.lo.z);
-  if (cell_key(cell_of(lx, a.h), cell_of(ly, a.h), cell_of(lz, a.h)) != cell) return false;
+  if (cell_key(cell_of(lx, a.inv_h), cell_of(ly, a.inv_h), cell_of(lz, a.inv_h)) != cell) return false;
   uint32_t jc = a.edges[2 * f], jd = a.edges[2 * f + 1];
   if (ia == jc || ia == jd || ib == jc || ib == jd) return false;
   if ((ia_ex && (is_excluded(a, ia, jc) || is_excluded(a, ia, jd))) ||
@@ -425,7 +429,7 @@ __global__ void k_items(DetectArgs a, uint32_t* len, uint32_t* start, unsigned l
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < a.n_total) {
     D3 p = ld3(a.x, i);
-    unsigned long long key = cell_key(cell_of(p.x, a.h), cell_of(p.y, a.h), cell_of(p.z, a.h));
+    unsigned long long key = cell_key(cell_of(p.x, a.inv_h), cell_of(p.y, a.inv_h), cell_of(p.z, a.inv_h));
     uint32_t bk = bucket_of(key, a.log2t);
     uint32_t s0 = a.bstart[bk];
     start[i] = s0;
@@ -437,7 +441,7 @@ __global__ void k_items(DetectArgs a, uint32_t* len, uint32_t* start, unsigned l
     int c = 0;
     if (a.edge_moving[e]) {
       long long r[6];
-      cell_range(ldbox(a.ebox, e), a.h, r);
+      cell_range(ldbox(a.ebox, e), a.inv_h, r);
       long long cnt = (r[3] - r[0] + 1) * (r[4] - r[1] + 1) * (r[5] - r[2] + 1);
       if (cnt > kEdgeCells) {
         over[atomicAdd(n_over, 1ull)] = e;
@@ -580,7 +584,7 @@ __global__ __launch_bounds__(TPB) void k_ee_overflow(DetectArgs a, const uint32_
     uint32_t e = over[gi];
     ogm::Box be = ldbox(a.ebox, e);
     long long r[6];
-    cell_range(be, a.h, r);
+    cell_range(be, a.inv_h, r);
     uint32_t ia = a.edges[2 * e], ib = a.edges[2 * e + 1];
     bool ia_ex = a.ex_off[ia] != a.ex_off[ia + 1], ib_ex = a.ex_off[ib] != a.ex_off[ib + 1];
     uint32_t table = 1u << a.log2t;
@@ -792,7 +796,8 @@ int build_grid(cv_gpu& g, uint32_t F, uint32_t E, double pr, double h) {
   uint32_t N = F + E;
   b.cell_count.reserve(N + 1);
   b.cell_off.reserve(N + 1);
-  k_reg_count<<<blocks(N), TPB, 0, st>>>(b.fbox.p, b.ebox.p, F, E, pr, h, b.cell_count.p);
+  const double inv_h = 1.0 / h;
+  k_reg_count<<<blocks(N), TPB, 0, st>>>(b.fbox.p, b.ebox.p, F, E, pr, inv_h, b.cell_count.p);
   CV_HIP(hipGetLastError());
   CV_HIP(hipMemsetAsync(b.cell_count.p + N, 0, sizeof(uint32_t), st));
   scan_u32(g, b.cell_count.p, b.cell_off.p, (size_t)N + 1);
@@ -820,7 +825,7 @@ int build_grid(cv_gpu& g, uint32_t F, uint32_t E, double pr, double h) {
   b.bstart.reserve(T2 + 1);
   CV_HIP(hipMemsetAsync(b.bcount.p, 0, (T2 + 1) * sizeof(uint32_t), st));
   k_reg_owner<<<blocks(N), TPB, 0, st>>>(b.cell_off.p, N, b.cell_owner.p, cap, g.d_scal);
-  k_reg_fill<<<blocks(cap), TPB, 0, st>>>(b.fbox.p, b.ebox.p, F, pr, h, b.cell_off.p, N, b.cell_owner.p, cap,
+  k_reg_fill<<<blocks(cap), TPB, 0, st>>>(b.fbox.p, b.ebox.p, F, pr, inv_h, b.cell_off.p, N, b.cell_owner.p, cap,
                                           log2t, b.cell_key.p, b.cell_val.p, b.cell_rank.p, b.bcount.p, g.d_scal);
   CV_HIP(hipGetLastError());
   scan_u32(g, b.bcount.p, b.bstart.p, T2 + 1);
@@ -934,7 +939,7 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   a.rq = rq;
   a.pr = pr_reg;
   a.pr2 = pr2;
-  a.h = h;
+  a.inv_h = 1.0 / h;
   a.dist_bits = b.dist_bits.p;
   a.raw_ids = b.raw_ids.p;
   a.raw_count = b.raw_count.p;
