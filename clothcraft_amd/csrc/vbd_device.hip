@@ -355,21 +355,7 @@ __global__ void k_iter_end_batch(const IterEnd* e, uint32_t count) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const IterEnd q = e[i];
-  Scalars* sc = q.sc;
-  if (sc->stop) return;
-  unsigned long long c = sc->clipped;
-  sc->iters_run += 1;
-  sc->truncated_total += c;
-  sc->clipped = 0;
-  if (q.tol > 0.0 && sqrt(sc->true_residual_sum / q.nfree3) <= q.tol) {
-    sc->stop = 2;
-    sc->stop_it = (unsigned int)(q.it + 1);
-    return;
-  }
-  if (c > 0 && (double)c >= q.threshold) {
-    sc->stop = 1;
-    sc->stop_it = (unsigned int)(q.it + 1);
-  }
+  iter_end(q.sc, q.sc->clipped, q.it, q.threshold, q.tol, q.nfree3);
 }
 
 // Gradient probe (src/vbd.wgsl:142-147): (g, E) of the local objective.
