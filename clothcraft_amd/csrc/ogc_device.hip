@@ -213,9 +213,6 @@ __device__ __forceinline__ bool is_excluded(const DetectArgs& a, uint32_t u, uin
   }
   return false;
 }
-__device__ __forceinline__ bool adjacent(const DetectArgs& a, uint32_t u, uint32_t w) {
-  return u == w || (a.ex_off[u] != a.ex_off[u + 1] && is_excluded(a, u, w));
-}
 // Per-vertex minimum distance as the bit pattern of a non-negative double
 // (ordered like the value).  Values only decrease, so a plain read that is
 // already <= d (even a stale one) proves the atomic would change nothing:

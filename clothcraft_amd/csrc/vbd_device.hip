@@ -1769,7 +1769,6 @@ cv_status cv_gpu_probe(cv_gpu* g, cv_solver* s, const cv_state* st, const cv_set
     stage_state(*g, *s, st, set);
     g->pred_valid = false;
     size_t n = s->n;
-    hipStream_t stream = g->stream;
     double dt = set->dt, damp = 1.0 + set->damping * dt;
     prologue_reduce(*g, (uint32_t)n, dt, damp, set->gravity);
     read_scalars(*g);
