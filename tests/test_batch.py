@@ -14,10 +14,13 @@ def _instances():
             scenes.small_sheet_on_sphere(24, 0.26, seed=8)]
 
 
-def test_batched_step_equals_individual_steps(gpu):
+@pytest.mark.parametrize("flags", [0, 4], ids=["reference_colouring", "fast_colouring"])
+def test_batched_step_equals_individual_steps(gpu, flags):
+    """With either colouring (flags 4: CV_FLAG_FAST_COLORING, coloured on each
+    solver's own stream in the batch)."""
     from clothcraft_amd import GpuVbd, VbdSolver
     scs = _instances()
-    settings = dict(scenes.default_settings(dt=1 / 60, iterations=12, radius=0.002, stiffness=1e5))
+    settings = dict(scenes.default_settings(dt=1 / 60, iterations=12, radius=0.002, stiffness=1e5), flags=flags)
     one = GpuVbd(0)
     a_states = [sc.state.copy() for sc in scs]
     b_states = [sc.state.copy() for sc in scs]
