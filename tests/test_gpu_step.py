@@ -276,6 +276,10 @@ def test_colourings_on_the_device_equal_the_oracle(gpu):
     xg = g2.state.x + np.random.default_rng(5).normal(0.0, 0.004, g2.state.x.shape)
     cases.append((g2, g2.state, xg, 0.008))
     grab = dict(vertices=(3, 40, 77), weights=(0.2, 0.3, 0.5), target=(0, 0, 0), stiffness=10.0)
+    from clothcraft_amd import VbdError
+    with pytest.raises(VbdError) as e:  # no detection of this solver yet
+        gpu.colors(VbdSolver(sc.garment, sc.avatar, cases[0][1]), None, fast=True)
+    assert "cv_gpu_detect" in e.value.message
     for scn, st, xs, rq in cases:
         sol = VbdSolver(scn.garment, scn.avatar, st)
         ref = O.Solver(scn.garment, scn.avatar, st)
