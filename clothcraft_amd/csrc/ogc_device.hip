@@ -374,7 +374,8 @@ __device__ __forceinline__ void ee_narrow(const DetectArgs& a, uint32_t e, uint3
 // device.  VF entries grow from the front of the global list, EE entries
 // from the back (the two never meet: the list holds one slot per broad-phase
 // pair, and each pair yields at most one candidate).
-constexpr int kListBuf = 1024;
+constexpr int kPairUnroll = 4;  // pairs per thread per broad-pass step
+constexpr int kListBuf = 2048;
 struct ListStage {
   uint2 vf[kListBuf], ee[kListBuf];
   uint32_t n_vf, n_ee;
@@ -514,35 +515,49 @@ __global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, const uint32_t* own
   PairCounts cnt;
   const uint32_t X = 8, x = blockIdx.x % X, nbx = gridDim.x / X;
   const uint64_t lo = W * x / X, hi = W * (x + 1) / X, step = (uint64_t)nbx * blockDim.x;
-  // Block-uniform trip count (the loop holds barriers).
-  for (uint64_t j0 = lo + (uint64_t)(blockIdx.x / X) * blockDim.x; j0 < hi; j0 += step) {
-    uint64_t j = j0 + threadIdx.x;
-    bool vf = false, ee = false;
-    uint32_t q = 0, prim = 0;
-    if (j < hi) {
-      uint32_t item = owner[j];
-      uint32_t it = start[item] + (uint32_t)(j - off[item]);
-      unsigned long long key = ikey[item];
-      prim = a.sprim[it];
-      if (a.skey[it] == key) {  // else another cell hashed to the same bucket
-        const double* bx = a.sbox + 6 * (size_t)it;
-        ogm::Box eb{{bx[0], bx[1], bx[2]}, {bx[3], bx[4], bx[5]}};
-        if (item < a.n_total) {
-          q = item;
-          D3 p = ld3(a.x, q);
-          vf = vf_filter(a, q, q < a.n, a.ex_off[q] != a.ex_off[q + 1], ogm::box_of(p), prim, eb, cnt);
-        } else {
-          q = (item - a.n_total) / kEdgeCells;
-          uint32_t ia = a.edges[2 * q], ib = a.edges[2 * q + 1];
-          ee = ee_filter(a, q, ldbox(a.ebox, q), ia, ib, a.ex_off[ia] != a.ex_off[ia + 1],
-                         a.ex_off[ib] != a.ex_off[ib + 1], prim, eb, key, cnt);
+  // Block-uniform trip count (the loop holds barriers).  Each step covers
+  // kPairUnroll consecutive block-wide spans: the threads' pair tests are
+  // independent, so their loads overlap, and the staging barriers are paid
+  // once per step.
+  for (uint64_t j0 = lo + (uint64_t)(blockIdx.x / X) * blockDim.x * kPairUnroll; j0 < hi;
+       j0 += step * kPairUnroll) {
+    bool vf[kPairUnroll], ee[kPairUnroll];
+    uint32_t qs[kPairUnroll], prims[kPairUnroll];
+#pragma unroll
+    for (int u = 0; u < kPairUnroll; ++u) {
+      uint64_t j = j0 + (uint64_t)u * blockDim.x + threadIdx.x;
+      vf[u] = ee[u] = false;
+      uint32_t q = 0, prim = 0;
+      if (j < hi) {
+        uint32_t item = owner[j];
+        uint32_t it = start[item] + (uint32_t)(j - off[item]);
+        unsigned long long key = ikey[item];
+        prim = a.sprim[it];
+        if (a.skey[it] == key) {  // else another cell hashed to the same bucket
+          const double* bx = a.sbox + 6 * (size_t)it;
+          ogm::Box eb{{bx[0], bx[1], bx[2]}, {bx[3], bx[4], bx[5]}};
+          if (item < a.n_total) {
+            q = item;
+            D3 p = ld3(a.x, q);
+            vf[u] = vf_filter(a, q, q < a.n, a.ex_off[q] != a.ex_off[q + 1], ogm::box_of(p), prim, eb, cnt);
+          } else {
+            q = (item - a.n_total) / kEdgeCells;
+            uint32_t ia = a.edges[2 * q], ib = a.edges[2 * q + 1];
+            ee[u] = ee_filter(a, q, ldbox(a.ebox, q), ia, ib, a.ex_off[ia] != a.ex_off[ia + 1],
+                              a.ex_off[ib] != a.ex_off[ib + 1], prim, eb, key, cnt);
+          }
         }
       }
+      qs[u] = q;
+      prims[u] = prim;
     }
-    stage_pair(stage.vf, &stage.n_vf, vf, q, prim);
-    stage_pair(stage.ee, &stage.n_ee, ee, q, prim);
+#pragma unroll
+    for (int u = 0; u < kPairUnroll; ++u) {
+      stage_pair(stage.vf, &stage.n_vf, vf[u], qs[u], prims[u]);
+      stage_pair(stage.ee, &stage.n_ee, ee[u], qs[u], prims[u]);
+    }
     __syncthreads();
-    bool full = stage.n_vf > kListBuf - TPB || stage.n_ee > kListBuf - TPB;
+    bool full = stage.n_vf > kListBuf - kPairUnroll * TPB || stage.n_ee > kListBuf - kPairUnroll * TPB;
     __syncthreads();  // every thread has read the counts before they change again
     if (full) flush_stage(stage, a, list, W);
   }
