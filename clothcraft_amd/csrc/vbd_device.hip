@@ -60,7 +60,6 @@ struct GrabArgs {
 template <class R>
 struct SweepArgs {
   using T4 = typename P4<R>::T;
-  const uint32_t* order;
   const uint4* vinfo;  // per colour-ordered slot: vertex, s0, ns, c0 (k_vertex_info)
   const uint32_t* vnc;  // and its contact-record count
   T4* pos;
@@ -712,7 +711,6 @@ struct PBufs<double> {
 template <class R>
 SweepArgs<R> sweep_args(cv_gpu& g, const cv_settings* set, const cv_grab* grab) {
   SweepArgs<R> a{};
-  a.order = g.b.order.p;
   a.vinfo = g.b.vinfo.p;
   a.vnc = g.b.vnc.p;
   a.pos = PBufs<R>::pos(g);
