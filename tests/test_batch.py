@@ -38,6 +38,27 @@ def test_batched_step_equals_individual_steps(gpu, flags):
     print("batched == individual over 4 steps for", [sc.name for sc in scs])
 
 
+def test_full_batch_of_64_uneven_instances_equals_individual_steps(gpu):
+    """The batched workload at its full size (64 seeded instances, 1k-50k
+    vertices each, ~0.78 M vertices): two batched steps equal each
+    instance's own steps bit for bit."""
+    from clothcraft_amd import GpuVbd, VbdSolver
+    scs = garments.batched_scenes(64, seed=3)
+    settings = scs[0].settings
+    one = GpuVbd(0)
+    a_states = [sc.state.copy() for sc in scs]
+    b_states = [sc.state.copy() for sc in scs]
+    a_sol = [VbdSolver(sc.garment, sc.avatar, st) for sc, st in zip(scs, a_states)]
+    b_sol = [VbdSolver(sc.garment, sc.avatar, st) for sc, st in zip(scs, b_states)]
+    print("vertices", sum(sc.garment.n for sc in scs))
+    for k in range(2):
+        for s_, st in zip(a_sol, a_states):
+            one.step(s_, st, settings)
+        gpu.step_batch(b_sol, b_states, settings)
+        for i, (a, b) in enumerate(zip(a_states, b_states)):
+            assert np.array_equal(a.x, b.x) and np.array_equal(a.v, b.v), (k, i)
+
+
 def test_batch_failure_leaves_only_the_failing_state_unchanged(gpu):
     from clothcraft_amd import VbdError, VbdSolver
     scs = _instances()[:3]
