@@ -113,6 +113,16 @@ def test_full_size_sheet_on_sphere_in_contact_matches_the_oracle(gpu):
     assert mg["contact_terms"] == mr["contact_terms"] > 0
     assert mg["detection_updates"] == mr["detection_updates"]
     assert err <= 1e-5
+    # both colourings of a device detection at this state equal the oracle's
+    from clothcraft_amd import OgcSettings
+    rq = 0.006
+    det = gpu.detect(sol2, st.x, rq, OgcSettings(radius=0.002), candidates=False)
+    for fast in (False, True):
+        exp, kexp, c = ref.detect_and_color(st.x, rq, O.ogc_c(radius=0.002), None, fast=fast)
+        assert np.array_equal(c["kind_ids"], det["kind_ids"])
+        got, k = gpu.colors(sol2, None, fast=fast)
+        print(f"full size colouring fast={fast}: {k} colours, {len(det['kind_ids'])} contacts")
+        assert k == kexp and np.array_equal(got, exp)
     c1, c2 = st.copy(), st.copy()
     gpu.run(VbdSolver(sc.garment, sc.avatar, c1), c1, sc.settings, 5)
     gpu.run(VbdSolver(sc.garment, sc.avatar, c2), c2, sc.settings, 5)
