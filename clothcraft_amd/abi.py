@@ -23,7 +23,7 @@ STATUS_NAMES = {
     15: "CV_ERR_NONFINITE_RESULT", 16: "CV_ERR_TRUST_REGION", 17: "CV_ERR_FIXED_CHANGED",
     18: "CV_ERR_QUERY_RADIUS", 19: "CV_ERR_INVALID_POSITIONS", 20: "CV_ERR_DEVICE",
     21: "CV_ERR_CAPACITY", 22: "CV_ERR_INVALID_SEAM", 23: "CV_ERR_NOT_BUILT",
-    24: "CV_ERR_INVALID_MATERIAL",
+    24: "CV_ERR_INVALID_MATERIAL", 25: "CV_ERR_LINE_SEARCH",
 }
 
 CV_FLAG_DIAGNOSTICS_OFF = 1 << 0
@@ -105,6 +105,19 @@ class cv_detection(C.Structure):
                 ("vertex_face_candidates", C.c_uint64), ("edge_edge_candidates", C.c_uint64),
                 ("n_contacts", C.c_uint64), ("active_contacts", C.c_uint64),
                 ("minimum_distance", C.c_double)]
+
+
+class cv_newton_settings(C.Structure):
+    _fields_ = [("newton_iterations", C.c_uint32), ("pcg_iterations", C.c_uint32),
+                ("absolute_tolerance", C.c_double), ("relative_tolerance", C.c_double)]
+
+
+class cv_newton_metrics(C.Structure):
+    _fields_ = [("newton_iterations", C.c_uint32), ("pcg_iterations", C.c_uint32),
+                ("line_search_backtracks", C.c_uint32), ("converged", C.c_uint32),
+                ("residual_rms_n", C.c_double), ("tolerance_n", C.c_double),
+                ("objective_initial_j", C.c_double), ("objective_final_j", C.c_double),
+                ("max_regularization", C.c_double)]
 
 
 def ptr(a: np.ndarray | None, ctype):

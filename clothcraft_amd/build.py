@@ -24,6 +24,7 @@ SOURCES = [
     ("ogc_device.hip", ["-ffp-contract=off"]),
     ("mlp_device.hip", []),
     ("color_device.hip", []),
+    ("newton_device.hip", []),
     ("solver.cpp", ["-ffp-contract=off", "-x", "hip"]),
     ("api.cpp", ["-x", "hip"]),
 ]

@@ -179,6 +179,12 @@ struct StepBufs {
 
 // Warm-start network state on the device (FP32 master parameters, Adam
 // moments, BF16 operand tiles, gradient scratch, training data).
+// Workspace of the device Newton reference (cv_gpu_newton).
+struct NewtonBufs {
+  DBuf<double4> X, Xt;  // iterate; trial and difference points (garment, then avatar)
+  DBuf<double> x0, v0, g, gp, gm, diag, diag0, sol, r, z, p, ap, epart, parts;
+};
+
 struct Mlp {
   DBuf<float> master, grad, m1, m2, partial, loss_partial, loss;
   DBuf<uint16_t> fw, bw;
@@ -307,6 +313,7 @@ struct cv_gpu {
   uint64_t timed_counter = 0;  // iterations queued with profiling on
   double static_bytes32 = 0, static_bytes64 = 0;  // algorithmic bytes of one sweep, static terms
   bool f64 = true;  // sweep precision (f64 reproduces the f64 reference step)
+  NewtonBufs nt;
   Mlp mlp;
   DBuf<float> feat, wpred;  // warm-start features [12n] and predictions [3n]
   // Batched steps: one context per solver of the batch, each with its own
@@ -357,6 +364,9 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
                 size_t cand_cap);
 // Exclusive scan of count uint32 values on the context's stream.
 void device_scan_u32(cv_gpu& g, const uint32_t* in, uint32_t* out, size_t count);
+// Device Newton reference of one step (newton_device.hip).
+void newton_reference(cv_gpu& g, const cv_solver& s, const cv_state* st, const cv_settings* set,
+                      const cv_newton_settings* ns, double* x_out, cv_newton_metrics* out);
 // Fast colouring (CV_FLAG_FAST_COLORING) of the last detection's contacts and
 // the grab, on the device: the colour-grouped vertex order into b.order and
 // the colour offsets into color_off; returns the colour count.

@@ -64,6 +64,9 @@ def lib():
         L.cv_gpu_detect.argtypes = [vp, vp, abi.f64p, C.c_double, C.POINTER(abi.cv_ogc_settings),
                                     C.POINTER(abi.cv_detection), abi.u32p, C.c_uint64, abi.u32p,
                                     C.c_uint64, abi.u32p, C.c_uint64, abi.f64p, abi.f64p]
+        L.cv_gpu_newton.argtypes = [vp, vp, C.POINTER(abi.cv_state), C.POINTER(abi.cv_settings),
+                                    C.POINTER(abi.cv_newton_settings), abi.f64p,
+                                    C.POINTER(abi.cv_newton_metrics)]
         L.cv_gpu_colors.argtypes = [vp, vp, C.POINTER(abi.cv_grab), C.c_uint32, abi.u32p, C.c_uint32,
                                     abi.u32p]
         L.cv_gpu_step_batch.argtypes = [vp, C.POINTER(vp), C.POINTER(C.POINTER(abi.cv_state)),
@@ -290,6 +293,23 @@ class GpuVbd:
         if rc != 0:
             self._raise(rc)
         return out
+
+    def newton(self, solver: VbdSolver, start, settings, newton_iterations: int = 30,
+               pcg_iterations: int = 250, absolute_tolerance: float = 1e-6,
+               relative_tolerance: float = 1e-4):
+        """Newton energy reference of one step on the device
+        (ReferenceSolver::step, src/solver.rs:422-647; defaults of
+        SolverSettings, src/solver.rs:25-36): minimises the step objective
+        from `start` with the contacts of this solver's last device step or
+        detection frozen.  Returns (x, metrics); `start` is not changed."""
+        x = np.zeros((solver.n, 3))
+        ns = abi.cv_newton_settings(newton_iterations, pcg_iterations, absolute_tolerance, relative_tolerance)
+        m = abi.cv_newton_metrics()
+        rc = lib().cv_gpu_newton(self.h, solver.h, C.byref(start.to_c()), C.byref(_settings(settings)),
+                                 C.byref(ns), abi.ptr(x, C.c_double), C.byref(m))
+        if rc != 0:
+            self._raise(rc)
+        return x, {name: getattr(m, name) for name, _ in m._fields_}
 
     def detect(self, solver: VbdSolver, x, rq, ogc: OgcSettings | None = None,
                candidates: bool = True, capacity: int = 1 << 22) -> dict:

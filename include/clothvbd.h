@@ -52,7 +52,8 @@ typedef enum cv_status {
   CV_ERR_CAPACITY = 21,            /* a request exceeds a documented size limit             */
   CV_ERR_INVALID_SEAM = 22,        /* stitch weights invalid                                */
   CV_ERR_NOT_BUILT = 23,           /* a feature (e.g. warm-start network) is not set up     */
-  CV_ERR_INVALID_MATERIAL = 24     /* Material::validate (src/model.rs:40-63)               */
+  CV_ERR_INVALID_MATERIAL = 24,    /* Material::validate (src/model.rs:40-63)               */
+  CV_ERR_LINE_SEARCH = 25          /* "Newton line search failed; ..." (src/solver.rs:547)   */
 } cv_status;
 
 /* ------------------------------------------------------------- inputs */
@@ -256,6 +257,39 @@ cv_status cv_gpu_detect(cv_gpu* gpu, cv_solver* solver, const double* x, double 
  * the colour of each moving vertex; returns the colour count in *n_colors. */
 cv_status cv_gpu_colors(cv_gpu* gpu, cv_solver* solver, const cv_grab* grab, uint32_t flags,
                         uint32_t* color_of_vertex, uint32_t n, uint32_t* n_colors);
+
+/* Newton energy reference on the device — ReferenceSolver::step
+ * (src/solver.rs:422-647, SolverSettings of src/solver.rs:15-58 split
+ * between cv_newton_settings and the step fields of cv_settings): the f64 Newton-PCG minimiser of one
+ * step's objective from `start` (inertial form, membrane, bending, seams)
+ * plus the OGC contacts that the last cv_gpu_step or cv_gpu_detect of this
+ * solver found, frozen (the reference Newton has no OGC term; with them the
+ * objective is the one a contact step minimises).  dt, damping, gravity and
+ * the contact radius and stiffness come from `settings`.  Writes the
+ * minimiser to x_out [3n] and the solver's statistics to `metrics`; the
+ * handle's step state is not touched.  Needs the f64 sweep.  Errors and
+ * messages are the reference's ("Invalid solver settings", "Non-finite FEM
+ * assembly; step rolled back", "Newton line search failed; ..."). */
+typedef struct cv_newton_settings {
+  uint32_t newton_iterations;  /* 1..200  */
+  uint32_t pcg_iterations;     /* 1..5000 */
+  double absolute_tolerance;   /* > 0, RMS gradient (N)  */
+  double relative_tolerance;   /* [0, 1), of the initial RMS gradient */
+} cv_newton_settings;
+typedef struct cv_newton_metrics {  /* the solver fields of Metrics, src/solver.rs:60-75 */
+  uint32_t newton_iterations;
+  uint32_t pcg_iterations;
+  uint32_t line_search_backtracks;
+  uint32_t converged;
+  double residual_rms_n;
+  double tolerance_n;
+  double objective_initial_j;
+  double objective_final_j;
+  double max_regularization;
+} cv_newton_metrics;
+cv_status cv_gpu_newton(cv_gpu* gpu, cv_solver* solver, const cv_state* start,
+                        const cv_settings* settings, const cv_newton_settings* newton,
+                        double* x_out, cv_newton_metrics* metrics);
 
 /* Batched uneven scenes (no reference counterpart): one step of `count`
  * independent solvers of any sizes.  Each solver gets exactly the device work
