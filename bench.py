@@ -83,10 +83,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nx", type=int, default=128, help="quads per side (128 is the workload)")
     ap.add_argument("--workload", default="sheet_on_sphere",
-                    choices=("sheet_on_sphere", "sewn_garment", "batch64", "warm_infer"),
+                    choices=("sheet_on_sphere", "sewn_garment", "batch64", "warm_infer", "newton_sheet"),
                     help="sewn_garment: two ~50k-vertex Poisson-disk panels with side and shoulder "
                          "seams on a capsule body, 40 iterations; batch64: 64 independent sheets and "
-                         "sewn tubes of 1k-50k vertices in batched steps (not the headline line)")
+                         "sewn tubes of 1k-50k vertices in batched steps; newton_sheet: the device "
+                         "Newton energy reference of one headline step in contact, one solve per "
+                         "--steps (not the headline line)")
     ap.add_argument("--fast-coloring", action="store_true",
                     help="fast colouring on the device (fewer colours; not the reference's colouring order)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -116,6 +118,8 @@ def main():
         return bench_batch(args)
     if args.workload == "warm_infer":
         return bench_warm_infer(args)
+    if args.workload == "newton_sheet":
+        return bench_newton(args)
 
     if args.workload == "sewn_garment":
         from clothcraft_amd import garments
@@ -295,6 +299,38 @@ def bench_warm_infer(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
         "data": "synthetic normal features", "config": {"workload": f"MLP 12-128-128-128-3 GELU, {n} rows per call"},
         "algorithmic_flop_per_call": flop}), flush=True)
+
+
+def bench_newton(args):
+    """The Newton energy reference (cv_gpu_newton, the reference's Newton
+    settings: 30 iterations, 250 PCG iterations) of one step of the
+    128x128 sheet on the sphere, in contact after 140 device steps, with the
+    step's contacts frozen.  A "step" of this line is one Newton solve."""
+    from clothcraft_amd import GpuVbd, VbdSolver, scenes
+    sc = scenes.sheet_on_sphere(nx=args.nx)
+    s = sc.settings
+    gpu = GpuVbd(0)
+    st = sc.state.copy()
+    gpu.run(VbdSolver(sc.garment, sc.avatar, st), st, s, 140)
+    sol = VbdSolver(sc.garment, sc.avatar, st)
+    gpu.step(sol, st.copy(), s)
+    for _ in range(args.warmup):
+        gpu.newton(sol, st, s)
+    K = args.steps
+    t0 = time.perf_counter()
+    for _ in range(K):
+        _, m = gpu.newton(sol, st, s)
+    wall = time.perf_counter() - t0
+    print(json.dumps({
+        "metric": "device Newton energy reference solves/s", "value": K / wall, "unit": "solves/s",
+        "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": 1e3 * wall / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (procedural sheet and sphere)",
+        "config": {"workload": f"Newton-PCG of one step of the {args.nx}x{args.nx} sheet on the sphere "
+                               f"({sc.garment.n} vertices) in contact, contacts frozen"},
+        "newton_iterations": m["newton_iterations"], "pcg_iterations": m["pcg_iterations"],
+        "gradient_passes_per_s": K * (2 * m["pcg_iterations"] + m["newton_iterations"] + 1) / wall,
+        "converged": bool(m["converged"])}), flush=True)
 
 
 def bench_batch(args):
