@@ -180,9 +180,15 @@ struct StepBufs {
 // Warm-start network state on the device (FP32 master parameters, Adam
 // moments, BF16 operand tiles, gradient scratch, training data).
 // Workspace of the device Newton reference (cv_gpu_newton).
+struct NtScal {  // PCG scalars kept on the device between read-backs
+  double rz, tol, h, inv2h, alpha, beta;
+  unsigned int state;  // 0 running, 1 converged, 2 non-positive curvature
+  unsigned int its;    // iterations taken when state became non-zero
+};
 struct NewtonBufs {
   DBuf<double4> X, Xt;  // iterate; trial and difference points (garment, then avatar)
   DBuf<double> x0, v0, g, gp, gm, diag, diag0, sol, r, z, p, ap, epart, parts;
+  DBuf<NtScal> sc;
 };
 
 struct Mlp {

@@ -19,8 +19,9 @@
 //    exact gradient, so the inexact products change the iteration count, not
 //    the minimiser.
 // The Newton loop, PCG, regularisation shifts and Armijo backtracking are the
-// reference's; reductions run as fixed-size per-block partials summed on the
-// host in block order (deterministic).
+// reference's; reductions run as fixed-size per-block partials summed in
+// block order (deterministic), and the PCG scalars stay on the device with
+// the host reading the PCG state once per batch of iterations.
 #include <cmath>
 #include <vector>
 
@@ -75,7 +76,8 @@ __device__ __forceinline__ double lane_sum(double x) {
 // diagonal of its Hessian block plus the inertial mass term, and the vertex's
 // share of the objective (its inertial term and the terms it owns).
 __global__ __launch_bounds__(NT_TPB) void k_nt_eval(NtArgs a, const double4* X, double* grad, double* diag,
-                                                    double* epart) {
+                                                    double* epart, const NtScal* sc) {
+  if (sc && sc->state) return;  // a stopped PCG's queued difference passes
   const int lane = threadIdx.x & (NT_L - 1);
   const uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / NT_L;
   if (v >= a.n) return;  // whole groups leave together
@@ -207,11 +209,50 @@ __global__ __launch_bounds__(NT_TPB) void k_nt_move(const double4* X, const doub
     Xt[v] = q.w > 0.5 ? q : make_double4(q.x + s * p[3 * v], q.y + s * p[3 * v + 1], q.z + s * p[3 * v + 2], 0.0);
   }
 }
+// The PCG kernels below read their scalars from NtScal and do nothing once
+// its state is set, so the host can queue several iterations between
+// read-backs.  The *_fin kernels (one thread) reduce the 256 block partials
+// in block order, as a host sum would.
+__device__ __forceinline__ double parts_sum(const double* parts, int K, int j) {
+  double s = parts[j];
+  for (uint32_t b = 1; b < NT_PARTS; ++b) s += parts[(size_t)K * b + j];
+  return s;
+}
+__device__ __forceinline__ double parts_max(const double* parts, int K, int j) {
+  double s = parts[j];
+  for (uint32_t b = 1; b < NT_PARTS; ++b) s = fmax(s, parts[(size_t)K * b + j]);
+  return s;
+}
+// Difference step h = delta / max|p| (delta = 1e-7 of the scene scale).
+__device__ __forceinline__ void set_step(NtScal* sc, double pmax, double delta) {
+  const double h = delta / (pmax > 0.0 ? pmax : 1.0);
+  sc->h = h;
+  sc->inv2h = 0.5 / h;
+}
+__global__ void k_nt_init_fin(const double* parts, double delta, NtScal* sc) {
+  sc->rz = parts_sum(parts, 3, 1);
+  sc->tol = 1e-5 * sqrt(parts_sum(parts, 3, 0));
+  set_step(sc, parts_max(parts, 3, 2), delta);
+  sc->state = 0;
+  sc->its = 0;
+}
+// Xt = X +- h p on the free garment vertices.
+__global__ __launch_bounds__(NT_TPB) void k_nt_probe(const double4* X, const double* p, const NtScal* sc,
+                                                     double sign, uint32_t n, double4* Xt) {
+  if (sc->state) return;
+  const double s = sign * sc->h;
+  NT_LOOP(v, n) {
+    const double4 q = X[v];
+    Xt[v] = q.w > 0.5 ? q : make_double4(q.x + s * p[3 * v], q.y + s * p[3 * v + 1], q.z + s * p[3 * v + 2], 0.0);
+  }
+}
 // A p = (g(x + h p) - g(x - h p)) / 2h + shift p (p on pinned degrees of
 // freedom, where p is 0); partial p.Ap.
 __global__ __launch_bounds__(NT_TPB) void k_nt_ap(const double* gp, const double* gm, const double* p,
-                                                  const double4* X, double inv2h, double shift, uint32_t m,
+                                                  const double4* X, const NtScal* sc, double shift, uint32_t m,
                                                   double* ap, double* parts) {
+  if (sc->state) return;
+  const double inv2h = sc->inv2h;
   double v[1] = {0};
   NT_LOOP(i, m) {
     const double a = X[i / 3].w > 0.5 ? p[i] : (gp[i] - gm[i]) * inv2h + shift * p[i];
@@ -220,10 +261,22 @@ __global__ __launch_bounds__(NT_TPB) void k_nt_ap(const double* gp, const double
   }
   block_parts<1>(v, parts);
 }
+__global__ void k_nt_ap_fin(const double* parts, uint32_t it, NtScal* sc) {
+  if (sc->state) return;
+  const double curv = parts_sum(parts, 1, 0);
+  if (!isfinite(curv) || curv <= 0.0) {
+    sc->state = 2;
+    sc->its = it + 1;
+    return;
+  }
+  sc->alpha = sc->rz / curv;
+}
 // x += alpha p, r -= alpha Ap, z = r / d; partials r.r, r.z.
-__global__ __launch_bounds__(NT_TPB) void k_nt_update(double alpha, const double* p, const double* ap,
+__global__ __launch_bounds__(NT_TPB) void k_nt_update(const NtScal* sc, const double* p, const double* ap,
                                                       const double* diag0, const double4* X, double shift,
                                                       uint32_t m, double* sol, double* r, double* z, double* parts) {
+  if (sc->state) return;
+  const double alpha = sc->alpha;
   double v[2] = {0, 0};
   NT_LOOP(i, m) {
     sol[i] += alpha * p[i];
@@ -235,9 +288,22 @@ __global__ __launch_bounds__(NT_TPB) void k_nt_update(double alpha, const double
   }
   block_parts<2>(v, parts);
 }
+__global__ void k_nt_update_fin(const double* parts, uint32_t it, NtScal* sc) {
+  if (sc->state) return;
+  const double rr = parts_sum(parts, 2, 0), rz = parts_sum(parts, 2, 1);
+  if (sqrt(rr) <= sc->tol) {
+    sc->state = 1;
+    sc->its = it + 1;
+    return;
+  }
+  sc->beta = rz / sc->rz;
+  sc->rz = rz;
+}
 // p = z + beta p; partial max |p|.
-__global__ __launch_bounds__(NT_TPB) void k_nt_dir(double beta, const double* z, uint32_t m, double* p,
+__global__ __launch_bounds__(NT_TPB) void k_nt_dir(const NtScal* sc, const double* z, uint32_t m, double* p,
                                                    double* parts) {
+  if (sc->state) return;
+  const double beta = sc->beta;
   double v[1] = {0};
   NT_LOOP(i, m) {
     const double pi = z[i] + beta * p[i];
@@ -245,6 +311,10 @@ __global__ __launch_bounds__(NT_TPB) void k_nt_dir(double beta, const double* z,
     v[0] = fmax(v[0], fabs(pi));
   }
   block_parts<1, 1u>(v, parts);
+}
+__global__ void k_nt_dir_fin(const double* parts, double delta, NtScal* sc) {
+  if (sc->state) return;
+  set_step(sc, parts_max(parts, 1, 0), delta);
 }
 
 struct Newton {
@@ -278,7 +348,7 @@ struct Newton {
   }
   void eval(const double4* X, double* grad) {
     k_nt_eval<<<(unsigned)(((uint64_t)n * NT_L + NT_TPB - 1) / NT_TPB), NT_TPB, 0, g.stream>>>(a, X, grad, w.diag.p,
-                                                                                             w.epart.p);
+                                                                                             w.epart.p, nullptr);
   }
   double objective(const double4* X, double* grad) {
     eval(X, grad);
@@ -291,40 +361,39 @@ struct Newton {
   }
   // src/solver.rs:189-224 with A p by differences of the gradient.  The
   // diagonal of the current assembly is in w.diag0.  Returns false when the
-  // curvature along a direction is not positive.
+  // curvature along a direction is not positive.  Iterations are queued in
+  // batches of kBatch between read-backs of the state.
   bool pcg(double shift, uint32_t limit, uint32_t& its) {
-    double v3[3];
-    k_nt_init<<<NT_PARTS, NT_TPB, 0, g.stream>>>(w.g.p, w.diag0.p, w.X.p, n3, shift, w.sol.p, w.r.p, w.z.p, w.p.p,
-                                                 w.parts.p);
-    fetch<3>(v3, 4u);
-    double rz = v3[1], pmax = v3[2];
-    const double tol = 1e-5 * std::sqrt(v3[0]);
-    for (uint32_t it = 0; it < limit; ++it) {
-      const double h = NT_DELTA * scale / (pmax > 0.0 ? pmax : 1.0);
-      k_nt_move<<<NT_PARTS, NT_TPB, 0, g.stream>>>(w.X.p, w.p.p, h, n, w.Xt.p);
-      eval(w.Xt.p, w.gp.p);
-      k_nt_move<<<NT_PARTS, NT_TPB, 0, g.stream>>>(w.X.p, w.p.p, -h, n, w.Xt.p);
-      eval(w.Xt.p, w.gm.p);
-      k_nt_ap<<<NT_PARTS, NT_TPB, 0, g.stream>>>(w.gp.p, w.gm.p, w.p.p, w.X.p, 0.5 / h, shift, n3, w.ap.p,
-                                                 w.parts.p);
-      const double curv = fetch1();
-      if (!std::isfinite(curv) || curv <= 0.0) {
-        its = it + 1;
-        return false;
+    constexpr uint32_t kBatch = 16;
+    const double delta = NT_DELTA * scale;
+    NtScal* sc = w.sc.p;
+    hipStream_t st = g.stream;
+    k_nt_init<<<NT_PARTS, NT_TPB, 0, st>>>(w.g.p, w.diag0.p, w.X.p, n3, shift, w.sol.p, w.r.p, w.z.p, w.p.p,
+                                           w.parts.p);
+    k_nt_init_fin<<<1, 1, 0, st>>>(w.parts.p, delta, sc);
+    const unsigned eb = (unsigned)(((uint64_t)n * NT_L + NT_TPB - 1) / NT_TPB);
+    NtScal h{};
+    for (uint32_t it0 = 0; it0 < limit; it0 += kBatch) {
+      for (uint32_t it = it0; it < std::min(limit, it0 + kBatch); ++it) {
+        k_nt_probe<<<NT_PARTS, NT_TPB, 0, st>>>(w.X.p, w.p.p, sc, 1.0, n, w.Xt.p);
+        k_nt_eval<<<eb, NT_TPB, 0, st>>>(a, w.Xt.p, w.gp.p, w.diag.p, w.epart.p, sc);
+        k_nt_probe<<<NT_PARTS, NT_TPB, 0, st>>>(w.X.p, w.p.p, sc, -1.0, n, w.Xt.p);
+        k_nt_eval<<<eb, NT_TPB, 0, st>>>(a, w.Xt.p, w.gm.p, w.diag.p, w.epart.p, sc);
+        k_nt_ap<<<NT_PARTS, NT_TPB, 0, st>>>(w.gp.p, w.gm.p, w.p.p, w.X.p, sc, shift, n3, w.ap.p, w.parts.p);
+        k_nt_ap_fin<<<1, 1, 0, st>>>(w.parts.p, it, sc);
+        k_nt_update<<<NT_PARTS, NT_TPB, 0, st>>>(sc, w.p.p, w.ap.p, w.diag0.p, w.X.p, shift, n3, w.sol.p, w.r.p,
+                                                 w.z.p, w.parts.p);
+        k_nt_update_fin<<<1, 1, 0, st>>>(w.parts.p, it, sc);
+        k_nt_dir<<<NT_PARTS, NT_TPB, 0, st>>>(sc, w.z.p, n3, w.p.p, w.parts.p);
+        k_nt_dir_fin<<<1, 1, 0, st>>>(w.parts.p, delta, sc);
       }
-      const double alpha = rz / curv;
-      double v2[2];
-      k_nt_update<<<NT_PARTS, NT_TPB, 0, g.stream>>>(alpha, w.p.p, w.ap.p, w.diag0.p, w.X.p, shift, n3, w.sol.p,
-                                                     w.r.p, w.z.p, w.parts.p);
-      fetch<2>(v2);
-      if (std::sqrt(v2[0]) <= tol) {
-        its = it + 1;
-        return true;
+      CV_HIP(hipGetLastError());
+      CV_HIP(hipMemcpyAsync(&h, sc, sizeof(NtScal), hipMemcpyDeviceToHost, st));
+      CV_HIP(hipStreamSynchronize(st));
+      if (h.state) {
+        its = h.its;
+        return h.state == 1;
       }
-      const double beta = v2[1] / rz;
-      k_nt_dir<<<NT_PARTS, NT_TPB, 0, g.stream>>>(beta, w.z.p, n3, w.p.p, w.parts.p);
-      pmax = fetch1(1u);
-      rz = v2[1];
     }
     its = limit;
     return true;
@@ -347,6 +416,7 @@ void newton_reference(cv_gpu& g, const cv_solver& s, const cv_state* st, const c
     b->reserve(3 * (size_t)n);
   w.epart.reserve(n);
   w.parts.reserve(3 * NT_PARTS);
+  w.sc.reserve(1);
 
   // Start positions with pins at their targets (src/solver.rs:437-440); the
   // avatar as fixed vertices after the garment.
