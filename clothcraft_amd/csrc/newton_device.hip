@@ -19,8 +19,8 @@
 //    exact gradient, so the inexact products change the iteration count, not
 //    the minimiser.
 // The Newton loop, PCG, regularisation shifts and Armijo backtracking are the
-// reference's; reductions run as fixed-size per-block partials summed in
-// block order (deterministic), and the PCG scalars stay on the device with
+// reference's; reductions run as fixed-size per-block partials reduced in a
+// fixed order (deterministic), and the PCG scalars stay on the device with
 // the host reading the PCG state once per batch of iterations.
 #include <cmath>
 #include <vector>
@@ -211,17 +211,21 @@ __global__ __launch_bounds__(NT_TPB) void k_nt_move(const double4* X, const doub
 }
 // The PCG kernels below read their scalars from NtScal and do nothing once
 // its state is set, so the host can queue several iterations between
-// read-backs.  The *_fin kernels (one thread) reduce the 256 block partials
-// in block order, as a host sum would.
-__device__ __forceinline__ double parts_sum(const double* parts, int K, int j) {
-  double s = parts[j];
-  for (uint32_t b = 1; b < NT_PARTS; ++b) s += parts[(size_t)K * b + j];
-  return s;
-}
-__device__ __forceinline__ double parts_max(const double* parts, int K, int j) {
-  double s = parts[j];
-  for (uint32_t b = 1; b < NT_PARTS; ++b) s = fmax(s, parts[(size_t)K * b + j]);
-  return s;
+// read-backs.  The *_fin kernels (one block of NT_PARTS threads) reduce the
+// block partials in a fixed tree order (deterministic) and thread 0 updates
+// the scalars.
+template <bool MAX>
+__device__ __forceinline__ double parts_tree(const double* parts, int K, int j) {
+  __shared__ double t[NT_PARTS];
+  t[threadIdx.x] = parts[(size_t)K * threadIdx.x + j];
+  __syncthreads();
+  for (uint32_t s = NT_PARTS / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) t[threadIdx.x] = MAX ? fmax(t[threadIdx.x], t[threadIdx.x + s]) : t[threadIdx.x] + t[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = t[0];
+  __syncthreads();  // t is reused by the next call
+  return r;
 }
 // Difference step h = delta / max|p| (delta = 1e-7 of the scene scale).
 __device__ __forceinline__ void set_step(NtScal* sc, double pmax, double delta) {
@@ -229,10 +233,13 @@ __device__ __forceinline__ void set_step(NtScal* sc, double pmax, double delta) 
   sc->h = h;
   sc->inv2h = 0.5 / h;
 }
-__global__ void k_nt_init_fin(const double* parts, double delta, NtScal* sc) {
-  sc->rz = parts_sum(parts, 3, 1);
-  sc->tol = 1e-5 * sqrt(parts_sum(parts, 3, 0));
-  set_step(sc, parts_max(parts, 3, 2), delta);
+__global__ __launch_bounds__(NT_PARTS) void k_nt_init_fin(const double* parts, double delta, NtScal* sc) {
+  const double rr = parts_tree<false>(parts, 3, 0), rz = parts_tree<false>(parts, 3, 1);
+  const double pmax = parts_tree<true>(parts, 3, 2);
+  if (threadIdx.x) return;
+  sc->rz = rz;
+  sc->tol = 1e-5 * sqrt(rr);
+  set_step(sc, pmax, delta);
   sc->state = 0;
   sc->its = 0;
 }
@@ -261,9 +268,10 @@ __global__ __launch_bounds__(NT_TPB) void k_nt_ap(const double* gp, const double
   }
   block_parts<1>(v, parts);
 }
-__global__ void k_nt_ap_fin(const double* parts, uint32_t it, NtScal* sc) {
+__global__ __launch_bounds__(NT_PARTS) void k_nt_ap_fin(const double* parts, uint32_t it, NtScal* sc) {
   if (sc->state) return;
-  const double curv = parts_sum(parts, 1, 0);
+  const double curv = parts_tree<false>(parts, 1, 0);
+  if (threadIdx.x) return;
   if (!isfinite(curv) || curv <= 0.0) {
     sc->state = 2;
     sc->its = it + 1;
@@ -288,9 +296,10 @@ __global__ __launch_bounds__(NT_TPB) void k_nt_update(const NtScal* sc, const do
   }
   block_parts<2>(v, parts);
 }
-__global__ void k_nt_update_fin(const double* parts, uint32_t it, NtScal* sc) {
+__global__ __launch_bounds__(NT_PARTS) void k_nt_update_fin(const double* parts, uint32_t it, NtScal* sc) {
   if (sc->state) return;
-  const double rr = parts_sum(parts, 2, 0), rz = parts_sum(parts, 2, 1);
+  const double rr = parts_tree<false>(parts, 2, 0), rz = parts_tree<false>(parts, 2, 1);
+  if (threadIdx.x) return;
   if (sqrt(rr) <= sc->tol) {
     sc->state = 1;
     sc->its = it + 1;
@@ -312,9 +321,10 @@ __global__ __launch_bounds__(NT_TPB) void k_nt_dir(const NtScal* sc, const doubl
   }
   block_parts<1, 1u>(v, parts);
 }
-__global__ void k_nt_dir_fin(const double* parts, double delta, NtScal* sc) {
+__global__ __launch_bounds__(NT_PARTS) void k_nt_dir_fin(const double* parts, double delta, NtScal* sc) {
   if (sc->state) return;
-  set_step(sc, parts_max(parts, 1, 0), delta);
+  const double pmax = parts_tree<true>(parts, 1, 0);
+  if (threadIdx.x == 0) set_step(sc, pmax, delta);
 }
 
 struct Newton {
@@ -370,7 +380,7 @@ struct Newton {
     hipStream_t st = g.stream;
     k_nt_init<<<NT_PARTS, NT_TPB, 0, st>>>(w.g.p, w.diag0.p, w.X.p, n3, shift, w.sol.p, w.r.p, w.z.p, w.p.p,
                                            w.parts.p);
-    k_nt_init_fin<<<1, 1, 0, st>>>(w.parts.p, delta, sc);
+    k_nt_init_fin<<<1, NT_PARTS, 0, st>>>(w.parts.p, delta, sc);
     const unsigned eb = (unsigned)(((uint64_t)n * NT_L + NT_TPB - 1) / NT_TPB);
     NtScal h{};
     for (uint32_t it0 = 0; it0 < limit; it0 += kBatch) {
@@ -380,12 +390,12 @@ struct Newton {
         k_nt_probe<<<NT_PARTS, NT_TPB, 0, st>>>(w.X.p, w.p.p, sc, -1.0, n, w.Xt.p);
         k_nt_eval<<<eb, NT_TPB, 0, st>>>(a, w.Xt.p, w.gm.p, w.diag.p, w.epart.p, sc);
         k_nt_ap<<<NT_PARTS, NT_TPB, 0, st>>>(w.gp.p, w.gm.p, w.p.p, w.X.p, sc, shift, n3, w.ap.p, w.parts.p);
-        k_nt_ap_fin<<<1, 1, 0, st>>>(w.parts.p, it, sc);
+        k_nt_ap_fin<<<1, NT_PARTS, 0, st>>>(w.parts.p, it, sc);
         k_nt_update<<<NT_PARTS, NT_TPB, 0, st>>>(sc, w.p.p, w.ap.p, w.diag0.p, w.X.p, shift, n3, w.sol.p, w.r.p,
                                                  w.z.p, w.parts.p);
-        k_nt_update_fin<<<1, 1, 0, st>>>(w.parts.p, it, sc);
+        k_nt_update_fin<<<1, NT_PARTS, 0, st>>>(w.parts.p, it, sc);
         k_nt_dir<<<NT_PARTS, NT_TPB, 0, st>>>(sc, w.z.p, n3, w.p.p, w.parts.p);
-        k_nt_dir_fin<<<1, 1, 0, st>>>(w.parts.p, delta, sc);
+        k_nt_dir_fin<<<1, NT_PARTS, 0, st>>>(w.parts.p, delta, sc);
       }
       CV_HIP(hipGetLastError());
       CV_HIP(hipMemcpyAsync(&h, sc, sizeof(NtScal), hipMemcpyDeviceToHost, st));
