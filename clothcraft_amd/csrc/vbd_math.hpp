@@ -328,7 +328,7 @@ CV_HD Deriv<R> stitch_d(V3<R> d, R wv, R k) {
 }
 
 // -------------------------------------------------------------- contacts
-// Kinds follow src/ogc/potential.rs:171-177.
+// Kinds follow src/ogc/potential.rs:25-30.
 enum { KIND_PP = 0, KIND_PE = 1, KIND_PF = 2, KIND_EE = 3 };
 
 // src/ogc/potential.rs:8-22.
@@ -343,7 +343,7 @@ CV_HD R activation_v(R d, R r, R k) {
   }
   return R(0.5) * k * tau * tau - k * tau * tau * R(log(d / tau));
 }
-// Contact::energy (src/ogc/potential.rs:186-212).
+// Contact::energy (src/ogc/potential.rs:39-65).
 template <class R>
 CV_HD R contact_e(int kind, const V3<R>* p, R eps, R r, R k) {
   R d, nn = R(0);

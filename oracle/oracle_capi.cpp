@@ -451,7 +451,7 @@ double cvo_activation(double d, double r, double k, double* g, double* h) {
   return j.v;
 }
 double cvo_activation_value(double d, double r, double k) { return activation<double>(d, r, k); }
-// Contact::block (src/ogc/potential.rs:216-232) on 4 points x12.
+// Contact::block (src/ogc/potential.rs:69-85) on 4 points x12.
 double cvo_contact_block(int kind, const double* x12, int count, double eps, int vertex, double r,
                          double k, double* g3, double* h9) {
   Contact c{kind, {0, 1, 2, 3}, count, eps};

@@ -314,12 +314,12 @@ S edge_edge_distance(const A3<S>& a, const A3<S>& b, const A3<S>& c, const A3<S>
   return T::value(h) < 0.0 ? -h : h;
 }
 
-struct Contact {  // src/ogc/potential.rs:178-184
+struct Contact {  // src/ogc/potential.rs:31-37
   int kind;
   std::array<int, 4> vertices;
   int count;
   double mollifier_epsilon;
-  // src/ogc/potential.rs:186-212
+  // src/ogc/potential.rs:39-65
   template <class S>
   S energy(const std::array<A3<S>, 4>& p, double r, double k) const {
     using T = Sc<S>;
