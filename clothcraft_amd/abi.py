@@ -23,7 +23,7 @@ STATUS_NAMES = {
     15: "CV_ERR_NONFINITE_RESULT", 16: "CV_ERR_TRUST_REGION", 17: "CV_ERR_FIXED_CHANGED",
     18: "CV_ERR_QUERY_RADIUS", 19: "CV_ERR_INVALID_POSITIONS", 20: "CV_ERR_DEVICE",
     21: "CV_ERR_CAPACITY", 22: "CV_ERR_INVALID_SEAM", 23: "CV_ERR_NOT_BUILT",
-    24: "CV_ERR_INVALID_MATERIAL", 25: "CV_ERR_LINE_SEARCH",
+    24: "CV_ERR_INVALID_MATERIAL", 25: "CV_ERR_LINE_SEARCH", 26: "CV_ERR_GRADIENT_CHECK",
 }
 
 CV_FLAG_DIAGNOSTICS_OFF = 1 << 0

@@ -134,7 +134,7 @@ struct StepBufs {
   DBuf<unsigned long long> dist_bits;
   DBuf<float4> pos, tgt, ball;        // f32 sweep
   DBuf<double4> pos_d, tgt_d, ball_d;  // f64 sweep
-  DBuf<float> gsq;
+  DBuf<double> gsq;
   DBuf<double> gtrue;  // per-vertex |g|^2 of the true residual
   DBuf<uint32_t> order;
   DBuf<uint4> vinfo;  // colour-ordered (vertex, term offset, term count, contact offset)
@@ -283,6 +283,8 @@ struct cv_gpu {
   int device = 0;
   hipStream_t stream = nullptr;
   bool check_gradients = false;
+  double last_check_max_error = 0;  // largest gradient error of the last check (N)
+  DBuf<double> probe_out;           // [4n] probe of the gradient check
   DevSolver dev;
   StepBufs b;
   Scalars* d_scal = nullptr;  // device

@@ -44,6 +44,19 @@ int cvh_solver_create(const cv_garment* g, const cv_surface* avatar, const cv_st
 }
 void cvh_solver_destroy(cv_solver* s) { delete s; }
 
+// Host half of the runtime gradient check (check_probe) on given inputs.
+int cvh_check_probe(const cv_solver* s, const double* x, const double* t, const uint32_t* coff,
+                    const cvm::TermRec64* crec, const double* origin, const cv_settings* set, const cv_grab* grab,
+                    const double* probe, double* max_error, char* err, size_t errn) {
+  try {
+    *max_error = check_probe(*s, x, t, coff, crec, origin, set, grab, probe);
+  } catch (const CvError& e) {
+    if (err && errn) std::snprintf(err, errn, "%s", e.message.c_str());
+    return e.status;
+  }
+  return 0;
+}
+
 uint64_t cvh_solver_terms(const cv_solver* s, uint32_t* term_off, cvm::TermRec* recs, uint64_t cap) {
   if (term_off) std::memcpy(term_off, s->term_off.data(), s->term_off.size() * sizeof(uint32_t));
   if (recs && cap >= s->terms.size()) std::memcpy(recs, s->terms.data(), s->terms.size() * sizeof(cvm::TermRec));

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <set>
@@ -560,6 +561,63 @@ bool prediction_valid(const cv_solver& s, const cv_state* st, const cv_settings*
   for (size_t i = 0; i < 3 * (size_t)s.n; ++i)
     if (s.last_x[i] != st->x[i] || s.last_v[i] != st->v[i]) return false;
   return true;
+}
+
+namespace {
+// Rust's `{:?}` of an f64 prints the shortest decimal that reads back to the
+// same value; %.17g prints one that reads back exactly.
+std::string fmt_dvec3(double x, double y, double z) {
+  char buf[160];
+  std::snprintf(buf, sizeof buf, "DVec3(%.17g, %.17g, %.17g)", x, y, z);
+  return buf;
+}
+std::string fmt_g(double v) {
+  char buf[40];
+  std::snprintf(buf, sizeof buf, "%.17g", v);
+  return buf;
+}
+}  // namespace
+
+double check_probe(const cv_solver& s, const double* x, const double* t, const uint32_t* coff,
+                   const cvm::TermRec64* crec, const double* origin, const cv_settings* set, const cv_grab* grab,
+                   const double* probe) {
+  using V = cvm::V3<double>;
+  auto P = [&](int, uint32_t i) { return V{x[3 * i], x[3 * i + 1], x[3 * i + 2]}; };
+  const double cr = set->contact.radius, ck = set->contact.stiffness;
+  double max_error = 0.0;
+  for (uint32_t v = 0; v < s.n; ++v) {
+    const V xv = P(0, v);
+    cvm::Deriv<double> acc = cvm::deriv_zero<double>();
+    for (uint32_t k = s.term_off[v]; k < s.term_off[v + 1]; ++k)
+      cvm::deriv_add(acc, cvm::term_deriv<double>(s.terms64[k], xv, cr, ck, P));
+    for (uint32_t k = coff[v]; k < coff[v + 1]; ++k) cvm::deriv_add(acc, cvm::term_deriv<double>(crec[k], xv, cr, ck, P));
+    if (grab) {  // src/vbd.rs:237-246
+      int slot = -1;
+      for (int q = 0; q < 3; ++q)
+        if (grab->vertices[q] == v) slot = q;
+      if (slot >= 0) {
+        V d = xv - V{grab->target[0] - origin[0], grab->target[1] - origin[1], grab->target[2] - origin[2]};
+        for (int q = 0; q < 3; ++q)
+          if (q != slot) d = d + (P(0, grab->vertices[q]) - xv) * grab->weights[q];
+        cvm::deriv_add(acc, cvm::stitch_d<double>(d, grab->weights[slot], grab->stiffness));
+      }
+    }
+    const V tv{t[4 * v], t[4 * v + 1], t[4 * v + 2]};
+    const double inertia = t[4 * v + 3];
+    const V g = acc.g + (xv - tv) * inertia;
+    const V actual{probe[4 * v], probe[4 * v + 1], probe[4 * v + 2]};
+    const V diff = actual - g;
+    const double error = std::sqrt(cvm::dot(diff, diff));
+    if (!std::isfinite(error) || error > 0.003 + 0.003 * std::sqrt(cvm::dot(g, g)))
+      throw CvError{CV_ERR_GRADIENT_CHECK, "GPU gradient mismatch at " + std::to_string(v) + ": " +
+                                               fmt_dvec3(actual.x, actual.y, actual.z) + " vs " +
+                                               fmt_dvec3(g.x, g.y, g.z) + ", error " + fmt_g(error)};
+    max_error = std::max(max_error, error);
+    const double energy = acc.e + 0.5 * inertia * cvm::dot(xv - tv, xv - tv);
+    if (std::fabs(energy - probe[4 * v + 3]) > 1e-5 + 0.001 * std::fabs(energy))
+      throw CvError{CV_ERR_GRADIENT_CHECK, "GPU energy mismatch at " + std::to_string(v)};
+  }
+  return max_error;
 }
 
 uint32_t greedy_colors(const cv_solver& s, const uint32_t* cnb_off, const uint32_t* cnb,

@@ -79,6 +79,17 @@ void build_solver(cv_solver& s, const cv_garment* g, const cv_surface* avatar, c
 void validate_settings(const cv_settings* s);
 void validate_state(const cv_solver& s, const cv_state* st);
 void validate_grab(const cv_solver& s, const cv_grab* g);
+// Host half of GpuVbd.check_gradients (src/gpu_vbd.rs:336-372): every
+// vertex's local objective in f64 (closed-form term derivatives over the
+// solver's f64 records, the contact records, the grab, inertia) at the given
+// positions x ([3 (n + fixed)], relative to `origin`) and targets t ([4n]:
+// target relative to `origin`, inertia), compared with the device probe
+// ([4n]: g, E).  Throws CV_ERR_GRADIENT_CHECK with the reference's message on
+// a gradient error above 3 mN + 0.3 % or an energy error above 1e-5 + 0.1 %;
+// returns the largest gradient error.
+double check_probe(const cv_solver& s, const double* x, const double* t, const uint32_t* coff,
+                   const cvm::TermRec64* crec, const double* origin, const cv_settings* set, const cv_grab* grab,
+                   const double* probe);
 bool prediction_valid(const cv_solver& s, const cv_state* st, const cv_settings* set);
 
 // Greedy colouring in vertex order (src/vbd/coloring.rs:5-43), or with

@@ -69,7 +69,7 @@ struct SweepArgs {
   const cvm::TermRecT<R>* terms;
   const uint32_t* coff;
   const cvm::TermRecT<R>* crec;
-  float* gsq;
+  double* gsq;  // per-vertex |g|^2 of the sweep (f64 in both precisions)
   unsigned long long* clipped;
   const unsigned int* stop;  // set when a refresh is pending: the launch is a no-op
   R cr, ck;
@@ -217,7 +217,7 @@ __device__ __forceinline__ void solve_vertex(const SweepArgs<R>& a, uint4 vi, ui
   const T4 b4 = a.ball[v];
   const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt);
   if (p4.w > R(0.5)) {  // pinned: no update, zero residual contribution
-    if (lane == 0) a.gsq[v] = 0.0f;
+    if (lane == 0) a.gsq[v] = 0.0;
     return;
   }
   V3<R> x{p4.x, p4.y, p4.z};
@@ -246,7 +246,7 @@ __device__ __forceinline__ void solve_vertex(const SweepArgs<R>& a, uint4 vi, ui
   }
   if (lane == 0) {
     a.pos[v] = P4<R>::make(accepted.x, accepted.y, accepted.z, R(0));
-    a.gsq[v] = (float)gg;
+    a.gsq[v] = (double)gg;
     if (truncated) atomicAdd(a.clipped, 1ull);
   }
 }
@@ -664,20 +664,6 @@ __global__ void k_finalize(const typename P4<R>::T* pos, const double* x_old, co
     v_out[3 * i + k] = vn;
     acc[3 * i + k] = (vn - v_old[3 * i + k]) / dt + damping * vn;
   }
-}
-
-// Sum of |g|^2 in fixed order (single block).
-__global__ void k_sum_f32(const float* a, uint32_t n, double* out) {
-  __shared__ double red[1024];
-  double s = 0;
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s += (double)a[i];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
-    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = red[0];
 }
 
 void read_scalars(cv_gpu& g) {
@@ -1132,7 +1118,7 @@ struct StepCtx {
       true_residual<R>(g, a, n);
       out.true_residual = true;
     }
-    k_sum_f32<<<1, 1024, 0, g.stream>>>(b.gsq.p, n, &g.d_scal->residual_sum);
+    k_sum_f64<<<1, 1024, 0, g.stream>>>(b.gsq.p, n, &g.d_scal->residual_sum);
     // The trust-region check tolerance of the device path (src/gpu_vbd.rs:436).
     k_finalize<R><<<blocks(n), TPB, 0, g.stream>>>(PBufs<R>::pos(g), b.x_old64.p, b.v_old64.p, b.centers64.p,
                                                    b.radii64.p, d.pinned.p, n, set->dt, set->damping, 1e-8,
@@ -1149,6 +1135,59 @@ struct StepCtx {
   }
 };
 
+// GpuVbd.check_gradients (src/gpu_vbd.rs:305-379), device half: the probe
+// of every vertex's local objective (k_probe, the sweep's own derivative
+// code) at the positions and targets the sweep starts from, read back with
+// those positions, targets and the detection's contact records for the host
+// comparison (check_probe), which throws on a mismatch before anything is
+// committed.
+template <class R>
+void check_gradients_t(cv_gpu& g, const cv_solver& s, const SweepArgs<R>& a, const cv_settings* set,
+                       const cv_grab* grab) {
+  using T4 = typename P4<R>::T;
+  const size_t n = s.n, nall = n + g.dev.n_fixed;
+  hipStream_t st = g.stream;
+  g.probe_out.reserve(4 * n);
+  k_probe<32, R><<<blocks(n * 32), TPB, 0, st>>>(a, (uint32_t)n, g.probe_out.p);
+  CV_HIP(hipGetLastError());
+  std::vector<double> probe(4 * n);
+  std::vector<T4> pos(nall), tgt(n);
+  std::vector<uint32_t> coff(n + 1);
+  CV_HIP(hipMemcpyAsync(probe.data(), g.probe_out.p, probe.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  CV_HIP(hipMemcpyAsync(pos.data(), a.pos, nall * sizeof(T4), hipMemcpyDeviceToHost, st));
+  CV_HIP(hipMemcpyAsync(tgt.data(), a.tgt, n * sizeof(T4), hipMemcpyDeviceToHost, st));
+  CV_HIP(hipMemcpyAsync(coff.data(), a.coff, (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  CV_HIP(hipStreamSynchronize(st));
+  std::vector<cvm::TermRecT<R>> crec(coff[n]);
+  if (!crec.empty()) {
+    CV_HIP(hipMemcpyAsync(crec.data(), a.crec, crec.size() * sizeof(crec[0]), hipMemcpyDeviceToHost, st));
+    CV_HIP(hipStreamSynchronize(st));
+  }
+  std::vector<double> x(3 * nall), t(4 * n);
+  for (size_t i = 0; i < nall; ++i) {
+    x[3 * i] = (double)pos[i].x;
+    x[3 * i + 1] = (double)pos[i].y;
+    x[3 * i + 2] = (double)pos[i].z;
+  }
+  for (size_t i = 0; i < n; ++i) {
+    t[4 * i] = (double)tgt[i].x;
+    t[4 * i + 1] = (double)tgt[i].y;
+    t[4 * i + 2] = (double)tgt[i].z;
+    t[4 * i + 3] = (double)tgt[i].w;
+  }
+  std::vector<cvm::TermRec64> c64(crec.size());
+  for (size_t k = 0; k < crec.size(); ++k) {
+    c64[k] = cvm::TermRec64{};
+    c64[k].head = crec[k].head;
+    for (int q = 0; q < 4; ++q) c64[k].ids[q] = crec[k].ids[q];
+    c64[k].f[0] = (double)crec[k].f[0];  // the EE mollifier scale
+  }
+  g.last_check_max_error =
+      check_probe(s, x.data(), t.data(), coff.data(), c64.data(), g.h_scal->origin, set, grab, probe.data());
+  std::fprintf(stderr, "GPU/CPU local gradients verified: %zu vertices, maximum absolute error %.3e N\n", n,
+               g.last_check_max_error);
+}
+
 // carry_in: the previous step of a device-resident run already queued this
 // step's start state and prologue reduction and read them back with its own
 // final scalars; carry_out: queue them for the next step the same way (one
@@ -1163,6 +1202,7 @@ void run_step_t(cv_gpu& g, cv_solver& s, const cv_settings* set, const cv_grab* 
     CV_HIP(hipStreamSynchronize(g.stream));
   }
   c.begin();
+  if (g.check_gradients) check_gradients_t<R>(g, s, c.a, set, grab);
   uint32_t first, end;
   while (true) {
     c.read_back();

@@ -255,28 +255,42 @@ def merge_surfaces(parts) -> Surface:
     return Surface(np.vstack(vs), np.vstack(ts).astype(np.uint32))
 
 
-def mannequin(around: int = 112, along: int = 40) -> Surface:
+def mannequin(around: int = 112, along: int = 40, arm_x: float = 0.30) -> Surface:
     """Capsule-union body proxy: torso capsule r 0.15 m (centres y 0.85-1.30 m)
-    and two hanging arm capsules r 0.05 m at x = +-0.30 m (about 20k
+    and two hanging arm capsules r 0.05 m at x = +-arm_x (about 20k
     triangles at the default resolution)."""
     torso = capsule((0.0, 0.85, 0.0), (0.0, 1.30, 0.0), 0.15, around, along, 12)
-    arm_l = capsule((-0.30, 0.95, 0.0), (-0.30, 1.30, 0.0), 0.05, around // 2, along // 3, 4)
-    arm_r = capsule((0.30, 0.95, 0.0), (0.30, 1.30, 0.0), 0.05, around // 2, along // 3, 4)
+    arm_l = capsule((-arm_x, 0.95, 0.0), (-arm_x, 1.30, 0.0), 0.05, around // 2, along // 3, 4)
+    arm_r = capsule((arm_x, 0.95, 0.0), (arm_x, 1.30, 0.0), 0.05, around // 2, along // 3, 4)
     return merge_surfaces([torso, arm_l, arm_r])
+
+
+def _half_ellipse(a: float, b: float, m: int = 20001):
+    """Arc length s(theta) of x = a cos(theta), z = b sin(theta) on [0, pi]."""
+    t = np.linspace(0.0, math.pi, m)
+    ds = np.sqrt((a * np.sin(t)) ** 2 + (b * np.cos(t)) ** 2)
+    s = np.concatenate([[0.0], np.cumsum(0.5 * (ds[1:] + ds[:-1]) * np.diff(t))])
+    return t, s
 
 
 # --------------------------------------------------------------- workloads
 def sewn_garment_on_body(spacing: float = 0.0024, seed: int = 2, width: float = 0.6, height: float = 0.7,
-                         seam_stiffness: float = 1e4, iterations: int = 40, radius: float = 0.001) -> Scene:
+                         seam_stiffness: float = 1e4, iterations: int = 40, radius: float = 0.001,
+                         clearance: float = 0.01) -> Scene:
     """Two width x height panels (front and back, seeded Poisson-disk meshes;
-    the default spacing gives about 50k vertices each) wrapped as half
-    cylinders of radius width/pi around the mannequin's torso, joined by side
-    seams (the full vertical edges) and shoulder seams (the outer 0.12 m of
-    the top edges), penalty stitches of seam_stiffness N/m.  The side seams
-    start 4 mm apart and the panels 4 cm outside the torso.  Material:
-    StVK mu 2e4 / lambda 8e3 N/m, bending 2e-5 N m, 0.2 kg/m^2.  OGC radius
-    1 mm (below the mesh spacing, so neighbouring in-plane primitives are not
-    in contact at rest), stiffness 1e5 N/m; dt 1/60, 40 iterations."""
+    the default spacing gives about 50k vertices each), each wrapped without
+    stretch on half of an elliptic cylinder around the mannequin's torso and
+    joined by side seams (the full vertical edges) and shoulder seams (the
+    outer 0.12 m of the top edges), penalty stitches of seam_stiffness N/m.
+    The ellipse's front-back semi-axis is the torso radius plus `clearance`
+    (1 cm: the panels start 1 cm outside the body), its side semi-axis makes
+    the half perimeter the panel width plus the 4 mm the side seams start
+    apart, and the arm capsules hang `clearance` outside the side seams.
+    Material: StVK mu 2e4 / lambda 8e3 N/m, bending 2e-5 N m, 0.2 kg/m^2.
+    OGC radius 1 mm, stiffness 1e5 N/m (the radius is below the rest
+    distance of non-adjacent primitives of the 2.4 mm mesh, so the rest
+    state has no in-plane contacts; the reference default of 3 mm is not); dt 1/60, 40
+    iterations."""
     mat = Material.stvk(2e4, 8e3, 2e-5, 0.2)
     uv_f = poisson_disk(width, height, spacing, seed)
     uv_b = poisson_disk(width, height, spacing, seed + 1000)
@@ -284,29 +298,33 @@ def sewn_garment_on_body(spacing: float = 0.0024, seed: int = 2, width: float = 
     g = compile_panels(panels, lambda i: mat)
     g.materials = [mat]
     off = g.panel_offsets
-    R = width / math.pi
-    gap = 0.004
-    eps = math.asin(0.5 * gap / R)
-    top = 1.45
+    torso_r, arm_r, gap, top = 0.15, 0.05, 0.004, 1.45
+    b = torso_r + clearance
+    lo, hi = b, 2.0 * width / math.pi  # half perimeter monotone in the side semi-axis
+    for _ in range(60):
+        a = 0.5 * (lo + hi)
+        lo, hi = (a, hi) if _half_ellipse(a, b)[1][-1] < width + gap else (lo, a)
+    t, arc = _half_ellipse(a, b)
     X = np.zeros((g.n, 3))
     for p, sign in ((0, 1.0), (1, -1.0)):
         sl = slice(off[p], off[p + 1])
         u, v = g.uv[sl, 0], g.uv[sl, 1]
-        th = sign * (eps + (math.pi - 2 * eps) * (1.0 - u / width))
-        X[sl] = np.stack([R * np.cos(th), top - v, R * np.sin(th)], 1)
+        th = sign * np.interp(0.5 * gap + (width - u), arc, t)
+        X[sl] = np.stack([a * np.cos(th), top - v, b * np.sin(th)], 1)
     for side in ("left", "right"):
-        a = boundary_chain(g.uv[off[0]:off[1]], side, (0.0, height)) + off[0]
-        b = boundary_chain(g.uv[off[1]:off[2]], side, (0.0, height)) + off[1]
-        add_seam(g, a, b, seam_stiffness)
-    for lo, hi in ((0.0, 0.12), (width - 0.12, width)):
-        a = boundary_chain(g.uv[off[0]:off[1]], "top", (lo, hi)) + off[0]
-        b = boundary_chain(g.uv[off[1]:off[2]], "top", (lo, hi)) + off[1]
-        add_seam(g, a, b, seam_stiffness)
-    body = mannequin()
+        ca = boundary_chain(g.uv[off[0]:off[1]], side, (0.0, height)) + off[0]
+        cb = boundary_chain(g.uv[off[1]:off[2]], side, (0.0, height)) + off[1]
+        add_seam(g, ca, cb, seam_stiffness)
+    for lo_, hi_ in ((0.0, 0.12), (width - 0.12, width)):
+        ca = boundary_chain(g.uv[off[0]:off[1]], "top", (lo_, hi_)) + off[0]
+        cb = boundary_chain(g.uv[off[1]:off[2]], "top", (lo_, hi_)) + off[1]
+        add_seam(g, ca, cb, seam_stiffness)
+    body = mannequin(arm_x=a + clearance + arm_r)
     st = State(X, np.zeros_like(X), 0.0)
     settings = default_settings(dt=1.0 / 60.0, iterations=iterations, radius=radius, stiffness=1e5)
     return Scene(f"sewn_garment_{g.n}v", g, st, body, settings, 240,
-                 {"seed": seed, "spacing": spacing, "stitches": len(g.stitch_stiffness)})
+                 {"seed": seed, "spacing": spacing, "stitches": len(g.stitch_stiffness),
+                  "clearance_m": clearance, "ellipse_semi_axes_m": (a, b)})
 
 
 def grid_panel(width: float, height: float, nx: int, ny: int) -> tuple:

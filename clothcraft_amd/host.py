@@ -91,6 +91,30 @@ class HostSolver:
         lib().cvh_solver_terms(self.h, abi.ptr(off, C.c_uint32), recs.ctypes.data, k)
         return off, recs
 
+    def check_probe(self, x, targets, probe, settings, grab=None, origin=(0.0, 0.0, 0.0), coff=None,
+                    crec=None):
+        """The step's runtime gradient check on the host (check_probe):
+        positions x [n_all,3] and targets [n,4] relative to origin, probe
+        [n,4] (g, E).  Returns (status, message, max_error)."""
+        L = lib()
+        L.cvh_check_probe.argtypes = [C.c_void_p, abi.f64p, abi.f64p, abi.u32p, C.c_void_p, abi.f64p,
+                                      C.POINTER(abi.cv_settings), C.POINTER(abi.cv_grab), abi.f64p,
+                                      C.POINTER(C.c_double), C.c_char_p, C.c_size_t]
+        from .vbd import _grab as grab_c, _settings as settings_c
+        xx = np.ascontiguousarray(x, np.float64)
+        tt = np.ascontiguousarray(targets, np.float64)
+        pp = np.ascontiguousarray(probe, np.float64)
+        oo = np.ascontiguousarray(origin, np.float64)
+        co = np.zeros(self.n + 1, np.uint32) if coff is None else np.ascontiguousarray(coff, np.uint32)
+        cr = np.zeros(1, TERM64_DTYPE) if crec is None else crec
+        mx = C.c_double(0.0)
+        err = C.create_string_buffer(1024)
+        g = grab_c(grab)
+        rc = L.cvh_check_probe(self.h, _p(xx), _p(tt), abi.ptr(co, C.c_uint32), cr.ctypes.data, _p(oo),
+                               C.byref(settings_c(settings)), C.byref(g) if g else None, _p(pp),
+                               C.byref(mx), err, 1024)
+        return rc, err.value.decode(), mx.value
+
     def terms64(self):
         L = lib()
         L.cvh_solver_terms64.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]

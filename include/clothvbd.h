@@ -53,7 +53,8 @@ typedef enum cv_status {
   CV_ERR_INVALID_SEAM = 22,        /* stitch weights invalid                                */
   CV_ERR_NOT_BUILT = 23,           /* a feature (e.g. warm-start network) is not set up     */
   CV_ERR_INVALID_MATERIAL = 24,    /* Material::validate (src/model.rs:40-63)               */
-  CV_ERR_LINE_SEARCH = 25          /* "Newton line search failed; ..." (src/solver.rs:547)   */
+  CV_ERR_LINE_SEARCH = 25,         /* "Newton line search failed; ..." (src/solver.rs:547)   */
+  CV_ERR_GRADIENT_CHECK = 26       /* "GPU gradient/energy mismatch at i" (gpu_vbd.rs:361-372) */
 } cv_status;
 
 /* ------------------------------------------------------------- inputs */
@@ -207,9 +208,16 @@ cv_status cv_solver_colors(const cv_solver* solver, uint32_t* color_of_vertex, u
 cv_status cv_gpu_create(int device, cv_gpu** out);
 void cv_gpu_destroy(cv_gpu* gpu);
 
-/* GpuVbd.check_gradients — src/gpu_vbd.rs:22.  When set, the next step
- * verifies the device probe gradients against the f64 forward-mode
- * derivatives at the same f32-rounded geometry (src/gpu_vbd.rs:305-379). */
+/* GpuVbd.check_gradients — src/gpu_vbd.rs:22.  While set, every cv_gpu_step
+ * and every step of cv_gpu_run checks, after the detection and the truncated
+ * guess, the device probe of each vertex's local objective (gradient and
+ * energy) against the same objective evaluated in f64 on the host at the
+ * identical rounded positions and targets (src/gpu_vbd.rs:305-379): a
+ * gradient error above 3 mN + 0.3 % or an energy error above 1e-5 J + 0.1 %
+ * returns CV_ERR_GRADIENT_CHECK ("GPU gradient mismatch at i: ..." / "GPU
+ * energy mismatch at i") with the state unchanged; a passing check prints the
+ * reference's "GPU/CPU local gradients verified" line to stderr.  The batched
+ * step does not check. */
 cv_status cv_gpu_set_check_gradients(cv_gpu* gpu, int enabled);
 
 /* GpuVbd::step — src/gpu_vbd.rs:125-472.  `grab` and `metrics` may be NULL. */

@@ -31,6 +31,43 @@ def _one_step(gpu, sc, settings=None, bits=64):
     return s_gpu, s_ref, m_gpu, m_ref
 
 
+def _f32_step_within_reference_conditioning(gpu, sc, st, a, ref):
+    """The f32 device step from state st against the oracle step a (ref is
+    the oracle solver that produced a).  Positions: within 1e-5 of the box
+    or within twice the oracle's own deviation when its input is perturbed by
+    one f32 ulp of a position (3e-8 m) — the reference step's conditioning on
+    stiff draped cloth exceeds the bar (test_product_host.py demonstrates it on
+    the CPU).  Objective of the step (inertia, elastic and the oracle step's
+    final contacts frozen): within 1e-5 relative of the oracle step's."""
+    from clothcraft_amd import VbdSolver
+    s = sc.settings
+    p = st.copy()
+    rng = np.random.default_rng(1)
+    d = rng.uniform(-3e-8, 3e-8, p.x.shape)
+    d[sc.garment.pinned.astype(bool)] = 0.0
+    p.x = np.ascontiguousarray(p.x + d)
+    O.Solver(sc.garment, sc.avatar, p, threads=8).step(p, s)
+    b = st.copy()
+    gpu.set_precision(32)
+    try:
+        mg = gpu.step(VbdSolver(sc.garment, sc.avatar, b), b, s)
+    finally:
+        gpu.set_precision(64)
+    diag = _bbox_diag(a.x)
+    sens = np.abs(p.x - a.x).max() / diag
+    err = np.abs(b.x - a.x).max() / diag
+    ns = O.newton_settings(dt=s["dt"], gravity=s["gravity"], damping=s["damping"])
+    fz = O.frozen_c(ref.last_contacts(), s["radius"], s["stiffness"], sc.avatar.vertices)
+    ea = O.newton_objective(sc.garment, st, a.x, ns, None, fz)
+    eb = O.newton_objective(sc.garment, st, b.x, ns, None, fz)
+    rel = abs(eb - ea) / abs(ea)
+    print(f"{sc.name} f32: rel err {err:.3e} (oracle under a 1-ulp input perturbation {sens:.3e}), "
+          f"objective {eb:.9e} vs {ea:.9e} (rel {rel:.2e}), contacts {mg['contact_terms']}")
+    assert np.isfinite(b.x).all() and mg["minimum_distance_m"] > 0.0
+    assert err <= max(1e-5, 2.0 * sens)
+    assert rel <= 1e-5
+
+
 def test_free_fall_matches_damped_implicit_euler(gpu):
     # tests/gpu_motion.rs:140-190: mean displacement within 1e-5 of
     # g dt^2 / (1 + c dt), per vertex within 2e-5.  Large motion is limited by
@@ -71,6 +108,8 @@ def test_one_step_parity_hanging_sheet(gpu, bits, tol):
     assert mg["colors"] == mr["colors"]
     assert mg["detection_updates"] == mr["detection_updates"]
     assert err <= tol
+    if bits == 64:  # the lagging sweep residual (src/vbd.rs:396-397), summed in f64
+        assert abs(mg["residual_rms_n"] - mr["residual_rms_n"]) <= 1e-9 * mr["residual_rms_n"]
     # Extension in percent points: the f32 step's position error (a few 1e-6
     # of the diagonal, i.e. ~1e-5 of an edge) moves it by ~1e-4 points.
     dtol = 1e-6 if bits == 64 else 1e-3
@@ -86,6 +125,7 @@ def test_one_step_parity_sheet_on_sphere_with_contacts(gpu):
           f"updates {mg['detection_updates']}/{mr['detection_updates']}")
     assert mg["contact_terms"] == mr["contact_terms"] > 0
     assert err <= 1e-5
+    assert abs(mg["residual_rms_n"] - mr["residual_rms_n"]) <= 1e-9 * mr["residual_rms_n"]
 
 
 def test_full_size_sheet_on_sphere_in_contact_matches_the_oracle(gpu):
@@ -113,6 +153,7 @@ def test_full_size_sheet_on_sphere_in_contact_matches_the_oracle(gpu):
     assert mg["contact_terms"] == mr["contact_terms"] > 0
     assert mg["detection_updates"] == mr["detection_updates"]
     assert err <= 1e-5
+    _f32_step_within_reference_conditioning(gpu, sc, st, a, ref)
     # both colourings of a device detection at this state equal the oracle's
     from clothcraft_amd import OgcSettings
     rq = 0.006
@@ -419,6 +460,7 @@ def test_full_size_sewn_garment_step_matches_the_oracle(gpu):
     assert abs(mg["max_seam_gap_m"] - mr["max_seam_gap_m"]) <= 1e-12 + 1e-9 * mr["max_seam_gap_m"]
     for key in ("extension_p95_percent", "extension_max_percent"):
         assert abs(mg[key] - mr[key]) <= 1e-9 + 1e-9 * abs(mr[key]), key
+    _f32_step_within_reference_conditioning(gpu, sc, st, a, ref)
 
 
 def test_one_step_parity_sewn_garment_with_seams_and_body(gpu):
@@ -479,3 +521,36 @@ def test_fast_colouring_step_matches_the_oracle_in_the_same_mode(gpu):
         print(f"{sc.name} fast colouring: rel err {err:.3e} colours {mg['colors']} (vertex order {mplain['colors']})")
         assert mg["colors"] == mr["colors"] <= mplain["colors"]
         assert err <= tol
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_check_gradients_on_a_contact_step(gpu, bits, capfd):
+    """GpuVbd.check_gradients (src/gpu_vbd.rs:305-379): with the flag on, a
+    contact step with a grab checks every vertex's device probe against the
+    host's f64 evaluation at the same rounded geometry and prints the
+    reference's verification line.  The f64 step still gives the oracle's
+    step; the f32 step (whose trajectory inside the step is not the oracle's
+    at this conditioning) completes with the OGC invariants kept."""
+    sc = scenes.small_sheet_on_sphere(24, 0.2535)
+    grab = dict(vertices=(300, 301, 325), weights=(0.2, 0.3, 0.5),
+                target=tuple(sc.state.x[301] + [0.0, 0.01, 0.0]), stiffness=20.0)
+    gpu.check_gradients = True
+    try:
+        from clothcraft_amd import VbdSolver
+        a, b = sc.state.copy(), sc.state.copy()
+        mr = O.Solver(sc.garment, sc.avatar, a, threads=8).step(a, sc.settings, grab)
+        gpu.set_precision(bits)
+        mg = gpu.step(VbdSolver(sc.garment, sc.avatar, b), b, sc.settings, grab)
+    finally:
+        gpu.set_precision(64)
+        gpu.check_gradients = False
+    err = capfd.readouterr().err
+    assert f"GPU/CPU local gradients verified: {sc.garment.n} vertices" in err
+    rel = np.abs(a.x - b.x).max() / _bbox_diag(a.x)
+    print(f"check_gradients f{bits}: {err.strip()}; rel err {rel:.3e}, contacts {mg['contact_terms']}")
+    assert float(err.split("maximum absolute error ")[1].split()[0]) < (1e-9 if bits == 64 else 1e-3)
+    if bits == 64:
+        assert mg["contact_terms"] == mr["contact_terms"] > 0
+        assert rel <= 1e-9
+    else:
+        assert mg["contact_terms"] > 0 and mg["minimum_distance_m"] > 0.0 and np.isfinite(b.x).all()
