@@ -352,6 +352,9 @@ cv_status guard(std::string* err, F&& f) {
 
 // Warm-start network (mlp_device.hip).
 void mlp_ensure(cv_gpu& g);
+// Takes the device address of cv_warm_infer_noalloc's staging array on `device`
+// (current), once per device.
+void warm_scratch_setup(int device);
 void mlp_set_params(cv_gpu& g, const float* params);
 void mlp_get_params(cv_gpu& g, float* params);
 void mlp_infer_device(cv_gpu& g, const float* dX, uint32_t n, float* dY);

@@ -18,6 +18,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "gpu.hpp"
 #include "mlp.hpp"
@@ -517,4 +518,54 @@ void mlp_gather(cv_gpu& g, const float* src, const uint32_t* perm, uint32_t n, i
   if (n == 0) return;
   k_gather_rows<<<nblocks((uint64_t)n * width, 256), 256, 0, g.stream>>>(src, perm, n, width, dst);
   CV_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------ allocation-free inference
+// Staging of cv_warm_infer_noalloc: the features [12 r] and predictions
+// [3 r] of up to CV_WARM_NOALLOC_MAX_ROWS rows, in one file-scope device
+// array that every GPU the library runs on holds (15 MiB) until the process
+// ends.  In host code `scratch` is a host-side stand-in, never a device
+// address: host code uses only the per-device address warm_scratch_setup
+// takes with hipGetSymbolAddress.
+constexpr size_t kNoAllocRows = CV_WARM_NOALLOC_MAX_ROWS;
+static __device__ __align__(256) float scratch[(size_t)(12 + 3) * kNoAllocRows];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+float* scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+void warm_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) fail(CV_ERR_DEVICE, "device index beyond the library's table of 64");
+  std::lock_guard<std::mutex> l(scratch_mu);
+  if (scratch_ptr[device]) return;
+  void* p = nullptr;
+  CV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  scratch_ptr[device] = static_cast<float*>(p);
+}
+
+static float* warm_scratch(int device) {
+  std::lock_guard<std::mutex> l(scratch_mu);
+  return device >= 0 && device < kMaxDevices ? scratch_ptr[device] : nullptr;
+}
+
+extern "C" cv_status cv_warm_infer_noalloc(cv_gpu* g, const float* features, uint64_t n, float* out) {
+  if (!g || (n && (!features || !out))) return CV_ERR_INVALID_ARGUMENT;
+  return guard(&g->last_error, [&] {
+    if (n > kNoAllocRows)
+      fail(CV_ERR_CAPACITY, "cv_warm_infer_noalloc: " + std::to_string(n) + " rows exceed the limit of " +
+                                std::to_string(kNoAllocRows) + " (use cv_warm_infer)");
+    CV_HIP(hipSetDevice(g->device));
+    if (!g->mlp.ready) fail(CV_ERR_NOT_BUILT, "warm-start network not set up");
+    if (!n) return;
+    float* X = warm_scratch(g->device);
+    if (!X) fail(CV_ERR_DEVICE, "staging array of the device not set up");
+    float* Y = X + 12 * kNoAllocRows;
+    // Every call writes the rows it reads: nothing needs clearing.
+    CV_HIP(hipMemcpyAsync(X, features, 12 * n * sizeof(float), hipMemcpyHostToDevice, g->stream));
+    mlp_infer_device(*g, X, (uint32_t)n, Y);
+    CV_HIP(hipMemcpyAsync(out, Y, 3 * n * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    CV_HIP(hipStreamSynchronize(g->stream));
+  });
 }

@@ -1641,6 +1641,7 @@ cv_status cv_gpu_create(int device, cv_gpu** out) {
     if (device < 0 || device >= count) fail(CV_ERR_DEVICE, "no such HIP device");
     g->device = device;
     CV_HIP(hipSetDevice(device));
+    warm_scratch_setup(device);
     CV_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     CV_HIP(hipMalloc((void**)&g->d_scal, sizeof(Scalars)));
     CV_HIP(hipHostMalloc((void**)&g->h_scal, sizeof(Scalars)));

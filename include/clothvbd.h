@@ -342,6 +342,17 @@ cv_status cv_warm_gradient(cv_gpu* gpu, const float* features, const float* targ
  * epoch's mean squared error. */
 cv_status cv_warm_train(cv_gpu* gpu, const float* features, const float* targets, uint64_t n,
                         uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float* epoch_loss);
+/* Allocation-free variant of cv_warm_infer; it has no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the features and predictions are staged in a static device array of
+ * CV_WARM_NOALLOC_MAX_ROWS rows (15 MiB, one per GPU, held until the process
+ * ends) instead of buffers allocated per call, and no call allocates or frees
+ * device memory.  Same results as cv_warm_infer; n above the limit returns
+ * CV_ERR_CAPACITY.  The array is shared by every call on a GPU: calls that
+ * overlap on one GPU (any handles, any threads) race. */
+#define CV_WARM_NOALLOC_MAX_ROWS 262144
+cv_status cv_warm_infer_noalloc(cv_gpu* gpu, const float* features, uint64_t n, float* out);
 /* The features [12n] the step would feed the network at `state`. */
 cv_status cv_warm_features(cv_gpu* gpu, cv_solver* solver, const cv_state* state,
                            const cv_settings* settings, float* features);
