@@ -1,34 +1,56 @@
 #!/usr/bin/env python3
-"""Throughput of the device VBD step on the sheet-on-sphere workload.
+"""Throughput of the device VBD step on the sewn-garment workload at a fixed residual.
 
-Workload: a 1 m x 1 m sheet of 128 x 128 quads (16,641 vertices, 32,768
-triangles, 32,258 hinges) dropped from 0.3 m onto a static icosphere collider
-(radius 0.25 m, subdivision 4, 5,120 triangles); StVK mu 2e4 N/m scaled per
-triangle by a seeded U[0.8, 1.2], lambda 8e3 N/m, bending 2e-5 N m, 0.2 kg/m^2;
-OGC contact radius 2 mm, stiffness 1e5 N/m, self-contact on; 20 VBD iterations
-per step, dt 1/60 s.  Synthetic, seeded geometry (no external data).
+Workload (default): two 0.6 m x 0.7 m panels of seeded Poisson-disk meshes
+(99,287 vertices, 196,402 triangles) sewn by side and shoulder seams (1,368
+penalty stitches, 1e4 N/m), wrapped on half-ellipses 1 cm outside a static
+capsule-union mannequin (torso r 0.15 m and two arms, 19,488 triangles);
+StVK mu 2e4 / lambda 8e3 N/m, bending 2e-5 N m, 0.2 kg/m^2; OGC contact
+radius 1 mm (below the 1.2 mm rest distance of non-adjacent primitives of the
+mesh), stiffness 1e5 N/m, self- and body contact on; dt 1/60 s.  Synthetic,
+seeded geometry (no external data).
 
-One step is the complete VbdSolver::step of the reference (src/vbd.rs:194-436):
-prediction, OGC detection and colouring, 20 coloured Gauss-Seidel sweeps with
-the ball-truncated line search, mid-step re-detection, commit.  `--warmup`
-steps run untimed on a throw-away copy of the start state (buffers, code
-objects); the timed region then runs the workload's `--steps` (300) steps of
-the drop from the start state, device-resident.  The OGC trust regions
-(0.45 x 5.5 mm self-distance) hold the fall to about 2.5 mm per step, so the
-sheet reaches the sphere near step 120 and the rest of the drop is in
-contact.  `value` is steps/s from HIP events on the solver's stream around
-the device work, state already in HBM; the wall-clock rate including the one
-upload/download of the state is reported beside it.
+One step is the complete VbdSolver::step of the reference (src/vbd.rs:250-422):
+prediction (the adaptive guess, src/vbd/prediction.rs:15-41), OGC detection
+and colouring, coloured Gauss-Seidel sweeps with the ball-truncated line
+search, mid-step re-detection, commit.  Iterations run to a fixed true
+residual: after every sweep the post-sweep residual sqrt(sum |g|^2 / 3 n) of
+the step's objective (src/solver.rs:229-231 form) is computed, and the step
+ends once it is at most `--tol` (0.02 N, about what the configuration's 40
+iterations reach on this garment in contact) or after the configuration's
+40 iterations, whichever comes first.  (Beyond that the residual stops
+falling: vertices drift onto their trust-region boundaries and it rises
+again, so a larger cap only degrades the trajectory.)
+
+Phases: `--settle` (30) untimed steps of the configuration's own 40-iteration
+solve bring the garment from its start onto the body (the panels start 1 cm
+out; contacts appear within ~10 steps), then `--warmup` untimed steps and the
+`--steps` timed steps continue the trajectory at the fixed residual, so the
+timed window is in contact whatever `--steps` is.  `value` is steps/s from
+HIP events on the solver's stream around the device work, state resident in
+HBM; the rate including the state's upload/download is reported beside it.
+
+Warm start vs plain: after the timed window, from `--compare` successive
+states of the same trajectory, the iterations each start needs to reach the
+tolerance (at most `--compare-cap`, 100): the plain start is the reference's
+adaptive guess with its prediction history, the warm start that guess plus
+the committed network's correction (CV_FLAG_WARM_START;
+clothcraft_amd/warm_params.npy, trained by clothcraft_amd/train_warm.py).
 
 roofline: the sweep kernel (k_sweep), algorithmic bytes per launch at the f32
 minimum record sizes (64 B per free vertex + 72 per triangle, 52 per hinge,
-68 per stitch, 56 per contact incidence), divided by its mean launch time from
-HIP events recorded around every launch on the solver's stream.
+68 per stitch, 56 per contact incidence), divided by its mean launch duration
+from HIP events recorded around the sampled launches on the solver's stream;
+`traffic` is the measured HBM bytes per launch (rocprofv3 FETCH_SIZE +
+WRITE_SIZE, calibrated by a copy kernel) from the committed counter runs of
+this same command, null for other arguments.
 
 cpu_baseline: the f64 CPU restatement of VbdSolver::step (OpenMP within a
-colour) on the host, on a bounded sample of the same drop: steps from the
-start state (free fall) and from the end state (in contact), half the time
-budget each, combined in the drop's proportions of the two phases.
+colour) on the host, from the settled state with the same settings, on the
+process's CPU share (16 threads on a one-GPU box) and on one thread: steps
+capped at two iteration counts give each a per-step fixed cost and a cost per
+iteration (true residual included), evaluated at the GPU's mean iterations
+per step (a full step to the residual takes minutes on the CPU).
 """
 import argparse
 import json
@@ -40,55 +62,88 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
+WARM_PARAMS = os.path.join(ROOT, "clothcraft_amd", "warm_params.npy")
 
 
-def _traffic(kernel):
-    """HBM bytes per sweep launch from the committed rocprofv3 counter runs
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_sheet_on_sphere_128_traffic.json")
+def _traffic(kernel, workload, argv_key):
+    """HBM bytes per sweep launch from the committed rocprofv3 counter runs of
+    this same command (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
+    known size), or None when the arguments differ from the profiled ones."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
+            d = json.load(f)
     except (OSError, ValueError):
         return None
+    return d.get("bytes_per_launch") if d.get("args") == argv_key else None
 
 
-def cpu_time_per_step(sc, state, settings, budget_s, threads):
-    """Seconds per oracle f64 step on the host from `state` (bounded sample)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_share():
+    """Threads this process may use: its CPU affinity, capped by OMP_NUM_THREADS
+    (16 on a one-GPU box)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_step_seconds(sc, state, settings, threads, steps=1):
+    """Seconds per oracle f64 step on the host from `state`, and the mean
+    iterations those steps ran."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the CPU restatement: baseline only, never the product
     st = state.copy()
     ref = O.Solver(sc.garment, sc.avatar, st, threads=threads)
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        ref.step(st, settings)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 200:
-            break
-    return steps, el
+    its, t0 = 0, time.perf_counter()
+    for _ in range(steps):
+        m = ref.step(st, settings)
+        its += m["iterations_run"]
+    return (time.perf_counter() - t0) / steps, its / steps
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--settle", type=int, default=30,
+                    help="untimed steps of the configuration's own solve before the warm-up")
+    ap.add_argument("--tol", type=float, default=0.02, help="fixed true residual (N, RMS per coordinate)")
+    ap.add_argument("--max-iterations", type=int, default=0,
+                    help="iteration cap of a step (0: the configuration's own count, 40 for the garment)")
+    ap.add_argument("--compare-cap", type=int, default=100,
+                    help="iteration cap of the warm-vs-plain comparison steps")
+    ap.add_argument("--fixed-iterations", action="store_true",
+                    help="run the configuration's fixed iteration count instead of the fixed residual")
+    ap.add_argument("--compare", type=int, default=6, help="states of the warm-vs-plain comparison (0: off)")
     ap.add_argument("--precision", type=int, default=64, choices=(32, 64),
                     help="sweep arithmetic (64: f64 default; 32: f32 opt-in)")
     ap.add_argument("--lanes", type=int, default=32, choices=(8, 16, 32, 64),
                     help="lanes per vertex of the sweep kernel")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host CPUs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--nx", type=int, default=128, help="quads per side (128 is the workload)")
-    ap.add_argument("--workload", default="sheet_on_sphere",
-                    choices=("sheet_on_sphere", "sewn_garment", "batch64", "warm_infer", "newton_sheet"),
-                    help="sewn_garment: two ~50k-vertex Poisson-disk panels with side and shoulder "
-                         "seams on a capsule body, 40 iterations; batch64: 64 independent sheets and "
-                         "sewn tubes of 1k-50k vertices in batched steps; newton_sheet: the device "
-                         "Newton energy reference of one headline step in contact, one solve per "
-                         "--steps (not the headline line)")
+    ap.add_argument("--nx", type=int, default=128, help="sheet_on_sphere: quads per side")
+    ap.add_argument("--workload", default="sewn_garment",
+                    choices=("sewn_garment", "sheet_on_sphere", "batch64", "warm_infer", "newton_sheet"),
+                    help="sheet_on_sphere: the 128x128 sheet dropped on an icosphere (20 iterations, OGC "
+                         "r 2 mm); batch64: 64 independent sheets and sewn tubes of 1k-50k vertices in "
+                         "batched steps; warm_infer: the warm-start network on 4 Mi rows; newton_sheet: "
+                         "the device Newton energy reference of one sheet step in contact")
     ap.add_argument("--fast-coloring", action="store_true",
                     help="fast colouring on the device (fewer colours; not the reference's colouring order)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -111,87 +166,111 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the step has no multi-GPU path in this build)", file=sys.stderr)
         sys.exit(2)
-
-    from clothcraft_amd import GpuVbd, VbdSolver, scenes  # fails loudly if not built
-
     if args.workload == "batch64":
         return bench_batch(args)
     if args.workload == "warm_infer":
         return bench_warm_infer(args)
     if args.workload == "newton_sheet":
         return bench_newton(args)
+    return bench_step(args)
+
+
+def bench_step(args):
+    from clothcraft_amd import GpuVbd, VbdSolver, garments, scenes  # fails loudly if not built
+    from clothcraft_amd.abi import CV_FLAG_FAST_COLORING, CV_FLAG_TRUE_RESIDUAL, CV_FLAG_WARM_START
 
     if args.workload == "sewn_garment":
-        from clothcraft_amd import garments
         sc = garments.sewn_garment_on_body()
+        workload = (f"sewn garment on a body: {sc.garment.n} vertices, {len(sc.garment.tri_vertices)} "
+                    f"triangles, {len(sc.garment.stitch_stiffness)} seam stitches (1e4 N/m), panels 1 cm "
+                    f"outside a capsule-union body ({len(sc.avatar.triangles)} triangles), OGC r 1 mm k 1e5, "
+                    f"dt 1/60, self- and body contact on")
     else:
         sc = scenes.sheet_on_sphere(nx=args.nx)
-    settings = sc.settings
+        workload = (f"sheet on sphere {args.nx}x{args.nx} quads: {sc.garment.n} vertices, "
+                    f"{len(sc.garment.tri_vertices)} triangles, icosphere r 0.25 m "
+                    f"({len(sc.avatar.triangles)} triangles), OGC r 2 mm k 1e5, dt 1/60, self-contact on")
+    base = dict(sc.settings)
     if args.fast_coloring:
-        from clothcraft_amd.abi import CV_FLAG_FAST_COLORING
-        settings = dict(settings, flags=settings.get("flags", 0) | CV_FLAG_FAST_COLORING)
-    st = sc.state.copy()
+        base["flags"] = base.get("flags", 0) | CV_FLAG_FAST_COLORING
+    if args.fixed_iterations:
+        run_s = dict(base, flags=base.get("flags", 0) | CV_FLAG_TRUE_RESIDUAL)
+    else:
+        cap = args.max_iterations or base["iterations"]
+        run_s = dict(base, iterations=cap, residual_tolerance=args.tol,
+                     flags=base.get("flags", 0) | CV_FLAG_TRUE_RESIDUAL)
     gpu = GpuVbd(0)
     gpu.set_precision(args.precision)
     gpu.set_lanes(args.lanes)
+    st = sc.state.copy()
     sol = VbdSolver(sc.garment, sc.avatar, st)
-    print(f"[bench] {sc.name}: {sc.garment.n} vertices, warmup {args.warmup} steps", file=sys.stderr,
-          flush=True)
+    n_free = int(np.count_nonzero(sc.garment.pinned == 0))
+    log(f"{sc.name}: {sc.garment.n} vertices; settle {args.settle} steps at "
+        f"{base['iterations']} iterations, warm-up {args.warmup} steps")
+    if args.settle:
+        m = gpu.run(sol, st, base, args.settle)
+        log(f"settled: {m['contact_terms']} contact terms, minimum distance {m['minimum_distance_m']:.2e} m")
     if args.warmup:
-        gpu.run(sol, st.copy(), settings, args.warmup)
-    start_state = st.copy()
+        gpu.run(sol, st, run_s, args.warmup)
+    settled = st.copy()
 
-    print(f"[bench] timed: {args.steps} steps", file=sys.stderr, flush=True)
-    # The timed steps run as device-resident chunks of 10 (run() keeps the
-    # prediction history across calls, so results equal one long run); the
-    # chunk ends show when the drop reaches the sphere.
+    log(f"timed: {args.steps} steps")
     gpu.set_profiling(True)
     tot = dict(gpu_compute_ms=0.0, iterations_run=0, sweep_launches=0, sweep_ms=0.0, sweep_bytes=0.0,
                sweep_bytes_min=0.0, detection_updates=0, sweep_timed_launches=0, sweep_timed_bytes_min=0.0)
-    first_contact, done, m = None, 0, None
+    done, m, contacts, reached = 0, None, [], 0
     t0 = time.perf_counter()
     while done < args.steps:
-        k = min(10, args.steps - done)
-        m = gpu.run(sol, st, settings, k)
+        # Chunks of device-resident steps (run() keeps the prediction history
+        # across calls, so results equal one long run).
+        k = min(5, args.steps - done)
+        m = gpu.run(sol, st, run_s, k)
         for key in tot:
             tot[key] += m[key]
+        contacts.append(m["contact_terms"])
+        reached += int(m["true_residual_rms_n"] <= args.tol)
         done += k
-        if first_contact is None and m["contact_terms"] > 0:
-            first_contact = done - k
-        if done % 50 == 0:
-            print(f"[bench] {done}/{args.steps} steps", file=sys.stderr, flush=True)
+        log(f"{done}/{args.steps} steps, last step {m['iterations_run']} iterations" if k == 1 else
+            f"{done}/{args.steps} steps")
     wall_s = time.perf_counter() - t0
     gpu.set_profiling(False)
     if not np.all(np.isfinite(st.x)):
         raise RuntimeError("non-finite state after the timed steps")
-    # The reference's call pattern: one GpuVbd::step per frame, state in and
-    # out through host memory and the metrics (diagnostics included) each
-    # step; wall time per call, continuing from the final state on a copy.
-    api_st = st.copy()
-    api_sol = VbdSolver(sc.garment, sc.avatar, api_st)
-    api_ms = []
-    for _ in range(20):
-        t1 = time.perf_counter()
-        gpu.step(api_sol, api_st, settings)
-        api_ms.append(1e3 * (time.perf_counter() - t1))
-    api_ms = np.array(api_ms[2:])
+    last = m
     m = dict(m, **tot)
-
     K = args.steps
     dev_s = m["gpu_compute_ms"] / 1e3
-    n_free = int(np.count_nonzero(sc.garment.pinned == 0))
-    vi = n_free * m["iterations_run"]  # vertex-iterations in the timed window
-    # The sweep launches of every 4th iteration carry begin/end events: their
-    # mean duration and algorithmic bytes give the achieved bandwidth; the
-    # sweep's share of the step extrapolates to all launches.
+    vi = n_free * m["iterations_run"]
     launches = max(m["sweep_launches"], 1)
     timed = max(m["sweep_timed_launches"], 1)
     avg_launch_s = m["sweep_ms"] / 1e3 / timed
     bytes_per_launch = m["sweep_timed_bytes_min"] / timed
     achieved = bytes_per_launch / avg_launch_s / 1e9
     sweep_s = avg_launch_s * launches
-    kernel = "k_sweep"
 
+    # The reference's call pattern: one GpuVbd::step per frame, state in and
+    # out through host memory and the metrics each step (wall time per call).
+    api_st = st.copy()
+    api_sol = VbdSolver(sc.garment, sc.avatar, api_st)
+    api_ms = []
+    for _ in range(12):
+        t1 = time.perf_counter()
+        gpu.step(api_sol, api_st, run_s)
+        api_ms.append(1e3 * (time.perf_counter() - t1))
+    api_ms = np.array(api_ms[2:])
+
+    compare = None
+    if args.compare and not args.fixed_iterations:
+        # One more step of the trajectory's own solver gives a state with its
+        # previous velocity (the adaptive guess's history).
+        pv = st.v.copy()
+        gpu.step(sol, st, run_s)
+        compare = compare_starts(gpu, sc, st, pv, dict(run_s, iterations=args.compare_cap), args.compare)
+
+    kernel = "k_sweep"
+    argv_key = {"workload": args.workload, "steps": K, "warmup": args.warmup, "settle": args.settle,
+                "tol": args.tol, "precision": args.precision, "lanes": args.lanes,
+                "fixed_iterations": args.fixed_iterations, "fast_coloring": args.fast_coloring}
     out = {
         "metric": "steps/s + vertex-iters/s at fixed residual; warm-start vs plain iters-to-tol",
         "value": K / dev_s,
@@ -204,36 +283,34 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == 64 else "f32",
-        "data": "synthetic (seeded sheet and icosphere geometry)",
+        "data": "synthetic (seeded Poisson-disk panels, capsule body)" if args.workload == "sewn_garment"
+                else "synthetic (seeded sheet and icosphere geometry)",
         "config": {
-            "workload": (f"sheet on sphere {args.nx}x{args.nx} quads: {sc.garment.n} vertices, "
-                         f"{len(sc.garment.tri_vertices)} triangles, icosphere r 0.25 m "
-                         f"({len(sc.avatar.triangles)} triangles), OGC r 2 mm k 1e5, "
-                         f"{settings['iterations']} iterations, dt 1/60, self-contact on")
-            if args.workload == "sheet_on_sphere" else
-            (f"sewn garment: {sc.garment.n} vertices, {len(sc.garment.tri_vertices)} triangles, "
-             f"{len(sc.garment.stitch_stiffness)} seam stitches, capsule body "
-             f"({len(sc.avatar.triangles)} triangles), OGC r 1 mm k 1e5, "
-             f"{settings['iterations']} iterations, dt 1/60, self-contact on"),
-            "steps_timed": f"1..{K} of the drop (warmup on a discarded copy)",
-            "iteration_mode": "fixed iterations (residual tolerance 0)",
+            "workload": workload,
+            "steps_timed": f"{args.settle + args.warmup + 1}..{args.settle + args.warmup + K} of the trajectory "
+                           f"(in contact: {min(contacts)}-{max(contacts)} contact terms at the chunk ends)",
+            "iteration_mode": ("fixed iterations" if args.fixed_iterations else
+                               f"to true residual <= {args.tol} N or the configuration's "
+                               f"{run_s['iterations']} iterations, whichever comes first"),
             "coloring": ("fast: static smallest-last, device repair (CV_FLAG_FAST_COLORING)"
                          if args.fast_coloring else "reference vertex-order greedy"),
             "lanes_per_vertex": args.lanes,
         },
+        "iterations_per_step": m["iterations_run"] / K,
+        "chunk_end_steps_at_the_residual": f"{reached} of {len(contacts)}",
         "vertex_iters_per_s": vi / dev_s,
         "vertex_iters_per_s_sweep_only": vi / sweep_s if sweep_s > 0 else None,
         "steps_per_s_wall_incl_transfers": K / wall_s,
         "step_api_wall_ms": {"median": float(np.median(api_ms)), "p95": float(np.percentile(api_ms, 95)),
                              "calls": int(len(api_ms)), "warmup_calls": 2},
-        "residual_rms_n_last_step": m["residual_rms_n"],
-        "contacts_last_step": m["contact_terms"],
-        "colors_last_step": m["colors"],
-        "candidate_pairs_last_step": m["candidate_pairs"],
-        "grid_pairs_examined_last_step": m["candidate_pairs"] + m["aabb_rejections"],
+        "true_residual_rms_n_last_step": last["true_residual_rms_n"],
+        "residual_rms_n_last_step": last["residual_rms_n"],
+        "contacts_last_step": last["contact_terms"],
+        "colors_last_step": last["colors"],
+        "candidate_pairs_last_step": last["candidate_pairs"],
         "detections": m["detection_updates"],
-        "first_contact_by_step": None if first_contact is None else first_contact + 10,
         "sweep_share_of_step": sweep_s / dev_s,
+        "warm_vs_plain": compare,
         "roofline": {
             "kernel": kernel,
             "bound": "hbm",
@@ -241,9 +318,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": _traffic(kernel) if (args.workload == "sheet_on_sphere" and args.nx == 128
-                                             and args.precision == 64
-                                             and args.lanes == 32) else None,
+            "traffic": _traffic(kernel, args.workload, argv_key),
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_us": 1e6 * avg_launch_s,
             "launches": m["sweep_launches"],
@@ -253,27 +328,81 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        print(f"[bench] cpu baseline: oracle f64 step, {threads} threads, ~{args.cpu_seconds:.0f} s",
-              file=sys.stderr, flush=True)
-        half = args.cpu_seconds / 2
-        fs, fe = cpu_time_per_step(sc, start_state, settings, half, threads)
-        cs, ce = cpu_time_per_step(sc, st, settings, half, threads)
-        # The drop's two phases: free fall until the first contact, then contact.
-        k_fall = K if first_contact is None else first_contact
-        per_step = (k_fall * fe / fs + (K - k_fall) * ce / cs) / K
-        out["cpu_baseline"] = {
-            "value": 1.0 / per_step,
-            "unit": "steps/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"f64 C++ restatement of VbdSolver::step (OpenMP within a colour): {fs} steps "
-                      f"from the start state ({fe:.1f} s) and {cs} steps from the state after step {K} "
-                      f"({ce:.1f} s), weighted {k_fall}:{K - k_fall} as the drop's free-fall and "
-                      f"contact steps",
-            "vertex_iters_per_s": n_free * settings["iterations"] / per_step,
-        }
+        out["cpu_baseline"] = cpu_baseline(sc, settled, run_s, n_free, m["iterations_run"] / K)
     print(json.dumps(out), flush=True)
+
+
+def compare_starts(gpu, sc, st, prev_v, run_s, count):
+    """From `count` successive states of the plain trajectory: iterations to
+    the fixed residual with the plain adaptive start (prediction history
+    installed) and with the learned start, and their device times."""
+    from clothcraft_amd import VbdSolver
+    from clothcraft_amd.abi import CV_FLAG_WARM_START
+    if not os.path.exists(WARM_PARAMS):
+        log("no trained warm-start network; comparison skipped")
+        return None
+    gpu.warm.set_params(np.load(WARM_PARAMS))
+    gpu.warm.set_smoothing(1)  # the correction averaged over each one-ring once
+    warm_s = dict(run_s, flags=run_s["flags"] | CV_FLAG_WARM_START)
+    rows = []
+    st, pv = st.copy(), prev_v.copy()
+    for k in range(count):
+        plain = st.copy()
+        ps = VbdSolver(sc.garment, sc.avatar, plain)
+        ps.record_prediction(pv, plain, run_s)
+        mp = gpu.step(ps, plain, run_s)
+        warm = st.copy()
+        mw = gpu.step(VbdSolver(sc.garment, sc.avatar, warm), warm, warm_s)
+        rows.append((mp["iterations_run"], mw["iterations_run"], mp["gpu_compute_ms"], mw["gpu_compute_ms"],
+                     mp["true_residual_rms_n"], mw["true_residual_rms_n"]))
+        log(f"compare {k + 1}/{count}: iterations plain {mp['iterations_run']} warm {mw['iterations_run']}")
+        pv, st = st.v.copy(), plain
+    r = np.array(rows, dtype=np.float64)
+    return {"states": count, "plain_iterations_mean": float(r[:, 0].mean()),
+            "warm_iterations_mean": float(r[:, 1].mean()),
+            "iterations_gain": float(r[:, 0].mean() / max(r[:, 1].mean(), 1e-9)),
+            "plain_ms_mean": float(r[:, 2].mean()), "warm_ms_mean": float(r[:, 3].mean()),
+            "plain_final_residual_mean": float(r[:, 4].mean()), "warm_final_residual_mean": float(r[:, 5].mean()),
+            "network": "clothcraft_amd/warm_params.npy (clothcraft_amd/train_warm.py), correction "
+                       "averaged over the one-ring once"}
+
+
+def cpu_fit(sc, state, run_s, threads, lo, hi):
+    """Seconds of one oracle step from `state` capped at lo and at hi
+    iterations (the true residual computed after every iteration, as in the
+    fixed-residual mode, with a tolerance it never reaches): the step's fixed
+    cost and its cost per iteration."""
+    cap = lambda k: dict(run_s, iterations=k, residual_tolerance=1e-30)
+    t_lo, _ = cpu_step_seconds(sc, state, cap(lo), threads)
+    t_hi, _ = cpu_step_seconds(sc, state, cap(hi), threads)
+    per_it = max((t_hi - t_lo) / (hi - lo), 1e-9)
+    return max(t_lo - lo * per_it, 0.0), per_it, t_lo, t_hi
+
+
+def cpu_baseline(sc, state, run_s, n_free, gpu_iters_per_step):
+    share = cpu_share()
+    log(f"cpu baseline: oracle f64 step, {share} threads and 1 thread")
+    f_all, it_all, a_lo, a_hi = cpu_fit(sc, state, run_s, share, 2, 4)
+    f_one, it_one, o_lo, o_hi = cpu_fit(sc, state, run_s, 1, 1, 2)
+    t_all = f_all + it_all * gpu_iters_per_step
+    t_one = f_one + it_one * gpu_iters_per_step
+    return {
+        "value": 1.0 / t_all,
+        "unit": "steps/s",
+        "cores": share,
+        "kind": "port",
+        "cpu_model": cpu_model(),
+        "sample": f"f64 C++ restatement of VbdSolver::step (OpenMP within a colour) from the settled state, "
+                  f"same settings, on {share} threads (the process's CPU share): steps capped at 2 and 4 "
+                  f"iterations took {a_lo:.2f} s and {a_hi:.2f} s, i.e. {f_all:.2f} s per step + {it_all:.2f} s "
+                  f"per iteration, evaluated at the GPU's {gpu_iters_per_step:.1f} iterations per step",
+        "vertex_iters_per_s": n_free / it_all,
+        "single_core": {"value": 1.0 / t_one, "unit": "steps/s", "cores": 1,
+                        "sample": f"one thread: steps capped at 1 and 2 iterations took {o_lo:.2f} s and "
+                                  f"{o_hi:.2f} s ({f_one:.2f} s per step + {it_one:.2f} s per iteration), "
+                                  f"evaluated at {gpu_iters_per_step:.1f} iterations per step",
+                        "vertex_iters_per_s": n_free / it_one},
+    }
 
 
 def bench_warm_infer(args):

@@ -283,6 +283,9 @@ struct cv_gpu {
   int device = 0;
   hipStream_t stream = nullptr;
   bool check_gradients = false;
+  int warm_smooth = 0;  // one-ring averaging passes of the warm-start correction
+  double warm_scale = 1.0;        // factor of the warm-start correction
+  bool warm_normal_only = false;  // keep only the correction's component along the vertex normal
   double last_check_max_error = 0;  // largest gradient error of the last check (N)
   DBuf<double> probe_out;           // [4n] probe of the gradient check
   DevSolver dev;

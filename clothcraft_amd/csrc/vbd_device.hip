@@ -420,22 +420,23 @@ __global__ void k_sum_f64(const double* a, uint32_t n, double* out) {
 
 // Warm-start features of each moving vertex (12 floats, dimensionless), with
 // L = |g| dt^2 / (1 + c dt), the free-fall displacement of one step:
-//   0-2  inertial-prediction offset (v dt + g dt^2) / (1 + c dt) / L
+//   0-2  offset of the step's own guess from x / L: the adaptive inertial
+//        prediction of Prediction::guess (src/vbd/prediction.rs:15-41),
+//        (v dt + f g dt^2) / (1 + c dt), with f = 1 without history
 //   3-5  previous velocity v dt / L
 //   6-7  mean Green strain of the incident triangles along warp and weft, x 100
 //   8-10 mean velocity of the one-ring neighbours, v dt / L
 //   11   distance to the nearest contact primitive / query radius (<= 1)
-__global__ void k_warm_features(const double* x, const double* v, const uint32_t* term_off,
+__global__ void k_warm_features(const double* x, const double* v, const double* guess, const uint32_t* term_off,
                                 const cvm::TermRec64* terms, const uint32_t* nb_off, const uint32_t* nb,
-                                const unsigned long long* dist_bits, uint32_t n, double dt, double damp, double gx,
-                                double gy, double gz, double L, double rq, float* feat) {
+                                const unsigned long long* dist_bits, uint32_t n, double dt, double L, double rq,
+                                float* feat) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]}, w[3] = {v[3 * i], v[3 * i + 1], v[3 * i + 2]};
-  double g[3] = {gx, gy, gz};
   float* f = feat + 12 * (size_t)i;
   for (int k = 0; k < 3; ++k) {
-    f[k] = (float)((w[k] * dt + g[k] * dt * dt) / damp / L);
+    f[k] = (float)((guess[3 * i + k] - p[k]) / L);
     f[3 + k] = (float)(w[k] * dt / L);
   }
   double ew = 0, ev = 0;
@@ -468,15 +469,51 @@ __global__ void k_warm_features(const double* x, const double* v, const uint32_t
   double d = __longlong_as_double((long long)dist_bits[i]);
   f[11] = (float)(fmin(d, rq) / rq);
 }
-// Learned guess for moving vertices: the inertial prediction
-// x + (v dt + g dt^2) / (1 + c dt) plus L times the network's correction.
-__global__ void k_warm_guess(const float* pred, const double* x, const double* v, const uint8_t* pinned, uint32_t n,
-                             double dt, double damp, double gx, double gy, double gz, double L, double* guess) {
+// One pass of one-ring averaging of the network's correction (moving
+// neighbours only): out_i = (in_i + sum_j in_j) / (1 + #j).
+__global__ void k_warm_smooth(const float* in, const uint32_t* nb_off, const uint32_t* nb, uint32_t n, float* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s[3] = {in[3 * i], in[3 * i + 1], in[3 * i + 2]};
+  int cnt = 1;
+  for (uint32_t k = nb_off[i]; k < nb_off[i + 1]; ++k) {
+    uint32_t j = nb[k];
+    if (j >= n) continue;
+    for (int c = 0; c < 3; ++c) s[c] += in[3 * j + c];
+    ++cnt;
+  }
+  for (int c = 0; c < 3; ++c) out[3 * i + c] = s[c] / (float)cnt;
+}
+// Learned guess for moving vertices: the step's own guess (the adaptive
+// inertial prediction) plus scale * L times the network's correction, or with
+// normal_only its component along the area-weighted vertex normal at x (the
+// sum of the incident triangles' (x1 - x) x (x2 - x)).
+__global__ void k_warm_guess(const float* pred, const uint8_t* pinned, const double* x, const uint32_t* term_off,
+                             const cvm::TermRec64* terms, uint32_t n, double L, int normal_only, double* guess) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || pinned[i]) return;
-  double g[3] = {gx, gy, gz};
-  for (int k = 0; k < 3; ++k)
-    guess[3 * i + k] = x[3 * i + k] + (v[3 * i + k] * dt + g[k] * dt * dt) / damp + L * (double)pred[3 * i + k];
+  double c[3] = {L * (double)pred[3 * i], L * (double)pred[3 * i + 1], L * (double)pred[3 * i + 2]};
+  if (normal_only) {
+    double nx = 0, ny = 0, nz = 0;
+    for (uint32_t k = term_off[i]; k < term_off[i + 1]; ++k) {
+      const cvm::TermRec64& r = terms[k];
+      if (cvm::rec_kind(r.head) != cvm::TERM_MEMBRANE) continue;
+      double a[3], b[3];
+      for (int q = 0; q < 3; ++q) {
+        a[q] = x[3 * r.ids[0] + q] - x[3 * i + q];
+        b[q] = x[3 * r.ids[1] + q] - x[3 * i + q];
+      }
+      nx += a[1] * b[2] - a[2] * b[1];
+      ny += a[2] * b[0] - a[0] * b[2];
+      nz += a[0] * b[1] - a[1] * b[0];
+    }
+    double nn = nx * nx + ny * ny + nz * nz;
+    double d = nn > 0 ? (c[0] * nx + c[1] * ny + c[2] * nz) / nn : 0.0;
+    c[0] = d * nx;
+    c[1] = d * ny;
+    c[2] = d * nz;
+  }
+  for (int k = 0; k < 3; ++k) guess[3 * i + k] += c[k];
 }
 // Step prologue reductions (single block, fixed order => deterministic):
 // origin = centroid (src/gpu_vbd.rs:156), predicted motion max|target - x|
@@ -665,6 +702,7 @@ __global__ void k_finalize(const typename P4<R>::T* pos, const double* x_old, co
     acc[3 * i + k] = (vn - v_old[3 * i + k]) / dt + damping * vn;
   }
 }
+
 
 void read_scalars(cv_gpu& g) {
   CV_HIP(hipMemcpyAsync(g.h_scal, g.d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, g.stream));
@@ -905,15 +943,31 @@ inline double warm_length(const cv_settings* set) {
                        set->gravity[2] * set->gravity[2]);
   return std::max(g * set->dt * set->dt / (1.0 + set->damping * set->dt), 1e-9);
 }
-// Features at the positions in b.x64 / velocities in b.v64, using the last
-// detection's distances.
+// Features at the positions in b.x64 / velocities in b.v64 and the guess in
+// b.guess64 (k_prologue's), using the last detection's distances.
 void warm_features(cv_gpu& g, const cv_solver& s, const cv_settings* set, double rq) {
   uint32_t n = s.n;
   g.feat.reserve(12 * (size_t)n);
-  double dt = set->dt;
-  k_warm_features<<<blocks(n), TPB, 0, g.stream>>>(
-      g.b.x64.p, g.b.v64.p, g.dev.term_off.p, g.dev.terms_d.p, g.dev.nb_off.p, g.dev.nb.p, g.b.dist_bits.p, n, dt,
-      1.0 + set->damping * dt, set->gravity[0], set->gravity[1], set->gravity[2], warm_length(set), rq, g.feat.p);
+  k_warm_features<<<blocks(n), TPB, 0, g.stream>>>(g.b.x64.p, g.b.v64.p, g.b.guess64.p, g.dev.term_off.p,
+                                                   g.dev.terms_d.p, g.dev.nb_off.p, g.dev.nb.p, g.b.dist_bits.p, n,
+                                                   set->dt, warm_length(set), rq, g.feat.p);
+  CV_HIP(hipGetLastError());
+}
+// The network's correction for the features in g.feat, smoothed over the
+// one-ring g.warm_smooth times, added to the guess in b.guess64.
+void warm_guess(cv_gpu& g, const cv_solver& s, const cv_settings* set) {
+  uint32_t n = s.n;
+  g.wpred.reserve(6 * (size_t)n);
+  mlp_infer_device(g, g.feat.p, n, g.wpred.p);
+  float* a = g.wpred.p;
+  float* b = g.wpred.p + 3 * (size_t)n;
+  for (int k = 0; k < g.warm_smooth; ++k) {
+    k_warm_smooth<<<blocks(n), TPB, 0, g.stream>>>(a, g.dev.nb_off.p, g.dev.nb.p, n, b);
+    std::swap(a, b);
+  }
+  k_warm_guess<<<blocks(n), TPB, 0, g.stream>>>(a, g.dev.pinned.p, g.b.x64.p, g.dev.term_off.p, g.dev.terms_d.p, n,
+                                                g.warm_scale * warm_length(set), g.warm_normal_only ? 1 : 0,
+                                                g.b.guess64.p);
   CV_HIP(hipGetLastError());
 }
 
@@ -983,16 +1037,11 @@ struct StepCtx {
     CV_HIP(hipGetLastError());
     detect_and_color<R>(g, s, set, grab, rq, out, true);
     if (set->flags & CV_FLAG_WARM_START) {
-      // Learned guess in place of the adaptive one; clipped to the balls below
+      // Learned correction of the adaptive guess; clipped to the balls below
       // exactly as that guess is.
       if (!g.mlp.ready) fail(CV_ERR_NOT_BUILT, "warm-start network not set up");
       warm_features(g, s, set, rq);
-      g.wpred.reserve(3 * (size_t)n);
-      mlp_infer_device(g, g.feat.p, n, g.wpred.p);
-      k_warm_guess<<<blocks(n), TPB, 0, st>>>(g.wpred.p, b.x64.p, b.v64.p, d.pinned.p, n, dt, damp,
-                                              set->gravity[0], set->gravity[1], set->gravity[2], warm_length(set),
-                                              b.guess64.p);
-      CV_HIP(hipGetLastError());
+      warm_guess(g, s, set);
     }
     k_truncate_guess<R><<<blocks(n), TPB, 0, st>>>(b.guess64.p, b.centers64.p, b.guess64.p + 3 * (size_t)n,
                                                    d.pinned.p, n, g.d_scal, PBufs<R>::pos(g), &g.d_scal->clipped);
@@ -1685,6 +1734,19 @@ cv_status cv_gpu_set_check_gradients(cv_gpu* g, int enabled) {
   return CV_OK;
 }
 
+cv_status cv_warm_set_smoothing(cv_gpu* g, int passes) {
+  if (!g || passes < 0 || passes > 16) return CV_ERR_INVALID_ARGUMENT;
+  g->warm_smooth = passes;
+  return CV_OK;
+}
+
+cv_status cv_warm_set_correction(cv_gpu* g, double scale, int normal_only) {
+  if (!g || !(scale >= 0.0 && scale <= 4.0)) return CV_ERR_INVALID_ARGUMENT;
+  g->warm_scale = scale;
+  g->warm_normal_only = normal_only != 0;
+  return CV_OK;
+}
+
 cv_status cv_gpu_set_lanes(cv_gpu* g, int lanes) {
   if (!g || !(lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64)) return CV_ERR_INVALID_ARGUMENT;
   g->lanes = lanes;
@@ -2136,12 +2198,25 @@ cv_status cv_warm_features(cv_gpu* g, cv_solver* s, const cv_state* st, const cv
     validate_state(*s, st);
     ensure_device_solver(*g, *s);
     ensure_step_buffers(*g, *s);
-    stage_state(*g, *s, st, set);
-    g->pred_valid = false;
+    stage_state(*g, *s, st, set);  // the solver's prediction history, as a step
     uint32_t n = s->n;
     double dt = set->dt, damp = 1.0 + set->damping * dt;
     prologue_reduce(*g, n, dt, damp, set->gravity);
     read_scalars(*g);
+    DevSolver& d = g->dev;
+    StepBufs& b = g->b;
+    auto prologue = [&](auto tag) {
+      using R = decltype(tag);
+      k_prologue<R><<<blocks(n + d.n_fixed), TPB, 0, g->stream>>>(
+          b.x64.p, b.v64.p, d.acc.p, d.masses.p, d.pinned.p, n, d.fixed_pos.p, d.n_fixed, dt, set->damping,
+          set->gravity[0], set->gravity[1], set->gravity[2], g->pred_valid ? 1 : 0, g->d_scal, PBufs<R>::pos(*g),
+          PBufs<R>::tgt(*g), b.guess64.p, b.x_old64.p, b.v_old64.p);
+    };
+    if (g->f64)
+      prologue(0.0);
+    else
+      prologue(0.0f);
+    CV_HIP(hipGetLastError());
     double r = set->contact.radius;
     double rq = std::min(std::max(r + g->h_scal->motion, 2.0 * r), 4.0 * r);
     gpu_detect(*g, *s, rq, set->contact, 0);

@@ -381,6 +381,19 @@ class WarmStart:
         assert p.size == WARM_PARAMS
         self._call(lib().cv_warm_set_params(self.gpu.h, abi.ptr(p, C.c_float)))
 
+    def set_smoothing(self, passes: int):
+        """One-ring averaging passes of the network's correction (cv_warm_set_smoothing)."""
+        L = lib()
+        L.cv_warm_set_smoothing.argtypes = [C.c_void_p, C.c_int]
+        self._call(L.cv_warm_set_smoothing(self.gpu.h, int(passes)))
+
+    def set_correction(self, scale: float = 1.0, normal_only: bool = False):
+        """Factor of the network's correction and whether only its component
+        along the vertex normal is kept (cv_warm_set_correction)."""
+        L = lib()
+        L.cv_warm_set_correction.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        self._call(L.cv_warm_set_correction(self.gpu.h, float(scale), int(normal_only)))
+
     def params(self) -> np.ndarray:
         out = np.zeros(WARM_PARAMS, np.float32)
         self._call(lib().cv_warm_get_params(self.gpu.h, abi.ptr(out, C.c_float)))

@@ -145,7 +145,7 @@ enum {
                                         than the reference's vertex-order colouring
                                         (src/vbd/coloring.rs:5-43), so steps are not the
                                         reference's steps                                 */
-  CV_FLAG_WARM_START = 1u << 3       /* learned initial guess in place of the adaptive one  */
+  CV_FLAG_WARM_START = 1u << 3       /* learned correction added to the adaptive guess      */
 };
 
 /* Grab — src/vbd.rs:54-60. */
@@ -317,19 +317,28 @@ cv_status cv_gpu_step_batch(cv_gpu* gpu, cv_solver* const* solvers, cv_state* co
 /* No reference counterpart.  A per-vertex network (12 -> 128 -> 128 -> 128
  * -> 3, GELU, BF16 weights and activations, FP32 accumulation on the matrix
  * cores) predicts each step's displacement as a correction to the inertial
- * prediction; with CV_FLAG_WARM_START in the settings the step starts from
- * x + (v dt + g dt^2) / (1 + c dt) + L * prediction instead of the adaptive
- * guess of Prediction::guess (src/vbd/prediction.rs:15-41) and clips it to
- * the OGC balls as it clips that guess (src/vbd.rs:316-324); iterations,
- * residuals and energies are unchanged, so the solver itself judges the
- * network.  L = |g| dt^2 / (1 + c dt).  Features (12 floats per vertex):
- * inertial-prediction offset / L (3), v dt / L (3), mean Green strain of the
- * incident triangles along warp and weft x 100 (2), mean one-ring neighbour
- * velocity v dt / L (3), nearest contact distance / query radius (1).
+ * prediction of Prediction::guess (src/vbd/prediction.rs:15-41), the adaptive
+ * x + (v dt + f g dt^2) / (1 + c dt) (f = 1 without a valid history); with
+ * CV_FLAG_WARM_START in the settings the step starts from that prediction
+ * plus L * correction (the correction optionally averaged over each vertex's
+ * one-ring, cv_warm_set_smoothing) and clips it to the OGC balls as it clips
+ * the plain guess (src/vbd.rs:316-324); iterations, residuals and energies
+ * are unchanged, so the solver itself judges the network, and a network
+ * that outputs zero is the plain start.  L = |g| dt^2 / (1 + c dt).
+ * Features (12 floats per vertex): offset of the inertial prediction / L (3),
+ * v dt / L (3), mean Green strain of the incident triangles along warp and
+ * weft x 100 (2), mean one-ring neighbour velocity v dt / L (3), nearest
+ * contact distance / query radius (1).
  * Parameters: FP32 vector of CV_WARM_PARAMS values, W0 [12 x 128] row-major,
  * b0, W1 [128 x 128], b1, W2, b2, W3 [128 x 3], b3. */
 #define CV_WARM_PARAMS 35075
 cv_status cv_warm_set_params(cv_gpu* gpu, const float* params);
+/* Passes of one-ring averaging of the correction before it is added (0-16,
+ * default 0). */
+cv_status cv_warm_set_smoothing(cv_gpu* gpu, int passes);
+/* Factor of the correction (0-4, default 1) and whether only its component
+ * along the area-weighted vertex normal is added (default 0: all of it). */
+cv_status cv_warm_set_correction(cv_gpu* gpu, double scale, int normal_only);
 cv_status cv_warm_get_params(cv_gpu* gpu, float* params);
 /* Predictions [3n] for features [12n] (host arrays). */
 cv_status cv_warm_infer(cv_gpu* gpu, const float* features, uint64_t n, float* out);
