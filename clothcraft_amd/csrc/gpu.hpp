@@ -135,6 +135,7 @@ struct StepBufs {
   DBuf<float4> pos, tgt, ball;        // f32 sweep
   DBuf<double4> pos_d, tgt_d, ball_d;  // f64 sweep
   DBuf<double> gsq;
+  DBuf<double> sum_parts;  // partials of the deterministic sums
   DBuf<double> gtrue;  // per-vertex |g|^2 of the true residual
   DBuf<uint32_t> order;
   DBuf<uint4> vinfo;  // colour-ordered (vertex, term offset, term count, contact offset)

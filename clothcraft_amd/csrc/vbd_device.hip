@@ -405,6 +405,23 @@ __global__ __launch_bounds__(TPB) void k_true_grad(SweepArgs<R> a, uint32_t n, d
     out[v] = (double)cvm::dot(g, g);
   }
 }
+// Deterministic two-level sum: block b of k_sum_part sums the contiguous
+// chunk [b c, (b + 1) c) in a fixed order; k_sum_final sums the partials in
+// block order.
+constexpr uint32_t kSumChunk = 4096;
+__global__ __launch_bounds__(256) void k_sum_part(const double* a, uint32_t n, double* part) {
+  __shared__ double red[256];
+  const uint32_t lo = blockIdx.x * kSumChunk, hi = min(n, lo + kSumChunk);
+  double s = 0;
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) s += a[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
 __global__ void k_sum_f64(const double* a, uint32_t n, double* out) {
   __shared__ double red[1024];
   double s = 0;
@@ -877,16 +894,23 @@ void prologue_reduce(cv_gpu& g, uint32_t n, double dt, double damp, const double
   CV_HIP(hipGetLastError());
 }
 
+// Deterministic sum of a[0, n) into *out on stream st.
+void sum_f64(cv_gpu& g, const double* a, uint32_t n, double* out, hipStream_t st) {
+  const uint32_t parts = std::max<uint32_t>(1, (n + kSumChunk - 1) / kSumChunk);
+  g.b.sum_parts.reserve(parts);
+  k_sum_part<<<parts, 256, 0, st>>>(a, n, g.b.sum_parts.p);
+  k_sum_f64<<<1, 256, 0, st>>>(g.b.sum_parts.p, parts, out);
+  CV_HIP(hipGetLastError());
+}
 template <class R>
 void true_residual(cv_gpu& g, const SweepArgs<R>& a, uint32_t n, hipStream_t st) {
-  switch (g.lanes) {
-    case 8: k_true_grad<8, R><<<blocks((uint64_t)n * 8), TPB, 0, st>>>(a, n, g.b.gtrue.p); break;
-    case 16: k_true_grad<16, R><<<blocks((uint64_t)n * 16), TPB, 0, st>>>(a, n, g.b.gtrue.p); break;
-    case 64: k_true_grad<64, R><<<blocks((uint64_t)n * 64), TPB, 0, st>>>(a, n, g.b.gtrue.p); break;
-    default: k_true_grad<32, R><<<blocks((uint64_t)n * 32), TPB, 0, st>>>(a, n, g.b.gtrue.p); break;
-  }
-  k_sum_f64<<<1, 1024, 0, st>>>(g.b.gtrue.p, n, &g.d_scal->true_residual_sum);
-  CV_HIP(hipGetLastError());
+  // A derivative pass without the line search: 16 lanes per vertex (two
+  // records per lane for a typical vertex) unless the sweep uses fewer.
+  if (g.lanes == 8)
+    k_true_grad<8, R><<<blocks((uint64_t)n * 8), TPB, 0, st>>>(a, n, g.b.gtrue.p);
+  else
+    k_true_grad<16, R><<<blocks((uint64_t)n * 16), TPB, 0, st>>>(a, n, g.b.gtrue.p);
+  sum_f64(g, g.b.gtrue.p, n, &g.d_scal->true_residual_sum, st);
 }
 template <class R>
 void true_residual(cv_gpu& g, const SweepArgs<R>& a, uint32_t n) {
@@ -1167,7 +1191,7 @@ struct StepCtx {
       true_residual<R>(g, a, n);
       out.true_residual = true;
     }
-    k_sum_f64<<<1, 1024, 0, g.stream>>>(b.gsq.p, n, &g.d_scal->residual_sum);
+    sum_f64(g, b.gsq.p, n, &g.d_scal->residual_sum, g.stream);
     // The trust-region check tolerance of the device path (src/gpu_vbd.rs:436).
     k_finalize<R><<<blocks(n), TPB, 0, g.stream>>>(PBufs<R>::pos(g), b.x_old64.p, b.v_old64.p, b.centers64.p,
                                                    b.radii64.p, d.pinned.p, n, set->dt, set->damping, 1e-8,
