@@ -39,11 +39,16 @@ clothcraft_amd/warm_params.npy, trained by clothcraft_amd/train_warm.py).
 
 roofline: the sweep kernel (k_sweep), algorithmic bytes per launch at the f32
 minimum record sizes (64 B per free vertex + 72 per triangle, 52 per hinge,
-68 per stitch, 56 per contact incidence), divided by its mean launch duration
-from HIP events recorded around the sampled launches on the solver's stream;
+68 per stitch, 56 per contact incidence), divided by its mean duration over
+the sampled launches (every launch of every 4th iteration of the timed
+window): the execution span each launch stamps on the device's constant
+clock, the quantity a kernel trace reports; the HIP events recorded around
+the same launches on the solver's stream (which add the dispatch latency)
+give the launch interval reported beside it;
 `traffic` is the measured HBM bytes per launch (rocprofv3 FETCH_SIZE +
 WRITE_SIZE, calibrated by a copy kernel) from the committed counter runs of
-this same command, null for other arguments.
+this command (same workload, tolerance, precision and lanes; the timed step
+count does not change the per-launch work), null for other arguments.
 
 cpu_baseline: the f64 CPU restatement of VbdSolver::step (OpenMP within a
 colour) on the host, from the settled state with the same settings, on the
@@ -67,8 +72,9 @@ WARM_PARAMS = os.path.join(ROOT, "clothcraft_amd", "warm_params.npy")
 
 def _traffic(kernel, workload, argv_key):
     """HBM bytes per sweep launch from the committed rocprofv3 counter runs of
-    this same command (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
-    known size), or None when the arguments differ from the profiled ones."""
+    this command (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known
+    size), or None when the work-defining arguments differ from the profiled
+    ones."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
     try:
         with open(path) as f:
@@ -217,7 +223,8 @@ def bench_step(args):
     log(f"timed: {args.steps} steps")
     gpu.set_profiling(True)
     tot = dict(gpu_compute_ms=0.0, iterations_run=0, sweep_launches=0, sweep_ms=0.0, sweep_bytes=0.0,
-               sweep_bytes_min=0.0, detection_updates=0, sweep_timed_launches=0, sweep_timed_bytes_min=0.0)
+               sweep_bytes_min=0.0, detection_updates=0, sweep_timed_launches=0, sweep_timed_bytes_min=0.0,
+               sweep_kernel_ms=0.0, sweep_kernel_launches=0)
     done, m, contacts, reached = 0, None, [], 0
     t0 = time.perf_counter()
     while done < args.steps:
@@ -243,10 +250,15 @@ def bench_step(args):
     vi = n_free * m["iterations_run"]
     launches = max(m["sweep_launches"], 1)
     timed = max(m["sweep_timed_launches"], 1)
-    avg_launch_s = m["sweep_ms"] / 1e3 / timed
+    # Kernel duration: each sampled launch's execution span on the device's
+    # constant clock (first block's start to last block's end, what a kernel
+    # trace reports); the HIP events around the same launches also include
+    # the dispatch latency before and after the kernel (the launch interval).
+    avg_launch_s = m["sweep_kernel_ms"] / 1e3 / max(m["sweep_kernel_launches"], 1)
+    avg_interval_s = m["sweep_ms"] / 1e3 / timed
     bytes_per_launch = m["sweep_timed_bytes_min"] / timed
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    sweep_s = avg_launch_s * launches
+    sweep_s = avg_interval_s * launches
 
     # The reference's call pattern: one GpuVbd::step per frame, state in and
     # out through host memory and the metrics each step (wall time per call).
@@ -268,8 +280,10 @@ def bench_step(args):
         compare = compare_starts(gpu, sc, st, pv, dict(run_s, iterations=args.compare_cap), args.compare)
 
     kernel = "k_sweep"
-    argv_key = {"workload": args.workload, "steps": K, "warmup": args.warmup, "settle": args.settle,
-                "tol": args.tol, "precision": args.precision, "lanes": args.lanes,
+    # The arguments that define the per-launch work of the sweep (the step
+    # count and warm-up only change how many steps are timed).
+    argv_key = {"workload": args.workload, "settle": args.settle, "tol": args.tol,
+                "max_iterations": args.max_iterations, "precision": args.precision, "lanes": args.lanes,
                 "fixed_iterations": args.fixed_iterations, "fast_coloring": args.fast_coloring}
     out = {
         "metric": "steps/s + vertex-iters/s at fixed residual; warm-start vs plain iters-to-tol",
@@ -321,6 +335,7 @@ def bench_step(args):
             "traffic": _traffic(kernel, args.workload, argv_key),
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_us": 1e6 * avg_launch_s,
+            "avg_launch_interval_us_hip_events": 1e6 * avg_interval_s,
             "launches": m["sweep_launches"],
             "launches_timed": m["sweep_timed_launches"],
             "bytes_per_launch_in_sweep_precision": m["sweep_bytes"] / launches,

@@ -322,6 +322,8 @@ struct cv_gpu {
   // Live sweep-kernel timing (events around every sweep launch).
   bool profile = false;
   std::vector<hipEvent_t> sweep_events;
+  DBuf<unsigned long long> sweep_span;  // execution spans of the stamped sweep launches
+  double wall_khz = 1e5;                 // rate of the device's constant wall clock
   uint64_t timed_counter = 0;  // iterations queued with profiling on
   double static_bytes32 = 0, static_bytes64 = 0;  // algorithmic bytes of one sweep, static terms
   bool f64 = true;  // sweep precision (f64 reproduces the f64 reference step)

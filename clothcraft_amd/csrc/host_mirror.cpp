@@ -248,7 +248,7 @@ void sweep_color(uint32_t count, const uint32_t* order, R* pos, const R* tgt, co
       cvm::V3<R> cand = cvm::truncate_ball<R>(x + delta * scale, center, ball[4 * v + 3], tr);
       cvm::V3<R> dc = cand - target;
       R e = energy(cand) + R(0.5) * inertia * cvm::dot(dc, dc);
-      if (cvm::finite3(cand) && std::isfinite(e) && e <= old + R(1e-14)) {
+      if (cvm::finite3(cand) && std::isfinite(e) && e <= old + cvm::accept_slack<R>(old)) {
         accepted = cand;
         break;
       }

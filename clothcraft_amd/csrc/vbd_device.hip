@@ -177,7 +177,7 @@ __device__ __forceinline__ R local_energy(const SweepArgs<R>& a, const LaneTerm<
 // own values in the per-lane order and reduction local_energy uses, so it
 // serves as the line search's reference energy without a second pass over
 // the terms.  It can differ from local_energy at x in the last bits (the two
-// routines may contract differently); the accept slack of 1e-14 covers that.
+// routines may contract differently); the accept slack (accept_slack) covers that.
 // E = false leaves acc.e undefined.
 template <int L, class R, bool E = true>
 __device__ __forceinline__ cvm::Deriv<R> local_deriv(const SweepArgs<R>& a, const LaneTerm<R>& lt, uint32_t v, V3<R> x,
@@ -238,7 +238,7 @@ __device__ __forceinline__ void solve_vertex(const SweepArgs<R>& a, uint4 vi, ui
     V3<R> cand = cvm::truncate_ball<R>(x + delta * scale, center, b4.w, truncated);
     V3<R> dc = cand - target;
     R e = local_energy<L>(a, lt, v, cand, lane, s0, ns, c0, nt) + R(0.5) * inertia * cvm::dot(dc, dc);
-    if (cvm::finite3(cand) && isfinite(e) && e <= old + R(1e-14)) {
+    if (cvm::finite3(cand) && isfinite(e) && e <= old + cvm::accept_slack<R>(old)) {
       accepted = cand;
       break;
     }
@@ -302,9 +302,26 @@ __device__ __forceinline__ void iter_tail(const IterTail& t) {
   iter_end(t.sc, c, t.it, t.threshold, 0.0, 1.0);
 }
 
+// Execution span of a profiled launch on the device's constant-rate clock:
+// span[0] holds the complement of the earliest block start (so a zero fill
+// initialises both ends for atomicMax), span[1] the last block's end.  Only
+// the first eight blocks (dispatched first, one per XCD) stamp the start: a
+// same-address atomic from every block at once would serialise and delay the
+// launch it measures; block ends are spread out in time.
+__device__ __forceinline__ void span_begin(unsigned long long* span) {
+  if (span && blockIdx.x < 8 && threadIdx.x == 0) atomicMax(&span[0], ~(unsigned long long)wall_clock64());
+}
+__device__ __forceinline__ void span_end(unsigned long long* span) {
+  if (!span) return;  // block-uniform
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(&span[1], (unsigned long long)wall_clock64());
+}
+
 // One colour of one iteration.
 template <int L, class R>
-__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uint32_t count, IterTail tail) {
+__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uint32_t count, IterTail tail,
+                                               unsigned long long* span) {
+  span_begin(span);
   const int lane = threadIdx.x & (L - 1);
   // Vertices of a colour are in index order, so neighbouring groups share
   // neighbours: keep consecutive groups on one XCD's L2.
@@ -316,6 +333,7 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uin
     const uint32_t nc = a.vnc[off + grp];
     if (!*a.stop) solve_vertex<L, R>(a, vi, nc, lane);
   }
+  span_end(span);
   if (tail.sc) iter_tail(tail);  // block-uniform
 }
 
@@ -783,21 +801,21 @@ SweepArgs<R> sweep_args(cv_gpu& g, const cv_settings* set, const cv_grab* grab) 
 // summed time is kernel execution only, as a kernel trace reports it.
 template <int L, class R>
 void launch_sweep(const SweepArgs<R>& a, uint32_t off, uint32_t count, const IterTail& tail, hipStream_t st,
-                  hipEvent_t e0, hipEvent_t e1) {
+                  hipEvent_t e0, hipEvent_t e1, unsigned long long* span) {
   dim3 grid(blocks((uint64_t)count * L)), block(TPB);
   if (e0)
-    hipExtLaunchKernelGGL(k_sweep<L, R>, grid, block, 0, st, e0, e1, 0, a, off, count, tail);
+    hipExtLaunchKernelGGL(k_sweep<L, R>, grid, block, 0, st, e0, e1, 0, a, off, count, tail, span);
   else
-    k_sweep<L, R><<<grid, block, 0, st>>>(a, off, count, tail);
+    k_sweep<L, R><<<grid, block, 0, st>>>(a, off, count, tail, span);
 }
 template <class R>
 void sweep_color(cv_gpu& g, const SweepArgs<R>& a, uint32_t off, uint32_t count, const IterTail& tail = IterTail{},
-                 hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+                 hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, unsigned long long* span = nullptr) {
   switch (g.lanes) {
-    case 8: launch_sweep<8>(a, off, count, tail, g.stream, e0, e1); break;
-    case 16: launch_sweep<16>(a, off, count, tail, g.stream, e0, e1); break;
-    case 64: launch_sweep<64>(a, off, count, tail, g.stream, e0, e1); break;
-    default: launch_sweep<32>(a, off, count, tail, g.stream, e0, e1); break;
+    case 8: launch_sweep<8>(a, off, count, tail, g.stream, e0, e1, span); break;
+    case 16: launch_sweep<16>(a, off, count, tail, g.stream, e0, e1, span); break;
+    case 64: launch_sweep<64>(a, off, count, tail, g.stream, e0, e1, span); break;
+    default: launch_sweep<32>(a, off, count, tail, g.stream, e0, e1, span); break;
   }
 }
 
@@ -922,6 +940,8 @@ struct StepOut {
   uint32_t colors = 0, iterations_run = 0;
   uint64_t sweep_launches = 0, sweep_timed_launches = 0;
   double sweep_ms = 0, sweep_bytes = 0, sweep_bytes_min = 0, sweep_timed_bytes_min = 0;
+  double sweep_span_ms = 0;  // summed execution spans of the stamped launches (device clock)
+  uint64_t sweep_span_launches = 0;
   double detection_ms = 0, refit_ms = 0, candidates_ms = 0, narrow_ms = 0;
   uint64_t candidate_pairs = 0, grid_tests = 0;
   bool true_residual = false;
@@ -1028,8 +1048,10 @@ struct StepCtx {
   std::vector<Launch> launches;
   std::vector<uint32_t> timed_its;  // iterations whose launches carry events
   std::vector<size_t> pending_ev;   // completed timed launches not yet read
+  std::vector<unsigned long long> spans;  // host copy of the stamped spans
   size_t nev = 0;                   // event pairs used in this step
   static constexpr uint64_t kTimedStride = 4;
+  static constexpr size_t kMaxSpans = 4096;  // stamped launches per step
 
   StepCtx(cv_gpu& g_, cv_solver& s_, const cv_settings* set_, const cv_grab* grab_, StepOut& out_)
       : g(g_), s(s_), set(set_), grab(grab_), out(out_) {}
@@ -1079,6 +1101,10 @@ struct StepCtx {
     CV_HIP(hipGetLastError());
     a = sweep_args<R>(g, set, grab);
     iters = set->iterations;
+    if (g.profile) {
+      g.sweep_span.reserve(2 * kMaxSpans);
+      CV_HIP(hipMemsetAsync(g.sweep_span.p, 0, 2 * kMaxSpans * sizeof(unsigned long long), g.stream));
+    }
   }
   // Queues the scalar read-back that after_sync() consumes.
   void read_back() {
@@ -1091,12 +1117,24 @@ struct StepCtx {
     uint32_t ran = g.h_scal->iters_run;
     double inc = (double)g.h_scal->incidences;
     double it_bytes32 = g.static_bytes32 + 56.0 * inc;
+    size_t span_hi = 0;
+    for (const Launch& l : launches)
+      if (g.profile && l.ev != (size_t)-1 && l.ev < kMaxSpans) span_hi = std::max(span_hi, l.ev + 1);
+    if (span_hi) {  // the stream has drained: the stamped spans are final
+      spans.resize(2 * span_hi);
+      CV_HIP(hipMemcpy(spans.data(), g.sweep_span.p, 2 * span_hi * sizeof(unsigned long long),
+                       hipMemcpyDeviceToHost));
+    }
     for (const Launch& l : launches) {
       if (l.it >= ran) continue;
       out.sweep_launches++;
       if (g.profile && l.ev != (size_t)-1) {
         pending_ev.push_back(l.ev);  // read by drain_timing() once more work is queued
         out.sweep_timed_launches++;
+        if (l.ev < kMaxSpans && spans[2 * l.ev + 1]) {
+          out.sweep_span_ms += (double)(spans[2 * l.ev + 1] - ~spans[2 * l.ev]) / g.wall_khz;
+          out.sweep_span_launches++;
+        }
       }
     }
     for (uint32_t it : timed_its)
@@ -1154,7 +1192,8 @@ struct StepCtx {
             g.sweep_events.push_back(e0);
             g.sweep_events.push_back(e1);
           }
-          sweep_color<R>(g, a, off, cnt, tail, g.sweep_events[2 * nev], g.sweep_events[2 * nev + 1]);
+          sweep_color<R>(g, a, off, cnt, tail, g.sweep_events[2 * nev], g.sweep_events[2 * nev + 1],
+                         nev < kMaxSpans ? g.sweep_span.p + 2 * nev : nullptr);
           launches.push_back({nev++, it});
         } else {
           sweep_color<R>(g, a, off, cnt, tail);
@@ -1628,6 +1667,8 @@ void fill_metrics(cv_metrics* m, const cv_gpu& g, const cv_solver& s, const cv_s
   m->sweep_ms = g.profile ? o.sweep_ms : NAN;
   m->sweep_timed_launches = o.sweep_timed_launches;
   m->sweep_timed_bytes_min = o.sweep_timed_bytes_min;
+  m->sweep_kernel_ms = g.profile ? o.sweep_span_ms : NAN;
+  m->sweep_kernel_launches = o.sweep_span_launches;
   m->f64 = g.f64 ? 1 : 0;
   m->sweep_launches = o.sweep_launches;
   m->sweep_bytes = o.sweep_bytes;
@@ -1715,6 +1756,9 @@ cv_status cv_gpu_create(int device, cv_gpu** out) {
     g->device = device;
     CV_HIP(hipSetDevice(device));
     warm_scratch_setup(device);
+    int khz = 0;
+    CV_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+    g->wall_khz = khz > 0 ? (double)khz : 1e5;
     CV_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     CV_HIP(hipMalloc((void**)&g->d_scal, sizeof(Scalars)));
     CV_HIP(hipHostMalloc((void**)&g->h_scal, sizeof(Scalars)));
@@ -1825,6 +1869,8 @@ static cv_status run_steps(cv_gpu* g, cv_solver* s, cv_state* st, const cv_setti
       tot.sweep_ms += o.sweep_ms;
       tot.sweep_timed_launches += o.sweep_timed_launches;
       tot.sweep_timed_bytes_min += o.sweep_timed_bytes_min;
+      tot.sweep_span_ms += o.sweep_span_ms;
+      tot.sweep_span_launches += o.sweep_span_launches;
       tot.sweep_bytes += o.sweep_bytes;
       tot.sweep_bytes_min += o.sweep_bytes_min;
     }
@@ -1859,6 +1905,8 @@ static cv_status run_steps(cv_gpu* g, cv_solver* s, cv_state* st, const cv_setti
       o.sweep_ms = tot.sweep_ms;
       o.sweep_timed_launches = tot.sweep_timed_launches;
       o.sweep_timed_bytes_min = tot.sweep_timed_bytes_min;
+      o.sweep_span_ms = tot.sweep_span_ms;
+      o.sweep_span_launches = tot.sweep_span_launches;
       o.sweep_bytes = tot.sweep_bytes;
       o.sweep_bytes_min = tot.sweep_bytes_min;
     }

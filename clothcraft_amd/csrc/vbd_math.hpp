@@ -622,6 +622,24 @@ CV_HD V3<R> newton_direction(Deriv<R>& tot, V3<R> x, V3<R> target, R inertia) {
   return mk<R>(-(c00 * g.x + c01 * g.y + c02 * g.z) * inv, -(c01 * g.x + c11 * g.y + c12 * g.z) * inv,
                -(c02 * g.x + c12 * g.y + c22 * g.z) * inv);
 }
+// Slack of the line search's accept test `E <= E_old + slack`
+// (src/vbd.rs:375-386).  In f64 the reference's 1e-14.  In f32 that absolute
+// slack lies below one ulp of a local energy, so candidates whose energy
+// differs from E_old only by rounding would be rejected and converged
+// vertices would run all ten halvings; the f32 slack adds four ulps of E_old
+// (local energies are sums of non-negative terms), the rounding level of an
+// f32 sum, as the reference's slack sits above the rounding level of f64.
+template <class R>
+CV_HD R accept_slack(R old);
+template <>
+CV_HD double accept_slack<double>(double) {
+  return 1e-14;
+}
+template <>
+CV_HD float accept_slack<float>(float old) {
+  return 1e-14f + 4.0f * 1.1920928955078125e-07f * fabs(old);
+}
+
 // ContactSet::truncate (src/ogc/world.rs:91-100) against a ball (center, radius).
 template <class R>
 CV_HD V3<R> truncate_ball(V3<R> p, V3<R> c, R radius, bool& clipped) {

@@ -182,6 +182,10 @@ typedef struct cv_metrics {
   uint64_t sweep_timed_launches;  /* launches in sweep_ms: every launch of every 4th
                                      iteration queued while profiling is on           */
   double sweep_timed_bytes_min;   /* sweep_bytes_min of those launches                */
+  double sweep_kernel_ms;         /* profiling on: summed execution spans of the same
+                                     launches on the device's constant clock (first
+                                     block's start to last block's end), else NaN    */
+  uint64_t sweep_kernel_launches; /* launches in sweep_kernel_ms                      */
 } cv_metrics;
 
 /* ------------------------------------------------------------- solver */
