@@ -41,10 +41,10 @@ roofline: the sweep kernel (k_sweep), algorithmic bytes per launch at the f32
 minimum record sizes (64 B per free vertex + 72 per triangle, 52 per hinge,
 68 per stitch, 56 per contact incidence), divided by its mean duration over
 the sampled launches (every launch of every 4th iteration of the timed
-window): the execution span each launch stamps on the device's constant
-clock, the quantity a kernel trace reports; the HIP events recorded around
-the same launches on the solver's stream (which add the dispatch latency)
-give the launch interval reported beside it;
+window): the HIP events recorded around each of those launches on the
+solver's stream (hipExtLaunchKernelGGL start/stop events), less the event
+interval of an empty launch on the same stream measured in the same run
+(the dispatch latency inside every interval; both are reported);
 `traffic` is the measured HBM bytes per launch (rocprofv3 FETCH_SIZE +
 WRITE_SIZE, calibrated by a copy kernel) from the committed counter runs of
 this command (same workload, tolerance, precision and lanes; the timed step
@@ -223,8 +223,7 @@ def bench_step(args):
     log(f"timed: {args.steps} steps")
     gpu.set_profiling(True)
     tot = dict(gpu_compute_ms=0.0, iterations_run=0, sweep_launches=0, sweep_ms=0.0, sweep_bytes=0.0,
-               sweep_bytes_min=0.0, detection_updates=0, sweep_timed_launches=0, sweep_timed_bytes_min=0.0,
-               sweep_kernel_ms=0.0, sweep_kernel_launches=0)
+               sweep_bytes_min=0.0, detection_updates=0, sweep_timed_launches=0, sweep_timed_bytes_min=0.0)
     done, m, contacts, reached = 0, None, [], 0
     t0 = time.perf_counter()
     while done < args.steps:
@@ -250,12 +249,11 @@ def bench_step(args):
     vi = n_free * m["iterations_run"]
     launches = max(m["sweep_launches"], 1)
     timed = max(m["sweep_timed_launches"], 1)
-    # Kernel duration: each sampled launch's execution span on the device's
-    # constant clock (first block's start to last block's end, what a kernel
-    # trace reports); the HIP events around the same launches also include
-    # the dispatch latency before and after the kernel (the launch interval).
-    avg_launch_s = m["sweep_kernel_ms"] / 1e3 / max(m["sweep_kernel_launches"], 1)
+    # Kernel duration: the HIP event interval of each sampled launch less the
+    # interval of an empty launch on the same stream (dispatch latency).
+    overhead_s = m["sweep_event_overhead_ms"] / 1e3
     avg_interval_s = m["sweep_ms"] / 1e3 / timed
+    avg_launch_s = max(avg_interval_s - overhead_s, 1e-9)
     bytes_per_launch = m["sweep_timed_bytes_min"] / timed
     achieved = bytes_per_launch / avg_launch_s / 1e9
     sweep_s = avg_interval_s * launches
@@ -336,6 +334,7 @@ def bench_step(args):
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_us": 1e6 * avg_launch_s,
             "avg_launch_interval_us_hip_events": 1e6 * avg_interval_s,
+            "empty_launch_interval_us_hip_events": 1e6 * overhead_s,
             "launches": m["sweep_launches"],
             "launches_timed": m["sweep_timed_launches"],
             "bytes_per_launch_in_sweep_precision": m["sweep_bytes"] / launches,

@@ -94,7 +94,7 @@ class cv_metrics(C.Structure):
         ("sweep_ms", C.c_double), ("sweep_launches", C.c_uint64), ("sweep_bytes", C.c_double),
         ("f64", C.c_uint32), ("sweep_bytes_min", C.c_double), ("steps", C.c_uint32),
         ("sweep_timed_launches", C.c_uint64), ("sweep_timed_bytes_min", C.c_double),
-        ("sweep_kernel_ms", C.c_double), ("sweep_kernel_launches", C.c_uint64),
+        ("sweep_event_overhead_ms", C.c_double), ("broad_phase_fallbacks", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

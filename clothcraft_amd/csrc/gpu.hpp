@@ -129,6 +129,15 @@ struct DevSolver {
 };
 
 // Device working set of one step (sized to the largest solver seen).
+// One linear BVH (ogc_device.hip): Morton keys, leaf order, Karras tree, boxes.
+struct BvhBufs {
+  DBuf<unsigned long long> key, key_sorted;
+  DBuf<uint32_t> idx, prim, flag;
+  DBuf<int32_t> child, parent;
+  DBuf<double> box;
+  uint32_t N = 0;
+};
+
 struct StepBufs {
   DBuf<double> x64, x_old64, v64, v_old64, guess64, xcur64, radii64, centers64;
   DBuf<unsigned long long> dist_bits;
@@ -146,6 +155,11 @@ struct StepBufs {
       fc_val;
   // detection
   DBuf<double> fbox, ebox;  // [6F], [6E]
+  BvhBufs bvh_faces, bvh_edges;          // LBVH broad phase (cv_gpu_set_broad_phase 1)
+  DBuf<uint32_t> vf_cnt, vf_off, ee_cnt, ee_off;
+  bool bvh_rebuild = true;               // rebuild (not refit) at the next LBVH detection
+  bool bvh_fallback = false;             // this detection runs the grid (an LBVH walk overflowed)
+  uint64_t bvh_fallbacks = 0;
   DBuf<uint32_t> cell_count, cell_off;
   DBuf<unsigned long long> cell_key, cell_key_sorted;
   DBuf<uint32_t> cell_val, cell_val_sorted, cell_rank, cell_owner;
@@ -284,6 +298,7 @@ struct cv_gpu {
   int device = 0;
   hipStream_t stream = nullptr;
   bool check_gradients = false;
+  int broad_phase = 0;  // 0: hash grid, 1: LBVH
   int warm_smooth = 0;  // one-ring averaging passes of the warm-start correction
   double warm_scale = 1.0;        // factor of the warm-start correction
   bool warm_normal_only = false;  // keep only the correction's component along the vertex normal
@@ -322,8 +337,7 @@ struct cv_gpu {
   // Live sweep-kernel timing (events around every sweep launch).
   bool profile = false;
   std::vector<hipEvent_t> sweep_events;
-  DBuf<unsigned long long> sweep_span;  // execution spans of the stamped sweep launches
-  double wall_khz = 1e5;                 // rate of the device's constant wall clock
+  double event_overhead_ms = 0;  // event interval of an empty launch (profiling calibration)
   uint64_t timed_counter = 0;  // iterations queued with profiling on
   double static_bytes32 = 0, static_bytes64 = 0;  // algorithmic bytes of one sweep, static terms
   bool f64 = true;  // sweep precision (f64 reproduces the f64 reference step)

@@ -306,6 +306,7 @@ __device__ __forceinline__ void vf_narrow(const DetectArgs& a, uint32_t v, uint3
   }
 }
 
+constexpr unsigned long long kNoCell = ~0ull;  // a pair met once per query (LBVH): no cell check
 // Edge-edge filter for a pair met in the grid cell `cell`
 // (src/ogc/world.rs:498-512).  The moving edge e is the first operand;
 // moving/moving pairs are taken once with e < f, and only in the cell
@@ -318,9 +319,11 @@ __device__ __forceinline__ bool ee_filter(const DetectArgs& a, uint32_t e, const
   if (ogm::box_dist2(be, bfr) > a.pr2) return false;
   if (f == e) return false;
   if (a.edge_moving[f] && f < e) return false;
-  ogm::Box bf = inflate(bfr, a.pr);
-  double lx = fmax(be.lo.x, bf.lo.x), ly = fmax(be.lo.y, bf.lo.y), lz = fmax(be.lo.z, bf.lo.z);
-  if (cell_key(cell_of(lx, a.inv_h), cell_of(ly, a.inv_h), cell_of(lz, a.inv_h)) != cell) return false;
+  if (cell != kNoCell) {
+    ogm::Box bf = inflate(bfr, a.pr);
+    double lx = fmax(be.lo.x, bf.lo.x), ly = fmax(be.lo.y, bf.lo.y), lz = fmax(be.lo.z, bf.lo.z);
+    if (cell_key(cell_of(lx, a.inv_h), cell_of(ly, a.inv_h), cell_of(lz, a.inv_h)) != cell) return false;
+  }
   uint32_t jc = a.edges[2 * f], jd = a.edges[2 * f + 1];
   if (ia == jc || ia == jd || ib == jc || ib == jd) return false;
   if ((ia_ex && (is_excluded(a, ia, jc) || is_excluded(a, ia, jd))) ||
@@ -618,6 +621,218 @@ __global__ __launch_bounds__(TPB) void k_ee_overflow(DetectArgs a, const uint32_
   add_counts(a, cnt);
 }
 
+// ------------------------------------------------------- LBVH broad phase
+// The alternative to the hash grid (cv_gpu_set_broad_phase): one linear BVH
+// per primitive kind (faces, edges) over the same f64 boxes.  Leaves are
+// ordered by 63-bit Morton codes of the box centres in the cube
+// [-scale, scale]^3 of the padded radius, the internal nodes form a Karras
+// radix tree (a node's range and split from the longest common prefixes of
+// the sorted codes, ties broken by leaf index), and node boxes are fitted
+// bottom up (the second child to arrive unions both).  A query walks its
+// tree with a stack and descends only into nodes with
+// box_dist2(query box, node box) <= pr^2: node boxes are exact f64 min/max
+// unions, so a node's distance never exceeds a leaf's below it and the walk
+// keeps every pair the reference's predicate keeps — candidate sets stay
+// bit-exact.  Candidates are counted per query in a first walk, placed by an
+// exclusive scan and written by a second walk (no atomics, deterministic
+// order).  Nodes: internal 0..N-2, leaves N-1..2N-2 (root 0 in both cases).
+struct Bvh {
+  const double* box;     // [6 (2N - 1)]
+  const int32_t* child;  // [2 (N - 1)]
+  const uint32_t* prim;  // [N] primitive of each leaf, in leaf order
+  uint32_t N;
+};
+constexpr int kBvhStack = 64;
+
+__device__ __forceinline__ unsigned long long spread21(unsigned long long v) {
+  v &= 0x1FFFFFull;
+  v = (v | v << 32) & 0x1F00000000FFFFull;
+  v = (v | v << 16) & 0x1F0000FF0000FFull;
+  v = (v | v << 8) & 0x100F00F00F00F00Full;
+  v = (v | v << 4) & 0x10C30C30C30C30C3ull;
+  v = (v | v << 2) & 0x1249249249249249ull;
+  return v;
+}
+__global__ void k_morton(const double* box, uint32_t N, double lo, double scale21, unsigned long long* key,
+                         uint32_t* idx) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double* b = box + 6 * (size_t)i;
+  unsigned long long q[3];
+  for (int k = 0; k < 3; ++k) {
+    double c = ((0.5 * (b[k] + b[3 + k])) - lo) * scale21;
+    q[k] = (unsigned long long)fmin(fmax(c, 0.0), 2097151.0);
+  }
+  key[i] = spread21(q[0]) << 2 | spread21(q[1]) << 1 | spread21(q[2]);
+  idx[i] = i;
+}
+__device__ __forceinline__ int bvh_delta(const unsigned long long* k, int N, int i, int j) {
+  if (j < 0 || j >= N) return -1;
+  const unsigned long long a = k[i], b = k[j];
+  if (a == b) return 64 + __clz((uint32_t)(i ^ j));
+  return __clzll((long long)(a ^ b));
+}
+__global__ void k_karras(const unsigned long long* k, int N, int32_t* child, int32_t* parent) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) parent[0] = -1;
+  if (i >= N - 1) return;
+  const int d = bvh_delta(k, N, i, i + 1) - bvh_delta(k, N, i, i - 1) >= 0 ? 1 : -1;
+  const int dmin = bvh_delta(k, N, i, i - d);
+  int lmax = 2;
+  while (bvh_delta(k, N, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (bvh_delta(k, N, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = bvh_delta(k, N, i, j);
+  int s = 0;
+  for (int div = 2;; div <<= 1) {
+    const int t = (l + div - 1) / div;
+    if (bvh_delta(k, N, i, i + (s + t) * d) > dnode) s += t;
+    if (t <= 1) break;
+  }
+  const int gamma = i + s * d + min(d, 0);
+  const int left = min(i, j) == gamma ? N - 1 + gamma : gamma;
+  const int right = max(i, j) == gamma + 1 ? N - 1 + gamma + 1 : gamma + 1;
+  child[2 * i] = left;
+  child[2 * i + 1] = right;
+  parent[left] = i;
+  parent[right] = i;
+}
+__device__ __forceinline__ double ld_coherent(const double* p) {
+  return __builtin_bit_cast(double, __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT));
+}
+// Leaf boxes, then unions up the tree; flag[] (N - 1 zeros) counts arrivals.
+__global__ void k_refit(const double* pbox, const uint32_t* prim, const int32_t* child, const int32_t* parent,
+                        uint32_t N, uint32_t* flag, double* box) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  int node = (int)(N - 1 + t);
+  const double* src = pbox + 6 * (size_t)prim[t];
+  for (int k = 0; k < 6; ++k) box[6 * (size_t)node + k] = src[k];
+  int par = parent[node];
+  // A path to the root has at most N - 1 internal nodes; the bound only
+  // guards the loop against a malformed tree.
+  for (uint32_t up = 0; par >= 0 && up < N; ++up) {
+    __threadfence();
+    if (atomicAdd(&flag[par], 1u) == 0) return;  // the sibling's thread completes the parent
+    __threadfence();
+    const double* a = box + 6 * (size_t)child[2 * par];
+    const double* b = box + 6 * (size_t)child[2 * par + 1];
+    double* o = box + 6 * (size_t)par;
+    for (int k = 0; k < 3; ++k) {
+      o[k] = fmin(ld_coherent(a + k), ld_coherent(b + k));
+      o[3 + k] = fmax(ld_coherent(a + 3 + k), ld_coherent(b + 3 + k));
+    }
+    par = parent[par];
+  }
+}
+// Vertex-face walk: every point (moving and avatar) against the face tree.
+template <bool WRITE>
+__global__ __launch_bounds__(TPB) void k_bvh_vf(DetectArgs a, Bvh t, uint32_t* count, const uint32_t* off,
+                                                uint2* list) {
+  PairCounts cnt;
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < a.n_total) {
+    const D3 p = ld3(a.x, q);
+    const ogm::Box pb = ogm::box_of(p);
+    const bool moving = q < a.n, ex = a.ex_off[q] != a.ex_off[q + 1];
+    uint32_t c = 0, base = WRITE ? off[q] : 0;
+    int stack[kBvhStack];
+    int sp = 0;
+    stack[sp++] = 0;
+    // A tree's walk pops each of its 2N - 1 nodes at most once; more pops
+    // (a malformed tree) end the walk like a stack overflow.
+    for (uint32_t pops = 0; sp; ++pops) {
+      if (pops >= 2 * t.N) {
+        atomicOr(&a.sc->err_overflow, 16u);
+        break;
+      }
+      const int nd = stack[--sp];
+      const ogm::Box nb = ldbox(t.box, (uint32_t)nd);
+      if (nd >= (int)t.N - 1) {
+        const uint32_t f = t.prim[nd - ((int)t.N - 1)];
+        if (vf_filter(a, q, moving, ex, pb, f, nb, cnt)) {
+          if (WRITE) list[base + c] = make_uint2(q, f);
+          ++c;
+        }
+        continue;
+      }
+      ++cnt.tests;
+      if (ogm::box_dist2(pb, nb) > a.pr2) continue;
+      if (sp + 2 > kBvhStack) {
+        atomicOr(&a.sc->err_overflow, 16u);
+        break;
+      }
+      stack[sp++] = t.child[2 * nd];
+      stack[sp++] = t.child[2 * nd + 1];
+    }
+    if (!WRITE) count[q] = c;
+  }
+  if (WRITE) add_counts(a, cnt);  // every lane reaches it
+}
+// Edge-edge walk: every moving edge against the edge tree.
+template <bool WRITE>
+__global__ __launch_bounds__(TPB) void k_bvh_ee(DetectArgs a, Bvh t, uint32_t* count, const uint32_t* off,
+                                                uint2* list) {
+  PairCounts cnt;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < a.E) {
+    uint32_t c = 0, base = WRITE ? off[e] : 0;
+    if (a.edge_moving[e]) {
+      const ogm::Box be = ldbox(a.ebox, e);
+      const uint32_t ia = a.edges[2 * e], ib = a.edges[2 * e + 1];
+      const bool ia_ex = a.ex_off[ia] != a.ex_off[ia + 1], ib_ex = a.ex_off[ib] != a.ex_off[ib + 1];
+      int stack[kBvhStack];
+      int sp = 0;
+      stack[sp++] = 0;
+      for (uint32_t pops = 0; sp; ++pops) {
+        if (pops >= 2 * t.N) {
+          atomicOr(&a.sc->err_overflow, 16u);
+          break;
+        }
+        const int nd = stack[--sp];
+        const ogm::Box nb = ldbox(t.box, (uint32_t)nd);
+        if (nd >= (int)t.N - 1) {
+          const uint32_t f = t.prim[nd - ((int)t.N - 1)];
+          if (ee_filter(a, e, be, ia, ib, ia_ex, ib_ex, f, nb, kNoCell, cnt)) {
+            if (WRITE) list[base + c] = make_uint2(e, f);
+            ++c;
+          }
+          continue;
+        }
+        ++cnt.tests;
+        if (ogm::box_dist2(be, nb) > a.pr2) continue;
+        if (sp + 2 > kBvhStack) {
+          atomicOr(&a.sc->err_overflow, 16u);
+          break;
+        }
+        stack[sp++] = t.child[2 * nd];
+        stack[sp++] = t.child[2 * nd + 1];
+      }
+    }
+    if (!WRITE) count[e] = c;
+  }
+  if (WRITE) add_counts(a, cnt);
+}
+// Narrow pass over the two candidate lists of the LBVH walk.
+__global__ __launch_bounds__(TPB) void k_narrow_lists(DetectArgs a, const uint2* vf, uint64_t nvf, const uint2* ee,
+                                                      uint64_t nee) {
+  PairCounts cnt;
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nvf + nee; j += step) {
+    if (j < nvf) {
+      const uint2 p = vf[j];
+      vf_narrow(a, p.x, p.y, cnt);
+    } else {
+      const uint2 p = ee[j - nvf];
+      ee_narrow(a, p.x, p.y, cnt);
+    }
+  }
+  add_counts(a, cnt);
+}
+
 __global__ void k_init_dist(unsigned long long* bits, uint32_t n, double rq) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) bits[i] = (unsigned long long)__double_as_longlong(rq);
@@ -895,6 +1110,79 @@ __global__ void k_seam_gap(const double* x, const uint32_t* off, const uint32_t*
 
 void device_scan_u32(cv_gpu& g, const uint32_t* in, uint32_t* out, size_t count) { scan_u32(g, in, out, count); }
 
+// The LBVH broad phase and the narrow pass over its candidates.  The trees
+// are rebuilt at the first detection of a step (b.bvh_rebuild) and refitted
+// at the refresh detections within it (same leaf order, new boxes).
+void lbvh_tree(cv_gpu& g, BvhBufs& t, const double* pbox, uint32_t N, double scale, bool rebuild) {
+  hipStream_t st = g.stream;
+  if (N == 0) {
+    t.N = 0;
+    return;
+  }
+  if (rebuild || t.N != N) {
+    t.N = N;
+    t.key.reserve(N);
+    t.key_sorted.reserve(N);
+    t.idx.reserve(N);
+    t.prim.reserve(N);
+    t.child.reserve(2 * (size_t)std::max<uint32_t>(N - 1, 1));
+    t.parent.reserve(2 * (size_t)N);
+    t.box.reserve(6 * (2 * (size_t)N));
+    t.flag.reserve(std::max<uint32_t>(N - 1, 1));
+    const double half = scale > 0 ? scale : 1.0;  // box centres lie in [-scale, scale]^3
+    k_morton<<<blocks(N), TPB, 0, st>>>(pbox, N, -half, 2097152.0 / (2.0 * half), t.key.p, t.idx.p);
+    CV_HIP(hipGetLastError());
+    with_temp(g, [&](void* tmp, size_t& bytes) {
+      return rocprim::radix_sort_pairs(tmp, bytes, t.key.p, t.key_sorted.p, t.idx.p, t.prim.p, (size_t)N, 0, 63,
+                                       st);
+    });
+    k_karras<<<blocks(std::max<uint32_t>(N - 1, 1)), TPB, 0, st>>>(t.key_sorted.p, (int)N, t.child.p, t.parent.p);
+    CV_HIP(hipGetLastError());
+  }
+  if (N > 1) CV_HIP(hipMemsetAsync(t.flag.p, 0, (N - 1) * sizeof(uint32_t), st));
+  k_refit<<<blocks(N), TPB, 0, st>>>(pbox, t.prim.p, t.child.p, t.parent.p, N, t.flag.p, t.box.p);
+  CV_HIP(hipGetLastError());
+}
+
+void lbvh_broad_narrow(cv_gpu& g, const DetectArgs& a, double scale) {
+  DevSolver& d = g.dev;
+  StepBufs& b = g.b;
+  hipStream_t st = g.stream;
+  const bool rebuild = b.bvh_rebuild;
+  b.bvh_rebuild = false;
+  lbvh_tree(g, b.bvh_faces, b.fbox.p, d.F, scale, rebuild);
+  lbvh_tree(g, b.bvh_edges, b.ebox.p, d.E, scale, rebuild);
+  Bvh tf{b.bvh_faces.box.p, b.bvh_faces.child.p, b.bvh_faces.prim.p, d.F};
+  Bvh te{b.bvh_edges.box.p, b.bvh_edges.child.p, b.bvh_edges.prim.p, d.E};
+  DetectArgs counting = a;  // the counting walk exports nothing
+  counting.cand_vf = counting.cand_ee = nullptr;
+  b.vf_cnt.reserve(d.n_total + 1);
+  b.vf_off.reserve(d.n_total + 1);
+  b.ee_cnt.reserve(d.E + 1);
+  b.ee_off.reserve(d.E + 1);
+  // A walk over an empty tree finds nothing: its counts are zeros.
+  CV_HIP(hipMemsetAsync(b.vf_cnt.p + (d.F ? d.n_total : 0), 0, (d.F ? 1 : d.n_total + 1) * sizeof(uint32_t), st));
+  CV_HIP(hipMemsetAsync(b.ee_cnt.p + d.E, 0, sizeof(uint32_t), st));
+  if (d.F) k_bvh_vf<false><<<blocks(d.n_total), TPB, 0, st>>>(counting, tf, b.vf_cnt.p, nullptr, nullptr);
+  if (d.E) k_bvh_ee<false><<<blocks(d.E), TPB, 0, st>>>(counting, te, b.ee_cnt.p, nullptr, nullptr);
+  CV_HIP(hipGetLastError());
+  scan_u32(g, b.vf_cnt.p, b.vf_off.p, (size_t)d.n_total + 1);
+  scan_u32(g, b.ee_cnt.p, b.ee_off.p, (size_t)d.E + 1);
+  uint32_t tot[2] = {0, 0};
+  CV_HIP(hipMemcpyAsync(&tot[0], b.vf_off.p + d.n_total, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  CV_HIP(hipMemcpyAsync(&tot[1], b.ee_off.p + d.E, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  CV_HIP(hipStreamSynchronize(st));
+  const uint64_t nvf = tot[0], nee = tot[1];
+  b.pair_list.reserve(std::max<uint64_t>(nvf + nee, 1));
+  uint2* vf = b.pair_list.p;
+  uint2* ee = b.pair_list.p + nvf;
+  if (d.F) k_bvh_vf<true><<<blocks(d.n_total), TPB, 0, st>>>(a, tf, nullptr, b.vf_off.p, vf);
+  if (d.E) k_bvh_ee<true><<<blocks(d.E), TPB, 0, st>>>(a, te, nullptr, b.ee_off.p, ee);
+  CV_HIP(hipGetLastError());
+  k_narrow_lists<<<8 * 256, TPB, 0, st>>>(a, vf, nvf, ee, nee);
+  CV_HIP(hipGetLastError());
+}
+
 void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings& ogc, size_t cand_cap) {
   DevSolver& d = g.dev;
   StepBufs& b = g.b;
@@ -961,43 +1249,57 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   a.cand_ee = cand_cap ? b.cand_ee.p : nullptr;
   a.cand_cap = cand_cap;
   a.sc = g.d_scal;
-  // Faces and edges registered with the padded inflation; points look up
-  // their own cell, edges the cells of their own box.
-  a.log2t = build_grid(g, d.F, d.E, pr_reg, h);
-  a.skey = b.cell_key_sorted.p;
-  a.sprim = b.cell_val_sorted.p;
-  a.sbox = b.cell_box_sorted.p;
-  a.bstart = b.bstart.p;
-  // Flattened broad phase: query items, their pair counts, owners, pairs.
-  uint32_t NI = d.n_total + kEdgeCells * d.E;
-  b.item_len.reserve(NI + 1);
-  b.item_off.reserve(NI + 1);
-  b.item_start.reserve(NI);
-  b.item_key.reserve(NI);
-  b.over.reserve(std::max<uint32_t>(d.E, 1));
-  CV_HIP(hipMemsetAsync(b.item_len.p + NI, 0, sizeof(uint32_t), st));
-  CV_HIP(hipMemsetAsync(&g.d_scal->n_over, 0, sizeof(unsigned long long), st));
-  k_items<<<blocks(d.n_total + d.E), TPB, 0, st>>>(a, b.item_len.p, b.item_start.p, b.item_key.p, b.over.p,
-                                                   &g.d_scal->n_over);
-  CV_HIP(hipGetLastError());
-  scan_u32(g, b.item_len.p, b.item_off.p, (size_t)NI + 1);
-  // Pair-slot capacity: grown from the last detection's need; an overflow is
-  // flagged on the device and the detection is run again with more room.
-  if (b.pair_cap == 0) b.pair_cap = std::max<uint64_t>(4ull * b.grid_cap + (1u << 20), 1u << 22);
-  b.pair_owner.reserve(b.pair_cap);
-  b.pair_list.reserve(b.pair_cap);
-  k_item_owner<<<blocks(NI), TPB, 0, st>>>(b.item_off.p, NI, b.pair_owner.p, b.pair_cap, g.d_scal);
-  CV_HIP(hipGetLastError());
-  k_pairs<<<8 * 256, TPB, 0, st>>>(
-      a, b.pair_owner.p, b.item_off.p, b.item_start.p, b.item_key.p, NI, b.pair_cap, b.pair_list.p);
-  CV_HIP(hipGetLastError());
-  k_narrow<<<8 * 256, TPB, 0, st>>>(a, b.item_off.p, NI, b.pair_cap, b.pair_list.p);
-  CV_HIP(hipGetLastError());
-  constexpr int G = 16;
-  k_ee_overflow<G><<<64, TPB, 0, st>>>(a, b.over.p, &g.d_scal->n_over);
-  CV_HIP(hipGetLastError());
+  uint32_t NI = 0;  // grid query items
+  if (g.broad_phase == 1 && !g.b.bvh_fallback) {
+    lbvh_broad_narrow(g, a, scale);
+  } else {
+    // Faces and edges registered with the padded inflation; points look up
+    // their own cell, edges the cells of their own box.
+    a.log2t = build_grid(g, d.F, d.E, pr_reg, h);
+    a.skey = b.cell_key_sorted.p;
+    a.sprim = b.cell_val_sorted.p;
+    a.sbox = b.cell_box_sorted.p;
+    a.bstart = b.bstart.p;
+    // Flattened broad phase: query items, their pair counts, owners, pairs.
+    NI = d.n_total + kEdgeCells * d.E;
+    b.item_len.reserve(NI + 1);
+    b.item_off.reserve(NI + 1);
+    b.item_start.reserve(NI);
+    b.item_key.reserve(NI);
+    b.over.reserve(std::max<uint32_t>(d.E, 1));
+    CV_HIP(hipMemsetAsync(b.item_len.p + NI, 0, sizeof(uint32_t), st));
+    CV_HIP(hipMemsetAsync(&g.d_scal->n_over, 0, sizeof(unsigned long long), st));
+    k_items<<<blocks(d.n_total + d.E), TPB, 0, st>>>(a, b.item_len.p, b.item_start.p, b.item_key.p, b.over.p,
+                                                     &g.d_scal->n_over);
+    CV_HIP(hipGetLastError());
+    scan_u32(g, b.item_len.p, b.item_off.p, (size_t)NI + 1);
+    // Pair-slot capacity: grown from the last detection's need; an overflow is
+    // flagged on the device and the detection is run again with more room.
+    if (b.pair_cap == 0) b.pair_cap = std::max<uint64_t>(4ull * b.grid_cap + (1u << 20), 1u << 22);
+    b.pair_owner.reserve(b.pair_cap);
+    b.pair_list.reserve(b.pair_cap);
+    k_item_owner<<<blocks(NI), TPB, 0, st>>>(b.item_off.p, NI, b.pair_owner.p, b.pair_cap, g.d_scal);
+    CV_HIP(hipGetLastError());
+    k_pairs<<<8 * 256, TPB, 0, st>>>(
+        a, b.pair_owner.p, b.item_off.p, b.item_start.p, b.item_key.p, NI, b.pair_cap, b.pair_list.p);
+    CV_HIP(hipGetLastError());
+    k_narrow<<<8 * 256, TPB, 0, st>>>(a, b.item_off.p, NI, b.pair_cap, b.pair_list.p);
+    CV_HIP(hipGetLastError());
+    constexpr int G = 16;
+    k_ee_overflow<G><<<64, TPB, 0, st>>>(a, b.over.p, &g.d_scal->n_over);
+    CV_HIP(hipGetLastError());
+  }
   read_scalars(g);
-  b.grid_last = g.h_scal->grid_entries;
+  if (g.h_scal->err_overflow & 16u) {
+    // An LBVH walk ran out of stack (a tree deeper than kBvhStack along one
+    // path) or out of its pop bound: this detection is redone with the grid.
+    g.b.bvh_fallback = true;
+    g.b.bvh_fallbacks++;
+    gpu_detect(g, s, rq, ogc, cand_cap);
+    g.b.bvh_fallback = false;
+    return;
+  }
+  if (g.broad_phase != 1 || g.b.bvh_fallback) b.grid_last = g.h_scal->grid_entries;
   if (g.h_scal->err_overflow & 12u) {
     // More grid registrations (8) or (query, entry) pairs (4) than room:
     // grow and detect again.

@@ -302,26 +302,9 @@ __device__ __forceinline__ void iter_tail(const IterTail& t) {
   iter_end(t.sc, c, t.it, t.threshold, 0.0, 1.0);
 }
 
-// Execution span of a profiled launch on the device's constant-rate clock:
-// span[0] holds the complement of the earliest block start (so a zero fill
-// initialises both ends for atomicMax), span[1] the last block's end.  Only
-// the first eight blocks (dispatched first, one per XCD) stamp the start: a
-// same-address atomic from every block at once would serialise and delay the
-// launch it measures; block ends are spread out in time.
-__device__ __forceinline__ void span_begin(unsigned long long* span) {
-  if (span && blockIdx.x < 8 && threadIdx.x == 0) atomicMax(&span[0], ~(unsigned long long)wall_clock64());
-}
-__device__ __forceinline__ void span_end(unsigned long long* span) {
-  if (!span) return;  // block-uniform
-  __syncthreads();
-  if (threadIdx.x == 0) atomicMax(&span[1], (unsigned long long)wall_clock64());
-}
-
 // One colour of one iteration.
 template <int L, class R>
-__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uint32_t count, IterTail tail,
-                                               unsigned long long* span) {
-  span_begin(span);
+__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uint32_t count, IterTail tail) {
   const int lane = threadIdx.x & (L - 1);
   // Vertices of a colour are in index order, so neighbouring groups share
   // neighbours: keep consecutive groups on one XCD's L2.
@@ -333,7 +316,6 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uin
     const uint32_t nc = a.vnc[off + grp];
     if (!*a.stop) solve_vertex<L, R>(a, vi, nc, lane);
   }
-  span_end(span);
   if (tail.sc) iter_tail(tail);  // block-uniform
 }
 
@@ -801,21 +783,21 @@ SweepArgs<R> sweep_args(cv_gpu& g, const cv_settings* set, const cv_grab* grab) 
 // summed time is kernel execution only, as a kernel trace reports it.
 template <int L, class R>
 void launch_sweep(const SweepArgs<R>& a, uint32_t off, uint32_t count, const IterTail& tail, hipStream_t st,
-                  hipEvent_t e0, hipEvent_t e1, unsigned long long* span) {
+                  hipEvent_t e0, hipEvent_t e1) {
   dim3 grid(blocks((uint64_t)count * L)), block(TPB);
   if (e0)
-    hipExtLaunchKernelGGL(k_sweep<L, R>, grid, block, 0, st, e0, e1, 0, a, off, count, tail, span);
+    hipExtLaunchKernelGGL(k_sweep<L, R>, grid, block, 0, st, e0, e1, 0, a, off, count, tail);
   else
-    k_sweep<L, R><<<grid, block, 0, st>>>(a, off, count, tail, span);
+    k_sweep<L, R><<<grid, block, 0, st>>>(a, off, count, tail);
 }
 template <class R>
 void sweep_color(cv_gpu& g, const SweepArgs<R>& a, uint32_t off, uint32_t count, const IterTail& tail = IterTail{},
-                 hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, unsigned long long* span = nullptr) {
+                 hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   switch (g.lanes) {
-    case 8: launch_sweep<8>(a, off, count, tail, g.stream, e0, e1, span); break;
-    case 16: launch_sweep<16>(a, off, count, tail, g.stream, e0, e1, span); break;
-    case 64: launch_sweep<64>(a, off, count, tail, g.stream, e0, e1, span); break;
-    default: launch_sweep<32>(a, off, count, tail, g.stream, e0, e1, span); break;
+    case 8: launch_sweep<8>(a, off, count, tail, g.stream, e0, e1); break;
+    case 16: launch_sweep<16>(a, off, count, tail, g.stream, e0, e1); break;
+    case 64: launch_sweep<64>(a, off, count, tail, g.stream, e0, e1); break;
+    default: launch_sweep<32>(a, off, count, tail, g.stream, e0, e1); break;
   }
 }
 
@@ -940,10 +922,10 @@ struct StepOut {
   uint32_t colors = 0, iterations_run = 0;
   uint64_t sweep_launches = 0, sweep_timed_launches = 0;
   double sweep_ms = 0, sweep_bytes = 0, sweep_bytes_min = 0, sweep_timed_bytes_min = 0;
-  double sweep_span_ms = 0;  // summed execution spans of the stamped launches (device clock)
-  uint64_t sweep_span_launches = 0;
+
   double detection_ms = 0, refit_ms = 0, candidates_ms = 0, narrow_ms = 0;
   uint64_t candidate_pairs = 0, grid_tests = 0;
+  uint64_t bvh_fallbacks = 0;  // LBVH detections redone with the grid
   bool true_residual = false;
   bool diagnostics = false;  // garment diagnostics computed after the commit
 };
@@ -959,7 +941,10 @@ void detect_and_color(cv_gpu& g, const cv_solver& s, const cv_settings* set, con
     ~Restore() { g.host_contacts = true; }
   } restore{g};
   g.host_contacts = !fast;  // the fast colouring reads the contacts on the device
+  if (first) g.b.bvh_rebuild = true;  // LBVH: new trees per step, refits at refreshes
+  const uint64_t fb = g.b.bvh_fallbacks;
   gpu_detect(g, s, rq, set->contact, 0);
+  out.bvh_fallbacks += g.b.bvh_fallbacks - fb;
   out.detections++;
   out.candidate_pairs += g.h_scal->vf_cand + g.h_scal->ee_cand;
   out.grid_tests += g.h_scal->grid_tests;
@@ -1048,10 +1033,8 @@ struct StepCtx {
   std::vector<Launch> launches;
   std::vector<uint32_t> timed_its;  // iterations whose launches carry events
   std::vector<size_t> pending_ev;   // completed timed launches not yet read
-  std::vector<unsigned long long> spans;  // host copy of the stamped spans
   size_t nev = 0;                   // event pairs used in this step
   static constexpr uint64_t kTimedStride = 4;
-  static constexpr size_t kMaxSpans = 4096;  // stamped launches per step
 
   StepCtx(cv_gpu& g_, cv_solver& s_, const cv_settings* set_, const cv_grab* grab_, StepOut& out_)
       : g(g_), s(s_), set(set_), grab(grab_), out(out_) {}
@@ -1101,10 +1084,6 @@ struct StepCtx {
     CV_HIP(hipGetLastError());
     a = sweep_args<R>(g, set, grab);
     iters = set->iterations;
-    if (g.profile) {
-      g.sweep_span.reserve(2 * kMaxSpans);
-      CV_HIP(hipMemsetAsync(g.sweep_span.p, 0, 2 * kMaxSpans * sizeof(unsigned long long), g.stream));
-    }
   }
   // Queues the scalar read-back that after_sync() consumes.
   void read_back() {
@@ -1117,24 +1096,12 @@ struct StepCtx {
     uint32_t ran = g.h_scal->iters_run;
     double inc = (double)g.h_scal->incidences;
     double it_bytes32 = g.static_bytes32 + 56.0 * inc;
-    size_t span_hi = 0;
-    for (const Launch& l : launches)
-      if (g.profile && l.ev != (size_t)-1 && l.ev < kMaxSpans) span_hi = std::max(span_hi, l.ev + 1);
-    if (span_hi) {  // the stream has drained: the stamped spans are final
-      spans.resize(2 * span_hi);
-      CV_HIP(hipMemcpy(spans.data(), g.sweep_span.p, 2 * span_hi * sizeof(unsigned long long),
-                       hipMemcpyDeviceToHost));
-    }
     for (const Launch& l : launches) {
       if (l.it >= ran) continue;
       out.sweep_launches++;
       if (g.profile && l.ev != (size_t)-1) {
         pending_ev.push_back(l.ev);  // read by drain_timing() once more work is queued
         out.sweep_timed_launches++;
-        if (l.ev < kMaxSpans && spans[2 * l.ev + 1]) {
-          out.sweep_span_ms += (double)(spans[2 * l.ev + 1] - ~spans[2 * l.ev]) / g.wall_khz;
-          out.sweep_span_launches++;
-        }
       }
     }
     for (uint32_t it : timed_its)
@@ -1192,8 +1159,7 @@ struct StepCtx {
             g.sweep_events.push_back(e0);
             g.sweep_events.push_back(e1);
           }
-          sweep_color<R>(g, a, off, cnt, tail, g.sweep_events[2 * nev], g.sweep_events[2 * nev + 1],
-                         nev < kMaxSpans ? g.sweep_span.p + 2 * nev : nullptr);
+          sweep_color<R>(g, a, off, cnt, tail, g.sweep_events[2 * nev], g.sweep_events[2 * nev + 1]);
           launches.push_back({nev++, it});
         } else {
           sweep_color<R>(g, a, off, cnt, tail);
@@ -1667,8 +1633,8 @@ void fill_metrics(cv_metrics* m, const cv_gpu& g, const cv_solver& s, const cv_s
   m->sweep_ms = g.profile ? o.sweep_ms : NAN;
   m->sweep_timed_launches = o.sweep_timed_launches;
   m->sweep_timed_bytes_min = o.sweep_timed_bytes_min;
-  m->sweep_kernel_ms = g.profile ? o.sweep_span_ms : NAN;
-  m->sweep_kernel_launches = o.sweep_span_launches;
+  m->sweep_event_overhead_ms = g.profile ? g.event_overhead_ms : NAN;
+  m->broad_phase_fallbacks = o.bvh_fallbacks;
   m->f64 = g.f64 ? 1 : 0;
   m->sweep_launches = o.sweep_launches;
   m->sweep_bytes = o.sweep_bytes;
@@ -1724,6 +1690,7 @@ void probe_t(cv_gpu& g, cv_solver& s, const cv_state* st, const cv_settings* set
       PBufs<R>::pos(g), PBufs<R>::tgt(g), g.b.guess64.p, g.b.x_old64.p, g.b.v_old64.p);
   double r = set->contact.radius;
   double rq = std::min(std::max(r + g.h_scal->motion, 2.0 * r), 4.0 * r);
+  g.b.bvh_rebuild = true;
   gpu_detect(g, s, rq, set->contact, 0);
   // Probe at the state's positions rounded to the sweep precision.
   using T4 = typename P4<R>::T;
@@ -1741,6 +1708,29 @@ void probe_t(cv_gpu& g, cv_solver& s, const cv_state* st, const cv_settings* set
   CV_HIP(hipStreamSynchronize(stream));
 }
 
+// The part of a launch's event interval (hipExtLaunchKernelGGL start/stop
+// events on the stream) that is not the kernel: the mean interval of
+// back-to-back empty launches on the same stream, after one untimed launch.
+__global__ void k_empty() {}
+void calibrate_event_overhead(cv_gpu& g) {
+  CV_HIP(hipSetDevice(g.device));
+  constexpr int K = 32;
+  std::vector<hipEvent_t> ev(2 * K);
+  for (auto& e : ev) CV_HIP(hipEventCreate(&e));
+  hipLaunchKernelGGL(k_empty, dim3(1), dim3(64), 0, g.stream);
+  for (int k = 0; k < K; ++k)
+    hipExtLaunchKernelGGL(k_empty, dim3(1), dim3(64), 0, g.stream, ev[2 * k], ev[2 * k + 1], 0);
+  CV_HIP(hipStreamSynchronize(g.stream));
+  double sum = 0;
+  for (int k = 0; k < K; ++k) {
+    float ms = 0;
+    CV_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+    sum += ms;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  g.event_overhead_ms = sum / K;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1756,9 +1746,7 @@ cv_status cv_gpu_create(int device, cv_gpu** out) {
     g->device = device;
     CV_HIP(hipSetDevice(device));
     warm_scratch_setup(device);
-    int khz = 0;
-    CV_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
-    g->wall_khz = khz > 0 ? (double)khz : 1e5;
+
     CV_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     CV_HIP(hipMalloc((void**)&g->d_scal, sizeof(Scalars)));
     CV_HIP(hipHostMalloc((void**)&g->h_scal, sizeof(Scalars)));
@@ -1815,6 +1803,13 @@ cv_status cv_warm_set_correction(cv_gpu* g, double scale, int normal_only) {
   return CV_OK;
 }
 
+cv_status cv_gpu_set_broad_phase(cv_gpu* g, int kind) {
+  if (!g || (kind != CV_BROAD_PHASE_GRID && kind != CV_BROAD_PHASE_LBVH)) return CV_ERR_INVALID_ARGUMENT;
+  g->broad_phase = kind;
+  g->b.bvh_rebuild = true;
+  return CV_OK;
+}
+
 cv_status cv_gpu_set_lanes(cv_gpu* g, int lanes) {
   if (!g || !(lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64)) return CV_ERR_INVALID_ARGUMENT;
   g->lanes = lanes;
@@ -1829,8 +1824,10 @@ cv_status cv_gpu_set_precision(cv_gpu* g, int bits) {
 
 cv_status cv_gpu_set_profiling(cv_gpu* g, int enabled) {
   if (!g) return CV_ERR_INVALID_ARGUMENT;
-  g->profile = enabled != 0;
-  return CV_OK;
+  return guard(&g->last_error, [&] {
+    g->profile = enabled != 0;
+    if (g->profile) calibrate_event_overhead(*g);
+  });
 }
 
 // Steps `steps` times; the state crosses PCIe once each way.  Each step has
@@ -1863,14 +1860,13 @@ static cv_status run_steps(cv_gpu* g, cv_solver* s, cv_state* st, const cv_setti
       }
       run_step(*g, *s, set, grab, o, k + 1 == steps, k > 0, k + 1 < steps);
       tot.detections += o.detections;
+      tot.bvh_fallbacks += o.bvh_fallbacks;
       tot.truncated += o.truncated;
       tot.iterations_run += o.iterations_run;
       tot.sweep_launches += o.sweep_launches;
       tot.sweep_ms += o.sweep_ms;
       tot.sweep_timed_launches += o.sweep_timed_launches;
       tot.sweep_timed_bytes_min += o.sweep_timed_bytes_min;
-      tot.sweep_span_ms += o.sweep_span_ms;
-      tot.sweep_span_launches += o.sweep_span_launches;
       tot.sweep_bytes += o.sweep_bytes;
       tot.sweep_bytes_min += o.sweep_bytes_min;
     }
@@ -1899,14 +1895,13 @@ static cv_status run_steps(cv_gpu* g, cv_solver* s, cv_state* st, const cv_setti
     if (steps > 1) {
       // Run totals (see cv_gpu_run).
       o.detections = tot.detections;
+      o.bvh_fallbacks = tot.bvh_fallbacks;
       o.truncated = tot.truncated;
       o.iterations_run = tot.iterations_run;
       o.sweep_launches = tot.sweep_launches;
       o.sweep_ms = tot.sweep_ms;
       o.sweep_timed_launches = tot.sweep_timed_launches;
       o.sweep_timed_bytes_min = tot.sweep_timed_bytes_min;
-      o.sweep_span_ms = tot.sweep_span_ms;
-      o.sweep_span_launches = tot.sweep_span_launches;
       o.sweep_bytes = tot.sweep_bytes;
       o.sweep_bytes_min = tot.sweep_bytes_min;
     }
@@ -1998,6 +1993,7 @@ cv_status cv_gpu_detect(cv_gpu* g, cv_solver* s, const double* x, double rq, con
     for (size_t i = 0; i < 3 * n; ++i) mabs = std::max(mabs, std::fabs(x[i]));
     g->h_scal->scale = mabs;
     size_t cap = std::max(vf_capacity, ee_capacity);
+    g->b.bvh_rebuild = true;
     gpu_detect(*g, *s, rq, c, (vf_pairs || ee_pairs) ? std::max<size_t>(cap, 1) : 0);
     const Scalars& hs = *g->h_scal;
     out->n_vf_candidates = hs.vf_cand;
@@ -2291,6 +2287,7 @@ cv_status cv_warm_features(cv_gpu* g, cv_solver* s, const cv_state* st, const cv
     CV_HIP(hipGetLastError());
     double r = set->contact.radius;
     double rq = std::min(std::max(r + g->h_scal->motion, 2.0 * r), 4.0 * r);
+    g->b.bvh_rebuild = true;
     gpu_detect(*g, *s, rq, set->contact, 0);
     warm_features(*g, *s, set, rq);
     CV_HIP(hipMemcpyAsync(out, g->feat.p, 12 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, g->stream));

@@ -53,6 +53,7 @@ def lib():
         L.cv_gpu_destroy.argtypes = [vp]
         L.cv_gpu_set_check_gradients.argtypes = [vp, C.c_int]
         L.cv_gpu_set_lanes.argtypes = [vp, C.c_int]
+        L.cv_gpu_set_broad_phase.argtypes = [vp, C.c_int]
         L.cv_gpu_set_profiling.argtypes = [vp, C.c_int]
         L.cv_gpu_set_precision.argtypes = [vp, C.c_int]
         L.cv_gpu_step.argtypes = [vp, vp, C.POINTER(abi.cv_state), C.POINTER(abi.cv_settings),
@@ -235,6 +236,18 @@ class GpuVbd:
         rc = lib().cv_gpu_set_lanes(self.h, lanes)
         if rc != 0:
             raise VbdError(rc, "lanes must be 8, 16, 32 or 64")
+
+    BROAD_PHASES = {"grid": 0, "lbvh": 1}
+
+    def set_broad_phase(self, kind: str):
+        """OGC broad phase: "grid" (default hashed grid) or "lbvh" (linear BVH
+        rebuilt per step, refitted at refreshes).  Both give the same
+        candidate pairs."""
+        if kind not in self.BROAD_PHASES:
+            raise ValueError(f"broad phase must be one of {sorted(self.BROAD_PHASES)}")
+        rc = lib().cv_gpu_set_broad_phase(self.h, self.BROAD_PHASES[kind])
+        if rc != 0:
+            raise VbdError(rc, "invalid broad phase")
 
     def set_precision(self, bits: int):
         """64 (default, reference parity) or 32 (faster f32 sweep)."""

@@ -182,10 +182,10 @@ typedef struct cv_metrics {
   uint64_t sweep_timed_launches;  /* launches in sweep_ms: every launch of every 4th
                                      iteration queued while profiling is on           */
   double sweep_timed_bytes_min;   /* sweep_bytes_min of those launches                */
-  double sweep_kernel_ms;         /* profiling on: summed execution spans of the same
-                                     launches on the device's constant clock (first
-                                     block's start to last block's end), else NaN    */
-  uint64_t sweep_kernel_launches; /* launches in sweep_kernel_ms                      */
+  double sweep_event_overhead_ms; /* profiling on: event interval of an empty launch on
+                                     the same stream (the dispatch latency inside each
+                                     timed launch's interval), else NaN              */
+  uint64_t broad_phase_fallbacks; /* LBVH detections redone with the grid          */
 } cv_metrics;
 
 /* ------------------------------------------------------------- solver */
@@ -383,8 +383,22 @@ cv_status cv_gpu_set_precision(cv_gpu* gpu, int bits);
 
 /* Begin/end events around the sweep launches of every 4th iteration;
  * metrics.sweep_ms reports their summed duration, sweep_timed_launches their
- * count and sweep_timed_bytes_min their algorithmic bytes. */
+ * count and sweep_timed_bytes_min their algorithmic bytes.  Enabling it
+ * measures the event interval of an empty launch on the same stream once
+ * (metrics.sweep_event_overhead_ms), the part of each timed interval that is
+ * dispatch rather than kernel. */
 cv_status cv_gpu_set_profiling(cv_gpu* gpu, int enabled);
+
+/* Broad phase of the OGC detection: CV_BROAD_PHASE_GRID (default; a hashed
+ * uniform grid of cell size max(cell, 2 padded radii), the candidate walk of
+ * src/ogc/world.rs:407-470) or CV_BROAD_PHASE_LBVH (a linear BVH over the
+ * same f64 boxes, rebuilt at each step's first detection and refitted at its
+ * refreshes; a walk deeper than its stack redoes that detection with the
+ * grid, counted in metrics.broad_phase_fallbacks).  Both produce the same
+ * candidate pairs; only their cost differs. */
+#define CV_BROAD_PHASE_GRID 0
+#define CV_BROAD_PHASE_LBVH 1
+cv_status cv_gpu_set_broad_phase(cv_gpu* gpu, int kind);
 
 /* Lanes per vertex of the sweep kernel (8, 16, 32 or 64); the autotuner sets it. */
 cv_status cv_gpu_set_lanes(cv_gpu* gpu, int lanes);

@@ -204,7 +204,19 @@ def _contact_scene():
     return sc, x
 
 
-def test_detection_candidates_and_contacts_are_bit_exact(gpu):
+@pytest.mark.parametrize("broad", ["grid", "lbvh"])
+def test_detection_candidates_and_contacts_are_bit_exact(gpu, broad):
+    """Both broad phases (hashed grid, LBVH) give the oracle's pruned
+    candidate pairs and its contacts, bit for bit."""
+    from clothcraft_amd import OgcSettings, VbdSolver
+    gpu.set_broad_phase(broad)
+    try:
+        _detection_bit_exact(gpu)
+    finally:
+        gpu.set_broad_phase("grid")
+
+
+def _detection_bit_exact(gpu):
     from clothcraft_amd import OgcSettings, VbdSolver
     sc, x = _contact_scene()
     st = scenes.State(np.ascontiguousarray(x), np.zeros_like(x))
