@@ -70,7 +70,8 @@ class cv_ogc_settings(C.Structure):
 class cv_settings(C.Structure):
     _fields_ = [("dt", C.c_double), ("iterations", C.c_uint32), ("damping", C.c_double),
                 ("gravity", C.c_double * 3), ("contact", cv_ogc_settings),
-                ("residual_tolerance", C.c_double), ("flags", C.c_uint32)]
+                ("residual_tolerance", C.c_double), ("flags", C.c_uint32),
+                ("residual_interval", C.c_uint32)]
 
 
 class cv_grab(C.Structure):

@@ -132,8 +132,8 @@ struct DevSolver {
 // One linear BVH (ogc_device.hip): Morton keys, leaf order, Karras tree, boxes.
 struct BvhBufs {
   DBuf<unsigned long long> key, key_sorted;
-  DBuf<uint32_t> idx, prim, flag;
-  DBuf<int32_t> child, parent;
+  DBuf<uint32_t> idx, prim, flag, top;
+  DBuf<int32_t> child, parent, range;
   DBuf<double> box;
   uint32_t N = 0;
 };
@@ -156,7 +156,8 @@ struct StepBufs {
   // detection
   DBuf<double> fbox, ebox;  // [6F], [6E]
   BvhBufs bvh_faces, bvh_edges;          // LBVH broad phase (cv_gpu_set_broad_phase 1)
-  DBuf<uint32_t> vf_cnt, vf_off, ee_cnt, ee_off;
+  BvhBufs bvh_points;                    // points in Morton order (the VF walk's query order)
+  uint64_t bvh_cap_vf = 0, bvh_cap_ee = 0;  // candidate list regions of the LBVH walks
   bool bvh_rebuild = true;               // rebuild (not refit) at the next LBVH detection
   bool bvh_fallback = false;             // this detection runs the grid (an LBVH walk overflowed)
   uint64_t bvh_fallbacks = 0;

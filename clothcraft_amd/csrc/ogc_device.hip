@@ -672,7 +672,7 @@ __device__ __forceinline__ int bvh_delta(const unsigned long long* k, int N, int
   if (a == b) return 64 + __clz((uint32_t)(i ^ j));
   return __clzll((long long)(a ^ b));
 }
-__global__ void k_karras(const unsigned long long* k, int N, int32_t* child, int32_t* parent) {
+__global__ void k_karras(const unsigned long long* k, int N, int32_t* child, int32_t* parent, int32_t* range) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) parent[0] = -1;
   if (i >= N - 1) return;
@@ -698,128 +698,203 @@ __global__ void k_karras(const unsigned long long* k, int N, int32_t* child, int
   child[2 * i + 1] = right;
   parent[left] = i;
   parent[right] = i;
+  range[2 * i] = min(i, j);  // the node's leaves (sorted order)
+  range[2 * i + 1] = max(i, j);
 }
-__device__ __forceinline__ double ld_coherent(const double* p) {
-  return __builtin_bit_cast(double, __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT));
+// Refit: node boxes as unions of their children, in two launches that need
+// only workgroup-scope ordering (device-scope fences per tree level would
+// write back the L2 each time).  k_refit_chunks: one block per kRefitChunk
+// consecutive sorted leaves writes the leaf boxes and refits every internal
+// node whose leaf range lies inside its chunk, bottom up: the second of the
+// two children to finish (arrival count in flag[], workgroup-scope atomics)
+// forms the union.  A node whose parent's range leaves the chunk is listed
+// in `top`.  k_refit_top: one block climbs from the listed nodes through the
+// nodes that span chunks, the same way.  flag[] (N - 1 zeros) and *n_top
+// (zero) are cleared before the first launch.
+constexpr int kRefitChunk = 1024;
+__device__ __forceinline__ void union_children(const int32_t* child, int par, double* box) {
+  const double* a = box + 6 * (size_t)child[2 * par];
+  const double* b = box + 6 * (size_t)child[2 * par + 1];
+  double* o = box + 6 * (size_t)par;
+  for (int k = 0; k < 3; ++k) {
+    o[k] = fmin(a[k], b[k]);
+    o[3 + k] = fmax(a[3 + k], b[3 + k]);
+  }
 }
-// Leaf boxes, then unions up the tree; flag[] (N - 1 zeros) counts arrivals.
-__global__ void k_refit(const double* pbox, const uint32_t* prim, const int32_t* child, const int32_t* parent,
-                        uint32_t N, uint32_t* flag, double* box) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= N) return;
-  int node = (int)(N - 1 + t);
+__device__ __forceinline__ bool second_arrival(uint32_t* flag) {
+  return __hip_atomic_fetch_add(flag, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+}
+__global__ __launch_bounds__(kRefitChunk) void k_refit_chunks(const double* pbox, const uint32_t* prim,
+                                                              const int32_t* child, const int32_t* parent,
+                                                              const int32_t* range, uint32_t N, uint32_t* flag,
+                                                              double* box, uint32_t* top, uint32_t* n_top) {
+  const uint32_t t = blockIdx.x * kRefitChunk + threadIdx.x;
+  if (t >= N) return;  // no block barriers below
+  const int lo = (int)(blockIdx.x * kRefitChunk), hi = (int)min(N, (blockIdx.x + 1) * kRefitChunk) - 1;
+  int cur = (int)(N - 1 + t);
   const double* src = pbox + 6 * (size_t)prim[t];
-  for (int k = 0; k < 6; ++k) box[6 * (size_t)node + k] = src[k];
-  int par = parent[node];
-  // A path to the root has at most N - 1 internal nodes; the bound only
-  // guards the loop against a malformed tree.
-  for (uint32_t up = 0; par >= 0 && up < N; ++up) {
-    __threadfence();
-    if (atomicAdd(&flag[par], 1u) == 0) return;  // the sibling's thread completes the parent
-    __threadfence();
-    const double* a = box + 6 * (size_t)child[2 * par];
-    const double* b = box + 6 * (size_t)child[2 * par + 1];
-    double* o = box + 6 * (size_t)par;
-    for (int k = 0; k < 3; ++k) {
-      o[k] = fmin(ld_coherent(a + k), ld_coherent(b + k));
-      o[3 + k] = fmax(ld_coherent(a + 3 + k), ld_coherent(b + 3 + k));
+  for (int k = 0; k < 6; ++k) box[6 * (size_t)cur + k] = src[k];
+  for (uint32_t up = 0; up < N; ++up) {  // a path has at most N - 1 internal nodes
+    const int par = parent[cur];
+    if (par < 0) return;  // the root (a tree within one chunk)
+    if (range[2 * par] < lo || range[2 * par + 1] > hi) {
+      top[atomicAdd(n_top, 1u)] = (uint32_t)cur;
+      return;
     }
-    par = parent[par];
+    if (!second_arrival(&flag[par])) return;
+    union_children(child, par, box);
+    cur = par;
   }
 }
-// Vertex-face walk: every point (moving and avatar) against the face tree.
-template <bool WRITE>
-__global__ __launch_bounds__(TPB) void k_bvh_vf(DetectArgs a, Bvh t, uint32_t* count, const uint32_t* off,
-                                                uint2* list) {
-  PairCounts cnt;
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < a.n_total) {
-    const D3 p = ld3(a.x, q);
-    const ogm::Box pb = ogm::box_of(p);
-    const bool moving = q < a.n, ex = a.ex_off[q] != a.ex_off[q + 1];
-    uint32_t c = 0, base = WRITE ? off[q] : 0;
-    int stack[kBvhStack];
-    int sp = 0;
-    stack[sp++] = 0;
-    // A tree's walk pops each of its 2N - 1 nodes at most once; more pops
-    // (a malformed tree) end the walk like a stack overflow.
-    for (uint32_t pops = 0; sp; ++pops) {
-      if (pops >= 2 * t.N) {
-        atomicOr(&a.sc->err_overflow, 16u);
-        break;
-      }
-      const int nd = stack[--sp];
-      const ogm::Box nb = ldbox(t.box, (uint32_t)nd);
-      if (nd >= (int)t.N - 1) {
-        const uint32_t f = t.prim[nd - ((int)t.N - 1)];
-        if (vf_filter(a, q, moving, ex, pb, f, nb, cnt)) {
-          if (WRITE) list[base + c] = make_uint2(q, f);
-          ++c;
-        }
-        continue;
-      }
-      ++cnt.tests;
-      if (ogm::box_dist2(pb, nb) > a.pr2) continue;
-      if (sp + 2 > kBvhStack) {
-        atomicOr(&a.sc->err_overflow, 16u);
-        break;
-      }
-      stack[sp++] = t.child[2 * nd];
-      stack[sp++] = t.child[2 * nd + 1];
+__global__ __launch_bounds__(kRefitChunk) void k_refit_top(const int32_t* child, const int32_t* parent, uint32_t N,
+                                                           uint32_t* flag, double* box, const uint32_t* top,
+                                                           const uint32_t* n_top) {
+  const uint32_t m = *n_top;
+  for (uint32_t k = threadIdx.x; k < m; k += kRefitChunk) {
+    int cur = (int)top[k];
+    for (uint32_t up = 0; up < N; ++up) {
+      const int par = parent[cur];
+      if (par < 0) break;
+      if (!second_arrival(&flag[par])) break;
+      union_children(child, par, box);
+      cur = par;
     }
-    if (!WRITE) count[q] = c;
   }
-  if (WRITE) add_counts(a, cnt);  // every lane reaches it
 }
-// Edge-edge walk: every moving edge against the edge tree.
-template <bool WRITE>
-__global__ __launch_bounds__(TPB) void k_bvh_ee(DetectArgs a, Bvh t, uint32_t* count, const uint32_t* off,
-                                                uint2* list) {
+// Morton codes of points (the VF walk's query order).
+__global__ void k_morton_points(const double* x, uint32_t N, double lo, double scale21, unsigned long long* key,
+                                uint32_t* idx) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  unsigned long long q[3];
+  for (int k = 0; k < 3; ++k) q[k] = (unsigned long long)fmin(fmax((x[3 * (size_t)i + k] - lo) * scale21, 0.0), 2097151.0);
+  key[i] = spread21(q[0]) << 2 | spread21(q[1]) << 1 | spread21(q[2]);
+  idx[i] = i;
+}
+
+// Per-wavefront candidate staging of the LBVH walks: a lane with a candidate
+// appends it to its wavefront's LDS buffer (one LDS atomic per wavefront and
+// step), and a wavefront whose buffer could overflow on its next step writes
+// the buffer out with one global atomic.  The walks' loops are
+// wavefront-uniform and need no block barrier.  Entries beyond `cap` are
+// counted but not written: the capacity flag (32) makes the host grow the
+// region and detect again.
+constexpr int kWaveBuf = 512;
+struct WaveStage {
+  uint2 buf[kWaveBuf];
+  uint32_t n;
+};
+__device__ __forceinline__ void wave_stage(WaveStage& ws, bool take, uint32_t q, uint32_t p) {
+  const uint64_t mask = __ballot(take);
+  if (!mask) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t base = ws.n;  // wavefront-uniform
+  if (take) ws.buf[base + __popcll(mask & ((1ull << lane) - 1))] = make_uint2(q, p);
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) ws.n = base + (uint32_t)__popcll(mask);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ void wave_flush(WaveStage& ws, unsigned long long* total, uint64_t cap, uint2* region,
+                                           Scalars* sc) {
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const uint32_t n = ws.n, lane = threadIdx.x & 63;
+  if (!n) return;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(total, (unsigned long long)n);
+  base = __shfl(base, 0, 64);
+  for (uint32_t k = lane; k < n; k += 64)
+    if (base + k < cap) region[base + k] = ws.buf[k];
+  if (lane == 0 && base + n > cap) atomicOr(&sc->err_overflow, 32u);
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) ws.n = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// One LBVH walk per query, queries in Morton order (`order`), so the lanes
+// of a wavefront descend the same subtrees.  A popped internal node is known
+// to be within reach; its children are tested before they are pushed, and a
+// popped leaf goes through the pair filter.  KIND 0: points against the face
+// tree (vertex-face); KIND 1: moving edges against the edge tree (edge-edge).
+// Every wavefront runs the loop until all its lanes are done.
+template <int KIND>
+__global__ __launch_bounds__(TPB) void k_bvh_walk(DetectArgs a, Bvh t, const uint32_t* order, uint32_t nq,
+                                                  uint64_t cap, uint2* region) {
+  __shared__ WaveStage stage[TPB / 64];
+  WaveStage& ws = stage[threadIdx.x >> 6];
+  if ((threadIdx.x & 63) == 0) ws.n = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
   PairCounts cnt;
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < a.E) {
-    uint32_t c = 0, base = WRITE ? off[e] : 0;
-    if (a.edge_moving[e]) {
-      const ogm::Box be = ldbox(a.ebox, e);
-      const uint32_t ia = a.edges[2 * e], ib = a.edges[2 * e + 1];
-      const bool ia_ex = a.ex_off[ia] != a.ex_off[ia + 1], ib_ex = a.ex_off[ib] != a.ex_off[ib + 1];
-      int stack[kBvhStack];
-      int sp = 0;
+  const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t q = 0;
+  ogm::Box qb{};
+  bool moving = false, ex = false;
+  uint32_t ia = 0, ib = 0;
+  bool ia_ex = false, ib_ex = false;
+  int stack[kBvhStack];
+  int sp = 0;
+  if (qi < nq && t.N) {
+    q = order[qi];
+    if (KIND == 0) {
+      qb = ogm::box_of(ld3(a.x, q));
+      moving = q < a.n;
+      ex = a.ex_off[q] != a.ex_off[q + 1];
       stack[sp++] = 0;
-      for (uint32_t pops = 0; sp; ++pops) {
-        if (pops >= 2 * t.N) {
-          atomicOr(&a.sc->err_overflow, 16u);
-          break;
-        }
-        const int nd = stack[--sp];
-        const ogm::Box nb = ldbox(t.box, (uint32_t)nd);
-        if (nd >= (int)t.N - 1) {
-          const uint32_t f = t.prim[nd - ((int)t.N - 1)];
-          if (ee_filter(a, e, be, ia, ib, ia_ex, ib_ex, f, nb, kNoCell, cnt)) {
-            if (WRITE) list[base + c] = make_uint2(e, f);
-            ++c;
+    } else if (a.edge_moving[q]) {
+      qb = ldbox(a.ebox, q);
+      ia = a.edges[2 * q];
+      ib = a.edges[2 * q + 1];
+      ia_ex = a.ex_off[ia] != a.ex_off[ia + 1];
+      ib_ex = a.ex_off[ib] != a.ex_off[ib + 1];
+      stack[sp++] = 0;
+    }
+  }
+  const int leaf0 = (int)t.N - 1;
+  // A walk pops each of the tree's 2N - 1 nodes at most once; more pops (a
+  // malformed tree) or a full stack end it with the fallback flag (16).
+  uint32_t pops = 0;
+  while (__ballot(sp > 0)) {
+    bool take = false;
+    uint32_t prim = 0;
+    if (sp > 0) {
+      const int nd = stack[--sp];
+      if (++pops > 2 * t.N) {
+        atomicOr(&a.sc->err_overflow, 16u);
+        sp = 0;
+      } else if (nd >= leaf0) {
+        prim = t.prim[nd - leaf0];
+        const ogm::Box lb = ldbox(t.box, (uint32_t)nd);
+        take = KIND == 0 ? vf_filter(a, q, moving, ex, qb, prim, lb, cnt)
+                         : ee_filter(a, q, qb, ia, ib, ia_ex, ib_ex, prim, lb, kNoCell, cnt);
+      } else {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const int ch = t.child[2 * nd + c];
+          ++cnt.tests;
+          if (ogm::box_dist2(qb, ldbox(t.box, (uint32_t)ch)) > a.pr2) continue;
+          if (sp == kBvhStack) {
+            atomicOr(&a.sc->err_overflow, 16u);
+            sp = 0;
+            break;
           }
-          continue;
+          stack[sp++] = ch;
         }
-        ++cnt.tests;
-        if (ogm::box_dist2(be, nb) > a.pr2) continue;
-        if (sp + 2 > kBvhStack) {
-          atomicOr(&a.sc->err_overflow, 16u);
-          break;
-        }
-        stack[sp++] = t.child[2 * nd];
-        stack[sp++] = t.child[2 * nd + 1];
       }
     }
-    if (!WRITE) count[e] = c;
+    wave_stage(ws, take, q, prim);
+    if (ws.n > kWaveBuf - 64) wave_flush(ws, KIND == 0 ? &a.sc->vf_list_n : &a.sc->ee_list_n, cap, region, a.sc);
   }
-  if (WRITE) add_counts(a, cnt);
+  wave_flush(ws, KIND == 0 ? &a.sc->vf_list_n : &a.sc->ee_list_n, cap, region, a.sc);
+  add_counts(a, cnt);  // every lane reaches it
 }
-// Narrow pass over the two candidate lists of the LBVH walk.
-__global__ __launch_bounds__(TPB) void k_narrow_lists(DetectArgs a, const uint2* vf, uint64_t nvf, const uint2* ee,
-                                                      uint64_t nee) {
+// Narrow pass over the two candidate regions of the LBVH walks (the counts
+// are the device's; a region that overflowed is redone by the host).
+__global__ __launch_bounds__(TPB) void k_narrow_lists(DetectArgs a, const uint2* vf, uint64_t cap_vf, const uint2* ee,
+                                                      uint64_t cap_ee) {
   PairCounts cnt;
+  const uint64_t nvf = min((uint64_t)a.sc->vf_list_n, cap_vf), nee = min((uint64_t)a.sc->ee_list_n, cap_ee);
   const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nvf + nee; j += step) {
     if (j < nvf) {
@@ -1126,9 +1201,11 @@ void lbvh_tree(cv_gpu& g, BvhBufs& t, const double* pbox, uint32_t N, double sca
     t.idx.reserve(N);
     t.prim.reserve(N);
     t.child.reserve(2 * (size_t)std::max<uint32_t>(N - 1, 1));
+    t.range.reserve(2 * (size_t)std::max<uint32_t>(N - 1, 1));
     t.parent.reserve(2 * (size_t)N);
     t.box.reserve(6 * (2 * (size_t)N));
-    t.flag.reserve(std::max<uint32_t>(N - 1, 1));
+    t.flag.reserve(std::max<uint32_t>(N - 1, 1) + 1);  // arrival counts, then the top-list count
+    t.top.reserve(2 * (size_t)N);
     const double half = scale > 0 ? scale : 1.0;  // box centres lie in [-scale, scale]^3
     k_morton<<<blocks(N), TPB, 0, st>>>(pbox, N, -half, 2097152.0 / (2.0 * half), t.key.p, t.idx.p);
     CV_HIP(hipGetLastError());
@@ -1136,12 +1213,20 @@ void lbvh_tree(cv_gpu& g, BvhBufs& t, const double* pbox, uint32_t N, double sca
       return rocprim::radix_sort_pairs(tmp, bytes, t.key.p, t.key_sorted.p, t.idx.p, t.prim.p, (size_t)N, 0, 63,
                                        st);
     });
-    k_karras<<<blocks(std::max<uint32_t>(N - 1, 1)), TPB, 0, st>>>(t.key_sorted.p, (int)N, t.child.p, t.parent.p);
+    k_karras<<<blocks(std::max<uint32_t>(N - 1, 1)), TPB, 0, st>>>(t.key_sorted.p, (int)N, t.child.p, t.parent.p,
+                                                                  t.range.p);
     CV_HIP(hipGetLastError());
   }
-  if (N > 1) CV_HIP(hipMemsetAsync(t.flag.p, 0, (N - 1) * sizeof(uint32_t), st));
-  k_refit<<<blocks(N), TPB, 0, st>>>(pbox, t.prim.p, t.child.p, t.parent.p, N, t.flag.p, t.box.p);
+  uint32_t* n_top = t.flag.p + std::max<uint32_t>(N - 1, 1);
+  CV_HIP(hipMemsetAsync(t.flag.p, 0, (std::max<uint32_t>(N - 1, 1) + 1) * sizeof(uint32_t), st));
+  const unsigned nb = (N + kRefitChunk - 1) / kRefitChunk;
+  k_refit_chunks<<<nb, kRefitChunk, 0, st>>>(pbox, t.prim.p, t.child.p, t.parent.p, t.range.p, N, t.flag.p, t.box.p,
+                                             t.top.p, n_top);
   CV_HIP(hipGetLastError());
+  if (nb > 1) {
+    k_refit_top<<<1, kRefitChunk, 0, st>>>(t.child.p, t.parent.p, N, t.flag.p, t.box.p, t.top.p, n_top);
+    CV_HIP(hipGetLastError());
+  }
 }
 
 void lbvh_broad_narrow(cv_gpu& g, const DetectArgs& a, double scale) {
@@ -1152,34 +1237,36 @@ void lbvh_broad_narrow(cv_gpu& g, const DetectArgs& a, double scale) {
   b.bvh_rebuild = false;
   lbvh_tree(g, b.bvh_faces, b.fbox.p, d.F, scale, rebuild);
   lbvh_tree(g, b.bvh_edges, b.ebox.p, d.E, scale, rebuild);
+  BvhBufs& pts = b.bvh_points;
+  if (rebuild || pts.N != d.n_total) {
+    // Points in Morton order of their positions at the step's first detection.
+    const uint32_t N = d.n_total;
+    pts.N = N;
+    pts.key.reserve(N);
+    pts.key_sorted.reserve(N);
+    pts.idx.reserve(N);
+    pts.prim.reserve(N);
+    const double half = scale > 0 ? scale : 1.0;
+    k_morton_points<<<blocks(N), TPB, 0, st>>>(a.x, N, -half, 2097152.0 / (2.0 * half), pts.key.p, pts.idx.p);
+    CV_HIP(hipGetLastError());
+    with_temp(g, [&](void* tmp, size_t& bytes) {
+      return rocprim::radix_sort_pairs(tmp, bytes, pts.key.p, pts.key_sorted.p, pts.idx.p, pts.prim.p, (size_t)N,
+                                       0, 63, st);
+    });
+  }
   Bvh tf{b.bvh_faces.box.p, b.bvh_faces.child.p, b.bvh_faces.prim.p, d.F};
   Bvh te{b.bvh_edges.box.p, b.bvh_edges.child.p, b.bvh_edges.prim.p, d.E};
-  DetectArgs counting = a;  // the counting walk exports nothing
-  counting.cand_vf = counting.cand_ee = nullptr;
-  b.vf_cnt.reserve(d.n_total + 1);
-  b.vf_off.reserve(d.n_total + 1);
-  b.ee_cnt.reserve(d.E + 1);
-  b.ee_off.reserve(d.E + 1);
-  // A walk over an empty tree finds nothing: its counts are zeros.
-  CV_HIP(hipMemsetAsync(b.vf_cnt.p + (d.F ? d.n_total : 0), 0, (d.F ? 1 : d.n_total + 1) * sizeof(uint32_t), st));
-  CV_HIP(hipMemsetAsync(b.ee_cnt.p + d.E, 0, sizeof(uint32_t), st));
-  if (d.F) k_bvh_vf<false><<<blocks(d.n_total), TPB, 0, st>>>(counting, tf, b.vf_cnt.p, nullptr, nullptr);
-  if (d.E) k_bvh_ee<false><<<blocks(d.E), TPB, 0, st>>>(counting, te, b.ee_cnt.p, nullptr, nullptr);
-  CV_HIP(hipGetLastError());
-  scan_u32(g, b.vf_cnt.p, b.vf_off.p, (size_t)d.n_total + 1);
-  scan_u32(g, b.ee_cnt.p, b.ee_off.p, (size_t)d.E + 1);
-  uint32_t tot[2] = {0, 0};
-  CV_HIP(hipMemcpyAsync(&tot[0], b.vf_off.p + d.n_total, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  CV_HIP(hipMemcpyAsync(&tot[1], b.ee_off.p + d.E, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  CV_HIP(hipStreamSynchronize(st));
-  const uint64_t nvf = tot[0], nee = tot[1];
-  b.pair_list.reserve(std::max<uint64_t>(nvf + nee, 1));
+  // Candidate regions: VF [0, cap_vf), EE [cap_vf, cap_vf + cap_ee), sized
+  // from the last detection's counts (grown and redone on overflow).
+  if (b.bvh_cap_vf == 0) b.bvh_cap_vf = 8ull * d.n_total + (1u << 20);
+  if (b.bvh_cap_ee == 0) b.bvh_cap_ee = 8ull * d.E + (1u << 20);
+  b.pair_list.reserve(b.bvh_cap_vf + b.bvh_cap_ee);
   uint2* vf = b.pair_list.p;
-  uint2* ee = b.pair_list.p + nvf;
-  if (d.F) k_bvh_vf<true><<<blocks(d.n_total), TPB, 0, st>>>(a, tf, nullptr, b.vf_off.p, vf);
-  if (d.E) k_bvh_ee<true><<<blocks(d.E), TPB, 0, st>>>(a, te, nullptr, b.ee_off.p, ee);
+  uint2* ee = b.pair_list.p + b.bvh_cap_vf;
+  if (d.F) k_bvh_walk<0><<<blocks(d.n_total), TPB, 0, st>>>(a, tf, pts.prim.p, d.n_total, b.bvh_cap_vf, vf);
+  if (d.E) k_bvh_walk<1><<<blocks(d.E), TPB, 0, st>>>(a, te, b.bvh_edges.prim.p, d.E, b.bvh_cap_ee, ee);
   CV_HIP(hipGetLastError());
-  k_narrow_lists<<<8 * 256, TPB, 0, st>>>(a, vf, nvf, ee, nee);
+  k_narrow_lists<<<8 * 256, TPB, 0, st>>>(a, vf, b.bvh_cap_vf, ee, b.bvh_cap_ee);
   CV_HIP(hipGetLastError());
 }
 
@@ -1297,6 +1384,15 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
     g.b.bvh_fallbacks++;
     gpu_detect(g, s, rq, ogc, cand_cap);
     g.b.bvh_fallback = false;
+    return;
+  }
+  if (g.h_scal->err_overflow & 32u) {
+    // An LBVH candidate region was too small: grow it to the count the walks
+    // found and detect again.
+    const Scalars& hs = *g.h_scal;
+    if (hs.vf_list_n > b.bvh_cap_vf) b.bvh_cap_vf = hs.vf_list_n + hs.vf_list_n / 4 + 4096;
+    if (hs.ee_list_n > b.bvh_cap_ee) b.bvh_cap_ee = hs.ee_list_n + hs.ee_list_n / 4 + 4096;
+    gpu_detect(g, s, rq, ogc, cand_cap);
     return;
   }
   if (g.broad_phase != 1 || g.b.bvh_fallback) b.grid_last = g.h_scal->grid_entries;

@@ -405,6 +405,66 @@ __global__ __launch_bounds__(TPB) void k_true_grad(SweepArgs<R> a, uint32_t n, d
     out[v] = (double)cvm::dot(g, g);
   }
 }
+// The true residual of an iteration in two launches: k_true_resid sums the
+// |g_v|^2 of each block's vertices in a fixed order into part[block];
+// k_resid_end sums the partials in block order into sc->true_residual_sum
+// and, with `end`, runs the iteration's end (iter_end).  With `end`, a
+// pending refresh or a converged step (the stop flag, set before the
+// launches) makes both no-ops, as it makes the sweeps: the residual would not
+// be read.  (A last-block ticket would save the second launch but needs a
+// device-scope release per block, an L2 write-back each on this GPU.)
+struct ResidEnd {
+  int end;
+  int it;
+  double threshold, tol, nfree3;
+};
+template <int L, class R>
+__global__ __launch_bounds__(TPB) void k_true_resid(SweepArgs<R> a, uint32_t n, double* part, int honor_stop) {
+  using T4 = typename P4<R>::T;
+  if (honor_stop && *a.stop) return;  // grid-uniform
+  constexpr int G = TPB / L;
+  __shared__ double red[G];
+  const int lane = threadIdx.x & (L - 1);
+  const uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / L;
+  double gg = 0.0;
+  if (v < n) {
+    T4 p4 = a.pos[v];
+    if (p4.w <= R(0.5)) {
+      V3<R> x{p4.x, p4.y, p4.z};
+      uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
+      uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
+      const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt);
+      cvm::Deriv<R> acc = local_deriv<L, R, false>(a, lt, v, x, lane, s0, ns, c0, nt);
+      T4 tg = a.tgt[v];
+      V3<R> g = acc.g + (x - V3<R>{tg.x, tg.y, tg.z}) * tg.w;
+      gg = (double)cvm::dot(g, g);
+    }
+  }
+  if (lane == 0) red[threadIdx.x / L] = gg;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+    for (int k = 0; k < G; ++k) sum += red[k];
+    part[blockIdx.x] = sum;
+  }
+}
+__global__ __launch_bounds__(TPB) void k_resid_end(const double* part, uint32_t np, Scalars* sc, ResidEnd e) {
+  if (e.end && sc->stop) return;
+  __shared__ double red[TPB];
+  double sum = 0.0;
+  for (uint32_t k = threadIdx.x; k < np; k += TPB) sum += part[k];
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int k = TPB / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sc->true_residual_sum = red[0];
+    if (e.end) iter_end(sc, sc->clipped, e.it, e.threshold, e.tol, e.nfree3);
+  }
+}
+
 // Deterministic two-level sum: block b of k_sum_part sums the contiguous
 // chunk [b c, (b + 1) c) in a fixed order; k_sum_final sums the partials in
 // block order.
@@ -916,6 +976,20 @@ template <class R>
 void true_residual(cv_gpu& g, const SweepArgs<R>& a, uint32_t n) {
   true_residual<R>(g, a, n, g.stream);
 }
+// The two-launch form on the solver's stream: residual, sum and (with
+// e.end) the iteration's end.
+template <class R>
+void true_residual_fused(cv_gpu& g, const SweepArgs<R>& a, uint32_t n, ResidEnd e) {
+  const int L = g.lanes == 8 ? 8 : 16;
+  const unsigned nb = blocks((uint64_t)std::max<uint32_t>(n, 1) * L);
+  g.b.sum_parts.reserve(nb);
+  if (L == 8)
+    k_true_resid<8, R><<<nb, TPB, 0, g.stream>>>(a, n, g.b.sum_parts.p, e.end);
+  else
+    k_true_resid<16, R><<<nb, TPB, 0, g.stream>>>(a, n, g.b.sum_parts.p, e.end);
+  k_resid_end<<<1, TPB, 0, g.stream>>>(g.b.sum_parts.p, nb, g.d_scal, e);
+  CV_HIP(hipGetLastError());
+}
 
 struct StepOut {
   uint64_t detections = 0, truncated = 0;
@@ -1179,9 +1253,18 @@ struct StepCtx {
     }
     pending_ev.clear();
   }
+  // With a tolerance the residual is evaluated after sweeps k, 2k, ... and
+  // the last (settings.residual_interval = k).
+  bool checks(uint32_t it) const {
+    const uint32_t every = std::max<uint32_t>(set->residual_interval, 1);
+    return tol > 0.0 && ((it + 1) % every == 0 || it + 1 == iters);
+  }
   void end_iteration(uint32_t it) {
-    if (tol > 0.0) true_residual<R>(g, a, s.n);
-    k_iter_end<<<1, 1, 0, g.stream>>>(g.d_scal, (int)it, threshold, tol, nfree3);
+    if (checks(it)) {
+      true_residual_fused<R>(g, a, s.n, ResidEnd{1, (int)it, threshold, tol, nfree3});
+      return;
+    }
+    k_iter_end<<<1, 1, 0, g.stream>>>(g.d_scal, (int)it, threshold, 0.0, nfree3);
     CV_HIP(hipGetLastError());
   }
   // Residuals and commit kernels; then read_back() and a drain, then check().
@@ -1193,7 +1276,7 @@ struct StepCtx {
     out.iterations_run += g.h_scal->iters_run;
     if (want_true) {
       // Post-sweep residual of the final positions.
-      true_residual<R>(g, a, n);
+      true_residual_fused<R>(g, a, n, ResidEnd{0, 0, 0.0, 0.0, 1.0});
       out.true_residual = true;
     }
     sum_f64(g, b.gsq.p, n, &g.d_scal->residual_sum, g.stream);
@@ -1428,7 +1511,8 @@ void run_batch_t(cv_gpu& P, const std::vector<cv_gpu*>& G, const std::vector<cv_
       size_t e0 = ends.size();
       for (auto& r : rr)
         if (r.first + t < r.end)
-          ends.push_back({G[r.i]->d_scal, (int)(r.first + t), C[r.i]->threshold, C[r.i]->tol, C[r.i]->nfree3});
+          ends.push_back({G[r.i]->d_scal, (int)(r.first + t), C[r.i]->threshold,
+                          C[r.i]->checks(r.first + t) ? C[r.i]->tol : 0.0, C[r.i]->nfree3});
       end_range[t] = {e0, ends.size() - e0};
     }
     P.batch_args.upload((const unsigned char*)args.data(), N * sizeof(SweepArgs<R>), st);
@@ -1450,7 +1534,7 @@ void run_batch_t(cv_gpu& P, const std::vector<cv_gpu*>& G, const std::vector<cv_
         }
       }
       for (auto& r : rr)
-        if (r.first + t < r.end && C[r.i]->tol > 0.0) true_residual<R>(*G[r.i], C[r.i]->a, S[r.i]->n, st);
+        if (r.first + t < r.end && C[r.i]->checks(r.first + t)) true_residual<R>(*G[r.i], C[r.i]->a, S[r.i]->n, st);
       auto [e0, ne] = end_range[t];
       if (ne) k_iter_end_batch<<<blocks(ne), TPB, 0, st>>>(dends + e0, (uint32_t)ne);
       CV_HIP(hipGetLastError());

@@ -119,18 +119,20 @@ class VbdSettings:
     contact: OgcSettings = field(default_factory=OgcSettings)
     residual_tolerance: float = 0.0
     flags: int = 0
+    residual_interval: int = 0
 
     @staticmethod
     def from_dict(d: dict) -> "VbdSettings":
         return VbdSettings(d["dt"], d["iterations"], d["damping"], tuple(d["gravity"]),
                            OgcSettings(d["radius"], d["stiffness"], d["relaxation"],
                                        d["refresh_fraction"]),
-                           d.get("residual_tolerance", 0.0), d.get("flags", 0))
+                           d.get("residual_tolerance", 0.0), d.get("flags", 0),
+                           d.get("residual_interval", 0))
 
     def to_c(self):
         return abi.cv_settings(self.dt, self.iterations, self.damping,
                                (C.c_double * 3)(*self.gravity), self.contact.to_c(),
-                               self.residual_tolerance, self.flags)
+                               self.residual_tolerance, self.flags, self.residual_interval)
 
 
 def _settings(s) -> abi.cv_settings:

@@ -131,6 +131,11 @@ typedef struct cv_settings {
   /* extensions */
   double residual_tolerance;  /* >0: stop once the post-sweep residual <= tol (max `iterations`) */
   uint32_t flags;             /* CV_FLAG_* */
+  uint32_t residual_interval; /* with a tolerance: the residual is evaluated after every k-th
+                                 sweep (sweeps k, 2k, ...) and after the last; 0 or 1: every
+                                 sweep.  k > 1 trades up to k - 1 extra sweeps for 1/k of the
+                                 residual passes (a residual pass costs about a quarter of a
+                                 sweep over all colours). */
 } cv_settings;
 
 enum {

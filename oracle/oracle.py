@@ -177,7 +177,7 @@ def settings_c(s: dict) -> abi.cv_settings:
     return abi.cv_settings(
         s["dt"], s["iterations"], s["damping"], (C.c_double * 3)(*s["gravity"]),
         abi.cv_ogc_settings(s["radius"], s["stiffness"], s["relaxation"], s["refresh_fraction"]),
-        s.get("residual_tolerance", 0.0), s.get("flags", 0))
+        s.get("residual_tolerance", 0.0), s.get("flags", 0), s.get("residual_interval", 0))
 
 
 def ogc_c(radius=0.003, stiffness=1e5, relaxation=0.45, refresh_fraction=0.01):

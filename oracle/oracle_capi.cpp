@@ -131,6 +131,7 @@ VbdSettings settings_of(const cv_settings* s) {
   o.contact.relaxation = s->contact.relaxation;
   o.contact.refresh_fraction = s->contact.refresh_fraction;
   o.residual_tolerance = s->residual_tolerance;
+  o.residual_interval = s->residual_interval;
   o.fast_coloring = (s->flags & CV_FLAG_FAST_COLORING) != 0;
   return o;
 }

@@ -519,7 +519,8 @@ VbdMetrics VbdSolver::step(State& state, const VbdSettings& st, const GrabT* gra
     m.truncated_vertices += clipped;
     m.residual_rms_n = std::sqrt(residual / (3.0 * n));
     m.iterations_run = it + 1;
-    if (st.residual_tolerance > 0.0) {
+    const int every = (int)std::max<uint32_t>(st.residual_interval, 1);
+    if (st.residual_tolerance > 0.0 && ((it + 1) % every == 0 || it + 1 == st.iterations)) {
       // Extension: true residual after the sweep, against the current contacts.
       m.true_residual_rms_n = true_residual(x, target, inertia, contacts, st, grab);
       if (m.true_residual_rms_n <= st.residual_tolerance) break;

@@ -931,6 +931,7 @@ struct VbdSettings {  // src/vbd.rs:18-52
   // Extension with no reference counterpart: > 0 ends the iterations once the
   // post-sweep (true) residual is <= this value.
   double residual_tolerance = 0.0;
+  uint32_t residual_interval = 0;  // residual after sweeps k, 2k, ... and the last (0, 1: every sweep)
   // Extension: the fast colouring (fast_colors) in place of the reference's.
   bool fast_coloring = false;
   void validate() const {

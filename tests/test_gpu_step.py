@@ -411,12 +411,19 @@ def test_true_residual_and_tolerance_mode_match_oracle(gpu):
     assert np.isfinite(mg["true_residual_rms_n"])
     assert abs(mg["true_residual_rms_n"] - mr["true_residual_rms_n"]) <= 1e-6 * mr["true_residual_rms_n"] + 1e-9
     tol = 3.0 * mr["true_residual_rms_n"]
-    st = dict(sc.settings, residual_tolerance=tol, iterations=40)
-    sg, sr, mg, mr = _one_step(gpu, sc, st)
-    print(f"tolerance {tol:.3e}: iterations gpu {mg['iterations_run']} oracle {mr['iterations_run']}")
-    assert mg["iterations_run"] == mr["iterations_run"] < 40
-    assert mg["true_residual_rms_n"] <= tol
-    assert np.abs(sg.x - sr.x).max() / _bbox_diag(sr.x) <= 1e-9
+    runs = {}
+    for k in (1, 4):
+        st = dict(sc.settings, residual_tolerance=tol, iterations=40, residual_interval=k)
+        sg, sr, mg, mr = _one_step(gpu, sc, st)
+        print(f"tolerance {tol:.3e}, residual every {k}: iterations gpu {mg['iterations_run']} "
+              f"oracle {mr['iterations_run']}")
+        assert mg["iterations_run"] == mr["iterations_run"] < 40
+        assert mg["true_residual_rms_n"] <= tol
+        assert np.abs(sg.x - sr.x).max() / _bbox_diag(sr.x) <= 1e-9
+        runs[k] = mg["iterations_run"]
+    # Checked after sweeps 4, 8, ... only: a multiple of 4, no earlier than
+    # the every-sweep stop (the sweeps themselves are the same).
+    assert runs[4] % 4 == 0 and runs[4] >= runs[1]
 
 
 # (config, tolerance N, pre-steps, measured relative gap of the step to the

@@ -113,3 +113,27 @@ def test_residual_tolerance_ends_iterations_early():
     assert 0 < m["iterations_run"] < 30
     assert m["true_residual_rms_n"] <= tol
     assert full["iterations_run"] == 30 and full["true_residual_rms_n"] < m["true_residual_rms_n"]
+
+
+def test_residual_interval_checks_only_every_kth_sweep():
+    """Extension: with residual_interval = 4 the residual is evaluated after
+    sweeps 4, 8, ... only.  The step stops at the first such sweep within the
+    tolerance: the same number of sweeps without a tolerance gives the same
+    positions and residual, and four sweeps fewer a residual above it."""
+    sc = scenes.hanging_sheet()
+    a = sc.state.copy()
+    full = O.Solver(sc.garment, None, a).step(a, dict(sc.settings, iterations=30, flags=2))
+    tol = 10.0 * full["true_residual_rms_n"]
+    b = sc.state.copy()
+    m = O.Solver(sc.garment, None, b).step(
+        b, dict(sc.settings, iterations=30, residual_tolerance=tol, residual_interval=4))
+    k = m["iterations_run"]
+    assert 4 <= k < 30 and k % 4 == 0
+    assert m["true_residual_rms_n"] <= tol
+    c = sc.state.copy()
+    same = O.Solver(sc.garment, None, c).step(c, dict(sc.settings, iterations=k, flags=2))
+    assert np.array_equal(c.x, b.x) and same["true_residual_rms_n"] == m["true_residual_rms_n"]
+    if k > 4:
+        d = sc.state.copy()
+        before = O.Solver(sc.garment, None, d).step(d, dict(sc.settings, iterations=k - 4, flags=2))
+        assert before["true_residual_rms_n"] > tol
