@@ -13,7 +13,11 @@ seeded geometry (no external data).
 One step is the complete VbdSolver::step of the reference (src/vbd.rs:250-422):
 prediction (the adaptive guess, src/vbd/prediction.rs:15-41), OGC detection
 and colouring, coloured Gauss-Seidel sweeps with the ball-truncated line
-search, mid-step re-detection, commit.  Iterations run to a fixed true
+search, mid-step re-detection, commit.  The sweep computes in f32
+(`--precision 32`, the default here: positions stay f64 and each vertex's
+local solve runs in f32 on coordinates relative to that vertex; `--precision
+64` is the f64 sweep that reproduces the f64 reference step to round-off).
+Iterations run to a fixed true
 residual: after every `--residual-interval`-th sweep (4: sweeps 4, 8, ...,
 and the last) the post-sweep residual sqrt(sum |g|^2 / 3 n) of the step's
 objective (src/solver.rs:229-231 form) is computed, and the step ends once
@@ -143,8 +147,9 @@ def main():
     ap.add_argument("--fixed-iterations", action="store_true",
                     help="run the configuration's fixed iteration count instead of the fixed residual")
     ap.add_argument("--compare", type=int, default=6, help="states of the warm-vs-plain comparison (0: off)")
-    ap.add_argument("--precision", type=int, default=64, choices=(32, 64),
-                    help="sweep arithmetic (64: f64 default; 32: f32 opt-in)")
+    ap.add_argument("--precision", type=int, default=32, choices=(32, 64),
+                    help="sweep arithmetic (32: f32 solves in each vertex's frame over f64 positions, the "
+                         "default here; 64: f64, the library's default)")
     ap.add_argument("--lanes", type=int, default=32, choices=(8, 16, 32, 64),
                     help="lanes per vertex of the sweep kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -157,6 +162,8 @@ def main():
                          "the device Newton energy reference of one sheet step in contact")
     ap.add_argument("--fast-coloring", action="store_true",
                     help="fast colouring on the device (fewer colours; not the reference's colouring order)")
+    ap.add_argument("--broad-phase", default="grid", choices=("grid", "lbvh"),
+                    help="OGC broad phase: hashed grid or linear BVH (same candidate sets)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batch64: use --lanes instead of timing the lanes per vertex first")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -214,6 +221,7 @@ def bench_step(args):
     gpu = GpuVbd(0)
     gpu.set_precision(args.precision)
     gpu.set_lanes(args.lanes)
+    gpu.set_broad_phase(args.broad_phase)
     st = sc.state.copy()
     sol = VbdSolver(sc.garment, sc.avatar, st)
     n_free = int(np.count_nonzero(sc.garment.pinned == 0))
@@ -289,7 +297,8 @@ def bench_step(args):
     argv_key = {"workload": args.workload, "settle": args.settle, "tol": args.tol,
                 "residual_interval": args.residual_interval,
                 "max_iterations": args.max_iterations, "precision": args.precision, "lanes": args.lanes,
-                "fixed_iterations": args.fixed_iterations, "fast_coloring": args.fast_coloring}
+                "fixed_iterations": args.fixed_iterations, "fast_coloring": args.fast_coloring,
+                "broad_phase": args.broad_phase}
     out = {
         "metric": "steps/s + vertex-iters/s at fixed residual; warm-start vs plain iters-to-tol",
         "value": K / dev_s,
@@ -315,6 +324,7 @@ def bench_step(args):
             "coloring": ("fast: static smallest-last, device repair (CV_FLAG_FAST_COLORING)"
                          if args.fast_coloring else "reference vertex-order greedy"),
             "lanes_per_vertex": args.lanes,
+            "broad_phase": args.broad_phase,
         },
         "iterations_per_step": m["iterations_run"] / K,
         "chunk_end_steps_at_the_residual": f"{reached} of {len(contacts)}",

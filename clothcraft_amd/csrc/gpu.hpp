@@ -141,8 +141,7 @@ struct BvhBufs {
 struct StepBufs {
   DBuf<double> x64, x_old64, v64, v_old64, guess64, xcur64, radii64, centers64;
   DBuf<unsigned long long> dist_bits;
-  DBuf<float4> pos, tgt, ball;        // f32 sweep
-  DBuf<double4> pos_d, tgt_d, ball_d;  // f64 sweep
+  DBuf<double4> pos_d, tgt_d, ball_d;  // sweep positions, targets, balls (both precisions)
   DBuf<double> gsq;
   DBuf<double> sum_parts;  // partials of the deterministic sums
   DBuf<double> gtrue;  // per-vertex |g|^2 of the true residual
@@ -340,7 +339,9 @@ struct cv_gpu {
   std::vector<hipEvent_t> sweep_events;
   double event_overhead_ms = 0;  // event interval of an empty launch (profiling calibration)
   uint64_t timed_counter = 0;  // iterations queued with profiling on
-  double static_bytes32 = 0, static_bytes64 = 0;  // algorithmic bytes of one sweep, static terms
+  // Algorithmic bytes of one sweep over the static terms: the f32 minimum,
+  // the f32 sweep as stored, the f64 sweep.
+  double static_bytes32 = 0, static_bytes32own = 0, static_bytes64 = 0;
   bool f64 = true;  // sweep precision (f64 reproduces the f64 reference step)
   NewtonBufs nt;
   Mlp mlp;

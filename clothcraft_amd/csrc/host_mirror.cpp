@@ -201,9 +201,9 @@ void cvh_closest_segments(const double* a, const double* b, const double* c, con
 
 }  // extern "C"
 
-// Sequential host mirror of one colour of the device sweep (k_sweep) in R:
-// same record math, Newton step, truncation and accept test; the only
-// difference from the device is the order of the per-vertex term sums.
+// Sequential host mirrors of one colour of the device sweep (k_sweep) in f64
+// and in f32: same record math, Newton step, truncation and accept test; the
+// only difference from the device is the order of the per-vertex term sums.
 namespace {
 template <class R>
 struct PosArr {
@@ -264,21 +264,99 @@ void sweep_color(uint32_t count, const uint32_t* order, R* pos, const R* tgt, co
     pos[4 * v + 2] = p.z;
   }
 }
+// The f32 sweep as the device runs it: positions, targets and ball centres
+// stay f64; each vertex solve works in f32 on coordinates relative to that
+// vertex's own position (every term, the inertia and the ball depend on
+// differences only), so f32 rounds lengths of the one-ring's size, not of
+// the scene's.
+struct PosRel {
+  const double* p;
+  double bx, by, bz;
+  cvm::V3<float> operator()(int, uint32_t i) const {
+    return {(float)(p[4 * i] - bx), (float)(p[4 * i + 1] - by), (float)(p[4 * i + 2] - bz)};
+  }
+};
+void sweep_color_local(uint32_t count, const uint32_t* order, double* pos, const double* tgt, const double* ball,
+                       const uint32_t* term_off, const cvm::TermRec* terms, const uint32_t* coff,
+                       const cvm::TermRec* crec, float cr, float ck, double* gsq, uint64_t* clipped,
+                       uint32_t* attempts) {
+  std::vector<std::pair<uint32_t, cvm::V3<double>>> upd;
+  for (uint32_t gi = 0; gi < count; ++gi) {
+    uint32_t v = order[gi];
+    if (pos[4 * v + 3] > 0.5) {
+      gsq[v] = 0;
+      continue;
+    }
+    const double bx = pos[4 * v], by = pos[4 * v + 1], bz = pos[4 * v + 2];
+    PosRel P{pos, bx, by, bz};
+    cvm::V3<float> x{0.f, 0.f, 0.f};
+    uint32_t s0 = term_off[v], ns = term_off[v + 1] - s0, c0 = coff[v], nt = ns + (coff[v + 1] - c0);
+    cvm::Deriv<float> acc = cvm::deriv_zero<float>();
+    // The line search accepts on the change of the local energy: records'
+    // energies at the candidate less their energies at x (ex), stitches in
+    // the exact difference form, the inertia as m/2 dx.(c + x - 2 t).
+    float ex = 0, emag = 0;
+    auto change = [&](cvm::V3<float> p) {
+      float e = -ex;
+      for (uint32_t i = 0; i < nt; ++i)
+        e += cvm::term_change_part<float>(i < ns ? terms[s0 + i] : crec[c0 + i - ns], x, p, cr, ck, P);
+      return e;
+    };
+    for (uint32_t i = 0; i < nt; ++i) {
+      const cvm::TermRec& t = i < ns ? terms[s0 + i] : crec[c0 + i - ns];
+      cvm::Deriv<float> d = cvm::term_deriv<float>(t, x, cr, ck, P);
+      if (cvm::rec_kind(t.head) != cvm::TERM_STITCH) {
+        ex += d.e;
+        emag += std::fabs(d.e);
+      }
+      cvm::deriv_add(acc, d);
+    }
+    cvm::V3<float> target{(float)(tgt[4 * v] - bx), (float)(tgt[4 * v + 1] - by), (float)(tgt[4 * v + 2] - bz)};
+    float inertia = (float)tgt[4 * v + 3];
+    cvm::V3<float> dx = x - target;
+    const float slack = 1e-14f + 4.0f * 1.1920928955078125e-07f * emag;
+    cvm::V3<float> delta = cvm::newton_direction<float>(acc, x, target, inertia);
+    gsq[v] = (double)cvm::dot(acc.g, acc.g);
+    cvm::V3<float> center{(float)(ball[4 * v] - bx), (float)(ball[4 * v + 1] - by), (float)(ball[4 * v + 2] - bz)};
+    bool tr = false;
+    cvm::V3<float> accepted = x;
+    float scale = 1;
+    int a = 0;
+    for (; a < 10; ++a) {
+      cvm::V3<float> cand = cvm::truncate_ball<float>(x + delta * scale, center, (float)ball[4 * v + 3], tr);
+      cvm::V3<float> dc = cand - target;
+      float e = change(cand) + 0.5f * inertia * cvm::dot(cand - x, dc + dx);
+      if (cvm::finite3(cand) && std::isfinite(e) && e <= slack) {
+        accepted = cand;
+        break;
+      }
+      scale *= 0.5f;
+    }
+    if (attempts) attempts[v] = a;
+    upd.push_back({v, cvm::V3<double>{bx + (double)accepted.x, by + (double)accepted.y, bz + (double)accepted.z}});
+    if (tr) (*clipped)++;
+  }
+  for (auto& [v, p] : upd) {
+    pos[4 * v] = p.x;
+    pos[4 * v + 1] = p.y;
+    pos[4 * v + 2] = p.z;
+  }
+}
 }  // namespace
 
 extern "C" {
-void cvh_sweep_color_f32(uint32_t count, const uint32_t* order, float* pos, const float* tgt, const float* ball,
-                         const uint32_t* term_off, const cvm::TermRec* terms, const uint32_t* coff,
-                         const cvm::TermRec* crec, double cr, double ck, float* gsq, uint64_t* clipped,
-                         uint32_t* attempts) {
-  sweep_color<float>(count, order, pos, tgt, ball, term_off, terms, coff, crec, (float)cr, (float)ck, gsq,
-                     clipped, attempts);
-}
 void cvh_sweep_color_f64(uint32_t count, const uint32_t* order, double* pos, const double* tgt,
                          const double* ball, const uint32_t* term_off, const cvm::TermRec64* terms,
                          const uint32_t* coff, const cvm::TermRec64* crec, double cr, double ck, double* gsq,
                          uint64_t* clipped, uint32_t* attempts) {
   sweep_color<double>(count, order, pos, tgt, ball, term_off, terms, coff, crec, cr, ck, gsq, clipped,
                       attempts);
+}
+void cvh_sweep_color_f32(uint32_t count, const uint32_t* order, double* pos, const double* tgt,
+                         const double* ball, const uint32_t* term_off, const cvm::TermRec* terms,
+                         const uint32_t* coff, const cvm::TermRec* crec, double cr, double ck, double* gsq,
+                         uint64_t* clipped, uint32_t* attempts) {
+  sweep_color_local(count, order, pos, tgt, ball, term_off, terms, coff, crec, (float)cr, (float)ck, gsq, clipped,
+                    attempts);
 }
 }

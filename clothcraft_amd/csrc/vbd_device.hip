@@ -8,10 +8,12 @@
 // identical sum), every lane forms the same Gershgorin-shifted Newton step,
 // and the ball-truncated backtracking search re-evaluates only energies.
 //
-// Positions are relative to the garment centroid.  The sweep is templated on
-// its scalar: f64 (default) reproduces the f64 reference step to round-off;
-// f32 is available as a faster mode whose strain resolution (one f32 ulp of
-// Green strain times the membrane stiffness) limits parity on stiff fabrics.
+// Positions, targets and ball centres are stored in f64, relative to the
+// garment centroid.  The sweep is templated on its arithmetic: f64 (default)
+// reproduces the f64 reference step to round-off; f32 solves each vertex in
+// its own frame (coordinates relative to the vertex's position, converted
+// from f64 once per load), so f32 rounds lengths of the one-ring's size
+// rather than of the scene's.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -36,35 +38,21 @@ double now_ms() {
 }
 
 template <class R>
-struct P4;
-template <>
-struct P4<float> {
-  using T = float4;
-  static __host__ __device__ T make(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
-};
-template <>
-struct P4<double> {
-  using T = double4;
-  static __host__ __device__ T make(double a, double b, double c, double d) { return make_double4(a, b, c, d); }
-};
-
-template <class R>
 struct GrabArgs {
   int active;
   uint32_t v[3];
   R w[3];
-  R target[3];  // relative to the origin
+  double target[3];  // relative to the origin
   R k;
 };
 
 template <class R>
 struct SweepArgs {
-  using T4 = typename P4<R>::T;
   const uint4* vinfo;  // per colour-ordered slot: vertex, s0, ns, c0 (k_vertex_info)
   const uint32_t* vnc;  // and its contact-record count
-  T4* pos;
-  const T4* tgt;
-  const T4* ball;
+  double4* pos;  // xyz relative to the origin, w: pinned flag
+  const double4* tgt;  // target, inertia
+  const double4* ball;  // centre, radius
   const uint32_t* term_off;
   const cvm::TermRecT<R>* terms;
   const uint32_t* coff;
@@ -76,11 +64,37 @@ struct SweepArgs {
   GrabArgs<R> grab;
 };
 
-template <class T4>
-__device__ __forceinline__ auto ldp(const T4* pos, uint32_t i) {
-  T4 q = pos[i];
-  using R = decltype(q.x);
-  return V3<R>{q.x, q.y, q.z};
+// Origin of a vertex solve's coordinates: the stored origin itself in f64
+// (no conversion: the f64 sweep works on the stored values), the solved
+// vertex's own position in f32.
+struct Frame {
+  double x, y, z;
+};
+template <class R>
+__device__ __forceinline__ V3<R> rel(double4 q, const Frame& f) {
+  if constexpr (sizeof(R) == 8)
+    return V3<R>{q.x, q.y, q.z};
+  else
+    return V3<R>{(R)(q.x - f.x), (R)(q.y - f.y), (R)(q.z - f.z)};
+}
+template <class R>
+__device__ __forceinline__ V3<R> ldp(const double4* pos, uint32_t i, const Frame& f) {
+  return rel<R>(pos[i], f);
+}
+template <class R>
+__device__ __forceinline__ Frame frame_of(double4 p) {
+  if constexpr (sizeof(R) == 8)
+    return Frame{0.0, 0.0, 0.0};
+  else
+    return Frame{p.x, p.y, p.z};
+}
+// A solved position in the frame back to stored f64 coordinates.
+template <class R>
+__device__ __forceinline__ double4 stored(V3<R> p, const Frame& f) {
+  if constexpr (sizeof(R) == 8)
+    return make_double4(p.x, p.y, p.z, 0.0);
+  else
+    return make_double4(f.x + (double)p.x, f.y + (double)p.y, f.z + (double)p.z, 0.0);
 }
 
 // Value of v in another lane of the same 16-lane row, by a DPP move (no LDS
@@ -116,17 +130,17 @@ __device__ __forceinline__ R group_sum(R v) {
 }
 
 // Grab term of vertex v (src/vbd.rs:237-246): d = sum w_j x_j - target.
-template <class R, class T4>
-__device__ __forceinline__ bool grab_d(const GrabArgs<R>& g, uint32_t v, V3<R> xv, const T4* pos, V3<R>& d,
-                                       R& wv) {
+template <class R>
+__device__ __forceinline__ bool grab_d(const GrabArgs<R>& g, uint32_t v, V3<R> xv, const double4* pos,
+                                       const Frame& f, V3<R>& d, R& wv) {
   if (!g.active) return false;
   int s = -1;
   for (int k = 0; k < 3; ++k)
     if (g.v[k] == v) s = k;
   if (s < 0) return false;
-  d = xv - V3<R>{g.target[0], g.target[1], g.target[2]};
+  d = xv - rel<R>(make_double4(g.target[0], g.target[1], g.target[2], 0.0), f);
   for (int k = 0; k < 3; ++k)
-    if (k != s) d = d + (ldp(pos, g.v[k]) - xv) * g.w[k];
+    if (k != s) d = d + (ldp<R>(pos, g.v[k], f) - xv) * g.w[k];
   wv = g.w[s];
   return true;
 }
@@ -144,21 +158,21 @@ struct LaneTerm {
 };
 template <class R>
 __device__ __forceinline__ LaneTerm<R> load_lane_term(const SweepArgs<R>& a, int lane, uint32_t s0, uint32_t ns,
-                                                      uint32_t c0, uint32_t nt) {
+                                                      uint32_t c0, uint32_t nt, const Frame& f) {
   LaneTerm<R> t;
   t.live = (uint32_t)lane < nt;
   if (t.live) {
     t.rec = (uint32_t)lane < ns ? a.terms[s0 + lane] : a.crec[c0 + (lane - ns)];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) t.q[k] = ldp(a.pos, t.rec.ids[k]);
+    for (int k = 0; k < 4; ++k) t.q[k] = ldp<R>(a.pos, t.rec.ids[k], f);
   }
   return t;
 }
 
 template <int L, class R>
 __device__ __forceinline__ R local_energy(const SweepArgs<R>& a, const LaneTerm<R>& lt, uint32_t v, V3<R> xv, int lane,
-                          uint32_t s0, uint32_t ns, uint32_t c0, uint32_t nt) {
-  auto P = [&](int, uint32_t i) { return ldp(a.pos, i); };
+                          uint32_t s0, uint32_t ns, uint32_t c0, uint32_t nt, const Frame& f) {
+  auto P = [&](int, uint32_t i) { return ldp<R>(a.pos, i, f); };
   auto Q = [&](int k, uint32_t) { return lt.q[k]; };
   R e = lt.live ? cvm::term_energy<R>(lt.rec, xv, a.cr, a.ck, Q) : R(0);
   for (uint32_t i = lane + L; i < nt; i += L) {
@@ -168,7 +182,31 @@ __device__ __forceinline__ R local_energy(const SweepArgs<R>& a, const LaneTerm<
   if (lane == 0) {
     V3<R> d;
     R wv;
-    if (grab_d(a.grab, v, xv, a.pos, d, wv)) e += R(0.5) * a.grab.k * cvm::dot(d, d);
+    if (grab_d(a.grab, v, xv, a.pos, f, d, wv)) e += R(0.5) * a.grab.k * cvm::dot(d, d);
+  }
+  return group_sum<L>(e);
+}
+
+// f32 line search: the change of the local energy (without the inertia term)
+// from x to the candidate xc, summed over the group: per lane, its records'
+// energies at xc less ex (their energies at x, from the derivative pass),
+// with stitches and the grab in the exact difference form (quad_change).
+template <int L, class R>
+__device__ __forceinline__ R local_change(const SweepArgs<R>& a, const LaneTerm<R>& lt, uint32_t v, V3<R> x, V3<R> xc,
+                                          int lane, uint32_t s0, uint32_t ns, uint32_t c0, uint32_t nt,
+                                          const Frame& f, R ex) {
+  auto P = [&](int, uint32_t i) { return ldp<R>(a.pos, i, f); };
+  auto Q = [&](int k, uint32_t) { return lt.q[k]; };
+  R e = -ex;
+  if (lt.live) e += cvm::term_change_part<R>(lt.rec, x, xc, a.cr, a.ck, Q);
+  for (uint32_t i = lane + L; i < nt; i += L) {
+    const cvm::TermRecT<R>& t = i < ns ? a.terms[s0 + i] : a.crec[c0 + (i - ns)];
+    e += cvm::term_change_part<R>(t, x, xc, a.cr, a.ck, P);
+  }
+  if (lane == 0) {
+    V3<R> d;
+    R wv;
+    if (grab_d(a.grab, v, x, a.pos, f, d, wv)) e += cvm::quad_change<R>(d, wv, a.grab.k, xc - x);
   }
   return group_sum<L>(e);
 }
@@ -178,24 +216,34 @@ __device__ __forceinline__ R local_energy(const SweepArgs<R>& a, const LaneTerm<
 // serves as the line search's reference energy without a second pass over
 // the terms.  It can differ from local_energy at x in the last bits (the two
 // routines may contract differently); the accept slack (accept_slack) covers that.
-// E = false leaves acc.e undefined.
-template <int L, class R, bool E = true>
+// E = false leaves acc.e undefined.  With X, *ex receives the lane's own
+// (unreduced) energy at x of its records other than stitches: what the f32
+// line search subtracts from their energies at a candidate.
+template <int L, class R, bool E = true, bool X = false>
 __device__ __forceinline__ cvm::Deriv<R> local_deriv(const SweepArgs<R>& a, const LaneTerm<R>& lt, uint32_t v, V3<R> x,
-                                     int lane, uint32_t s0, uint32_t ns, uint32_t c0, uint32_t nt) {
-  auto P = [&](int, uint32_t i) { return ldp(a.pos, i); };
+                                     int lane, uint32_t s0, uint32_t ns, uint32_t c0, uint32_t nt,
+                                     const Frame& f, R* ex = nullptr) {
+  auto P = [&](int, uint32_t i) { return ldp<R>(a.pos, i, f); };
   auto Q = [&](int k, uint32_t) { return lt.q[k]; };
   cvm::Deriv<R> acc = cvm::deriv_zero<R>();
-  if (lt.live) acc = cvm::term_deriv<R>(lt.rec, x, a.cr, a.ck, Q);
+  R e_other = R(0);
+  if (lt.live) {
+    acc = cvm::term_deriv<R>(lt.rec, x, a.cr, a.ck, Q);
+    if (X && cvm::rec_kind(lt.rec.head) != cvm::TERM_STITCH) e_other = acc.e;
+  }
   for (uint32_t i = lane + L; i < nt; i += L) {
     const cvm::TermRecT<R>& t = i < ns ? a.terms[s0 + i] : a.crec[c0 + (i - ns)];
-    cvm::deriv_add(acc, cvm::term_deriv<R>(t, x, a.cr, a.ck, P));
+    const cvm::Deriv<R> dt = cvm::term_deriv<R>(t, x, a.cr, a.ck, P);
+    if (X && cvm::rec_kind(t.head) != cvm::TERM_STITCH) e_other += dt.e;
+    cvm::deriv_add(acc, dt);
   }
+  if (X) *ex = e_other;
   if (lane == 0) {
     V3<R> d;
     R wv;
-    if (grab_d(a.grab, v, x, a.pos, d, wv)) cvm::deriv_add(acc, cvm::stitch_d<R>(d, wv, a.grab.k));
+    if (grab_d(a.grab, v, x, a.pos, f, d, wv)) cvm::deriv_add(acc, cvm::stitch_d<R>(d, wv, a.grab.k));
   }
-  if (E) acc.e = group_sum<L>(acc.e);
+  if (E && !X) acc.e = group_sum<L>(acc.e);
   acc.g.x = group_sum<L>(acc.g.x);
   acc.g.y = group_sum<L>(acc.g.y);
   acc.g.z = group_sum<L>(acc.g.z);
@@ -209,43 +257,60 @@ __device__ __forceinline__ cvm::Deriv<R> local_deriv(const SweepArgs<R>& a, cons
 // contact-record count.
 template <int L, class R>
 __device__ __forceinline__ void solve_vertex(const SweepArgs<R>& a, uint4 vi, uint32_t nc, int lane) {
-  using T4 = typename P4<R>::T;
   const uint32_t v = vi.x, s0 = vi.y, ns = vi.z, c0 = vi.w, nt = ns + nc;
   // The vertex's data and the lane's first term record are loaded at once.
-  const T4 p4 = a.pos[v];
-  const T4 tg = a.tgt[v];
-  const T4 b4 = a.ball[v];
-  const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt);
-  if (p4.w > R(0.5)) {  // pinned: no update, zero residual contribution
+  const double4 p4 = a.pos[v];
+  const double4 tg = a.tgt[v];
+  const double4 b4 = a.ball[v];
+  const Frame f = frame_of<R>(p4);
+  const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt, f);
+  if (p4.w > 0.5) {  // pinned: no update, zero residual contribution
     if (lane == 0) a.gsq[v] = 0.0;
     return;
   }
-  V3<R> x{p4.x, p4.y, p4.z};
-  cvm::Deriv<R> acc = local_deriv<L>(a, lt, v, x, lane, s0, ns, c0, nt);
-  V3<R> target{tg.x, tg.y, tg.z};
-  R inertia = tg.w;
-  // The accept test compares energies from one evaluation routine, so a
-  // candidate equal to x reproduces `old` exactly.
+  V3<R> x = rel<R>(p4, f);
+  constexpr bool F32 = sizeof(R) == 4;
+  R ex = R(0);
+  cvm::Deriv<R> acc = local_deriv<L, R, true, F32>(a, lt, v, x, lane, s0, ns, c0, nt, f, &ex);
+  V3<R> target = rel<R>(tg, f);
+  R inertia = (R)tg.w;
+  // f64: the reference's test E <= E_old + 1e-14 on energies from one
+  // evaluation routine (a candidate equal to x reproduces `old` exactly).
+  // f32: the same test on the change E - E_old, accumulated as changes (see
+  // local_change; the inertia term as m/2 dx.(c + x - 2 t)), with a slack of
+  // four f32 ulps of the non-quadratic energies it differences.
   V3<R> dx = x - target;
-  R old = acc.e + R(0.5) * inertia * cvm::dot(dx, dx);
+  R old = R(0), slack = R(0);
+  if constexpr (F32) {
+    slack = R(1e-14) + R(4.0 * 1.1920928955078125e-07) * group_sum<L>(fabs(ex));
+  } else {
+    old = acc.e + R(0.5) * inertia * cvm::dot(dx, dx);
+    slack = cvm::accept_slack<R>(old);
+  }
   V3<R> delta = cvm::newton_direction<R>(acc, x, target, inertia);
   R gg = cvm::dot(acc.g, acc.g);
-  V3<R> center{b4.x, b4.y, b4.z};
+  V3<R> center = rel<R>(b4, f);
+  const R radius = (R)b4.w;
   bool truncated = false;
   V3<R> accepted = x;
   R scale = R(1);
   for (int attempt = 0; attempt < 10; ++attempt) {
-    V3<R> cand = cvm::truncate_ball<R>(x + delta * scale, center, b4.w, truncated);
+    V3<R> cand = cvm::truncate_ball<R>(x + delta * scale, center, radius, truncated);
     V3<R> dc = cand - target;
-    R e = local_energy<L>(a, lt, v, cand, lane, s0, ns, c0, nt) + R(0.5) * inertia * cvm::dot(dc, dc);
-    if (cvm::finite3(cand) && isfinite(e) && e <= old + cvm::accept_slack<R>(old)) {
+    R e;
+    if constexpr (F32)
+      e = local_change<L>(a, lt, v, x, cand, lane, s0, ns, c0, nt, f, ex) +
+          R(0.5) * inertia * cvm::dot(cand - x, dc + dx);
+    else
+      e = local_energy<L>(a, lt, v, cand, lane, s0, ns, c0, nt, f) + R(0.5) * inertia * cvm::dot(dc, dc);
+    if (cvm::finite3(cand) && isfinite(e) && e <= old + slack) {
       accepted = cand;
       break;
     }
     scale *= R(0.5);
   }
   if (lane == 0) {
-    a.pos[v] = P4<R>::make(accepted.x, accepted.y, accepted.z, R(0));
+    a.pos[v] = stored<R>(accepted, f);
     a.gsq[v] = (double)gg;
     if (truncated) atomicAdd(a.clipped, 1ull);
   }
@@ -360,23 +425,24 @@ __global__ void k_iter_end_batch(const IterEnd* e, uint32_t count) {
 // Gradient probe (src/vbd.wgsl:142-147): (g, E) of the local objective.
 template <int L, class R>
 __global__ void k_probe(SweepArgs<R> a, uint32_t n, double* out) {
-  using T4 = typename P4<R>::T;
   const int lane = threadIdx.x & (L - 1);
   const uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / L;
   if (v >= n) return;
-  T4 p4 = a.pos[v];
-  V3<R> x{p4.x, p4.y, p4.z};
+  const double4 p4 = a.pos[v];
+  const Frame f = frame_of<R>(p4);
+  V3<R> x = rel<R>(p4, f);
   uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
   uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
-  const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt);
-  cvm::Deriv<R> acc = local_deriv<L>(a, lt, v, x, lane, s0, ns, c0, nt);
+  const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt, f);
+  cvm::Deriv<R> acc = local_deriv<L>(a, lt, v, x, lane, s0, ns, c0, nt, f);
   if (lane == 0) {
-    T4 tg = a.tgt[v];
-    V3<R> dx = x - V3<R>{tg.x, tg.y, tg.z};
-    out[4 * v] = acc.g.x + dx.x * tg.w;
-    out[4 * v + 1] = acc.g.y + dx.y * tg.w;
-    out[4 * v + 2] = acc.g.z + dx.z * tg.w;
-    out[4 * v + 3] = acc.e + R(0.5) * tg.w * cvm::dot(dx, dx);
+    const double4 tg = a.tgt[v];
+    const R m = (R)tg.w;
+    V3<R> dx = x - rel<R>(tg, f);
+    out[4 * v] = acc.g.x + dx.x * m;
+    out[4 * v + 1] = acc.g.y + dx.y * m;
+    out[4 * v + 2] = acc.g.z + dx.z * m;
+    out[4 * v + 3] = acc.e + R(0.5) * m * cvm::dot(dx, dx);
   }
 }
 
@@ -385,23 +451,23 @@ __global__ void k_probe(SweepArgs<R> a, uint32_t n, double* out) {
 // 0 for pinned vertices: the Newton-style rms of src/solver.rs:229-231.
 template <int L, class R>
 __global__ __launch_bounds__(TPB) void k_true_grad(SweepArgs<R> a, uint32_t n, double* out) {
-  using T4 = typename P4<R>::T;
   const int lane = threadIdx.x & (L - 1);
   const uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / L;
   if (v >= n) return;
-  T4 p4 = a.pos[v];
-  if (p4.w > R(0.5)) {
+  const double4 p4 = a.pos[v];
+  if (p4.w > 0.5) {
     if (lane == 0) out[v] = 0.0;
     return;
   }
-  V3<R> x{p4.x, p4.y, p4.z};
+  const Frame f = frame_of<R>(p4);
+  V3<R> x = rel<R>(p4, f);
   uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
   uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
-  const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt);
-  cvm::Deriv<R> acc = local_deriv<L, R, false>(a, lt, v, x, lane, s0, ns, c0, nt);
+  const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt, f);
+  cvm::Deriv<R> acc = local_deriv<L, R, false>(a, lt, v, x, lane, s0, ns, c0, nt, f);
   if (lane == 0) {
-    T4 tg = a.tgt[v];
-    V3<R> g = acc.g + (x - V3<R>{tg.x, tg.y, tg.z}) * tg.w;
+    const double4 tg = a.tgt[v];
+    V3<R> g = acc.g + (x - rel<R>(tg, f)) * (R)tg.w;
     out[v] = (double)cvm::dot(g, g);
   }
 }
@@ -420,7 +486,6 @@ struct ResidEnd {
 };
 template <int L, class R>
 __global__ __launch_bounds__(TPB) void k_true_resid(SweepArgs<R> a, uint32_t n, double* part, int honor_stop) {
-  using T4 = typename P4<R>::T;
   if (honor_stop && *a.stop) return;  // grid-uniform
   constexpr int G = TPB / L;
   __shared__ double red[G];
@@ -428,15 +493,16 @@ __global__ __launch_bounds__(TPB) void k_true_resid(SweepArgs<R> a, uint32_t n, 
   const uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / L;
   double gg = 0.0;
   if (v < n) {
-    T4 p4 = a.pos[v];
-    if (p4.w <= R(0.5)) {
-      V3<R> x{p4.x, p4.y, p4.z};
+    const double4 p4 = a.pos[v];
+    if (p4.w <= 0.5) {
+      const Frame f = frame_of<R>(p4);
+      V3<R> x = rel<R>(p4, f);
       uint32_t s0 = a.term_off[v], ns = a.term_off[v + 1] - s0;
       uint32_t c0 = a.coff[v], nt = ns + (a.coff[v + 1] - c0);
-      const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt);
-      cvm::Deriv<R> acc = local_deriv<L, R, false>(a, lt, v, x, lane, s0, ns, c0, nt);
-      T4 tg = a.tgt[v];
-      V3<R> g = acc.g + (x - V3<R>{tg.x, tg.y, tg.z}) * tg.w;
+      const LaneTerm<R> lt = load_lane_term(a, lane, s0, ns, c0, nt, f);
+      cvm::Deriv<R> acc = local_deriv<L, R, false>(a, lt, v, x, lane, s0, ns, c0, nt, f);
+      const double4 tg = a.tgt[v];
+      V3<R> g = acc.g + (x - rel<R>(tg, f)) * (R)tg.w;
       gg = (double)cvm::dot(g, g);
     }
   }
@@ -651,18 +717,16 @@ __global__ void k_prologue_final(const double* part, uint32_t nparts, uint32_t n
 }
 
 // Targets, inertia, adaptive guess (src/vbd.rs:283-297, src/vbd/prediction.rs:15-41).
-template <class R>
 __global__ void k_prologue(const double* x, const double* v, const double* acc, const double* masses,
                            const uint8_t* pinned, uint32_t n, const double* fixed, uint32_t n_fixed, double dt,
                            double damping, double gx, double gy, double gz, int pred_valid, const Scalars* sc,
-                           typename P4<R>::T* pos, typename P4<R>::T* tgt, double* guess, double* x_old,
-                           double* v_old) {
+                           double4* pos, double4* tgt, double* guess, double* x_old, double* v_old) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   double ox = sc->origin[0], oy = sc->origin[1], oz = sc->origin[2];
   if (i >= n) {
     uint32_t j = i - n;
     if (j < n_fixed)
-      pos[i] = P4<R>::make((R)(fixed[3 * j] - ox), (R)(fixed[3 * j + 1] - oy), (R)(fixed[3 * j + 2] - oz), R(1));
+      pos[i] = make_double4(fixed[3 * j] - ox, fixed[3 * j + 1] - oy, fixed[3 * j + 2] - oz, 1.0);
     return;
   }
   double damp = 1.0 + damping * dt;
@@ -672,7 +736,7 @@ __global__ void k_prologue(const double* x, const double* v, const double* acc, 
   double t[3];
   for (int k = 0; k < 3; ++k) t[k] = p[k] + (w[k] * dt + g[k] * dt * dt) / damp;
   double inertia = masses[i] * damp / (dt * dt);
-  tgt[i] = P4<R>::make((R)(t[0] - ox), (R)(t[1] - oy), (R)(t[2] - oz), (R)inertia);
+  tgt[i] = make_double4(t[0] - ox, t[1] - oy, t[2] - oz, inertia);
   double g2 = gx * gx + gy * gy + gz * gz;
   double factor = 1.0;
   if (pred_valid && g2 > 1e-20) {
@@ -684,26 +748,24 @@ __global__ void k_prologue(const double* x, const double* v, const double* acc, 
     x_old[3 * i + k] = p[k];
     v_old[3 * i + k] = w[k];
   }
-  if (pinned[i]) pos[i] = P4<R>::make((R)(p[0] - ox), (R)(p[1] - oy), (R)(p[2] - oz), R(1));
+  if (pinned[i]) pos[i] = make_double4(p[0] - ox, p[1] - oy, p[2] - oz, 1.0);
 }
 
-// Balls from the last detection.  In f32 the radius is shrunk by two ulps of
-// the coordinate magnitude so a truncated f32 point, converted back, never
+// Balls from the last detection.  For the f32 sweep the radius is shrunk by
+// four f32 ulps of itself: a point truncated in a vertex's frame (where the
+// centre is within the radius of the origin) and converted back then never
 // leaves the f64 ball (the f32 safety margin of src/gpu_vbd.rs:258-274); the
 // f64 sweep uses the radius unchanged.
 template <class R>
-__global__ void k_balls(const double* xdet, const double* radii, uint32_t n, const Scalars* sc,
-                        typename P4<R>::T* ball, double* centers, double* radii_used) {
+__global__ void k_balls(const double* xdet, const double* radii, uint32_t n, const Scalars* sc, double4* ball,
+                        double* centers, double* radii_used) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double cx = xdet[3 * i] - sc->origin[0], cy = xdet[3 * i + 1] - sc->origin[1],
          cz = xdet[3 * i + 2] - sc->origin[2];
   double r = radii[i];
-  if (sizeof(R) == 4) {
-    double m = fmax(fmax(fabs(cx), fabs(cy)), fabs(cz)) + r;
-    r = fmax(r - 2.0 * 1.1920928955078125e-07 * m, 0.0);
-  }
-  ball[i] = P4<R>::make((R)cx, (R)cy, (R)cz, (R)r);
+  if (sizeof(R) == 4) r -= 4.0 * 1.1920928955078125e-07 * r;
+  ball[i] = make_double4(cx, cy, cz, r);
   centers[3 * i] = xdet[3 * i];
   centers[3 * i + 1] = xdet[3 * i + 1];
   centers[3 * i + 2] = xdet[3 * i + 2];
@@ -711,9 +773,8 @@ __global__ void k_balls(const double* xdet, const double* radii, uint32_t n, con
 }
 
 // Conservative warm start: the guess clipped into its ball (src/vbd.rs:316-324).
-template <class R>
 __global__ void k_truncate_guess(const double* guess, const double* centers, const double* radii_used,
-                                 const uint8_t* pinned, uint32_t n, const Scalars* sc, typename P4<R>::T* pos,
+                                 const uint8_t* pinned, uint32_t n, const Scalars* sc, double4* pos,
                                  unsigned long long* clipped) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || pinned[i]) return;
@@ -726,13 +787,12 @@ __global__ void k_truncate_guess(const double* guess, const double* centers, con
   double p[3];
   bool clip = len > r;
   for (int k = 0; k < 3; ++k) p[k] = clip ? centers[3 * i + k] + d[k] * (r / len) : guess[3 * i + k];
-  pos[i] = P4<R>::make((R)(p[0] - sc->origin[0]), (R)(p[1] - sc->origin[1]), (R)(p[2] - sc->origin[2]), R(0));
+  pos[i] = make_double4(p[0] - sc->origin[0], p[1] - sc->origin[1], p[2] - sc->origin[2], 0.0);
   if (clip) atomicAdd(clipped, 1ull);
 }
 
 // Current f64 positions for a mid-step detection.
-template <class R>
-__global__ void k_current(const typename P4<R>::T* pos, const double* x_old, const uint8_t* pinned, uint32_t n,
+__global__ void k_current(const double4* pos, const double* x_old, const uint8_t* pinned, uint32_t n,
                           const Scalars* sc, double* x64) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -748,8 +808,7 @@ __global__ void k_current(const typename P4<R>::T* pos, const double* x_old, con
 
 // Commit (src/vbd.rs:399-411, src/vbd/prediction.rs:43-54): finiteness,
 // trust-region invariant, velocity, and the acceleration history.
-template <class R>
-__global__ void k_finalize(const typename P4<R>::T* pos, const double* x_old, const double* v_old,
+__global__ void k_finalize(const double4* pos, const double* x_old, const double* v_old,
                            const double* centers, const double* radii, const uint8_t* pinned, uint32_t n,
                            double dt, double damping, double tol, double* x_out, double* v_out, double* acc,
                            Scalars* sc) {
@@ -794,9 +853,9 @@ template <class R>
 struct PBufs;
 template <>
 struct PBufs<float> {
-  static float4* pos(cv_gpu& g) { return g.b.pos.p; }
-  static float4* tgt(cv_gpu& g) { return g.b.tgt.p; }
-  static float4* ball(cv_gpu& g) { return g.b.ball.p; }
+  static double4* pos(cv_gpu& g) { return g.b.pos_d.p; }
+  static double4* tgt(cv_gpu& g) { return g.b.tgt_d.p; }
+  static double4* ball(cv_gpu& g) { return g.b.ball_d.p; }
   static const cvm::TermRec* terms(cv_gpu& g) { return g.dev.terms.p; }
   static const cvm::TermRec* crec(cv_gpu& g) { return g.b.crec.p; }
 };
@@ -831,7 +890,7 @@ SweepArgs<R> sweep_args(cv_gpu& g, const cv_settings* set, const cv_grab* grab) 
     for (int k = 0; k < 3; ++k) {
       a.grab.v[k] = grab->vertices[k];
       a.grab.w[k] = (R)grab->weights[k];
-      a.grab.target[k] = (R)(grab->target[k] - g.h_scal->origin[k]);
+      a.grab.target[k] = grab->target[k] - g.h_scal->origin[k];
     }
     a.grab.k = (R)grab->stiffness;
   }
@@ -1133,7 +1192,7 @@ struct StepCtx {
     g.h_scal->stop = g.h_scal->stop_it = g.h_scal->iters_run = 0;
     g.h_scal->err_nonfinite = g.h_scal->err_trust = 0;
     write_scalars(g);
-    k_prologue<R><<<blocks(n + d.n_fixed), TPB, 0, st>>>(
+    k_prologue<<<blocks(n + d.n_fixed), TPB, 0, st>>>(
         b.x64.p, b.v64.p, d.acc.p, d.masses.p, d.pinned.p, n, d.fixed_pos.p, d.n_fixed, dt, set->damping,
         set->gravity[0], set->gravity[1], set->gravity[2], g.pred_valid ? 1 : 0, g.d_scal, PBufs<R>::pos(g),
         PBufs<R>::tgt(g), b.guess64.p, b.x_old64.p, b.v_old64.p);
@@ -1146,7 +1205,7 @@ struct StepCtx {
       warm_features(g, s, set, rq);
       warm_guess(g, s, set);
     }
-    k_truncate_guess<R><<<blocks(n), TPB, 0, st>>>(b.guess64.p, b.centers64.p, b.guess64.p + 3 * (size_t)n,
+    k_truncate_guess<<<blocks(n), TPB, 0, st>>>(b.guess64.p, b.centers64.p, b.guess64.p + 3 * (size_t)n,
                                                    d.pinned.p, n, g.d_scal, PBufs<R>::pos(g), &g.d_scal->clipped);
     CV_HIP(hipGetLastError());
     threshold = set->contact.refresh_fraction * n;
@@ -1182,7 +1241,8 @@ struct StepCtx {
       if (it < ran) out.sweep_timed_bytes_min += it_bytes32;
     launches.clear();
     timed_its.clear();
-    out.sweep_bytes += (ran - done) * (sizeof(R) == 4 ? it_bytes32 : g.static_bytes64 + 100.0 * inc);
+    out.sweep_bytes += (ran - done) * (sizeof(R) == 4 ? g.static_bytes32own + 92.0 * inc
+                                                      : g.static_bytes64 + 100.0 * inc);
     out.sweep_bytes_min += (ran - done) * it_bytes32;
     done = ran;
     if (g.h_scal->stop == 2) return finished = true, false;  // residual tolerance reached
@@ -1193,7 +1253,7 @@ struct StepCtx {
       g.h_scal->stop = 0;
       CV_HIP(hipMemcpyAsync(&g.d_scal->stop, &g.h_scal->stop, sizeof(unsigned int), hipMemcpyHostToDevice,
                             g.stream));
-      k_current<R><<<blocks(s.n), TPB, 0, g.stream>>>(PBufs<R>::pos(g), g.b.x_old64.p, g.dev.pinned.p, s.n,
+      k_current<<<blocks(s.n), TPB, 0, g.stream>>>(PBufs<R>::pos(g), g.b.x_old64.p, g.dev.pinned.p, s.n,
                                                       g.d_scal, g.b.x64.p);
       CV_HIP(hipGetLastError());
       detect_and_color<R>(g, s, set, grab, rq, out, false);
@@ -1281,7 +1341,7 @@ struct StepCtx {
     }
     sum_f64(g, b.gsq.p, n, &g.d_scal->residual_sum, g.stream);
     // The trust-region check tolerance of the device path (src/gpu_vbd.rs:436).
-    k_finalize<R><<<blocks(n), TPB, 0, g.stream>>>(PBufs<R>::pos(g), b.x_old64.p, b.v_old64.p, b.centers64.p,
+    k_finalize<<<blocks(n), TPB, 0, g.stream>>>(PBufs<R>::pos(g), b.x_old64.p, b.v_old64.p, b.centers64.p,
                                                    b.radii64.p, d.pinned.p, n, set->dt, set->damping, 1e-8,
                                                    b.xcur64.p, b.v64.p, d.acc.p, g.d_scal);
     CV_HIP(hipGetLastError());
@@ -1305,7 +1365,7 @@ struct StepCtx {
 template <class R>
 void check_gradients_t(cv_gpu& g, const cv_solver& s, const SweepArgs<R>& a, const cv_settings* set,
                        const cv_grab* grab) {
-  using T4 = typename P4<R>::T;
+  using T4 = double4;
   const size_t n = s.n, nall = n + g.dev.n_fixed;
   hipStream_t st = g.stream;
   g.probe_out.reserve(4 * n);
@@ -1615,23 +1675,29 @@ void ensure_device_solver(cv_gpu& g, const cv_solver& s) {
   }
   // Algorithmic bytes of one vertex-iteration (each term's data and the
   // neighbour positions read once, position written once):
-  //   f32: 64 fixed + 72 per triangle, 52 per hinge, 68 per stitch;
-  //   f64: 120 fixed + 136 per triangle, 92 per hinge, 124 per stitch.
+  //   f32 minimum (f32 positions and records): 64 fixed + 72 per triangle,
+  //     52 per hinge, 68 per stitch (56 per contact incidence);
+  //   f32 sweep as stored (f64 positions, f32 records): 120 fixed + 96 per
+  //     triangle, 88 per hinge, 104 per stitch (92 per contact);
+  //   f64: 120 fixed + 136 per triangle, 92 per hinge, 124 per stitch (100).
   // A pinned vertex reads its position only.
-  g.static_bytes32 = g.static_bytes64 = 0;
+  g.static_bytes32 = g.static_bytes32own = g.static_bytes64 = 0;
   for (uint32_t v = 0; v < s.n; ++v) {
     if (s.pinned[v]) {
       g.static_bytes32 += 16;
+      g.static_bytes32own += 32;
       g.static_bytes64 += 32;
       continue;
     }
-    double b32 = 64, b64 = 120;
+    double b32 = 64, b32own = 120, b64 = 120;
     for (uint32_t k = s.term_off[v]; k < s.term_off[v + 1]; ++k) {
       uint32_t kind = cvm::rec_kind(s.terms[k].head);
       b32 += kind == cvm::TERM_MEMBRANE ? 72 : (kind == cvm::TERM_HINGE ? 52 : 68);
+      b32own += kind == cvm::TERM_MEMBRANE ? 96 : (kind == cvm::TERM_HINGE ? 88 : 104);
       b64 += kind == cvm::TERM_MEMBRANE ? 136 : (kind == cvm::TERM_HINGE ? 92 : 124);
     }
     g.static_bytes32 += b32;
+    g.static_bytes32own += b32own;
     g.static_bytes64 += b64;
   }
   d.acc.reserve(3 * (size_t)s.n);
@@ -1652,9 +1718,6 @@ void ensure_step_buffers(cv_gpu& g, const cv_solver& s) {
   b.xcur64.reserve(3 * n);
   b.radii64.reserve(n);
   b.centers64.reserve(3 * n);
-  b.pos.reserve(nt);
-  b.tgt.reserve(n);
-  b.ball.reserve(n);
   b.pos_d.reserve(nt);
   b.tgt_d.reserve(n);
   b.ball_d.reserve(n);
@@ -1768,7 +1831,7 @@ void probe_t(cv_gpu& g, cv_solver& s, const cv_state* st, const cv_settings* set
   size_t n = s.n;
   hipStream_t stream = g.stream;
   double dt = set->dt;
-  k_prologue<R><<<blocks(n + g.dev.n_fixed), TPB, 0, stream>>>(
+  k_prologue<<<blocks(n + g.dev.n_fixed), TPB, 0, stream>>>(
       g.b.x64.p, g.b.v64.p, g.dev.acc.p, g.dev.masses.p, g.dev.pinned.p, (uint32_t)n, g.dev.fixed_pos.p,
       g.dev.n_fixed, dt, set->damping, set->gravity[0], set->gravity[1], set->gravity[2], 0, g.d_scal,
       PBufs<R>::pos(g), PBufs<R>::tgt(g), g.b.guess64.p, g.b.x_old64.p, g.b.v_old64.p);
@@ -1776,13 +1839,12 @@ void probe_t(cv_gpu& g, cv_solver& s, const cv_state* st, const cv_settings* set
   double rq = std::min(std::max(r + g.h_scal->motion, 2.0 * r), 4.0 * r);
   g.b.bvh_rebuild = true;
   gpu_detect(g, s, rq, set->contact, 0);
-  // Probe at the state's positions rounded to the sweep precision.
-  using T4 = typename P4<R>::T;
-  std::vector<T4> pos(n);
+  // Probe at the state's positions (relative to the origin).
+  std::vector<double4> pos(n);
   for (size_t i = 0; i < n; ++i)
-    pos[i] = P4<R>::make((R)(st->x[3 * i] - g.h_scal->origin[0]), (R)(st->x[3 * i + 1] - g.h_scal->origin[1]),
-                         (R)(st->x[3 * i + 2] - g.h_scal->origin[2]), s.pinned[i] ? R(1) : R(0));
-  CV_HIP(hipMemcpyAsync(PBufs<R>::pos(g), pos.data(), n * sizeof(T4), hipMemcpyHostToDevice, stream));
+    pos[i] = make_double4(st->x[3 * i] - g.h_scal->origin[0], st->x[3 * i + 1] - g.h_scal->origin[1],
+                          st->x[3 * i + 2] - g.h_scal->origin[2], s.pinned[i] ? 1.0 : 0.0);
+  CV_HIP(hipMemcpyAsync(PBufs<R>::pos(g), pos.data(), n * sizeof(double4), hipMemcpyHostToDevice, stream));
   SweepArgs<R> a = sweep_args<R>(g, set, grab);
   DBuf<double> outd;
   outd.reserve(4 * n);
@@ -2359,7 +2421,7 @@ cv_status cv_warm_features(cv_gpu* g, cv_solver* s, const cv_state* st, const cv
     StepBufs& b = g->b;
     auto prologue = [&](auto tag) {
       using R = decltype(tag);
-      k_prologue<R><<<blocks(n + d.n_fixed), TPB, 0, g->stream>>>(
+      k_prologue<<<blocks(n + d.n_fixed), TPB, 0, g->stream>>>(
           b.x64.p, b.v64.p, d.acc.p, d.masses.p, d.pinned.p, n, d.fixed_pos.p, d.n_fixed, dt, set->damping,
           set->gravity[0], set->gravity[1], set->gravity[2], g->pred_valid ? 1 : 0, g->d_scal, PBufs<R>::pos(*g),
           PBufs<R>::tgt(*g), b.guess64.p, b.x_old64.p, b.v_old64.p);

@@ -599,6 +599,30 @@ CV_HD Deriv<R> term_deriv(const TermRecT<F>& t, V3<R> xv, R cr, R ck, PosFn&& P)
   return contact_d<R>(rec_ckind(t.head), x, s, R(t.f[0]), cr, ck);
 }
 
+// Energy changes for the f32 line search.  A stitch (or grab) energy
+// 1/2 k |d|^2 is linear in the vertex through d = rel + w_v x_v, so moving
+// the vertex by dx changes it by k w_v (d . dx) + 1/2 k w_v^2 |dx|^2 exactly:
+// no difference of two large energies (a seam still open by centimetres
+// holds ~1e2 J against ~1e-7 J of membrane energy, far beyond f32's reach).
+template <class R>
+CV_HD R quad_change(V3<R> d, R wv, R k, V3<R> dx) {
+  return k * wv * dot(d, dx) + R(0.5) * k * wv * wv * dot(dx, dx);
+}
+// A record's contribution to the change of the local energy from xv to xc:
+// the exact change for a stitch; for the other kinds the energy at xc (the
+// caller subtracts their energies at xv, which it already holds).
+template <class R, class F, class PosFn>
+CV_HD R term_change_part(const TermRecT<F>& t, V3<R> xv, V3<R> xc, R cr, R ck, PosFn&& P) {
+  if (rec_kind(t.head) != TERM_STITCH) return term_energy<R>(t, xc, cr, ck, P);
+  int s = rec_slot(t.head), n = rec_count(t.head);
+  V3<R> d = mk<R>(0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (i < n && i != s) d = d + (P(i, t.ids[i]) - xv) * R(t.f[i]);
+  R ws = s == 0 ? R(t.f[0]) : s == 1 ? R(t.f[1]) : s == 2 ? R(t.f[2]) : R(t.f[3]);
+  return quad_change<R>(d, ws, R(t.f[4]), xc - xv);
+}
+
 // ------------------------------------------------------ local Newton
 // src/vbd.rs:353-371: inertia, Gershgorin shift and the cofactor solve.
 template <class R>

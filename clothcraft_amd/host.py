@@ -196,19 +196,21 @@ def sym(h6) -> np.ndarray:
 
 def sweep_color(order, pos, tgt, ball, term_off, terms, coff, crec, cr, ck, gsq, f32=True):
     """Host mirror of one device sweep launch (k_sweep) over `order`.
-    pos/tgt/ball are [n,4] arrays of the precision used (float32 or float64),
-    updated in place; returns (clipped count, per-vertex attempt index)."""
+    pos/tgt/ball are float64 [n,4] arrays (positions relative to the origin,
+    updated in place) in both precisions, as on the device; terms/crec are
+    the records of the precision used.  In f32 each vertex solve works in f32
+    relative to that vertex's own position.  Returns (clipped count,
+    per-vertex attempt index)."""
     L = lib()
     fn = L.cvh_sweep_color_f32 if f32 else L.cvh_sweep_color_f64
-    ct = C.c_float if f32 else C.c_double
-    fp = C.POINTER(ct)
+    fp = C.POINTER(C.c_double)
     fn.argtypes = [C.c_uint32, abi.u32p, fp, fp, fp, abi.u32p, C.c_void_p, abi.u32p, C.c_void_p,
                    C.c_double, C.c_double, fp, C.POINTER(C.c_uint64), abi.u32p]
     clipped = C.c_uint64(0)
     att = np.zeros(len(pos), np.uint32)
     order = np.ascontiguousarray(order, np.uint32)
-    fn(len(order), abi.ptr(order, C.c_uint32), abi.ptr(pos, ct), abi.ptr(tgt, ct), abi.ptr(ball, ct),
-       abi.ptr(term_off, C.c_uint32), terms.ctypes.data, abi.ptr(coff, C.c_uint32),
-       crec.ctypes.data if len(crec) else None, cr, ck, abi.ptr(gsq, ct), C.byref(clipped),
+    fn(len(order), abi.ptr(order, C.c_uint32), abi.ptr(pos, C.c_double), abi.ptr(tgt, C.c_double),
+       abi.ptr(ball, C.c_double), abi.ptr(term_off, C.c_uint32), terms.ctypes.data, abi.ptr(coff, C.c_uint32),
+       crec.ctypes.data if len(crec) else None, cr, ck, abi.ptr(gsq, C.c_double), C.byref(clipped),
        abi.ptr(att, C.c_uint32))
     return int(clipped.value), att

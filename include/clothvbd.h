@@ -178,7 +178,8 @@ typedef struct cv_metrics {
   double sweep_ms;            /* profiling on: summed device time of the sweep launches
                                  stamped with events (sweep_timed_launches of them), else NaN */
   uint64_t sweep_launches;    /* sweep kernel launches in the step                     */
-  double sweep_bytes;         /* algorithmic bytes those launches move in the sweep's precision */
+  double sweep_bytes;         /* algorithmic bytes those launches move as stored (f64
+                                 positions; f32 or f64 term records)                */
   uint32_t f64;               /* 1 when the sweep ran in f64, 0 for f32                */
   double sweep_bytes_min;     /* the same launches priced at the f32 record sizes:
                                  64 B per free vertex + 72 per triangle, 52 per hinge,
@@ -245,7 +246,8 @@ cv_status cv_gpu_run(cv_gpu* gpu, cv_solver* solver, cv_state* state,
                      const cv_settings* settings, uint32_t steps, cv_metrics* last_metrics);
 
 /* Device probe: per moving vertex (g.x, g.y, g.z, E) of the local objective at
- * the state's f32-rounded geometry — the WGSL `probe` entry (src/vbd.wgsl:142-147).
+ * the state's geometry, in the sweep's precision — the WGSL `probe` entry
+ * (src/vbd.wgsl:142-147).
  * `out` is [4n] f64. */
 cv_status cv_gpu_probe(cv_gpu* gpu, cv_solver* solver, const cv_state* state,
                        const cv_settings* settings, const cv_grab* grab, double* out);
@@ -383,7 +385,10 @@ size_t cv_last_error(const cv_gpu* gpu, char* buf, size_t n);
 size_t cv_solver_last_error(const cv_solver* solver, char* buf, size_t n);
 
 /* Sweep precision: 64 (default; reproduces the f64 reference step to
- * round-off) or 32 (faster; strain resolution limits parity on stiff fabrics). */
+ * round-off) or 32 (faster: positions stay f64, each vertex solve runs in f32
+ * on coordinates relative to that vertex, so f32 rounds one-ring lengths;
+ * the step then differs from the f64 step by about what the reference step
+ * itself moves when its input moves by 1e-10 m). */
 cv_status cv_gpu_set_precision(cv_gpu* gpu, int bits);
 
 /* Begin/end events around the sweep launches of every 4th iteration;

@@ -9,12 +9,14 @@ from clothcraft_amd import host
 
 
 def emulate_step(sc, state, f32=True):
+    """f32: the device's f32 sweep (f64 positions, each vertex solved in f32
+    in its own frame); otherwise the f64 sweep."""
     g, s = sc.garment, sc.settings
     n = g.n
     dt, c = s["dt"], s["damping"]
     damp = 1 + c * dt
     grav = np.array(s["gravity"])
-    dtype = np.float32 if f32 else np.float64
+    dtype = np.float64
     origin = state.x.mean(axis=0)
     target = state.x + (state.v * dt + grav * dt * dt) / damp
     inertia = g.masses * damp / (dt * dt)
@@ -37,9 +39,8 @@ def emulate_step(sc, state, f32=True):
         assert len(cs["kind_ids"]) == 0, "contact-free scenes only"
         radii = cs["radii"].copy()
         ball[:, :3] = x64 - origin
-        if f32:  # the f32 sweep's 2-ulp radius shrink (k_balls)
-            m = np.abs(ball[:, :3].astype(np.float64)).max(axis=1) + radii
-            radii = np.maximum(radii - 2 * 1.1920928955078125e-07 * m, 0)
+        if f32:  # the f32 sweep's radius shrink by four f32 ulps of itself (k_balls)
+            radii = radii - 4 * 1.1920928955078125e-07 * radii
         ball[:, 3] = radii
         return [np.nonzero(cols == cc)[0] for cc in range(k)], radii, x64.copy()
 

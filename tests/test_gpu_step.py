@@ -41,12 +41,16 @@ def _f32_step_within_reference_conditioning(gpu, sc, st, a, ref):
     final contacts frozen): within 1e-5 relative of the oracle step's."""
     from clothcraft_amd import VbdSolver
     s = sc.settings
-    p = st.copy()
-    rng = np.random.default_rng(1)
-    d = rng.uniform(-3e-8, 3e-8, p.x.shape)
-    d[sc.garment.pinned.astype(bool)] = 0.0
-    p.x = np.ascontiguousarray(p.x + d)
-    O.Solver(sc.garment, sc.avatar, p, threads=8).step(p, s)
+    def perturbed(amp):
+        p = st.copy()
+        rng = np.random.default_rng(1)
+        d = rng.uniform(-amp, amp, p.x.shape)
+        d[sc.garment.pinned.astype(bool)] = 0.0
+        p.x = np.ascontiguousarray(p.x + d)
+        O.Solver(sc.garment, sc.avatar, p, threads=8).step(p, s)
+        return p
+    p = perturbed(3e-8)
+    q = perturbed(1e-10)
     b = st.copy()
     gpu.set_precision(32)
     try:
@@ -55,13 +59,15 @@ def _f32_step_within_reference_conditioning(gpu, sc, st, a, ref):
         gpu.set_precision(64)
     diag = _bbox_diag(a.x)
     sens = np.abs(p.x - a.x).max() / diag
+    sens_sub = np.abs(q.x - a.x).max() / diag
     err = np.abs(b.x - a.x).max() / diag
     ns = O.newton_settings(dt=s["dt"], gravity=s["gravity"], damping=s["damping"])
     fz = O.frozen_c(ref.last_contacts(), s["radius"], s["stiffness"], sc.avatar.vertices)
     ea = O.newton_objective(sc.garment, st, a.x, ns, None, fz)
     eb = O.newton_objective(sc.garment, st, b.x, ns, None, fz)
     rel = abs(eb - ea) / abs(ea)
-    print(f"{sc.name} f32: rel err {err:.3e} (oracle under a 1-ulp input perturbation {sens:.3e}), "
+    print(f"{sc.name} f32: rel err {err:.3e} (oracle under a 1-ulp input perturbation {sens:.3e}, "
+          f"under 1e-10 m {sens_sub:.3e}), "
           f"objective {eb:.9e} vs {ea:.9e} (rel {rel:.2e}), contacts {mg['contact_terms']}")
     assert np.isfinite(b.x).all() and mg["minimum_distance_m"] > 0.0
     assert err <= max(1e-5, 2.0 * sens)
@@ -98,7 +104,7 @@ def test_free_fall_matches_damped_implicit_euler(gpu):
         assert np.abs(a.x - st.x).max() / _bbox_diag(a.x) < 1e-9
 
 
-@pytest.mark.parametrize("bits,tol", [(64, 1e-9), (32, 1e-5)])
+@pytest.mark.parametrize("bits,tol", [(64, 1e-9), (32, 1e-6)])
 def test_one_step_parity_hanging_sheet(gpu, bits, tol):
     sc = scenes.hanging_sheet()
     sg, sr, mg, mr = _one_step(gpu, sc, bits=bits)
@@ -110,9 +116,9 @@ def test_one_step_parity_hanging_sheet(gpu, bits, tol):
     assert err <= tol
     if bits == 64:  # the lagging sweep residual (src/vbd.rs:396-397), summed in f64
         assert abs(mg["residual_rms_n"] - mr["residual_rms_n"]) <= 1e-9 * mr["residual_rms_n"]
-    # Extension in percent points: the f32 step's position error (a few 1e-6
-    # of the diagonal, i.e. ~1e-5 of an edge) moves it by ~1e-4 points.
-    dtol = 1e-6 if bits == 64 else 1e-3
+    # Extension in percent points: the f32 step's position error (below 1e-6
+    # of the diagonal, i.e. ~1e-5 of an edge) moves it by less than 1e-4 points.
+    dtol = 1e-6 if bits == 64 else 1e-4
     for key in ("extension_p95_percent", "extension_max_percent"):
         assert abs(mg[key] - mr[key]) <= dtol * (1.0 + abs(mr[key])), key
 

@@ -218,8 +218,9 @@ def test_create_errors_match_the_reference_messages():
 def test_device_step_logic_replayed_on_cpu_matches_the_oracle():
     """The device step replayed on the CPU with the kernel's own math
     (host build of vbd_math.hpp): in f64 it reproduces the oracle step to
-    round-off; in f32 it stays within the 1e-5 relative parity bar on the
-    32x32 hanging sheet."""
+    round-off; in f32 (each vertex solved in its own frame, positions kept in
+    f64) it stays within 1e-6 of the box on the 32x32 hanging sheet, a tenth
+    of the 1e-5 parity bar (measured 6.6e-8)."""
     from device_emulation import emulate_step
     sc = scenes.hanging_sheet()
     a = sc.state.copy()
@@ -228,7 +229,7 @@ def test_device_step_logic_replayed_on_cpu_matches_the_oracle():
     x64 = emulate_step(sc, sc.state.copy(), f32=False)
     assert np.abs(x64 - a.x).max() / diag < 1e-10
     x32 = emulate_step(sc, sc.state.copy(), f32=True)
-    assert np.abs(x32 - a.x).max() / diag < 1e-5
+    assert np.abs(x32 - a.x).max() / diag < 1e-6
 
 
 def _step_objective(sc, x):
@@ -250,8 +251,8 @@ def _oracle_step_after_perturbation(sc, amp, seed=1):
 
 
 def test_reference_step_conditioning_bounds_fp32_position_parity():
-    """Why f32 positions cannot meet 1e-5 of the box on the sheet-on-sphere
-    material, and what f32 does meet there.
+    """Why no f32 step can meet 1e-5 of the box on the sheet-on-sphere
+    material, and what the f32 step does meet there.
 
     The reference step itself (the f64 oracle) amplifies a 3e-12 m input
     perturbation, far below one f32 ulp of a position, past the 1e-5 bar:
@@ -262,23 +263,25 @@ def test_reference_step_conditioning_bounds_fp32_position_parity():
     direction (inertia ~0.5 N/m), so Gauss-Seidel multiplies out-of-plane
     differences by ~1e4 in the first iteration.  The hanging sheet is in
     tension and amplifies ~1e2.  What is well-posed is the step's objective:
-    the f32 replay (the device's own math on the CPU) lands on the oracle's
-    objective to well within 1e-5 relative, and its positions differ from
-    the oracle's no more than the oracle's own step does under a one-ulp
-    (3e-8 m) input perturbation.  f64 reproduces the oracle to round-off."""
+    the f32 replay (the device's own math on the CPU, each vertex solved in
+    its own frame) lands on the oracle's objective to within 1e-6 relative,
+    and its positions differ from the oracle's no more than twice as much as
+    the oracle's own step does when its input moves by 1e-10 m (300 times
+    below one f32 ulp of an absolute position).  f64 reproduces the oracle to
+    round-off."""
     from device_emulation import emulate_step
     sc = scenes.small_sheet_on_sphere(32, 0.4)
     a = sc.state.copy()
     O.Solver(sc.garment, sc.avatar, a, threads=8).step(a, sc.settings)
     diag = np.linalg.norm(a.x.max(0) - a.x.min(0))
     sens_tiny = np.abs(_oracle_step_after_perturbation(sc, 3e-12) - a.x).max() / diag
-    sens_ulp = np.abs(_oracle_step_after_perturbation(sc, 3e-8) - a.x).max() / diag
+    sens_sub = np.abs(_oracle_step_after_perturbation(sc, 1e-10) - a.x).max() / diag
     assert sens_tiny > 1e-5  # the reference's own conditioning exceeds the bar
     e64 = np.abs(emulate_step(sc, sc.state.copy(), f32=False) - a.x).max() / diag
     x32 = emulate_step(sc, sc.state.copy(), f32=True)
     e32 = np.abs(x32 - a.x).max() / diag
     assert e64 < 1e-10
-    assert e32 <= 2 * sens_ulp
+    assert e32 <= 2 * sens_sub
     ea, e = _step_objective(sc, a.x), _step_objective(sc, x32)
     assert abs(e - ea) / abs(ea) < 1e-6
     # The well-conditioned hanging sheet: positions within the bar.
@@ -288,8 +291,8 @@ def test_reference_step_conditioning_bounds_fp32_position_parity():
     hdiag = np.linalg.norm(b.x.max(0) - b.x.min(0))
     assert np.abs(_oracle_step_after_perturbation(hs, 3e-8) - b.x).max() / hdiag < 1e-5
     y32 = emulate_step(hs, hs.state.copy(), f32=True)
-    assert np.abs(y32 - b.x).max() / hdiag < 1e-5
-    assert abs(_step_objective(hs, y32) - _step_objective(hs, b.x)) / _step_objective(hs, b.x) < 1e-5
+    assert np.abs(y32 - b.x).max() / hdiag < 1e-6
+    assert abs(_step_objective(hs, y32) - _step_objective(hs, b.x)) / _step_objective(hs, b.x) < 1e-7
 
 
 def test_runtime_gradient_check_host_half():
