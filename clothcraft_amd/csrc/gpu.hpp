@@ -177,7 +177,7 @@ struct StepBufs {
   DBuf<uint32_t> raw_ids;   // [5*cap] kind, ids
   DBuf<uint32_t> raw_count;
   DBuf<double> raw_eps;
-  DBuf<unsigned long long> key_hi, key_lo, key_tmp;
+  DBuf<unsigned long long> key_hi, key_lo, key_tmp, key_hi_sorted;
   DBuf<uint32_t> perm, perm_tmp;
   DBuf<uint32_t> uniq_flag, uniq_pos;
   DBuf<uint32_t> c_ids;     // unique contacts [5*cap]

@@ -1416,6 +1416,7 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   b.key_hi.reserve(m);
   b.key_lo.reserve(m);
   b.key_tmp.reserve(m);
+  b.key_hi_sorted.reserve(m);
   b.perm.reserve(m);
   b.perm_tmp.reserve(m);
   b.uniq_flag.reserve(m + 1);
@@ -1428,11 +1429,11 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
     });
     k_gather_u64<<<blocks(m), TPB, 0, st>>>(b.key_hi.p, b.perm_tmp.p, m, b.key_tmp.p);
     with_temp(g, [&](void* t, size_t& bytes) {  // stable by high key
-      return rocprim::radix_sort_pairs(t, bytes, b.key_tmp.p, b.cell_key.p /*scratch*/, b.perm_tmp.p,
-                                       b.perm.p, (size_t)m, 0, 64, st);
+      return rocprim::radix_sort_pairs(t, bytes, b.key_tmp.p, b.key_hi_sorted.p, b.perm_tmp.p, b.perm.p,
+                                       (size_t)m, 0, 64, st);
     });
-    // b.cell_key now holds hi keys in final order; perm the permutation.
-    k_unique_flags<<<blocks(m), TPB, 0, st>>>(b.cell_key.p, b.key_lo.p, b.perm.p, m, b.uniq_flag.p);
+    // key_hi_sorted holds the high keys in final order, perm the permutation.
+    k_unique_flags<<<blocks(m), TPB, 0, st>>>(b.key_hi_sorted.p, b.key_lo.p, b.perm.p, m, b.uniq_flag.p);
     CV_HIP(hipMemsetAsync(b.uniq_flag.p + m, 0, sizeof(uint32_t), st));
     scan_u32(g, b.uniq_flag.p, b.uniq_pos.p, (size_t)m + 1);
     k_unique_write<<<blocks(m), TPB, 0, st>>>(b.raw_ids.p, b.raw_count.p, b.raw_eps.p, b.perm.p,

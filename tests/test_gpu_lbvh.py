@@ -61,3 +61,23 @@ def test_lbvh_steps_are_bitwise_the_grid_steps(gpu):
     assert g["contact_terms"] == l["contact_terms"] > 0
     assert np.array_equal(out["grid"].x.view(np.uint64), out["lbvh"].x.view(np.uint64))
     assert np.array_equal(out["grid"].v.view(np.uint64), out["lbvh"].v.view(np.uint64))
+
+
+def test_lbvh_from_a_fresh_context_through_contact():
+    """A context whose every detection uses the LBVH (no grid detection ever
+    allocated the grid's arrays) steps a sheet onto the sphere through its
+    first contacts exactly as a grid context does: the contact dedup sort has
+    scratch of its own, sized to the raw contact count."""
+    from clothcraft_amd import GpuVbd, VbdSolver, scenes
+    sc = scenes.small_sheet_on_sphere(16, 0.2535)
+    out, met = {}, {}
+    for broad in ("lbvh", "grid"):
+        gpu = GpuVbd(0)
+        gpu.set_broad_phase(broad)
+        s = sc.state.copy()
+        met[broad] = gpu.run(VbdSolver(sc.garment, sc.avatar, s), s, sc.settings, 3)
+        out[broad] = s
+    print(f"lbvh: {met['lbvh']['contact_terms']} contact terms, {met['lbvh']['detection_updates']} detections")
+    assert met["lbvh"]["contact_terms"] == met["grid"]["contact_terms"] > 0
+    assert met["lbvh"]["broad_phase_fallbacks"] == 0
+    assert np.array_equal(out["grid"].x.view(np.uint64), out["lbvh"].x.view(np.uint64))
