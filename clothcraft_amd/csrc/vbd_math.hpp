@@ -167,6 +167,39 @@ CV_HD void deriv_add(Deriv<R>& a, const Deriv<R>& b) {
 
 // ------------------------------------------------------------- membrane
 // src/energy.rs:10-15 with derivatives of the smooth tension/compression split.
+// Reciprocals, quotients and square roots of the term math.  f64 keeps IEEE
+// division and square root (the reference's rounding).  The f32 sweep uses
+// the hardware reciprocal and square root (1 ulp each) on the device, a
+// fraction of the instructions of the correctly rounded sequences; its
+// parity bounds are set by the reference step's conditioning, far above an
+// ulp.  The host build (CPU tests) divides exactly.
+template <class R>
+CV_HD R recip(R x) {
+  return R(1) / x;
+}
+template <class R>
+CV_HD R quot(R a, R b) {
+  return a / b;
+}
+template <class R>
+CV_HD R root(R x) {
+  return sqrt(x);
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+template <>
+CV_HD float recip<float>(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+template <>
+CV_HD float quot<float>(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
+}
+template <>
+CV_HD float root<float>(float x) {
+  return __builtin_amdgcn_sqrtf(x);
+}
+#endif
+
 template <class R>
 CV_HD void axial3(R s, R k, R c, R delta, R& e, R& e1, R& e2) {
   if (c == R(1) && delta > R(0)) {
@@ -177,10 +210,10 @@ CV_HD void axial3(R s, R k, R c, R delta, R& e, R& e1, R& e2) {
     e2 = k * c;
     return;
   }
-  R r = sqrt(s * s + delta * delta);
+  R r = root(s * s + delta * delta);
   R p = (s + r - delta) * R(0.5);
-  R p1 = R(0.5) * (R(1) + s / r);
-  R p2 = R(0.5) * delta * delta / (r * r * r);
+  R p1 = R(0.5) * (R(1) + quot(s, r));
+  R p2 = quot(R(0.5) * delta * delta, r * r * r);
   e = R(0.5) * k * (c * s * s + (R(1) - c) * p * p);
   e1 = k * (c * s + (R(1) - c) * p * p1);
   e2 = k * (c + (R(1) - c) * (p1 * p1 + p * p2));
@@ -188,7 +221,7 @@ CV_HD void axial3(R s, R k, R c, R delta, R& e, R& e1, R& e2) {
 template <class R>
 CV_HD R axial_e(R s, R k, R c, R delta) {
   if (c == R(1) && delta > R(0)) return R(0.5) * k * (c * s * s);  // as in axial3
-  R r = sqrt(s * s + delta * delta);
+  R r = root(s * s + delta * delta);
   R p = (s + r - delta) * R(0.5);
   return R(0.5) * k * (c * s * s + (R(1) - c) * p * p);
 }
@@ -240,8 +273,8 @@ CV_HD R bending_e(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, const F* f) {
   V3<R> e = x1 - x0;
   V3<R> n1 = cross(e, x2 - x0);
   V3<R> n2 = cross(x3 - x0, e);
-  R l1 = sqrt(dot(n1, n1)), l2 = sqrt(dot(n2, n2));
-  V3<R> d = n1 * (R(1) / l1) - n2 * (R(1) / l2);
+  R l1 = root(dot(n1, n1)), l2 = root(dot(n2, n2));
+  V3<R> d = n1 * recip(l1) - n2 * recip(l2);
   return R(0.5) * R(f[0]) * dot(d, d);
 }
 template <class R, class F>
@@ -250,9 +283,9 @@ CV_HD Deriv<R> bending_d(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, int slot, const
   V3<R> e = x1 - x0, u = x2 - x0, w = x3 - x0;
   V3<R> n1 = cross(e, u);
   V3<R> n2 = cross(w, e);
-  R l1 = sqrt(dot(n1, n1)), l2 = sqrt(dot(n2, n2));
+  R l1 = root(dot(n1, n1)), l2 = root(dot(n2, n2));
   // One division per normal; the Jacobian and curvature reuse the reciprocals.
-  R il1 = R(1) / l1, il2 = R(1) / l2;
+  R il1 = recip(l1), il2 = recip(l2);
   V3<R> a = n1 * il1, b = n2 * il2;
   V3<R> D = a - b;
   // dn1 = a1 x delta, dn2 = a2 x delta for the moving slot.
@@ -341,7 +374,7 @@ CV_HD R activation_v(R d, R r, R k) {
     R g = r - d;
     return R(0.5) * k * g * g;
   }
-  return R(0.5) * k * tau * tau - k * tau * tau * R(log(d / tau));
+  return R(0.5) * k * tau * tau - k * tau * tau * R(log(quot(d, tau)));
 }
 // Contact::energy (src/ogc/potential.rs:39-65).
 template <class R>
@@ -349,12 +382,12 @@ CV_HD R contact_e(int kind, const V3<R>* p, R eps, R r, R k) {
   R d, nn = R(0);
   if (kind == KIND_PP) {
     V3<R> v = p[0] - p[1];
-    d = sqrt(dot(v, v));
+    d = root(dot(v, v));
   } else if (kind == KIND_PE) {
     V3<R> u = p[2] - p[1], w = p[0] - p[1];
-    R s = dot(w, u) / dot(u, u);
+    R s = quot(dot(w, u), dot(u, u));
     V3<R> dd = w - u * s;
-    d = sqrt(dot(dd, dd));
+    d = root(dot(dd, dd));
   } else {
     // PF: n = (p2 - p1) x (p3 - p1), r = p0 - p1; EE: n = (p1 - p0) x (p3 - p2),
     // r = p0 - p2 (one code path for both: a wavefront holding both kinds
@@ -363,12 +396,12 @@ CV_HD R contact_e(int kind, const V3<R>* p, R eps, R r, R k) {
     V3<R> n = cross(pf ? p[2] - p[1] : p[1] - p[0], pf ? p[3] - p[1] : p[3] - p[2]);
     nn = dot(n, n);
     if (!pf && nn < R(1e-30)) return R(0);
-    R h = dot(pf ? p[0] - p[1] : p[0] - p[2], n) / sqrt(nn);
+    R h = quot(dot(pf ? p[0] - p[1] : p[0] - p[2], n), root(nn));
     d = h < R(0) ? -h : h;
   }
   R e = activation_v(d, r, k);
   if (kind == KIND_EE) {
-    R qq = nn * (R(1) / eps);
+    R qq = nn * recip(eps);
     if (qq < R(1)) e = e * qq * (R(2) - qq);
   }
   return e;
@@ -382,8 +415,8 @@ CV_HD void activation_d(R d, R r, R k, R& e1, R& e2) {
     e2 = k;
   } else {
     R kt = k * tau * tau;
-    e1 = -kt / d;
-    e2 = kt / (d * d);
+    e1 = -quot(kt, d);
+    e2 = quot(kt, d * d);
   }
 }
 // Unsigned plane distance d = |r . n| / |n| and its derivatives in one
@@ -396,11 +429,11 @@ CV_HD void activation_d(R d, R r, R k, R& e1, R& e2) {
 // from dn^ = (I - n^ n^T) dn / m and dm = c_a . dx; d = sign(h) h.
 template <class R>
 CV_HD void plane_dist_d(V3<R> r, V3<R> n, R nn, V3<R> a, R c, R& d, V3<R>& g, S3<R>& H) {
-  R m = sqrt(nn);
-  R h = dot(r, n) / m;  // as contact_e
+  R m = root(nn);
+  R h = quot(dot(r, n), m);  // as contact_e
   R sg = h < R(0) ? R(-1) : R(1);
   d = h < R(0) ? -h : h;
-  R im = R(1) / m;
+  R im = recip(m);
   V3<R> nh = n * im;
   V3<R> ca = cross(nh, a);
   V3<R> w = cross(r - nh * h, a);
@@ -441,8 +474,8 @@ CV_HD Deriv<R> contact_d(int kind, const V3<R>* p, int slot, R eps, R r, R k) {
   R nn = R(0);
   if (kind == KIND_PP) {
     V3<R> v = p[0] - p[1];
-    d = sqrt(dot(v, v));
-    R id = R(1) / d;
+    d = root(dot(v, v));
+    R id = recip(d);
     V3<R> uh = v * id;
     gd = slot == 0 ? uh : -uh;
     s3_add_diag(hd, id);
@@ -450,21 +483,21 @@ CV_HD Deriv<R> contact_d(int kind, const V3<R>* p, int slot, R eps, R r, R k) {
   } else if (kind == KIND_PE) {
     V3<R> u = p[2] - p[1], w = p[0] - p[1];
     R uu = dot(u, u);
-    R s = dot(w, u) / uu;
+    R s = quot(dot(w, u), uu);
     V3<R> t = w - u * s;
-    d = sqrt(dot(t, t));
-    R id = R(1) / d;
+    d = root(dot(t, t));
+    R id = recip(d);
     V3<R> th = t * id;
     R b = slot == 0 ? R(1) : (slot == 1 ? R(1) - s : s);
     gd = slot == 0 ? th : th * (-b);
     R bb = b * b * id;
     s3_add_diag(hd, bb);
-    s3_add_outer(hd, u, -bb / uu);
+    s3_add_outer(hd, u, -quot(bb, uu));
     s3_add_outer(hd, th, -bb);
     if (slot != 0) {
       R gam = slot == 1 ? s - R(1) : s;
-      s3_add_sym(hd, th, u, gam / uu);
-      s3_add_outer(hd, th, -d / uu);
+      s3_add_sym(hd, th, u, quot(gam, uu));
+      s3_add_outer(hd, th, -quot(d, uu));
     }
   } else {  // PF and EE share one path (see contact_e)
     const bool pf = kind == KIND_PF;
@@ -482,7 +515,7 @@ CV_HD Deriv<R> contact_d(int kind, const V3<R>* p, int slot, R eps, R r, R k) {
     plane_dist_d(pf ? p[0] - p[1] : p[0] - p[2], n, nn, a, c, d, gd, hd);
   }
   R phi = activation_v(d, r, k);
-  R q = kind == KIND_EE ? nn * (R(1) / eps) : R(1);
+  R q = kind == KIND_EE ? nn * recip(eps) : R(1);
   bool moll = kind == KIND_EE && q < R(1);
   o.e = moll ? phi * q * (R(2) - q) : phi;
   if (d >= r) return o;
@@ -503,7 +536,7 @@ CV_HD Deriv<R> contact_d(int kind, const V3<R>* p, int slot, R eps, R r, R k) {
     return o;
   }
   R M = q * (R(2) - q), M1 = R(2) - R(2) * q;
-  R te = R(2) / eps;
+  R te = quot(R(2), eps);
   V3<R> gq = cross(n, a) * te;
   o.g = gd * (e1 * M) + gq * (phi * M1);
   s3_add_outer(o.h, gd, e2 * M);
@@ -641,7 +674,7 @@ CV_HD V3<R> newton_direction(Deriv<R>& tot, V3<R> x, V3<R> target, R inertia) {
   R c00 = d * f - e * e, c01 = c * e - b * f, c02 = b * e - c * d;
   R c11 = a * f - c * c, c12 = b * c - a * e, c22 = a * d - b * b;
   R det = a * c00 + b * c01 + c * c02;
-  R inv = R(1) / det;
+  R inv = recip(det);
   V3<R> g = tot.g;
   return mk<R>(-(c00 * g.x + c01 * g.y + c02 * g.z) * inv, -(c01 * g.x + c11 * g.y + c12 * g.z) * inv,
                -(c02 * g.x + c12 * g.y + c22 * g.z) * inv);
@@ -668,10 +701,10 @@ CV_HD float accept_slack<float>(float old) {
 template <class R>
 CV_HD V3<R> truncate_ball(V3<R> p, V3<R> c, R radius, bool& clipped) {
   V3<R> d = p - c;
-  R len = sqrt(dot(d, d));
+  R len = root(dot(d, d));
   if (len > radius) {
     clipped = true;
-    return c + d * (radius / len);
+    return c + d * quot(radius, len);
   }
   return p;
 }
