@@ -338,6 +338,7 @@ def bench_step(args):
         "contacts_last_step": last["contact_terms"],
         "colors_last_step": last["colors"],
         "candidate_pairs_last_step": last["candidate_pairs"],
+        "broad_phase_tests_last_step": last["candidate_pairs"] + last["aabb_rejections"],
         "detections": m["detection_updates"],
         "sweep_share_of_step": sweep_s / dev_s,
         "warm_vs_plain": compare,
