@@ -148,6 +148,9 @@ struct StepBufs {
   DBuf<uint32_t> order;
   DBuf<uint4> vinfo;  // colour-ordered (vertex, term offset, term count, contact offset)
   DBuf<uint32_t> vnc;  // colour-ordered contact-record counts
+  // 32-lane sweep: slots whose vertex has more than 32 records (flag, its
+  // exclusive scan over the slots, the slots in order), swept by 64 lanes
+  DBuf<uint32_t> hflag, hpos, hlist;
   // fast colouring: contact-neighbour CSR, round states, pending-round
   // flags, colour histogram, sort keys
   DBuf<uint32_t> fc_cnt, fc_off, fc_cur, fc_nb, fc_state, fc_pend, fc_hist, fc_key, fc_key_sorted,

@@ -60,6 +60,10 @@ struct SweepArgs {
   double* gsq;  // per-vertex |g|^2 of the sweep (f64 in both precisions)
   unsigned long long* clipped;
   const unsigned int* stop;  // set when a refresh is pending: the launch is a no-op
+  // 32-lane sweep only (else null): exclusive scan over the slots of the
+  // "more than 32 records" flag, and those slots in order (k_heavy_*).
+  const uint32_t* hpos;
+  const uint32_t* hlist;
   R cr, ck;
   GrabArgs<R> grab;
 };
@@ -367,19 +371,58 @@ __device__ __forceinline__ void iter_tail(const IterTail& t) {
   iter_end(t.sc, c, t.it, t.threshold, 0.0, 1.0);
 }
 
-// One colour of one iteration.
+// A vertex with more records than a 32-lane group holds would have its
+// lanes re-read the extra records on every pass (the records past the first
+// L are not kept in registers), which makes it several times slower than the
+// rest and, although such vertices are rare (~0.1 % on a garment in
+// contact), the slowest wavefront of most launches.  The 32-lane sweep
+// therefore leaves them to kHeavyBlocks extra blocks of the same launch,
+// where each takes a whole 64-lane wavefront.
+constexpr uint32_t kHeavyBlocks = 16;
+constexpr uint32_t kHeavyWaves = TPB / 64;
+__global__ void k_heavy_flags(const uint4* vinfo, const uint32_t* vnc, uint32_t n, uint32_t* flag) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= n) flag[i] = i < n && vinfo[i].z + vnc[i] > 32u ? 1u : 0u;
+}
+__global__ void k_heavy_list(const uint32_t* flag, const uint32_t* pos, uint32_t n, uint32_t* list) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) list[pos[i]] = i;
+}
+// Heavy wavefront w of W sweeps entries w, w + W, ... of [h0, h1) of the
+// heavy list (the heavy slots of slot range [off, off + count)).
+template <class R>
+__device__ __forceinline__ void sweep_heavy(const SweepArgs<R>& a, uint32_t h0, uint32_t h1, uint32_t w,
+                                            uint32_t W) {
+  const int lane = threadIdx.x & 63;
+  for (uint32_t k = h0 + w; k < h1; k += W) {
+    const uint32_t slot = a.hlist[k];
+    solve_vertex<64, R>(a, a.vinfo[slot], a.vnc[slot], lane);
+  }
+}
+
+// One colour of one iteration.  hb > 0 (32 lanes per vertex only): the last
+// hb logical blocks sweep the colour's vertices with more than 32 records,
+// 64 lanes each, and the 32-lane groups skip them.
 template <int L, class R>
-__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<R> a, uint32_t off, uint32_t count, IterTail tail) {
+__global__ __launch_bounds__(TPB, sizeof(R) == 4 ? 4 : 2) void k_sweep(SweepArgs<R> a, uint32_t off,
+                                                                       uint32_t count, IterTail tail, uint32_t hb) {
   const int lane = threadIdx.x & (L - 1);
   // Vertices of a colour are in index order, so neighbouring groups share
   // neighbours: keep consecutive groups on one XCD's L2.
-  const uint32_t grp = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / L;
-  if (grp < count) {
-    // The vertex's slot data and the stop flag are loaded together (one
-    // memory round trip instead of two before the solve starts).
-    const uint4 vi = a.vinfo[off + grp];
-    const uint32_t nc = a.vnc[off + grp];
-    if (!*a.stop) solve_vertex<L, R>(a, vi, nc, lane);
+  const uint32_t lb = xcd_block(blockIdx.x, gridDim.x), nb = gridDim.x - hb;
+  if (lb < nb) {
+    const uint32_t grp = (lb * blockDim.x + threadIdx.x) / L;
+    if (grp < count) {
+      // The vertex's slot data and the stop flag are loaded together (one
+      // memory round trip instead of two before the solve starts).
+      const uint4 vi = a.vinfo[off + grp];
+      const uint32_t nc = a.vnc[off + grp];
+      const bool heavy = L == 32 && hb && vi.z + nc > 32u;
+      if (!heavy && !*a.stop) solve_vertex<L, R>(a, vi, nc, lane);
+    }
+  } else if (L == 32 && !*a.stop) {
+    sweep_heavy(a, a.hpos[off], a.hpos[off + count], (lb - nb) * kHeavyWaves + threadIdx.x / 64,
+                hb * kHeavyWaves);
   }
   if (tail.sc) iter_tail(tail);  // block-uniform
 }
@@ -393,9 +436,25 @@ struct BatchSlot {
 };
 template <int L, class R>
 __global__ __launch_bounds__(TPB) void k_sweep_batch(const SweepArgs<R>* args, const BatchSlot* slots,
-                                                     uint32_t nslots, uint32_t total) {
+                                                     uint32_t nslots, uint32_t total, uint32_t hb) {
   const int lane = threadIdx.x & (L - 1);
-  const uint32_t grp = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / L;
+  const uint32_t lb = xcd_block(blockIdx.x, gridDim.x), nb = gridDim.x - hb;
+  if (L == 32 && lb >= nb) {
+    // The heavy vertices of every slot, as in k_sweep: heavy wavefront w of W
+    // takes entries w, w + W, ... of the slots' heavy ranges laid end to end.
+    const uint32_t w = (lb - nb) * kHeavyWaves + threadIdx.x / 64, W = hb * kHeavyWaves;
+    uint32_t base = 0;
+    for (uint32_t s = 0; s < nslots; ++s) {
+      const BatchSlot sl = slots[s];
+      const SweepArgs<R>& a = args[sl.inst];
+      const uint32_t h0 = a.hpos[sl.off], h1 = a.hpos[sl.off + sl.count];
+      const uint32_t first = base + (w >= base ? w - base : (W - (base - w) % W) % W);
+      if (!*a.stop && first < base + (h1 - h0)) sweep_heavy(a, h0 + (first - base), h1, 0, W);
+      base += h1 - h0;
+    }
+    return;
+  }
+  const uint32_t grp = (lb * blockDim.x + threadIdx.x) / L;
   if (grp >= total) return;
   uint32_t lo = 0, hi = nslots;
   while (hi - lo > 1) {
@@ -407,7 +466,10 @@ __global__ __launch_bounds__(TPB) void k_sweep_batch(const SweepArgs<R>* args, c
   const SweepArgs<R>& a = args[sl.inst];
   if (*a.stop) return;
   const uint32_t slot = sl.off + grp - sl.start;
-  solve_vertex<L, R>(a, a.vinfo[slot], a.vnc[slot], lane);
+  const uint4 vi = a.vinfo[slot];
+  const uint32_t nc = a.vnc[slot];
+  if (L == 32 && hb && vi.z + nc > 32u) return;  // swept by the heavy blocks
+  solve_vertex<L, R>(a, vi, nc, lane);
 }
 
 struct IterEnd {
@@ -883,6 +945,8 @@ SweepArgs<R> sweep_args(cv_gpu& g, const cv_settings* set, const cv_grab* grab) 
   a.gsq = g.b.gsq.p;
   a.clipped = &g.d_scal->clipped;
   a.stop = &g.d_scal->stop;
+  a.hpos = g.lanes == 32 ? g.b.hpos.p : nullptr;
+  a.hlist = g.lanes == 32 ? g.b.hlist.p : nullptr;
   a.cr = (R)set->contact.radius;
   a.ck = (R)set->contact.stiffness;
   a.grab.active = grab ? 1 : 0;
@@ -903,11 +967,12 @@ SweepArgs<R> sweep_args(cv_gpu& g, const cv_settings* set, const cv_grab* grab) 
 template <int L, class R>
 void launch_sweep(const SweepArgs<R>& a, uint32_t off, uint32_t count, const IterTail& tail, hipStream_t st,
                   hipEvent_t e0, hipEvent_t e1) {
-  dim3 grid(blocks((uint64_t)count * L)), block(TPB);
+  const uint32_t hb = L == 32 && a.hpos ? kHeavyBlocks : 0;
+  dim3 grid(blocks((uint64_t)count * L) + hb), block(TPB);
   if (e0)
-    hipExtLaunchKernelGGL(k_sweep<L, R>, grid, block, 0, st, e0, e1, 0, a, off, count, tail);
+    hipExtLaunchKernelGGL(k_sweep<L, R>, grid, block, 0, st, e0, e1, 0, a, off, count, tail, hb);
   else
-    k_sweep<L, R><<<grid, block, 0, st>>>(a, off, count, tail);
+    k_sweep<L, R><<<grid, block, 0, st>>>(a, off, count, tail, hb);
 }
 template <class R>
 void sweep_color(cv_gpu& g, const SweepArgs<R>& a, uint32_t off, uint32_t count, const IterTail& tail = IterTail{},
@@ -982,6 +1047,16 @@ void vertex_info(cv_gpu& g, uint32_t n) {
   k_vertex_info<<<blocks(n), TPB, 0, g.stream>>>(g.b.order.p, g.dev.term_off.p, g.b.coff.p, n, g.b.vinfo.p,
                                                  g.b.vnc.p);
   CV_HIP(hipGetLastError());
+  if (g.lanes == 32) {
+    g.b.hflag.reserve(n + 1);
+    g.b.hpos.reserve(n + 1);
+    g.b.hlist.reserve(std::max<uint32_t>(n, 1));
+    k_heavy_flags<<<blocks(n + 1), TPB, 0, g.stream>>>(g.b.vinfo.p, g.b.vnc.p, n, g.b.hflag.p);
+    CV_HIP(hipGetLastError());
+    device_scan_u32(g, g.b.hflag.p, g.b.hpos.p, (size_t)n + 1);
+    k_heavy_list<<<blocks(n), TPB, 0, g.stream>>>(g.b.hflag.p, g.b.hpos.p, n, g.b.hlist.p);
+    CV_HIP(hipGetLastError());
+  }
 }
 
 // The fast colouring (on the device, gpu_fast_colors) of the last detection;
@@ -1587,10 +1662,17 @@ void run_batch_t(cv_gpu& P, const std::vector<cv_gpu*>& G, const std::vector<cv_
         const BatchSlot* sl = dslots + l.slot0;
         uint32_t ns = (uint32_t)l.nslots;
         switch (P.lanes) {
-          case 8: k_sweep_batch<8, R><<<blocks((uint64_t)l.total * 8), TPB, 0, st>>>(dargs, sl, ns, l.total); break;
-          case 16: k_sweep_batch<16, R><<<blocks((uint64_t)l.total * 16), TPB, 0, st>>>(dargs, sl, ns, l.total); break;
-          case 64: k_sweep_batch<64, R><<<blocks((uint64_t)l.total * 64), TPB, 0, st>>>(dargs, sl, ns, l.total); break;
-          default: k_sweep_batch<32, R><<<blocks((uint64_t)l.total * 32), TPB, 0, st>>>(dargs, sl, ns, l.total); break;
+          case 8: k_sweep_batch<8, R><<<blocks((uint64_t)l.total * 8), TPB, 0, st>>>(dargs, sl, ns, l.total, 0); break;
+          case 16:
+            k_sweep_batch<16, R><<<blocks((uint64_t)l.total * 16), TPB, 0, st>>>(dargs, sl, ns, l.total, 0);
+            break;
+          case 64:
+            k_sweep_batch<64, R><<<blocks((uint64_t)l.total * 64), TPB, 0, st>>>(dargs, sl, ns, l.total, 0);
+            break;
+          default:  // every instance context sweeps with 32 lanes and holds its heavy lists
+            k_sweep_batch<32, R><<<blocks((uint64_t)l.total * 32) + kHeavyBlocks, TPB, 0, st>>>(
+                dargs, sl, ns, l.total, kHeavyBlocks);
+            break;
         }
       }
       for (auto& r : rr)
