@@ -48,14 +48,15 @@ roofline: the sweep kernel (k_sweep), algorithmic bytes per launch at the f32
 minimum record sizes (64 B per free vertex + 72 per triangle, 52 per hinge,
 68 per stitch, 56 per contact incidence), divided by its mean duration over
 the sampled launches (every launch of every 4th iteration of the timed
-window): the HIP events recorded around each of those launches on the
-solver's stream (hipExtLaunchKernelGGL start/stop events), less the event
-interval of an empty launch on the same stream measured in the same run
-(the dispatch latency inside every interval; both are reported);
-`traffic` is the measured HBM bytes per launch (rocprofv3 FETCH_SIZE +
-WRITE_SIZE, calibrated by a copy kernel) from the committed counter runs of
-this command (same workload, tolerance, precision and lanes; the timed step
-count does not change the per-launch work), null for other arguments.
+window): the interval between the HIP events recorded around each of those
+launches on the solver's stream (hipExtLaunchKernelGGL start/stop events).
+The interval of an empty launch on the same stream is reported beside it.
+Switching profiling on and off brackets the timed window with empty kernels
+(k_empty), so a rocprofv3 trace or counter run of this command can be cut to
+the window (`--window-stats`); `traffic` is the measured HBM bytes per
+launch (rocprofv3 FETCH_SIZE + WRITE_SIZE over the window, calibrated by a
+copy kernel) from the committed counter runs of this command (same workload,
+tolerance, precision, lanes and broad phase), null for other arguments.
 
 cpu_baseline: the f64 CPU restatement of VbdSolver::step (OpenMP within a
 colour) on the host, from the settled state with the same settings, on the
@@ -89,6 +90,57 @@ def _traffic(kernel, workload, argv_key):
     except (OSError, ValueError):
         return None
     return d.get("bytes_per_launch") if d.get("args") == argv_key else None
+
+
+def window_stats(path, real=0):
+    """Per-kernel mean duration (a kernel trace) or per-kernel, per-counter
+    mean (a counter collection) over the launches of the timed window: the
+    launches after the burst of empty kernels that switching profiling on
+    issues (its event calibration) and before the empty kernel that switching
+    it off issues."""
+    import csv
+    with open(path) as f:
+        rows = list(csv.DictReader(f))
+    trace = "Start_Timestamp" in rows[0] and "Counter_Name" not in rows[0]
+    key = (lambda r: int(r["Start_Timestamp"])) if trace else (lambda r: int(r["Dispatch_Id"]))
+    rows.sort(key=key)
+    empty = [i for i, r in enumerate(rows) if "k_empty" in r["Kernel_Name"]]
+    if not empty:
+        raise SystemExit("no profiling window (k_empty launches) in " + path)
+    # The window opens after the first burst of consecutive empty launches and
+    # closes at the next empty launch.
+    first = empty[0]
+    while first + 1 < len(rows) and "k_empty" in rows[first + 1]["Kernel_Name"]:
+        first += 1
+    later = [i for i in empty if i > first]
+    last = later[0] if later else len(rows)
+    out = {}
+    for r in rows[first + 1:last]:
+        name = r["Kernel_Name"].replace("(anonymous namespace)::", "").replace("void ", "").split("(")[0]
+        if trace:
+            d = out.setdefault(name, {"launches": 0, "total_ns": 0, "durations_ns": []})
+            d["launches"] += 1
+            d["total_ns"] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            d["durations_ns"].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        else:
+            d = out.setdefault(name, {}).setdefault(r["Counter_Name"], {"dispatches": 0, "sum": 0.0})
+            d["dispatches"] += 1
+            d["sum"] += float(r["Counter_Value"])
+    for name, d in out.items():
+        if trace:
+            d["mean_us"] = d["total_ns"] / d["launches"] / 1e3
+            dur = np.sort(np.array(d.pop("durations_ns"), np.float64))
+            d["p10_p50_p90_us"] = [float(v) for v in np.percentile(dur, [10, 50, 90]) / 1e3]
+            if real and name.startswith("k_sweep") and 0 < real <= len(dur):
+                d["mean_us_of_longest"] = {"launches": real, "mean_us": float(dur[-real:].mean() / 1e3)}
+        else:
+            for c in d.values():
+                c["mean"] = c["sum"] / c["dispatches"]
+                if real and name.startswith("k_sweep") and 0 < real <= c["dispatches"]:
+                    # the no-op launches read the iteration flag and exit, so
+                    # the window's counts belong to the real launches
+                    c["per_real_launch"] = c["sum"] / real
+    return {"source": path, "window_rows": last - first - 1, "kernels": out}
 
 
 def cpu_model():
@@ -166,10 +218,22 @@ def main():
                     help="OGC broad phase: hashed grid or linear BVH (same candidate sets)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batch64: use --lanes instead of timing the lanes per vertex first")
+    ap.add_argument("--window-real", type=int, default=0, metavar="N",
+                    help="with --window-stats: also the mean of the N longest k_sweep launches (a "
+                         "kernel trace) or each counter's window sum over N launches (a counter run); N "
+                         "is the bench's real launches, the others are no-ops queued behind a converged "
+                         "step")
+    ap.add_argument("--window-stats", metavar="CSV",
+                    help="only summarise a rocprofv3 kernel_trace.csv or counter_collection.csv of this "
+                         "command over the timed window (the launches between the profiling switch's "
+                         "empty kernels)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.window_stats:
+        print(json.dumps(window_stats(args.window_stats, args.window_real)))
+        return
     if args.calibrate_copy:
         import torch
         n = args.calibrate_copy << 20
@@ -263,11 +327,14 @@ def bench_step(args):
     vi = n_free * m["iterations_run"]
     launches = max(m["sweep_launches"], 1)
     timed = max(m["sweep_timed_launches"], 1)
-    # Kernel duration: the HIP event interval of each sampled launch less the
-    # interval of an empty launch on the same stream (dispatch latency).
+    # Kernel duration: the HIP event interval of each sampled launch (the
+    # start and stop events of hipExtLaunchKernelGGL).  It agrees with a
+    # rocprofv3 kernel trace of the same launches; the interval of an empty
+    # launch on the same stream is reported beside it but not subtracted (a
+    # queued launch's interval does not contain that much dispatch time).
     overhead_s = m["sweep_event_overhead_ms"] / 1e3
     avg_interval_s = m["sweep_ms"] / 1e3 / timed
-    avg_launch_s = max(avg_interval_s - overhead_s, 1e-9)
+    avg_launch_s = avg_interval_s
     bytes_per_launch = m["sweep_timed_bytes_min"] / timed
     achieved = bytes_per_launch / avg_launch_s / 1e9
     sweep_s = avg_interval_s * launches

@@ -2053,8 +2053,18 @@ cv_status cv_gpu_set_precision(cv_gpu* g, int bits) {
 cv_status cv_gpu_set_profiling(cv_gpu* g, int enabled) {
   if (!g) return CV_ERR_INVALID_ARGUMENT;
   return guard(&g->last_error, [&] {
+    const bool was = g->profile;
     g->profile = enabled != 0;
-    if (g->profile) calibrate_event_overhead(*g);
+    if (g->profile) {
+      calibrate_event_overhead(*g);
+    } else if (was) {
+      // An empty launch closes the profiled window, as the calibration's
+      // empty launches open it, so a kernel trace can be cut to the window.
+      CV_HIP(hipSetDevice(g->device));
+      hipLaunchKernelGGL(k_empty, dim3(1), dim3(64), 0, g->stream);
+      CV_HIP(hipGetLastError());
+      CV_HIP(hipStreamSynchronize(g->stream));
+    }
   });
 }
 

@@ -396,7 +396,9 @@ cv_status cv_gpu_set_precision(cv_gpu* gpu, int bits);
  * count and sweep_timed_bytes_min their algorithmic bytes.  Enabling it
  * measures the event interval of an empty launch on the same stream once
  * (metrics.sweep_event_overhead_ms), the part of each timed interval that is
- * dispatch rather than kernel. */
+ * dispatch rather than kernel.  Those empty launches (kernel k_empty) open
+ * the profiled window in a kernel trace and disabling it launches one more to
+ * close it, so a trace can be cut to the window. */
 cv_status cv_gpu_set_profiling(cv_gpu* gpu, int enabled);
 
 /* Broad phase of the OGC detection: CV_BROAD_PHASE_GRID (default; a hashed
