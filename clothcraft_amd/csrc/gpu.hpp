@@ -79,6 +79,7 @@ struct Scalars {
   double diag_p95, diag_max;                // extension percent of the committed positions
   unsigned long long diag_gap_bits;         // max seam gap (bit pattern of a non-negative double)
   unsigned int err_degenerate, err_overflow, err_nonfinite, err_trust;
+  unsigned int span_first[8], span_last[8], span_next[8];  // broad-pass spans per XCD (k_span_bounds)
 };
 
 // MI355X dispatches the workgroups of a launch to its 8 XCDs round-robin
@@ -168,7 +169,7 @@ struct StepBufs {
   DBuf<uint32_t> cell_val, cell_val_sorted, cell_rank, cell_owner;
   DBuf<double> cell_box_sorted;
   DBuf<uint32_t> bcount, bstart;  // hash-grid buckets [2T+1]
-  DBuf<uint32_t> item_len, item_off, item_start, over, pair_owner;  // flattened broad phase
+  DBuf<uint32_t> item_len, item_off, item_start, over;  // flattened broad phase
   DBuf<uint2> pair_list;  // candidates for the narrow phase: VF from the front, EE from the back
   DBuf<uint32_t> scan_sums;  // tile totals of the two-level scan
   DBuf<double> red_parts;    // per-block partials of the prologue reduction

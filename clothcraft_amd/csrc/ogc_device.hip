@@ -10,13 +10,15 @@
 //
 // Broad phase: a uniform hash grid.  Every triangle (edge) is registered in
 // the cells its box inflated by the padded radius covers; entries are bucketed
-// by a hash of the cell key with a counting sort (one atomic rank per entry,
-// one scan, one scatter), and each bucket entry keeps its cell key so hash
-// collisions are filtered exactly.  A point looks only in its own cell; an
-// edge scans the cells of its own box and keeps a pair only in the cell
-// holding the lower corner of the box intersection, so each pair is examined
-// once.  Each query is served by a group of G lanes that stride over the
-// bucket's entries, so a launch has G x (points or edges) lanes in flight.
+// by a hash of the cell key with a counting sort (bucket ranks from one
+// atomic per distinct bucket and wavefront, one scan, one scatter), and each
+// bucket entry keeps its cell key so hash collisions are filtered exactly.  A
+// point looks only in its own cell; an edge scans the cells of its own box
+// and keeps a pair only in the cell holding the lower corner of the box
+// intersection, so each pair is examined once.  The (query cell, bucket
+// entry) pairs are flattened into one list of slots, one thread per slot
+// (k_pairs); edges covering more cells than the list holds per edge are
+// served by groups of G lanes striding over their cells' entries.
 // Pairs are emitted with atomics and then put in the reference's canonical
 // (kind, ids) order by a two-key radix sort, so results are deterministic.
 #include <cstring>
@@ -145,7 +147,22 @@ __global__ void k_reg_fill(const double* fbox, const double* ebox, uint32_t F, d
   }
   keys[j] = key;
   prims[j] = prim;
-  ranks[j] = atomicAdd(&bcount[table + bucket_of(key, log2t)], 1u);
+  // Lanes of a wavefront with the same bucket (neighbouring primitives cover
+  // the same cells) take their ranks from one atomic: one returning atomic
+  // per distinct bucket and wavefront instead of one per entry on a hot
+  // counter.  The loop is wave-uniform over the active lanes.
+  const uint32_t bk = table + bucket_of(key, log2t), lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(1);
+  while (todo) {
+    const uint32_t leader = __ffsll(todo) - 1;
+    const uint32_t bl = __shfl(bk, leader, 64);
+    const unsigned long long peers = __ballot(bk == bl);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&bcount[bl], (uint32_t)__popcll(peers));
+    base = __shfl(base, leader, 64);
+    if (bk == bl) ranks[j] = base + (uint32_t)__popcll(peers & ((1ull << lane) - 1));
+    todo &= ~peers;
+  }
 }
 // Bucketed entries carry the primitive's (unpadded) box, so a query scans
 // contiguous memory and tests boxes without a dependent gather.
@@ -370,42 +387,43 @@ __device__ __forceinline__ void ee_narrow(const DetectArgs& a, uint32_t e, uint3
     add_contact(a, 3, ia, ib, jc, jd, 4, 1e-3 * a.rest_len2[e] * a.rest_len2[f]);
 }
 
-// Block-local staging of the candidate list.  Candidates are appended to
-// LDS (one LDS atomic per wavefront and kind) and written out in bulk with
-// one global atomic per block and kind when a buffer nears full and at the
-// end: same-address global atomics per wavefront serialise across the whole
-// device.  VF entries grow from the front of the global list, EE entries
-// from the back (the two never meet: the list holds one slot per broad-phase
-// pair, and each pair yields at most one candidate).
-constexpr int kPairUnroll = 4;  // pairs per thread per broad-pass step
-constexpr int kListBuf = 2048;
-struct ListStage {
-  uint2 vf[kListBuf], ee[kListBuf];
-  uint32_t n_vf, n_ee;
-  unsigned long long base_vf, base_ee;
-};
-__device__ __forceinline__ void stage_pair(uint2* buf, uint32_t* n, bool take, uint32_t q, uint32_t p) {
-  uint64_t mask = __ballot(take);
-  if (!mask) return;
-  uint32_t lane = threadIdx.x & 63;
-  uint32_t leader = __ffsll((unsigned long long)__ballot(1)) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(n, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader, 64);
-  if (take) buf[base + __popcll(mask & ((1ull << lane) - 1))] = make_uint2(q, p);
+// Wavefront-local staging of the candidate list.  Each wavefront appends its
+// candidates to its own LDS buffers (positions from a ballot prefix count; the
+// counts are wave-uniform registers) and writes them out in bulk with one
+// global atomic per kind when a buffer nears full and at the end:
+// same-address global atomics per test step would serialise across the whole
+// device.  VF entries grow from the front of the global list, EE entries from
+// the back (the two never meet: the list holds one slot per broad-phase pair,
+// and each pair yields at most one candidate).
+constexpr int kPairUnroll = 4;  // pairs per lane per broad-pass step
+constexpr int kWaveList = 512;  // staged candidates per kind and wavefront
+// LDS written by some lanes of a wavefront and read by others: a wavefront's
+// LDS operations execute in order, the fences keep the compiler from moving
+// them across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-// Called by every thread of the block, after a barrier.
-__device__ __forceinline__ void flush_stage(ListStage& st, const DetectArgs& a, uint2* list, uint64_t W) {
-  if (threadIdx.x == 0) {
-    st.base_vf = st.n_vf ? atomicAdd(&a.sc->vf_list_n, (unsigned long long)st.n_vf) : 0;
-    st.base_ee = st.n_ee ? atomicAdd(&a.sc->ee_list_n, (unsigned long long)st.n_ee) : 0;
-  }
-  __syncthreads();
-  for (uint32_t k = threadIdx.x; k < st.n_vf; k += blockDim.x) list[st.base_vf + k] = st.vf[k];
-  for (uint32_t k = threadIdx.x; k < st.n_ee; k += blockDim.x) list[W - 1 - (st.base_ee + k)] = st.ee[k];
-  __syncthreads();
-  if (threadIdx.x == 0) st.n_vf = st.n_ee = 0;
-  __syncthreads();
+// Called by every lane of the wavefront; n is wave-uniform.
+__device__ __forceinline__ void wave_stage(uint2* buf, uint32_t& n, bool take, uint32_t q, uint32_t p) {
+  const unsigned long long mask = __ballot(take);
+  const uint32_t lane = threadIdx.x & 63;
+  if (take) buf[n + __popcll(mask & ((1ull << lane) - 1))] = make_uint2(q, p);
+  n += (uint32_t)__popcll(mask);
+}
+// Called by every lane of the wavefront; n is wave-uniform and reset to 0.
+__device__ __forceinline__ void wave_flush(const uint2* buf, uint32_t& n, unsigned long long* counter, uint2* list,
+                                           uint64_t W, bool from_back) {
+  if (!n) return;
+  wave_sync();
+  const uint32_t lane = threadIdx.x & 63;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(counter, (unsigned long long)n);
+  base = (unsigned long long)__shfl((long long)base, 0, 64);
+  for (uint32_t k = lane; k < n; k += 64) list[from_back ? W - 1 - (base + k) : base + k] = buf[k];
+  wave_sync();
+  n = 0;
 }
 
 __device__ __forceinline__ void add_counts(const DetectArgs& a, const PairCounts& c) {
@@ -464,107 +482,123 @@ __global__ void k_items(DetectArgs a, uint32_t* len, uint32_t* start, unsigned l
     for (; c < kEdgeCells; ++c) len[base + c] = 0;
   }
 }
-// Owner item of every flattened pair slot.  Each wave takes 64 consecutive
-// items and writes their contiguous slot range together, one slot per lane
-// (an item's slots can number in the hundreds, a thread per item would write
-// them serially); a slot's item is found by binary search over the wave's 65
-// offsets staged in LDS.  A range ending past `cap` raises the capacity flag
-// (the host grows the buffer and detects again; the broad pass examines
-// nothing meanwhile).
-__global__ __launch_bounds__(TPB) void k_item_owner(const uint32_t* off, uint32_t NI, uint32_t* owner, uint64_t cap,
-                                                    Scalars* sc) {
-  constexpr uint32_t WPB = TPB / 64;
-  __shared__ uint32_t so[WPB][65];
-  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t base = (blockIdx.x * WPB + w) * 64;
-  const bool live = base < NI;
-  const uint32_t cnt = live ? min(64u, NI - base) : 0;
-  uint32_t* o = so[w];
-  if (live) {
-    if (lane < cnt) o[lane] = off[base + lane];
-    if (lane == 0) o[cnt] = off[base + cnt];
-  }
-  __syncthreads();
-  if (!live) return;
-  const uint32_t lo = o[0], hi = o[cnt];
-  if (hi > cap) {
-    if (lane == 0) atomicOr(&sc->err_overflow, 4u);
-    return;
-  }
-  for (uint32_t s = lo + lane; s < hi; s += 64) {
-    uint32_t a = 0, b = cnt;  // o[a] <= s < o[b]
+// Broad-pass work split.  The pair slots (item_off) are cut into 8 equal
+// ranges, one per XCD (blocks b with b % 8 == x run on XCD x), so
+// neighbouring items read the same buckets from one L2 and every XCD gets an
+// equal share however many slots are in use.  Items are taken in spans of
+// kSpan consecutive items; XCD x serves the spans that meet its slot range,
+// clipped to it: first[x] and last[x] are those spans, next[x] the counter its
+// wavefronts take spans from.  Slots past `cap` raise the capacity flag (the
+// host grows the list and detects again; the broad pass examines nothing).
+constexpr uint32_t kSpan = 64;
+__global__ void k_span_bounds(const uint32_t* off, uint32_t NI, uint64_t cap, Scalars* sc) {
+  const uint32_t x = threadIdx.x;
+  const uint64_t W = off[NI];
+  if (x == 0 && W > cap) atomicOr(&sc->err_overflow, 4u);
+  if (x >= 8) return;
+  const uint32_t NS = (NI + kSpan - 1) / kSpan;
+  const uint64_t lo = W * x / 8, hi = W * (x + 1) / 8;
+  // span holding slot t: the last span k with off[k kSpan] <= t
+  auto span_of = [&](uint64_t t) {
+    uint32_t a = 0, b = NS;  // off[a kSpan] <= t < off[b kSpan] (b = NS: W)
     while (b - a > 1) {
       uint32_t m = (a + b) >> 1;
-      if (o[m] <= s) a = m; else b = m;
+      if (off[(size_t)m * kSpan] <= t) a = m; else b = m;
     }
-    owner[s] = base + a;
-  }
+    return a;
+  };
+  const bool any = NS > 0 && hi > lo && W <= cap;
+  sc->span_first[x] = any ? span_of(lo) : 1u;
+  sc->span_last[x] = any ? span_of(hi - 1) : 0u;
+  sc->span_next[x] = 0u;
 }
-// Broad pass: one (query item, bucket entry) pair per thread, grid-stride;
-// the candidates go to the narrow-phase list.
-// The pair range is split into 8 contiguous chunks, one per XCD (blocks b
-// with b % 8 == x run on XCD x), and each XCD's blocks stride through their
-// chunk: neighbouring slots read the same buckets from one L2, and every XCD
-// gets an equal share however many slots are in use.  gridDim.x is a
-// multiple of 8.
-__global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, const uint32_t* owner, const uint32_t* off,
-                                               const uint32_t* start, const unsigned long long* ikey,
-                                               uint32_t NI, uint64_t cap, uint2* list) {
-  __shared__ ListStage stage;
-  uint64_t W = off[NI];
-  if (W > cap) W = 0;  // capacity flagged by k_item_owner: nothing is examined, the host retries
-  if (threadIdx.x == 0) stage.n_vf = stage.n_ee = 0;
-  __syncthreads();
+// Broad pass: one (query item, bucket entry) pair per lane and kPairUnroll
+// per lane per step, the candidates staged per wavefront for the
+// narrow-phase list.  A wavefront takes a span of items at a time; the
+// span's slot offsets, bucket starts and cell keys are staged in LDS and a
+// slot's item is found by binary search over the offsets, so a pair test
+// starts from its bucket entry's loads.
+__global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, const uint32_t* off, const uint32_t* start,
+                                               const unsigned long long* ikey, uint32_t NI, uint64_t cap,
+                                               uint2* list) {
+  constexpr uint32_t WPB = TPB / 64;
+  struct Stage {
+    uint2 vf[kWaveList], ee[kWaveList];
+    uint32_t off[kSpan + 1], start[kSpan];
+    unsigned long long key[kSpan];
+  };
+  __shared__ Stage stage[WPB];
+  const uint64_t W = off[NI];
+  if (W > cap) return;  // flagged by k_span_bounds
+  const uint32_t lane = threadIdx.x & 63, x = blockIdx.x % 8;
+  Stage& st = stage[threadIdx.x >> 6];
+  const uint64_t lo_x = W * x / 8, hi_x = W * (x + 1) / 8;
+  const uint32_t first = a.sc->span_first[x], last = a.sc->span_last[x];
+  uint32_t n_vf = 0, n_ee = 0;
   PairCounts cnt;
-  const uint32_t X = 8, x = blockIdx.x % X, nbx = gridDim.x / X;
-  const uint64_t lo = W * x / X, hi = W * (x + 1) / X, step = (uint64_t)nbx * blockDim.x;
-  // Block-uniform trip count (the loop holds barriers).  Each step covers
-  // kPairUnroll consecutive block-wide spans: the threads' pair tests are
-  // independent, so their loads overlap, and the staging barriers are paid
-  // once per step.
-  for (uint64_t j0 = lo + (uint64_t)(blockIdx.x / X) * blockDim.x * kPairUnroll; j0 < hi;
-       j0 += step * kPairUnroll) {
-    bool vf[kPairUnroll], ee[kPairUnroll];
-    uint32_t qs[kPairUnroll], prims[kPairUnroll];
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&a.sc->span_next[x], 1u);
+    const uint32_t span = first + (uint32_t)__shfl((int)t, 0, 64);
+    if (span > last) break;
+    const uint32_t base = span * kSpan, ni = min(kSpan, NI - base);
+    wave_sync();  // the previous span's staged values are no longer read
+    if (lane < ni) {
+      st.off[lane] = off[base + lane];
+      st.start[lane] = start[base + lane];
+      st.key[lane] = ikey[base + lane];
+    }
+    if (lane == 0) st.off[ni] = off[base + ni];
+    wave_sync();
+    const uint64_t s_lo = max((uint64_t)st.off[0], lo_x), s_hi = min((uint64_t)st.off[ni], hi_x);
+    for (uint64_t s0 = s_lo; s0 < s_hi; s0 += 64 * kPairUnroll) {
+      bool vf[kPairUnroll], ee[kPairUnroll];
+      uint32_t qs[kPairUnroll], prims[kPairUnroll];
 #pragma unroll
-    for (int u = 0; u < kPairUnroll; ++u) {
-      uint64_t j = j0 + (uint64_t)u * blockDim.x + threadIdx.x;
-      vf[u] = ee[u] = false;
-      uint32_t q = 0, prim = 0;
-      if (j < hi) {
-        uint32_t item = owner[j];
-        uint32_t it = start[item] + (uint32_t)(j - off[item]);
-        unsigned long long key = ikey[item];
-        prim = a.sprim[it];
-        if (a.skey[it] == key) {  // else another cell hashed to the same bucket
-          const double* bx = a.sbox + 6 * (size_t)it;
-          ogm::Box eb{{bx[0], bx[1], bx[2]}, {bx[3], bx[4], bx[5]}};
-          if (item < a.n_total) {
-            q = item;
-            D3 p = ld3(a.x, q);
-            vf[u] = vf_filter(a, q, q < a.n, a.ex_off[q] != a.ex_off[q + 1], ogm::box_of(p), prim, eb, cnt);
-          } else {
-            q = (item - a.n_total) / kEdgeCells;
-            uint32_t ia = a.edges[2 * q], ib = a.edges[2 * q + 1];
-            ee[u] = ee_filter(a, q, ldbox(a.ebox, q), ia, ib, a.ex_off[ia] != a.ex_off[ia + 1],
-                              a.ex_off[ib] != a.ex_off[ib + 1], prim, eb, key, cnt);
+      for (int u = 0; u < kPairUnroll; ++u) {
+        const uint64_t sl = s0 + (uint64_t)u * 64 + lane;
+        vf[u] = ee[u] = false;
+        uint32_t q = 0, prim = 0;
+        if (sl < s_hi) {
+          const uint32_t s32 = (uint32_t)sl;
+          uint32_t k = 0, kb = ni;  // st.off[k] <= s32 < st.off[kb]
+          while (kb - k > 1) {
+            const uint32_t m = (k + kb) >> 1;
+            if (st.off[m] <= s32) k = m; else kb = m;
+          }
+          const uint32_t item = base + k;
+          const uint32_t it = st.start[k] + (s32 - st.off[k]);
+          const unsigned long long key = st.key[k];
+          prim = a.sprim[it];
+          if (a.skey[it] == key) {  // else another cell hashed to the same bucket
+            const double* bx = a.sbox + 6 * (size_t)it;
+            ogm::Box eb{{bx[0], bx[1], bx[2]}, {bx[3], bx[4], bx[5]}};
+            if (item < a.n_total) {
+              q = item;
+              D3 p = ld3(a.x, q);
+              vf[u] = vf_filter(a, q, q < a.n, a.ex_off[q] != a.ex_off[q + 1], ogm::box_of(p), prim, eb, cnt);
+            } else {
+              q = (item - a.n_total) / kEdgeCells;
+              uint32_t ia = a.edges[2 * q], ib = a.edges[2 * q + 1];
+              ee[u] = ee_filter(a, q, ldbox(a.ebox, q), ia, ib, a.ex_off[ia] != a.ex_off[ia + 1],
+                                a.ex_off[ib] != a.ex_off[ib + 1], prim, eb, key, cnt);
+            }
           }
         }
+        qs[u] = q;
+        prims[u] = prim;
       }
-      qs[u] = q;
-      prims[u] = prim;
-    }
 #pragma unroll
-    for (int u = 0; u < kPairUnroll; ++u) {
-      stage_pair(stage.vf, &stage.n_vf, vf[u], qs[u], prims[u]);
-      stage_pair(stage.ee, &stage.n_ee, ee[u], qs[u], prims[u]);
+      for (int u = 0; u < kPairUnroll; ++u) {
+        wave_stage(st.vf, n_vf, vf[u], qs[u], prims[u]);
+        wave_stage(st.ee, n_ee, ee[u], qs[u], prims[u]);
+      }
+      if (n_vf > kWaveList - 64 * kPairUnroll) wave_flush(st.vf, n_vf, &a.sc->vf_list_n, list, W, false);
+      if (n_ee > kWaveList - 64 * kPairUnroll) wave_flush(st.ee, n_ee, &a.sc->ee_list_n, list, W, true);
     }
-    __syncthreads();
-    bool full = stage.n_vf > kListBuf - kPairUnroll * TPB || stage.n_ee > kListBuf - kPairUnroll * TPB;
-    __syncthreads();  // every thread has read the counts before they change again
-    if (full) flush_stage(stage, a, list, W);
   }
-  flush_stage(stage, a, list, W);
+  wave_flush(st.vf, n_vf, &a.sc->vf_list_n, list, W, false);
+  wave_flush(st.ee, n_ee, &a.sc->ee_list_n, list, W, true);
   add_counts(a, cnt);
 }
 // Narrow pass over the candidate list, VF and EE in separate grid-stride
@@ -1363,12 +1397,11 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
     // Pair-slot capacity: grown from the last detection's need; an overflow is
     // flagged on the device and the detection is run again with more room.
     if (b.pair_cap == 0) b.pair_cap = std::max<uint64_t>(4ull * b.grid_cap + (1u << 20), 1u << 22);
-    b.pair_owner.reserve(b.pair_cap);
     b.pair_list.reserve(b.pair_cap);
-    k_item_owner<<<blocks(NI), TPB, 0, st>>>(b.item_off.p, NI, b.pair_owner.p, b.pair_cap, g.d_scal);
+    k_span_bounds<<<1, 64, 0, st>>>(b.item_off.p, NI, b.pair_cap, g.d_scal);
     CV_HIP(hipGetLastError());
-    k_pairs<<<8 * 256, TPB, 0, st>>>(
-        a, b.pair_owner.p, b.item_off.p, b.item_start.p, b.item_key.p, NI, b.pair_cap, b.pair_list.p);
+    k_pairs<<<8 * 256, TPB, 0, st>>>(a, b.item_off.p, b.item_start.p, b.item_key.p, NI, b.pair_cap,
+                                     b.pair_list.p);
     CV_HIP(hipGetLastError());
     k_narrow<<<8 * 256, TPB, 0, st>>>(a, b.item_off.p, NI, b.pair_cap, b.pair_list.p);
     CV_HIP(hipGetLastError());
