@@ -388,15 +388,16 @@ __device__ __forceinline__ void ee_narrow(const DetectArgs& a, uint32_t e, uint3
 }
 
 // Wavefront-local staging of the candidate list.  Each wavefront appends its
-// candidates to its own LDS buffers (positions from a ballot prefix count; the
-// counts are wave-uniform registers) and writes them out in bulk with one
-// global atomic per kind when a buffer nears full and at the end:
-// same-address global atomics per test step would serialise across the whole
-// device.  VF entries grow from the front of the global list, EE entries from
-// the back (the two never meet: the list holds one slot per broad-phase pair,
-// and each pair yields at most one candidate).
+// candidates to its own LDS buffer, VF from the front and EE from the back
+// (positions from a ballot prefix count; the counts are wave-uniform
+// registers), and writes them out in bulk with one global atomic per kind
+// when the buffer nears full and at the end: same-address global atomics per
+// test step would serialise across the whole device.  In the global list VF
+// entries grow from the front and EE entries from the back (the two never
+// meet: the list holds one slot per broad-phase pair, and each pair yields at
+// most one candidate).
 constexpr int kPairUnroll = 4;  // pairs per lane per broad-pass step
-constexpr int kWaveList = 512;  // staged candidates per kind and wavefront
+constexpr int kWaveList = 512;  // staged candidates per wavefront, both kinds
 // LDS written by some lanes of a wavefront and read by others: a wavefront's
 // LDS operations execute in order, the fences keep the compiler from moving
 // them across this point.
@@ -406,22 +407,28 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 // Called by every lane of the wavefront; n is wave-uniform.
-__device__ __forceinline__ void wave_stage(uint2* buf, uint32_t& n, bool take, uint32_t q, uint32_t p) {
+__device__ __forceinline__ void wave_stage(uint2* buf, uint32_t& n, bool back, bool take, uint32_t q, uint32_t p) {
   const unsigned long long mask = __ballot(take);
   const uint32_t lane = threadIdx.x & 63;
-  if (take) buf[n + __popcll(mask & ((1ull << lane) - 1))] = make_uint2(q, p);
+  const uint32_t i = n + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+  if (take) buf[back ? kWaveList - 1 - i : i] = make_uint2(q, p);
   n += (uint32_t)__popcll(mask);
 }
 // Called by every lane of the wavefront; n is wave-uniform and reset to 0.
-__device__ __forceinline__ void wave_flush(const uint2* buf, uint32_t& n, unsigned long long* counter, uint2* list,
-                                           uint64_t W, bool from_back) {
+__device__ __forceinline__ void wave_flush(const uint2* buf, uint32_t& n, bool back, unsigned long long* counter,
+                                           uint2* list, uint64_t W) {
   if (!n) return;
   wave_sync();
   const uint32_t lane = threadIdx.x & 63;
   unsigned long long base = 0;
   if (lane == 0) base = atomicAdd(counter, (unsigned long long)n);
   base = (unsigned long long)__shfl((long long)base, 0, 64);
-  for (uint32_t k = lane; k < n; k += 64) list[from_back ? W - 1 - (base + k) : base + k] = buf[k];
+  for (uint32_t k = lane; k < n; k += 64) {
+    if (back)
+      list[W - 1 - (base + k)] = buf[kWaveList - 1 - k];
+    else
+      list[base + k] = buf[k];
+  }
   wave_sync();
   n = 0;
 }
@@ -514,18 +521,22 @@ __global__ void k_span_bounds(const uint32_t* off, uint32_t NI, uint64_t cap, Sc
 }
 // Broad pass: one (query item, bucket entry) pair per lane and kPairUnroll
 // per lane per step, the candidates staged per wavefront for the
-// narrow-phase list.  A wavefront takes a span of items at a time; the
-// span's slot offsets, bucket starts and cell keys are staged in LDS and a
-// slot's item is found by binary search over the offsets, so a pair test
-// starts from its bucket entry's loads.
+// narrow-phase list.  A wavefront takes a span of items at a time and stages
+// in LDS what every pair of an item shares: its slot offset, bucket start and
+// cell key, and the query's box, vertex ids and flags.  A slot's item is found
+// by binary search over the offsets, so a pair test starts from its bucket
+// entry's loads.
 __global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, const uint32_t* off, const uint32_t* start,
                                                const unsigned long long* ikey, uint32_t NI, uint64_t cap,
                                                uint2* list) {
   constexpr uint32_t WPB = TPB / 64;
   struct Stage {
-    uint2 vf[kWaveList], ee[kWaveList];
-    uint32_t off[kSpan + 1], start[kSpan];
+    uint2 cand[kWaveList];
+    double qbox[kSpan][6];       // the query's box (a point's: the point twice)
     unsigned long long key[kSpan];
+    uint32_t off[kSpan + 1], start[kSpan];
+    uint32_t qa[kSpan], qb[kSpan];  // an edge's vertices
+    uint32_t qf[kSpan];             // point: moving | excluded list << 1; edge: a excluded | b excluded << 1
   };
   __shared__ Stage stage[WPB];
   const uint64_t W = off[NI];
@@ -543,12 +554,39 @@ __global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, const uint32_t* off
     if (span > last) break;
     const uint32_t base = span * kSpan, ni = min(kSpan, NI - base);
     wave_sync();  // the previous span's staged values are no longer read
+    uint32_t my_off = 0, my_end = 0;
     if (lane < ni) {
-      st.off[lane] = off[base + lane];
+      my_off = off[base + lane];
+      my_end = off[base + lane + 1];
+      st.off[lane] = my_off;
       st.start[lane] = start[base + lane];
       st.key[lane] = ikey[base + lane];
     }
     if (lane == 0) st.off[ni] = off[base + ni];
+    if (lane < ni && my_end > my_off) {  // the query of an item with pairs
+      const uint32_t item = base + lane;
+      double* qb = st.qbox[lane];
+      if (item < a.n_total) {
+        const D3 p = ld3(a.x, item);
+        qb[0] = qb[3] = p.x;
+        qb[1] = qb[4] = p.y;
+        qb[2] = qb[5] = p.z;
+        st.qf[lane] = (item < a.n ? 1u : 0u) | (a.ex_off[item] != a.ex_off[item + 1] ? 2u : 0u);
+      } else {
+        const uint32_t e = (item - a.n_total) / kEdgeCells;
+        const ogm::Box be = ldbox(a.ebox, e);
+        qb[0] = be.lo.x;
+        qb[1] = be.lo.y;
+        qb[2] = be.lo.z;
+        qb[3] = be.hi.x;
+        qb[4] = be.hi.y;
+        qb[5] = be.hi.z;
+        const uint32_t ia = a.edges[2 * e], ib = a.edges[2 * e + 1];
+        st.qa[lane] = ia;
+        st.qb[lane] = ib;
+        st.qf[lane] = (a.ex_off[ia] != a.ex_off[ia + 1] ? 1u : 0u) | (a.ex_off[ib] != a.ex_off[ib + 1] ? 2u : 0u);
+      }
+    }
     wave_sync();
     const uint64_t s_lo = max((uint64_t)st.off[0], lo_x), s_hi = min((uint64_t)st.off[ni], hi_x);
     for (uint64_t s0 = s_lo; s0 < s_hi; s0 += 64 * kPairUnroll) {
@@ -572,16 +610,16 @@ __global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, const uint32_t* off
           prim = a.sprim[it];
           if (a.skey[it] == key) {  // else another cell hashed to the same bucket
             const double* bx = a.sbox + 6 * (size_t)it;
-            ogm::Box eb{{bx[0], bx[1], bx[2]}, {bx[3], bx[4], bx[5]}};
+            const ogm::Box eb{{bx[0], bx[1], bx[2]}, {bx[3], bx[4], bx[5]}};
+            const double* qx = st.qbox[k];
+            const ogm::Box qbx{{qx[0], qx[1], qx[2]}, {qx[3], qx[4], qx[5]}};
+            const uint32_t fl = st.qf[k];
             if (item < a.n_total) {
               q = item;
-              D3 p = ld3(a.x, q);
-              vf[u] = vf_filter(a, q, q < a.n, a.ex_off[q] != a.ex_off[q + 1], ogm::box_of(p), prim, eb, cnt);
+              vf[u] = vf_filter(a, q, fl & 1u, fl & 2u, qbx, prim, eb, cnt);
             } else {
               q = (item - a.n_total) / kEdgeCells;
-              uint32_t ia = a.edges[2 * q], ib = a.edges[2 * q + 1];
-              ee[u] = ee_filter(a, q, ldbox(a.ebox, q), ia, ib, a.ex_off[ia] != a.ex_off[ia + 1],
-                                a.ex_off[ib] != a.ex_off[ib + 1], prim, eb, key, cnt);
+              ee[u] = ee_filter(a, q, qbx, st.qa[k], st.qb[k], fl & 1u, fl & 2u, prim, eb, key, cnt);
             }
           }
         }
@@ -590,15 +628,17 @@ __global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, const uint32_t* off
       }
 #pragma unroll
       for (int u = 0; u < kPairUnroll; ++u) {
-        wave_stage(st.vf, n_vf, vf[u], qs[u], prims[u]);
-        wave_stage(st.ee, n_ee, ee[u], qs[u], prims[u]);
+        wave_stage(st.cand, n_vf, false, vf[u], qs[u], prims[u]);
+        wave_stage(st.cand, n_ee, true, ee[u], qs[u], prims[u]);
       }
-      if (n_vf > kWaveList - 64 * kPairUnroll) wave_flush(st.vf, n_vf, &a.sc->vf_list_n, list, W, false);
-      if (n_ee > kWaveList - 64 * kPairUnroll) wave_flush(st.ee, n_ee, &a.sc->ee_list_n, list, W, true);
+      if (n_vf + n_ee > kWaveList - 64 * kPairUnroll) {
+        wave_flush(st.cand, n_vf, false, &a.sc->vf_list_n, list, W);
+        wave_flush(st.cand, n_ee, true, &a.sc->ee_list_n, list, W);
+      }
     }
   }
-  wave_flush(st.vf, n_vf, &a.sc->vf_list_n, list, W, false);
-  wave_flush(st.ee, n_ee, &a.sc->ee_list_n, list, W, true);
+  wave_flush(st.cand, n_vf, false, &a.sc->vf_list_n, list, W);
+  wave_flush(st.cand, n_ee, true, &a.sc->ee_list_n, list, W);
   add_counts(a, cnt);
 }
 // Narrow pass over the candidate list, VF and EE in separate grid-stride
