@@ -54,6 +54,17 @@ struct DBuf {
 };
 
 // Step scalars shared by kernels and read back by the host in one copy.
+// One hash-grid registration in bucket order: the cell key (hash
+// collisions are filtered by it), the primitive and its unpadded box, one
+// 64-byte line, so the counting sort's scatter writes whole lines and a query
+// reads each entry with one contiguous load.
+struct alignas(64) GridEntry {
+  unsigned long long key;
+  uint32_t prim, pad;
+  double box[6];
+};
+static_assert(sizeof(GridEntry) == 64, "one grid entry per 64-byte line");
+
 struct Scalars {
   double origin[3];
   double rq;
@@ -165,9 +176,9 @@ struct StepBufs {
   bool bvh_fallback = false;             // this detection runs the grid (an LBVH walk overflowed)
   uint64_t bvh_fallbacks = 0;
   DBuf<uint32_t> cell_count, cell_off;
-  DBuf<unsigned long long> cell_key, cell_key_sorted;
-  DBuf<uint32_t> cell_val, cell_val_sorted, cell_rank, cell_owner;
-  DBuf<double> cell_box_sorted;
+  DBuf<unsigned long long> cell_key;
+  DBuf<uint32_t> cell_val, cell_rank, cell_owner;
+  DBuf<GridEntry> cell_entry;  // bucketed registrations
   DBuf<uint32_t> bcount, bstart;  // hash-grid buckets [2T+1]
   DBuf<uint32_t> item_len, item_off, item_start, over;  // flattened broad phase
   DBuf<uint2> pair_list;  // candidates for the narrow phase: VF from the front, EE from the back

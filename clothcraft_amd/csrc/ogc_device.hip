@@ -153,6 +153,7 @@ __global__ void k_reg_fill(const double* fbox, const double* ebox, uint32_t F, d
   // counter.  The loop is wave-uniform over the active lanes.
   const uint32_t bk = table + bucket_of(key, log2t), lane = threadIdx.x & 63;
   unsigned long long todo = __ballot(1);
+  uint32_t rank = 0;
   while (todo) {
     const uint32_t leader = __ffsll(todo) - 1;
     const uint32_t bl = __shfl(bk, leader, 64);
@@ -160,28 +161,31 @@ __global__ void k_reg_fill(const double* fbox, const double* ebox, uint32_t F, d
     uint32_t base = 0;
     if (lane == leader) base = atomicAdd(&bcount[bl], (uint32_t)__popcll(peers));
     base = __shfl(base, leader, 64);
-    if (bk == bl) ranks[j] = base + (uint32_t)__popcll(peers & ((1ull << lane) - 1));
+    if (bk == bl) rank = base + (uint32_t)__popcll(peers & ((1ull << lane) - 1));
     todo &= ~peers;
   }
+  ranks[j] = rank;  // one store per wavefront
 }
 // Bucketed entries carry the primitive's (unpadded) box, so a query scans
-// contiguous memory and tests boxes without a dependent gather.
+// contiguous memory and tests boxes without a dependent gather; each entry is
+// written as one 64-byte line (four 16-byte stores).
 __global__ void k_reg_scatter(const unsigned long long* keys, const uint32_t* prims, const uint32_t* ranks,
                               const uint32_t* off, uint32_t F, uint32_t N, uint64_t cap, int log2t,
-                              const uint32_t* bstart, const double* fbox, const double* ebox,
-                              unsigned long long* skey, uint32_t* sprim, double* sbox) {
+                              const uint32_t* bstart, const double* fbox, const double* ebox, GridEntry* ent) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t M = off[N], face_entries = off[F];
   if (M > cap || j >= M) return;
   bool face = j < face_entries;
-  uint32_t b = (face ? 0u : (1u << log2t)) + bucket_of(keys[j], log2t);
+  const unsigned long long key = keys[j];
+  uint32_t b = (face ? 0u : (1u << log2t)) + bucket_of(key, log2t);
   uint32_t pos = bstart[b] + ranks[j];
   uint32_t p = prims[j];
-  skey[pos] = keys[j];
-  sprim[pos] = p;
-  const double* src = face ? fbox + 6 * (size_t)p : ebox + 6 * (size_t)p;
-#pragma unroll
-  for (int k = 0; k < 6; ++k) sbox[6 * (size_t)pos + k] = src[k];
+  const double2* src = (const double2*)(face ? fbox + 6 * (size_t)p : ebox + 6 * (size_t)p);
+  double2* dst = (double2*)(ent + pos);
+  dst[0] = make_double2(__longlong_as_double((long long)key), __longlong_as_double((long long)p));
+  dst[1] = src[0];
+  dst[2] = src[1];
+  dst[3] = src[2];
 }
 
 struct DetectArgs {
@@ -201,9 +205,7 @@ struct DetectArgs {
   const double* fbox;
   const double* ebox;
   double rq, pr, pr2, inv_h;
-  const unsigned long long* skey;  // bucketed entries: cell key, primitive id and box
-  const uint32_t* sprim;
-  const double* sbox;
+  const GridEntry* ent;            // bucketed entries: cell key, primitive id and box
   const uint32_t* bstart;          // [2T+1] bucket starts, faces then edges
   int log2t;
   unsigned long long* dist_bits;
@@ -607,10 +609,10 @@ __global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, const uint32_t* off
           const uint32_t item = base + k;
           const uint32_t it = st.start[k] + (s32 - st.off[k]);
           const unsigned long long key = st.key[k];
-          prim = a.sprim[it];
-          if (a.skey[it] == key) {  // else another cell hashed to the same bucket
-            const double* bx = a.sbox + 6 * (size_t)it;
-            const ogm::Box eb{{bx[0], bx[1], bx[2]}, {bx[3], bx[4], bx[5]}};
+          const GridEntry en = a.ent[it];
+          prim = en.prim;
+          if (en.key == key) {  // else another cell hashed to the same bucket
+            const ogm::Box eb{{en.box[0], en.box[1], en.box[2]}, {en.box[3], en.box[4], en.box[5]}};
             const double* qx = st.qbox[k];
             const ogm::Box qbx{{qx[0], qx[1], qx[2]}, {qx[3], qx[4], qx[5]}};
             const uint32_t fl = st.qf[k];
@@ -684,10 +686,10 @@ __global__ __launch_bounds__(TPB) void k_ee_overflow(DetectArgs a, const uint32_
           uint32_t bk = table + bucket_of(key, a.log2t);
           uint32_t end = a.bstart[bk + 1];
           for (uint32_t it = a.bstart[bk] + lane; it < end; it += G) {
-            if (a.skey[it] != key) continue;
-            const double* bx = a.sbox + 6 * (size_t)it;
-            ogm::Box bfr{{bx[0], bx[1], bx[2]}, {bx[3], bx[4], bx[5]}};
-            uint32_t f = a.sprim[it];
+            const GridEntry en = a.ent[it];
+            if (en.key != key) continue;
+            ogm::Box bfr{{en.box[0], en.box[1], en.box[2]}, {en.box[3], en.box[4], en.box[5]}};
+            uint32_t f = en.prim;
             if (ee_filter(a, e, be, ia, ib, ia_ex, ib_ex, f, bfr, key, cnt)) ee_narrow(a, e, f, cnt);
           }
         }
@@ -1193,10 +1195,8 @@ int build_grid(cv_gpu& g, uint32_t F, uint32_t E, double pr, double h) {
   b.cell_key.reserve(cap);
   b.cell_val.reserve(cap);
   b.cell_rank.reserve(cap);
-  b.cell_key_sorted.reserve(cap);
-  b.cell_val_sorted.reserve(cap);
+  b.cell_entry.reserve(cap);
   b.cell_owner.reserve(cap);
-  b.cell_box_sorted.reserve(6 * cap);
   b.bcount.reserve(T2 + 1);
   b.bstart.reserve(T2 + 1);
   CV_HIP(hipMemsetAsync(b.bcount.p, 0, (T2 + 1) * sizeof(uint32_t), st));
@@ -1206,8 +1206,7 @@ int build_grid(cv_gpu& g, uint32_t F, uint32_t E, double pr, double h) {
   CV_HIP(hipGetLastError());
   scan_u32(g, b.bcount.p, b.bstart.p, T2 + 1);
   k_reg_scatter<<<blocks(cap), TPB, 0, st>>>(b.cell_key.p, b.cell_val.p, b.cell_rank.p, b.cell_off.p, F, N, cap,
-                                             log2t, b.bstart.p, b.fbox.p, b.ebox.p, b.cell_key_sorted.p,
-                                             b.cell_val_sorted.p, b.cell_box_sorted.p);
+                                             log2t, b.bstart.p, b.fbox.p, b.ebox.p, b.cell_entry.p);
   CV_HIP(hipGetLastError());
   return log2t;
 }
@@ -1417,9 +1416,7 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
     // Faces and edges registered with the padded inflation; points look up
     // their own cell, edges the cells of their own box.
     a.log2t = build_grid(g, d.F, d.E, pr_reg, h);
-    a.skey = b.cell_key_sorted.p;
-    a.sprim = b.cell_val_sorted.p;
-    a.sbox = b.cell_box_sorted.p;
+    a.ent = b.cell_entry.p;
     a.bstart = b.bstart.p;
     // Flattened broad phase: query items, their pair counts, owners, pairs.
     NI = d.n_total + kEdgeCells * d.E;
