@@ -126,6 +126,25 @@ def window_stats(path, real=0):
             d = out.setdefault(name, {}).setdefault(r["Counter_Name"], {"dispatches": 0, "sum": 0.0})
             d["dispatches"] += 1
             d["sum"] += float(r["Counter_Value"])
+    res = {"source": path, "window_rows": last - first - 1, "kernels": out}
+    if trace and last - first > 1:
+        # Device time between the window's dispatches: the gaps between one
+        # kernel's end and the next one's start (dependent launches; host
+        # waits show as the long ones).
+        iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in rows[first + 1:last])
+        busy, end, gaps = 0, iv[0][0], []
+        for s0, e0 in iv:
+            if s0 > end:
+                gaps.append(s0 - end)
+            busy += max(0, e0 - max(s0, end))
+            end = max(end, e0)
+        g = np.array(gaps, np.float64)
+        res["span_ms"] = (end - iv[0][0]) / 1e6
+        res["busy_ms"] = busy / 1e6
+        res["gaps_ms"] = {"under_10us": float(g[g < 1e4].sum() / 1e6),
+                          "10_to_100us": float(g[(g >= 1e4) & (g < 1e5)].sum() / 1e6),
+                          "over_100us": float(g[g >= 1e5].sum() / 1e6),
+                          "count_over_100us": int((g >= 1e5).sum())}
     for name, d in out.items():
         if trace:
             d["mean_us"] = d["total_ns"] / d["launches"] / 1e3
@@ -140,7 +159,7 @@ def window_stats(path, real=0):
                     # the no-op launches read the iteration flag and exit, so
                     # the window's counts belong to the real launches
                     c["per_real_launch"] = c["sum"] / real
-    return {"source": path, "window_rows": last - first - 1, "kernels": out}
+    return res
 
 
 def cpu_model():

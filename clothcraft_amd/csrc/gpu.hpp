@@ -158,6 +158,7 @@ struct StepBufs {
   DBuf<double> sum_parts;  // partials of the deterministic sums
   DBuf<double> gtrue;  // per-vertex |g|^2 of the true residual
   DBuf<uint32_t> order;
+  DBuf<uint32_t> color_dev, color_chg, color_keys, color_iota;  // colour of each vertex; changes; sort scratch
   DBuf<uint4> vinfo;  // colour-ordered (vertex, term offset, term count, contact offset)
   DBuf<uint32_t> vnc;  // colour-ordered contact-record counts
   // 32-lane sweep: slots whose vertex has more than 32 records (flag, its
@@ -338,8 +339,10 @@ struct cv_gpu {
   bool detected = false;        // a detection of the current device solver ran
   uint32_t* h_cont = nullptr;
   size_t h_cont_cap = 0;
-  std::vector<uint32_t> cnb_off, cnb;
-  std::vector<uint32_t> order, color_off;
+  uint32_t* h_cchg = nullptr;  // pinned staging of an incremental colouring's (vertex, colour) changes
+  bool order_host = false;     // b.order holds the host colouring's order for b.color_dev
+  size_t h_cchg_cap = 0;
+  std::vector<uint32_t> color_off;
   ColoringState coloring;  // incremental greedy colouring across detections
   // Inputs of the last colouring, to reuse it when they repeat.
   bool color_valid = false;
@@ -412,6 +415,8 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
                 size_t cand_cap);
 // Exclusive scan of count uint32 values on the context's stream.
 void device_scan_u32(cv_gpu& g, const uint32_t* in, uint32_t* out, size_t count);
+// b.order = the vertices 0..n-1 stably sorted by b.color_dev (colours below ncolors).
+void device_color_order(cv_gpu& g, uint32_t n, uint32_t ncolors);
 // Device Newton reference of one step (newton_device.hip).
 void newton_reference(cv_gpu& g, const cv_solver& s, const cv_state* st, const cv_settings* set,
                       const cv_newton_settings* ns, double* x_out, cv_newton_metrics* out);

@@ -75,26 +75,6 @@ uint32_t cvh_solver_edges(const cv_solver* s, uint32_t* out, uint32_t cap) {
 
 }  // extern "C"
 
-// Per-vertex contact neighbourhoods (CSR) from (kind, ids[4], count) rows.
-static void contact_csr(uint32_t n, const uint32_t* kind_ids, const uint32_t* counts, uint64_t m,
-                        std::vector<uint32_t>& off, std::vector<uint32_t>& nb) {
-  off.assign(n + 1, 0);
-  for (uint64_t c = 0; c < m; ++c)
-    for (uint32_t q = 0; q < counts[c]; ++q) {
-      uint32_t v = kind_ids[5 * c + 1 + q];
-      if (v < n) off[v + 1] += counts[c];
-    }
-  for (uint32_t v = 0; v < n; ++v) off[v + 1] += off[v];
-  nb.resize(off[n]);
-  std::vector<uint32_t> cur(off.begin(), off.end() - 1);
-  for (uint64_t c = 0; c < m; ++c)
-    for (uint32_t q = 0; q < counts[c]; ++q) {
-      uint32_t v = kind_ids[5 * c + 1 + q];
-      if (v < n)
-        for (uint32_t r = 0; r < counts[c]; ++r) nb[cur[v]++] = kind_ids[5 * c + 1 + r];
-    }
-}
-
 extern "C" {
 
 // Incremental colouring state for tests of update_colors.
@@ -102,10 +82,9 @@ void* cvh_coloring_new() { return new ColoringState(); }
 void cvh_coloring_free(void* p) { delete (ColoringState*)p; }
 uint32_t cvh_coloring_update(const cv_solver* s, void* state, const uint32_t* kind_ids, const uint32_t* counts,
                              uint64_t m, const cv_grab* grab, int smallest_last, uint32_t* color_of) {
-  std::vector<uint32_t> off, nb, order, coff;
-  contact_csr(s->n, kind_ids, counts, m, off, nb);
+  std::vector<uint32_t> coff;
   ColoringState& cs = *(ColoringState*)state;
-  uint32_t k = update_colors(*s, cs, off.data(), nb.data(), grab, smallest_last != 0, order, coff);
+  uint32_t k = update_colors(*s, cs, kind_ids, counts, m, grab, smallest_last != 0, coff);
   std::memcpy(color_of, cs.color_of.data(), s->n * sizeof(uint32_t));
   return k;
 }

@@ -1257,6 +1257,30 @@ __global__ void k_seam_gap(const double* x, const uint32_t* off, const uint32_t*
 }  // namespace
 
 void device_scan_u32(cv_gpu& g, const uint32_t* in, uint32_t* out, size_t count) { scan_u32(g, in, out, count); }
+namespace {
+__global__ void k_iota(uint32_t* a, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+}  // namespace
+void device_color_order(cv_gpu& g, uint32_t n, uint32_t ncolors) {
+  if (n == 0) return;
+  StepBufs& b = g.b;
+  hipStream_t st = g.stream;
+  if (b.color_iota.n < n) {
+    b.color_iota.reserve(n);
+    k_iota<<<blocks((uint32_t)b.color_iota.n), TPB, 0, st>>>(b.color_iota.p, (uint32_t)b.color_iota.n);
+    CV_HIP(hipGetLastError());
+  }
+  b.color_keys.reserve(n);
+  b.order.reserve(n);
+  int bits = 1;
+  while (bits < 32 && (1u << bits) < ncolors) ++bits;
+  with_temp(g, [&](void* t, size_t& bytes) {  // stable: vertex order within a colour
+    return rocprim::radix_sort_pairs(t, bytes, b.color_dev.p, b.color_keys.p, b.color_iota.p, b.order.p, (size_t)n,
+                                     0, bits, st);
+  });
+}
 
 // The LBVH broad phase and the narrow pass over its candidates.  The trees
 // are rebuilt at the first detection of a step (b.bvh_rebuild) and refitted

@@ -660,79 +660,176 @@ uint32_t greedy_colors(const cv_solver& s, const uint32_t* cnb_off, const uint32
 }
 
 
-uint32_t update_colors(const cv_solver& s, ColoringState& cs, const uint32_t* cnb_off, const uint32_t* cnb,
-                       const cv_grab* grab, bool smallest_last, std::vector<uint32_t>& order,
-                       std::vector<uint32_t>& color_off) {
+void ContactNb::build(uint32_t n, const uint32_t* kind_ids, const uint32_t* counts, uint64_t m) {
+  if (loc.size() != n) loc.assign(n, kNoLoc);
+  for (uint32_t v : vert) loc[v] = kNoLoc;
+  vert.clear();
+  len.clear();
+  for (uint64_t c = 0; c < m; ++c)
+    for (uint32_t q = 0; q < counts[c]; ++q) {
+      const uint32_t v = kind_ids[5 * c + 1 + q];
+      if (v >= n) continue;
+      if (loc[v] == kNoLoc) {
+        loc[v] = (uint32_t)vert.size();
+        vert.push_back(v);
+        len.push_back(0);
+      }
+      len[loc[v]]++;
+    }
+  off.assign(vert.size() + 1, 0);
+  for (size_t i = 0; i < vert.size(); ++i) off[i + 1] = off[i] + len[i];
+  rows.resize(off[vert.size()]);
+  for (size_t i = 0; i < vert.size(); ++i) len[i] = off[i];  // fill cursors
+  for (uint64_t c = 0; c < m; ++c)
+    for (uint32_t q = 0; q < counts[c]; ++q) {
+      const uint32_t v = kind_ids[5 * c + 1 + q];
+      if (v < n) rows[len[loc[v]]++] = (uint32_t)c;
+    }
+}
+
+uint32_t update_colors(const cv_solver& s, ColoringState& cs, const uint32_t* kind_ids, const uint32_t* counts,
+                       uint64_t m, const cv_grab* grab, bool smallest_last, std::vector<uint32_t>& color_off) {
   const uint32_t n = s.n;
   uint32_t key[4] = {grab ? 1u : 0u, grab ? grab->vertices[0] : 0u, grab ? grab->vertices[1] : 0u,
                      grab ? grab->vertices[2] : 0u};
-  auto remember = [&] {
+  cs.changed.clear();
+  auto finish = [&] {
+    cs.prev.resize(6 * m);
+    for (uint64_t c = 0; c < m; ++c) {
+      std::memcpy(&cs.prev[6 * c], kind_ids + 5 * c, 5 * sizeof(uint32_t));
+      cs.prev[6 * c + 5] = counts[c];
+    }
     cs.valid = true;
     cs.smallest_last = smallest_last;
     cs.generation = s.generation;
     std::memcpy(cs.grab_key, key, sizeof key);
-    cs.cnb_off.assign(cnb_off, cnb_off + n + 1);
-    cs.cnb.assign(cnb, cnb + cnb_off[n]);
+    uint32_t ncolors = 0;
+    for (uint32_t c = (uint32_t)cs.count.size(); c > 0; --c)
+      if (cs.count[c - 1]) {
+        ncolors = c;
+        break;
+      }
+    color_off.assign(ncolors + 1, 0);
+    for (uint32_t c = 0; c < ncolors; ++c) color_off[c + 1] = color_off[c] + cs.count[c];
+    return ncolors;
   };
   if (!cs.valid || cs.generation != s.generation || cs.smallest_last != smallest_last) {
-    uint32_t k = greedy_colors(s, cnb_off, cnb, grab, cs.color_of, order, color_off, smallest_last);
+    // From scratch, on the dense per-vertex contact neighbour lists.
+    std::vector<uint32_t> off(n + 1, 0), nb, order, coff;
+    for (uint64_t c = 0; c < m; ++c)
+      for (uint32_t q = 0; q < counts[c]; ++q) {
+        const uint32_t v = kind_ids[5 * c + 1 + q];
+        if (v < n) off[v + 1] += counts[c];
+      }
+    for (uint32_t v = 0; v < n; ++v) off[v + 1] += off[v];
+    nb.resize(off[n]);
+    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+    for (uint64_t c = 0; c < m; ++c)
+      for (uint32_t q = 0; q < counts[c]; ++q) {
+        const uint32_t v = kind_ids[5 * c + 1 + q];
+        if (v < n)
+          for (uint32_t r = 0; r < counts[c]; ++r) nb[cur[v]++] = kind_ids[5 * c + 1 + r];
+      }
+    greedy_colors(s, off.data(), nb.data(), grab, cs.color_of, order, coff, smallest_last);
     cs.pos.assign(n, 0);
     for (uint32_t i = 0; i < n; ++i) cs.pos[smallest_last ? s.sl_order[i] : i] = i;
-    remember();
-    return k;
+    cs.count.assign(coff.size() - 1, 0);
+    for (size_t c = 0; c + 1 < coff.size(); ++c) cs.count[c] = coff[c + 1] - coff[c];
+    cs.dirty.assign(n / 64 + 1, 0);
+    cs.seen.assign(n + 2, 0);
+    cs.stamp = 0;
+    cs.full = true;
+    return finish();
   }
-  // Dirty: vertices whose contact neighbour list changed (lists are in the
-  // canonical contact order, so an unchanged neighbourhood gives an equal
-  // list), and the grab vertices when the grab changed.
-  cs.dirty.assign(n, 0);
-  for (uint32_t v = 0; v < n; ++v) {
-    uint32_t a0 = cs.cnb_off[v], a1 = cs.cnb_off[v + 1], b0 = cnb_off[v], b1 = cnb_off[v + 1];
-    if (a1 - a0 != b1 - b0 || !std::equal(cnb + b0, cnb + b1, cs.cnb.begin() + a0)) cs.dirty[v] = 1;
+  cs.full = false;
+  const ContactNb& nx = cs.nb;
+  cs.nb.build(n, kind_ids, counts, m);
+  // Vertices to recolour, as a bitmap over colouring positions scanned upward
+  // (the order a from-scratch pass reaches them): a recoloured vertex only
+  // marks later positions, which the same scan reaches.
+  std::vector<uint64_t>& bits = cs.dirty;
+  auto enqueue = [&](uint32_t v) {
+    if (v >= n) return;
+    const uint32_t p = cs.pos[v];
+    bits[p >> 6] |= 1ull << (p & 63);
+  };
+  // Contacts in only one of the old and new lists: their vertices' contact
+  // neighbourhoods may have changed.
+  const uint64_t mo = cs.prev.size() / 6;
+  auto cmp = [&](uint64_t i, uint64_t j) {  // old row i against new row j
+    const uint32_t* a = &cs.prev[6 * i];
+    const uint32_t* b = kind_ids + 5 * j;
+    for (int k = 0; k < 5; ++k)
+      if (a[k] != b[k]) return a[k] < b[k] ? -1 : 1;
+    return a[5] == counts[j] ? 0 : (a[5] < counts[j] ? -1 : 1);
+  };
+  auto mark_old = [&](uint64_t i) {
+    for (uint32_t q = 0; q < cs.prev[6 * i + 5]; ++q) enqueue(cs.prev[6 * i + 1 + q]);
+  };
+  auto mark_new = [&](uint64_t j) {
+    for (uint32_t q = 0; q < counts[j]; ++q) enqueue(kind_ids[5 * j + 1 + q]);
+  };
+  for (uint64_t i = 0, j = 0; i < mo || j < m;) {
+    if (j == m) {
+      mark_old(i++);
+    } else if (i == mo) {
+      mark_new(j++);
+    } else {
+      const int c = cmp(i, j);
+      if (c < 0)
+        mark_old(i++);
+      else if (c > 0)
+        mark_new(j++);
+      else
+        ++i, ++j;
+    }
   }
   if (std::memcmp(key, cs.grab_key, sizeof key) != 0)
     for (int k = 1; k < 4; ++k) {
-      if (cs.grab_key[0] && cs.grab_key[k] < n) cs.dirty[cs.grab_key[k]] = 1;
-      if (key[0] && key[k] < n) cs.dirty[key[k]] = 1;
+      if (cs.grab_key[0] && cs.grab_key[k] < n) enqueue(cs.grab_key[k]);
+      if (key[0] && key[k] < n) enqueue(key[k]);
     }
   auto in_grab = [&](uint32_t v) {
     return grab && (grab->vertices[0] == v || grab->vertices[1] == v || grab->vertices[2] == v);
   };
-  const uint32_t UNSET = 0xFFFFFFFFu;
-  cs.scratch_seen.assign(n + 2, UNSET);
-  std::vector<uint32_t>& seen = cs.scratch_seen;
-  for (uint32_t idx = 0; idx < n; ++idx) {
-    const uint32_t v = smallest_last ? s.sl_order[idx] : idx;
-    if (!cs.dirty[v]) continue;
-    // Colour from the neighbours that come earlier in the order (the ones a
-    // from-scratch pass has coloured by now).
-    auto mark = [&](uint32_t i) {
-      if (i < n && cs.pos[i] < idx) seen[cs.color_of[i]] = v;
-    };
-    for (uint32_t k = s.graph_off[v]; k < s.graph_off[v + 1]; ++k) mark(s.graph[k]);
-    for (uint32_t k = cnb_off[v]; k < cnb_off[v + 1]; ++k) mark(cnb[k]);
-    if (in_grab(v))
-      for (int k = 0; k < 3; ++k) mark(grab->vertices[k]);
-    uint32_t c = 0;
-    while (seen[c] == v) ++c;
-    if (c == cs.color_of[v]) continue;
-    cs.color_of[v] = c;
-    // Later neighbours may now choose differently.
-    auto touch = [&](uint32_t i) {
-      if (i < n && cs.pos[i] > idx) cs.dirty[i] = 1;
-    };
-    for (uint32_t k = s.graph_off[v]; k < s.graph_off[v + 1]; ++k) touch(s.graph[k]);
-    for (uint32_t k = cnb_off[v]; k < cnb_off[v + 1]; ++k) touch(cnb[k]);
-    if (in_grab(v))
-      for (int k = 0; k < 3; ++k) touch(grab->vertices[k]);
+  for (size_t w = 0; w < bits.size(); ++w) {
+    while (bits[w]) {
+      const uint32_t pv = (uint32_t)(w * 64 + __builtin_ctzll(bits[w]));
+      bits[w] &= bits[w] - 1;
+      const uint32_t v = smallest_last ? s.sl_order[pv] : pv;
+      // Colour from the neighbours that come earlier in the order (the ones a
+      // from-scratch pass has coloured by now).
+      const uint64_t st = ++cs.stamp;
+      auto mark = [&](uint32_t i) {
+        if (i < n && cs.pos[i] < pv) cs.seen[cs.color_of[i]] = st;
+      };
+      for (uint32_t k = s.graph_off[v]; k < s.graph_off[v + 1]; ++k) mark(s.graph[k]);
+      for (uint32_t k = nx.begin(v); k < nx.end(v); ++k) {
+        const uint32_t c = nx.rows[k];
+        for (uint32_t q = 0; q < counts[c]; ++q) mark(kind_ids[5 * c + 1 + q]);
+      }
+      if (in_grab(v))
+        for (int k = 0; k < 3; ++k) mark(grab->vertices[k]);
+      uint32_t c = 0;
+      while (cs.seen[c] == st) ++c;
+      if (c == cs.color_of[v]) continue;
+      cs.count[cs.color_of[v]]--;
+      if (c >= cs.count.size()) cs.count.resize(c + 1, 0);
+      cs.count[c]++;
+      cs.color_of[v] = c;
+      cs.changed.push_back(v);
+      // Later neighbours may now choose differently.
+      auto touch = [&](uint32_t i) {
+        if (i < n && cs.pos[i] > pv) enqueue(i);
+      };
+      for (uint32_t k = s.graph_off[v]; k < s.graph_off[v + 1]; ++k) touch(s.graph[k]);
+      for (uint32_t k = nx.begin(v); k < nx.end(v); ++k) {
+        const uint32_t r = nx.rows[k];
+        for (uint32_t q = 0; q < counts[r]; ++q) touch(kind_ids[5 * r + 1 + q]);
+      }
+      if (in_grab(v))
+        for (int k = 0; k < 3; ++k) touch(grab->vertices[k]);
+    }
   }
-  remember();
-  uint32_t ncolors = 0;
-  for (uint32_t v = 0; v < n; ++v) ncolors = std::max(ncolors, cs.color_of[v] + 1);
-  color_off.assign(ncolors + 1, 0);
-  for (uint32_t v = 0; v < n; ++v) color_off[cs.color_of[v] + 1]++;
-  for (uint32_t c = 0; c < ncolors; ++c) color_off[c + 1] += color_off[c];
-  order.assign(n, 0);
-  std::vector<uint32_t> cur(color_off.begin(), color_off.end() - 1);
-  for (uint32_t v = 0; v < n; ++v) order[cur[cs.color_of[v]]++] = v;
-  return ncolors;
+  return finish();
 }

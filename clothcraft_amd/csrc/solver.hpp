@@ -106,20 +106,44 @@ uint32_t greedy_colors(const cv_solver& s, const uint32_t* cnb_off, const uint32
                        bool smallest_last = false);
 
 
+// The contacts incident to each vertex < n that is in contacts (indices into
+// the contact rows), grouped by vertex.  `loc` maps a vertex to its index in
+// `vert` (kNoLoc when it is in no contact).
+struct ContactNb {
+  static constexpr uint32_t kNoLoc = 0xFFFFFFFFu;
+  std::vector<uint32_t> vert, off, rows, loc, len;
+  void build(uint32_t n, const uint32_t* kind_ids, const uint32_t* counts, uint64_t m);
+  uint32_t begin(uint32_t v) const { return loc[v] == kNoLoc ? 0u : off[loc[v]]; }
+  uint32_t end(uint32_t v) const { return loc[v] == kNoLoc ? 0u : off[loc[v] + 1]; }
+};
+
 // Greedy colouring kept across detections of one solver: a vertex is
-// recoloured only when its contact neighbourhood or grab membership changed
-// or an earlier vertex (in colouring order) among its neighbours changed
-// colour, so the result equals greedy_colors from scratch while the work
-// follows the change.  A new solver generation or order mode starts over.
+// recoloured only when its contacts or grab membership changed or an earlier
+// vertex (in colouring order) among its neighbours changed colour, so the
+// result equals greedy_colors from scratch while the work follows the change.
+// The changed contacts are found by merging the old and new contact rows
+// (both in the canonical sorted order; a row met in only one list marks its
+// vertices, so any order is safe, sorted is fast); the vertices to recolour
+// are taken in colouring order from a bitmap, and only contact vertices and
+// their cascade are visited.  A new solver generation or order mode starts
+// over.
 struct ColoringState {
   bool valid = false, smallest_last = false;
   uint64_t generation = 0;
   uint32_t grab_key[4] = {0, 0, 0, 0};
-  std::vector<uint32_t> color_of, cnb_off, cnb, pos, scratch_seen;
-  std::vector<uint8_t> dirty;
+  std::vector<uint32_t> color_of, pos, count;  // colour and order position of each vertex; vertices per colour
+  std::vector<uint32_t> prev;                  // contact rows of the last colouring: kind, ids[4], count
+  ContactNb nb;                                // incident contacts of this colouring's rows
+  std::vector<uint64_t> dirty;                 // bitmap over colouring positions, all zero between calls
+  std::vector<uint64_t> seen;                  // [n + 2] colour marks, stamped per visited vertex
+  uint64_t stamp = 0;
+  bool full = false;              // the last update coloured from scratch
+  std::vector<uint32_t> changed;  // vertices whose colour the last incremental update changed
 };
-uint32_t update_colors(const cv_solver& s, ColoringState& cs, const uint32_t* cnb_off, const uint32_t* cnb,
-                       const cv_grab* grab, bool smallest_last, std::vector<uint32_t>& order,
-                       std::vector<uint32_t>& color_off);
+// Contact rows are (kind, ids[4]) with `counts` vertices each.  Writes the
+// colour offsets (vertices per colour, prefix-summed) and returns the colour
+// count; cs.color_of holds every vertex's colour.
+uint32_t update_colors(const cv_solver& s, ColoringState& cs, const uint32_t* kind_ids, const uint32_t* counts,
+                       uint64_t m, const cv_grab* grab, bool smallest_last, std::vector<uint32_t>& color_off);
 
 [[noreturn]] void fail(cv_status st, const std::string& msg);

@@ -987,6 +987,11 @@ void sweep_color(cv_gpu& g, const SweepArgs<R>& a, uint32_t off, uint32_t count,
 
 // Host greedy colouring of the current contact set (the detection left the
 // contact list in g.h_cont and its count in the scalars it read back).
+// Colours of the vertices an incremental colouring changed: (vertex, colour) pairs.
+__global__ void k_apply_colors(const uint32_t* chg, uint32_t k, uint32_t* color) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < k) color[chg[2 * i]] = chg[2 * i + 1];
+}
 uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
   uint64_t m = g.h_scal->contacts;
   const uint32_t* cids = g.h_cont;
@@ -1005,24 +1010,41 @@ uint32_t color_on_host(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
   g.color_valid = false;
   g.color_cids.assign(cids, cids + 5 * m);
   g.color_cids.insert(g.color_cids.end(), ccount, ccount + m);
-  g.cnb_off.assign(n + 1, 0);
-  for (uint64_t c = 0; c < m; ++c)
-    for (uint32_t q = 0; q < ccount[c]; ++q) {
-      uint32_t v = cids[5 * c + 1 + q];
-      if (v < n) g.cnb_off[v + 1] += ccount[c];
+  ColoringState& cs = g.coloring;
+  uint32_t k = update_colors(s, cs, cids, ccount, m, grab, false, g.color_off);
+  // The device keeps every vertex's colour; the colour-ordered slot table is
+  // its stable sort by colour (vertex order within a colour, as the host
+  // greedy lists them).  A full colouring uploads all colours, an incremental
+  // one only the vertices it recoloured.
+  // b.order is rebuilt from all colours when another colouring (the fast
+  // one) wrote it last or the previous update did not finish.
+  const bool resync = cs.full || !g.order_host;
+  g.order_host = false;
+  g.b.color_dev.reserve(n);
+  if (resync) {
+    g.b.color_dev.upload(cs.color_of, g.stream);
+  } else if (!cs.changed.empty()) {
+    const size_t nc = cs.changed.size();
+    if (g.h_cchg_cap < nc) {
+      if (g.h_cchg) CV_HIP(hipHostFree(g.h_cchg));
+      g.h_cchg = nullptr;
+      const size_t cap = nc + nc / 2 + 1024;
+      CV_HIP(hipHostMalloc((void**)&g.h_cchg, 2 * cap * sizeof(uint32_t)));
+      g.h_cchg_cap = cap;
     }
-  for (uint32_t v = 0; v < n; ++v) g.cnb_off[v + 1] += g.cnb_off[v];
-  g.cnb.resize(g.cnb_off[n]);
-  std::vector<uint32_t> cur(g.cnb_off.begin(), g.cnb_off.end() - 1);
-  for (uint64_t c = 0; c < m; ++c)
-    for (uint32_t q = 0; q < ccount[c]; ++q) {
-      uint32_t v = cids[5 * c + 1 + q];
-      if (v < n)
-        for (uint32_t r = 0; r < ccount[c]; ++r) g.cnb[cur[v]++] = cids[5 * c + 1 + r];
+    // The previous call's copy out of this buffer finished at the detection's
+    // read-back, so it can be rewritten.
+    for (size_t i = 0; i < nc; ++i) {
+      g.h_cchg[2 * i] = cs.changed[i];
+      g.h_cchg[2 * i + 1] = cs.color_of[cs.changed[i]];
     }
-  uint32_t k = update_colors(s, g.coloring, g.cnb_off.data(), g.cnb.data(), grab, false, g.order,
-                             g.color_off);
-  g.b.order.upload(g.order, g.stream);
+    g.b.color_chg.reserve(2 * nc);
+    CV_HIP(hipMemcpyAsync(g.b.color_chg.p, g.h_cchg, 2 * nc * sizeof(uint32_t), hipMemcpyHostToDevice, g.stream));
+    k_apply_colors<<<blocks((uint32_t)nc), TPB, 0, g.stream>>>(g.b.color_chg.p, (uint32_t)nc, g.b.color_dev.p);
+    CV_HIP(hipGetLastError());
+  }
+  if (resync || !cs.changed.empty()) device_color_order(g, n, k);
+  g.order_host = true;
   g.color_valid = true;
   g.color_generation = s.generation;
   g.color_m = m;
@@ -1070,6 +1092,7 @@ uint32_t color_on_device(cv_gpu& g, const cv_solver& s, const cv_grab* grab) {
     return (uint32_t)(g.color_off.size() - 1);
   g.color_valid = false;
   g.color_cids.clear();
+  g.order_host = false;  // b.order is about to hold the fast colouring's order
   uint32_t k = gpu_fast_colors(g, s, grab, g.color_off);
   g.color_valid = true;
   g.color_generation = s.generation;
@@ -2003,6 +2026,7 @@ void cv_gpu_destroy(cv_gpu* g) {
   if (g->h_v) (void)hipHostFree(g->h_v);
   if (g->h_acc) (void)hipHostFree(g->h_acc);
   if (g->h_cont) (void)hipHostFree(g->h_cont);
+  if (g->h_cchg) (void)hipHostFree(g->h_cchg);
   if (g->h_fc) (void)hipHostFree(g->h_fc);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
   if (g->ev1) (void)hipEventDestroy(g->ev1);

@@ -43,6 +43,11 @@ def test_trace_window(tmp_path):
     # the two longest launches are the real ones
     assert k["mean_us_of_longest"] == {"launches": 2, "mean_us": pytest.approx(12.0)}
     assert out["kernels"]["k_pairs"]["launches"] == 1
+    # device time of the window: four kernels 10 ns apart
+    assert out["busy_ms"] == pytest.approx(76_000 / 1e6)
+    assert out["span_ms"] == pytest.approx(76_030 / 1e6)
+    assert out["gaps_ms"]["under_10us"] == pytest.approx(30 / 1e6)
+    assert out["gaps_ms"]["count_over_100us"] == 0
 
 
 def test_counter_window(tmp_path):

@@ -168,10 +168,15 @@ def test_incremental_colouring_equals_greedy_from_scratch():
     # grow and shrink: subsets of one set, then the other sets
     ki, cn = sets[0]
     keep = rng.random(len(cn)) < 0.7
+    # rows out of the canonical order (the change detection merges sorted
+    # rows; unsorted ones may only cost work, never a missed change)
+    perm = rng.permutation(len(sets[1][1]))
+    shuffled = (sets[1][0][perm], sets[1][1][perm])
     seq = [sets[0], (ki[keep], cn[keep]), sets[1], (np.zeros((0, 5), np.uint32), np.zeros(0, np.uint32)),
-           sets[2], sets[3], sets[3]]
+           sets[2], sets[3], sets[3], shuffled, sets[0]]
     grabs = [None, None, dict(vertices=(3, 40, 77), weights=(0.2, 0.3, 0.5), target=(0, 0, 0), stiffness=10.0),
-             None, None, dict(vertices=(5, 6, 7), weights=(0.2, 0.3, 0.5), target=(0, 0, 0), stiffness=10.0), None]
+             None, None, dict(vertices=(5, 6, 7), weights=(0.2, 0.3, 0.5), target=(0, 0, 0), stiffness=10.0), None,
+             None, None]
     for sl in (False, True):
         inc = host.IncrementalColoring(hs)
         for (kid, cnt), grab in zip(seq, grabs):
