@@ -661,29 +661,20 @@ uint32_t greedy_colors(const cv_solver& s, const uint32_t* cnb_off, const uint32
 
 
 void ContactNb::build(uint32_t n, const uint32_t* kind_ids, const uint32_t* counts, uint64_t m) {
-  if (loc.size() != n) loc.assign(n, kNoLoc);
-  for (uint32_t v : vert) loc[v] = kNoLoc;
-  vert.clear();
-  len.clear();
+  const uint32_t nb = n / 64 + 1;
+  off.assign(nb + 1, 0);
   for (uint64_t c = 0; c < m; ++c)
     for (uint32_t q = 0; q < counts[c]; ++q) {
       const uint32_t v = kind_ids[5 * c + 1 + q];
-      if (v >= n) continue;
-      if (loc[v] == kNoLoc) {
-        loc[v] = (uint32_t)vert.size();
-        vert.push_back(v);
-        len.push_back(0);
-      }
-      len[loc[v]]++;
+      if (v < n) off[(v >> 6) + 1]++;
     }
-  off.assign(vert.size() + 1, 0);
-  for (size_t i = 0; i < vert.size(); ++i) off[i + 1] = off[i] + len[i];
-  rows.resize(off[vert.size()]);
-  for (size_t i = 0; i < vert.size(); ++i) len[i] = off[i];  // fill cursors
+  for (uint32_t b = 0; b < nb; ++b) off[b + 1] += off[b];
+  pairs.resize(off[nb]);
+  std::vector<uint32_t> cur(off.begin(), off.end() - 1);
   for (uint64_t c = 0; c < m; ++c)
     for (uint32_t q = 0; q < counts[c]; ++q) {
       const uint32_t v = kind_ids[5 * c + 1 + q];
-      if (v < n) rows[len[loc[v]]++] = (uint32_t)c;
+      if (v < n) pairs[cur[v >> 6]++] = (uint64_t)v << 32 | (uint32_t)c;
     }
 }
 
@@ -804,10 +795,9 @@ uint32_t update_colors(const cv_solver& s, ColoringState& cs, const uint32_t* ki
         if (i < n && cs.pos[i] < pv) cs.seen[cs.color_of[i]] = st;
       };
       for (uint32_t k = s.graph_off[v]; k < s.graph_off[v + 1]; ++k) mark(s.graph[k]);
-      for (uint32_t k = nx.begin(v); k < nx.end(v); ++k) {
-        const uint32_t c = nx.rows[k];
-        for (uint32_t q = 0; q < counts[c]; ++q) mark(kind_ids[5 * c + 1 + q]);
-      }
+      nx.for_rows(v, [&](uint32_t r) {
+        for (uint32_t q = 0; q < counts[r]; ++q) mark(kind_ids[5 * r + 1 + q]);
+      });
       if (in_grab(v))
         for (int k = 0; k < 3; ++k) mark(grab->vertices[k]);
       uint32_t c = 0;
@@ -823,10 +813,9 @@ uint32_t update_colors(const cv_solver& s, ColoringState& cs, const uint32_t* ki
         if (i < n && cs.pos[i] > pv) enqueue(i);
       };
       for (uint32_t k = s.graph_off[v]; k < s.graph_off[v + 1]; ++k) touch(s.graph[k]);
-      for (uint32_t k = nx.begin(v); k < nx.end(v); ++k) {
-        const uint32_t r = nx.rows[k];
+      nx.for_rows(v, [&](uint32_t r) {
         for (uint32_t q = 0; q < counts[r]; ++q) touch(kind_ids[5 * r + 1 + q]);
-      }
+      });
       if (in_grab(v))
         for (int k = 0; k < 3; ++k) touch(grab->vertices[k]);
     }

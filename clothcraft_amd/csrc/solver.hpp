@@ -106,15 +106,18 @@ uint32_t greedy_colors(const cv_solver& s, const uint32_t* cnb_off, const uint32
                        bool smallest_last = false);
 
 
-// The contacts incident to each vertex < n that is in contacts (indices into
-// the contact rows), grouped by vertex.  `loc` maps a vertex to its index in
-// `vert` (kNoLoc when it is in no contact).
+// The contacts incident to the vertices < n (indices into the contact rows),
+// as (vertex, row) pairs grouped by blocks of 64 vertices: a counting sort
+// whose counters stay in cache, and a lookup scans one block's pairs.
 struct ContactNb {
-  static constexpr uint32_t kNoLoc = 0xFFFFFFFFu;
-  std::vector<uint32_t> vert, off, rows, loc, len;
+  std::vector<uint32_t> off;      // [n / 64 + 2] start of each block's pairs
+  std::vector<uint64_t> pairs;    // vertex << 32 | row
   void build(uint32_t n, const uint32_t* kind_ids, const uint32_t* counts, uint64_t m);
-  uint32_t begin(uint32_t v) const { return loc[v] == kNoLoc ? 0u : off[loc[v]]; }
-  uint32_t end(uint32_t v) const { return loc[v] == kNoLoc ? 0u : off[loc[v] + 1]; }
+  template <class F>
+  void for_rows(uint32_t v, F&& f) const {
+    for (uint32_t k = off[v >> 6]; k < off[(v >> 6) + 1]; ++k)
+      if ((uint32_t)(pairs[k] >> 32) == v) f((uint32_t)pairs[k]);
+  }
 };
 
 // Greedy colouring kept across detections of one solver: a vertex is
