@@ -454,7 +454,7 @@ def bench_step(args):
 def compare_starts(gpu, sc, st, prev_v, run_s, count):
     """From `count` successive states of the plain trajectory: iterations to
     the fixed residual with the plain adaptive start (prediction history
-    installed) and with the learned start, and their device times."""
+    installed) and with the learned start, their device times and detections."""
     from clothcraft_amd import VbdSolver
     from clothcraft_amd.abi import CV_FLAG_WARM_START
     if not os.path.exists(WARM_PARAMS):
@@ -474,7 +474,8 @@ def compare_starts(gpu, sc, st, prev_v, run_s, count):
         warm = st.copy()
         mw = gpu.step(VbdSolver(sc.garment, sc.avatar, warm), warm, warm_s)
         rows.append((mp["iterations_run"], mw["iterations_run"], mp["gpu_compute_ms"], mw["gpu_compute_ms"],
-                     mp["true_residual_rms_n"], mw["true_residual_rms_n"]))
+                     mp["true_residual_rms_n"], mw["true_residual_rms_n"], mp["detection_updates"],
+                     mw["detection_updates"]))
         log(f"compare {k + 1}/{count}: iterations plain {mp['iterations_run']} warm {mw['iterations_run']}")
         pv, st = st.v.copy(), plain
     r = np.array(rows, dtype=np.float64)
@@ -483,6 +484,7 @@ def compare_starts(gpu, sc, st, prev_v, run_s, count):
             "iterations_gain": float(r[:, 0].mean() / max(r[:, 1].mean(), 1e-9)),
             "plain_ms_mean": float(r[:, 2].mean()), "warm_ms_mean": float(r[:, 3].mean()),
             "plain_final_residual_mean": float(r[:, 4].mean()), "warm_final_residual_mean": float(r[:, 5].mean()),
+            "plain_detections_mean": float(r[:, 6].mean()), "warm_detections_mean": float(r[:, 7].mean()),
             "network": "clothcraft_amd/warm_params.npy (clothcraft_amd/train_warm.py), correction "
                        "averaged over the one-ring once"}
 

@@ -351,6 +351,7 @@ struct cv_gpu {
   std::vector<uint32_t> color_cids;  // contact kinds/ids then counts
   // Sweep launch shape (lanes per vertex), chosen by the autotuner.
   int lanes = 32;
+  int batch_block = 256;  // threads per block of the batched sweep, chosen by the autotuner
   bool pred_valid = false;
   // Live sweep-kernel timing (events around every sweep launch).
   bool profile = false;

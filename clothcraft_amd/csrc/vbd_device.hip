@@ -441,8 +441,10 @@ __global__ __launch_bounds__(TPB) void k_sweep_batch(const SweepArgs<R>* args, c
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x), nb = gridDim.x - hb;
   if (L == 32 && lb >= nb) {
     // The heavy vertices of every slot, as in k_sweep: heavy wavefront w of W
-    // takes entries w, w + W, ... of the slots' heavy ranges laid end to end.
-    const uint32_t w = (lb - nb) * kHeavyWaves + threadIdx.x / 64, W = hb * kHeavyWaves;
+    // takes entries w, w + W, ... of the slots' heavy ranges laid end to end
+    // (the block size is the batch's tuned one, a multiple of 64).
+    const uint32_t wpb = blockDim.x / 64;
+    const uint32_t w = (lb - nb) * wpb + threadIdx.x / 64, W = hb * wpb;
     uint32_t base = 0;
     for (uint32_t s = 0; s < nslots; ++s) {
       const BatchSlot sl = slots[s];
@@ -1684,17 +1686,20 @@ void run_batch_t(cv_gpu& P, const std::vector<cv_gpu*>& G, const std::vector<cv_
       for (const Launch& l : per_t[t]) {
         const BatchSlot* sl = dslots + l.slot0;
         uint32_t ns = (uint32_t)l.nslots;
+        // Threads per block: the tuned batch block size (64, 128 or 256); the
+        // heavy blocks keep kHeavyBlocks * TPB / 64 heavy wavefronts.
+        const int B = P.batch_block;
+        const uint32_t hb = kHeavyBlocks * (TPB / B);
         switch (P.lanes) {
-          case 8: k_sweep_batch<8, R><<<blocks((uint64_t)l.total * 8), TPB, 0, st>>>(dargs, sl, ns, l.total, 0); break;
+          case 8: k_sweep_batch<8, R><<<blocks((uint64_t)l.total * 8, B), B, 0, st>>>(dargs, sl, ns, l.total, 0); break;
           case 16:
-            k_sweep_batch<16, R><<<blocks((uint64_t)l.total * 16), TPB, 0, st>>>(dargs, sl, ns, l.total, 0);
+            k_sweep_batch<16, R><<<blocks((uint64_t)l.total * 16, B), B, 0, st>>>(dargs, sl, ns, l.total, 0);
             break;
           case 64:
-            k_sweep_batch<64, R><<<blocks((uint64_t)l.total * 64), TPB, 0, st>>>(dargs, sl, ns, l.total, 0);
+            k_sweep_batch<64, R><<<blocks((uint64_t)l.total * 64, B), B, 0, st>>>(dargs, sl, ns, l.total, 0);
             break;
           default:  // every instance context sweeps with 32 lanes and holds its heavy lists
-            k_sweep_batch<32, R><<<blocks((uint64_t)l.total * 32) + kHeavyBlocks, TPB, 0, st>>>(
-                dargs, sl, ns, l.total, kHeavyBlocks);
+            k_sweep_batch<32, R><<<blocks((uint64_t)l.total * 32, B) + hb, B, 0, st>>>(dargs, sl, ns, l.total, hb);
             break;
         }
       }
@@ -2065,6 +2070,12 @@ cv_status cv_gpu_set_broad_phase(cv_gpu* g, int kind) {
 cv_status cv_gpu_set_lanes(cv_gpu* g, int lanes) {
   if (!g || !(lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64)) return CV_ERR_INVALID_ARGUMENT;
   g->lanes = lanes;
+  return CV_OK;
+}
+
+cv_status cv_gpu_set_batch_block(cv_gpu* g, int threads) {
+  if (!g || !(threads == 64 || threads == 128 || threads == 256)) return CV_ERR_INVALID_ARGUMENT;
+  g->batch_block = threads;
   return CV_OK;
 }
 

@@ -53,6 +53,7 @@ def lib():
         L.cv_gpu_destroy.argtypes = [vp]
         L.cv_gpu_set_check_gradients.argtypes = [vp, C.c_int]
         L.cv_gpu_set_lanes.argtypes = [vp, C.c_int]
+        L.cv_gpu_set_batch_block.argtypes = [vp, C.c_int]
         L.cv_gpu_set_broad_phase.argtypes = [vp, C.c_int]
         L.cv_gpu_set_profiling.argtypes = [vp, C.c_int]
         L.cv_gpu_set_precision.argtypes = [vp, C.c_int]
@@ -238,6 +239,12 @@ class GpuVbd:
         rc = lib().cv_gpu_set_lanes(self.h, lanes)
         if rc != 0:
             raise VbdError(rc, "lanes must be 8, 16, 32 or 64")
+
+    def set_batch_block(self, threads: int):
+        """Threads per block of the batched step's shared sweep launches."""
+        rc = lib().cv_gpu_set_batch_block(self.h, threads)
+        if rc != 0:
+            raise VbdError(rc, "batch block must be 64, 128 or 256 threads")
 
     BROAD_PHASES = {"grid": 0, "lbvh": 1}
 

@@ -415,6 +415,12 @@ cv_status cv_gpu_set_broad_phase(cv_gpu* gpu, int kind);
 /* Lanes per vertex of the sweep kernel (8, 16, 32 or 64); the autotuner sets it. */
 cv_status cv_gpu_set_lanes(cv_gpu* gpu, int lanes);
 
+/* Threads per block of the batched step's shared sweep launches (64, 128 or
+ * 256; default 256); the autotuner sets it with the lanes.  Results do not
+ * depend on it.  No counterpart in the reference: its batched dispatch
+ * (src/gpu_vbd.rs:383-405) has a fixed workgroup size. */
+cv_status cv_gpu_set_batch_block(cv_gpu* gpu, int threads);
+
 /* Library version string. */
 const char* cv_version(void);
 

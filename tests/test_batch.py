@@ -77,10 +77,11 @@ def test_batch_failure_leaves_only_the_failing_state_unchanged(gpu):
 
 
 def test_autotuner_picks_the_fastest_lanes_and_keeps_results(gpu):
-    """The lanes autotuner times every candidate shape on scratch copies
-    (callers' states untouched), leaves the fastest set, and every shape gives
-    the same batched step as 32 lanes to round-off (the lane-tree reduction
-    order depends on the lane count)."""
+    """The launch-shape autotuner times every (block size, lanes) candidate
+    of the batched step on scratch copies (callers' states untouched), leaves
+    the fastest set, and every shape gives the same batched step as 32 lanes
+    and 256-thread blocks to round-off (the lane-tree reduction order depends
+    on the lane count); the block size alone changes nothing, bit for bit."""
     from clothcraft_amd import GpuVbd, VbdSolver, autotune
     scs = _instances()[:3]
     settings = dict(scenes.default_settings(dt=1 / 60, iterations=8, radius=0.002, stiffness=1e5))
@@ -88,7 +89,9 @@ def test_autotuner_picks_the_fastest_lanes_and_keeps_results(gpu):
     res = autotune.tune_lanes(gpu, scs, settings, repeats=1)
     try:
         assert res["lanes"] in autotune.CANDIDATES and set(res["ms"]) == set(autotune.CANDIDATES)
-        assert res["ms"][res["lanes"]] == min(res["ms"].values())
+        assert res["block"] in autotune.BLOCKS and set(res["ms_by_block"]) == set(autotune.BLOCKS)
+        best = min(ms for per in res["ms_by_block"].values() for ms in per.values())
+        assert res["ms"][res["lanes"]] == best
         assert all(np.array_equal(b, sc.state.x) for b, sc in zip(before, scs))
         ref = GpuVbd(0)
         a = [sc.state.copy() for sc in scs]
@@ -98,5 +101,14 @@ def test_autotuner_picks_the_fastest_lanes_and_keeps_results(gpu):
         for x, y in zip(a, b):
             diag = float(np.linalg.norm(x.x.max(axis=0) - x.x.min(axis=0)))
             assert np.abs(x.x - y.x).max() / diag <= 1e-9
+        # the block size alone: bitwise, at 32 lanes (heavy blocks included)
+        gpu.set_lanes(32)
+        for B in (64, 128):
+            gpu.set_batch_block(B)
+            c = [sc.state.copy() for sc in scs]
+            gpu.step_batch([VbdSolver(sc.garment, sc.avatar, st) for sc, st in zip(scs, c)], c, settings)
+            for x, y in zip(a, c):
+                assert np.array_equal(x.x, y.x) and np.array_equal(x.v, y.v)
     finally:
         gpu.set_lanes(32)
+        gpu.set_batch_block(256)
