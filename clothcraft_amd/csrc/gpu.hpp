@@ -352,6 +352,7 @@ struct cv_gpu {
   // Sweep launch shape (lanes per vertex), chosen by the autotuner.
   int lanes = 32;
   int batch_block = 256;  // threads per block of the batched sweep, chosen by the autotuner
+  uint32_t tol_iters = 0;  // iterations the last residual-tolerance step ran (sizes the sweep queue)
   bool pred_valid = false;
   // Live sweep-kernel timing (events around every sweep launch).
   bool profile = false;

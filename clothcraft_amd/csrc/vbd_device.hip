@@ -1362,7 +1362,11 @@ struct StepCtx {
       return finished = true, false;
     }
     first = next;
-    end = next == 0 ? 1 : iters;
+    // With a residual tolerance the batch after iteration 0 runs to the
+    // iteration count the previous such step needed (a step that has not
+    // converged there queues the rest after one more read-back): launches
+    // queued past the convergence point would run as no-ops.
+    end = next == 0 ? 1 : (tol > 0.0 && g.tol_iters > next && g.tol_iters < iters ? g.tol_iters : iters);
     next = end;
     return true;
   }
@@ -1453,6 +1457,7 @@ struct StepCtx {
   void check() const {
     if (g.h_scal->err_nonfinite) fail(CV_ERR_NONFINITE_RESULT, "Non-finite VBD result; state unchanged");
     if (g.h_scal->err_trust) fail(CV_ERR_TRUST_REGION, "GPU OGC trust-region check failed; state unchanged");
+    if (tol > 0.0) g.tol_iters = out.iterations_run;
   }
 };
 
