@@ -351,8 +351,9 @@ __global__ void k_iter_end(Scalars* sc, int it, double threshold, double tol, do
   iter_end(sc, sc->clipped, it, threshold, tol, nfree3);
 }
 // The iteration's end folded into its last sweep launch: the last block to
-// finish (device-scope counter) runs iter_end.  Only without the residual
-// tolerance, which needs a residual pass between the sweep and the decision.
+// finish (device-scope counter) runs iter_end.  Not on iterations after which
+// the residual tolerance is checked, which need a residual pass between the
+// sweep and the decision.
 struct IterTail {
   Scalars* sc;  // null: no tail
   int it;
@@ -1378,12 +1379,13 @@ struct StepCtx {
       // time per launch than the kernel runs).
       bool timed = g.profile && (g.timed_counter++ % kTimedStride == 0);
       if (timed) timed_its.push_back(it);
-      // Without the residual tolerance the iteration's end rides on its last
-      // (non-empty) colour's launch.
+      // The iteration's end rides on its last (non-empty) colour's launch,
+      // except where the residual tolerance is checked after it (a residual
+      // pass comes between the sweep and the decision).
       size_t last = 0;
       for (size_t c = 0; c + 1 < g.color_off.size(); ++c)
         if (g.color_off[c + 1] > g.color_off[c]) last = c;
-      const bool fold = tol <= 0.0 && g.color_off.size() > 1 && g.color_off.back() > 0;
+      const bool fold = !checks(it) && g.color_off.size() > 1 && g.color_off.back() > 0;
       for (size_t c = 0; c + 1 < g.color_off.size(); ++c) {
         uint32_t off = g.color_off[c], cnt = g.color_off[c + 1] - off;
         if (!cnt) continue;
