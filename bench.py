@@ -376,6 +376,11 @@ def bench_step(args):
         pv = st.v.copy()
         gpu.step(sol, st, run_s)
         compare = compare_starts(gpu, sc, st, pv, dict(run_s, iterations=args.compare_cap), args.compare)
+        if compare is not None:
+            compare["trajectory"] = learned_trajectory(gpu, sc, settled, run_s, args, K)
+            compare["trajectory"]["plain_steps_per_s"] = K / dev_s
+            compare["trajectory"]["plain_iterations_per_step"] = m["iterations_run"] / K
+            compare["trajectory"]["plain_detections_per_step"] = m["detection_updates"] / K
 
     kernel = "k_sweep"
     # The arguments that define the per-launch work of the sweep (the step
@@ -487,6 +492,32 @@ def compare_starts(gpu, sc, st, prev_v, run_s, count):
             "plain_detections_mean": float(r[:, 6].mean()), "warm_detections_mean": float(r[:, 7].mean()),
             "network": "clothcraft_amd/warm_params.npy (clothcraft_amd/train_warm.py), correction "
                        "averaged over the one-ring once"}
+
+
+def learned_trajectory(gpu, sc, settled, run_s, args, K):
+    """The timed steps again from the settled state with the learned start
+    (the network compare_starts set up): steps/s, iterations and detections
+    per step, timed like the plain run."""
+    from clothcraft_amd import VbdSolver
+    from clothcraft_amd.abi import CV_FLAG_WARM_START
+    warm_s = dict(run_s, flags=run_s["flags"] | CV_FLAG_WARM_START)
+    st = settled.copy()
+    sol = VbdSolver(sc.garment, sc.avatar, st)
+    gpu.run(sol, st, warm_s, max(args.warmup, 1))
+    gpu.set_profiling(True)
+    ms = its = dets = 0.0
+    done = 0
+    while done < K:
+        k = min(5, K - done)
+        m = gpu.run(sol, st, warm_s, k)
+        ms += m["gpu_compute_ms"]
+        its += m["iterations_run"]
+        dets += m["detection_updates"]
+        done += k
+    gpu.set_profiling(False)
+    return {"steps": f"{K} steps from the settled state ({max(args.warmup, 1)} warm-up) with the learned start",
+            "learned_steps_per_s": K / (ms / 1e3), "learned_iterations_per_step": its / K,
+            "learned_detections_per_step": dets / K}
 
 
 def cpu_fit(sc, state, run_s, threads, lo, hi):
