@@ -60,7 +60,7 @@ tolerance, precision, lanes and broad phase), null for other arguments.
 
 cpu_baseline: the f64 CPU restatement of VbdSolver::step (OpenMP within a
 colour) on the host, from the settled state with the same settings, on the
-process's CPU share (16 threads on a one-GPU box) and on one thread: steps
+process's CPU share (16 threads where the machine gives each GPU 16) and on one thread: steps
 capped at two iteration counts give each a per-step fixed cost and a cost per
 iteration (true residual included), evaluated at the GPU's mean iterations
 per step (a full step to the residual takes minutes on the CPU).
@@ -175,7 +175,7 @@ def cpu_model():
 
 def cpu_share():
     """Threads this process may use: its CPU affinity, capped by OMP_NUM_THREADS
-    (16 on a one-GPU box)."""
+    (16 where the machine gives each GPU 16 threads)."""
     n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     omp = os.environ.get("OMP_NUM_THREADS")
     if omp and omp.isdigit():

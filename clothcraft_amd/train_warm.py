@@ -9,7 +9,7 @@ steps: about 3.3 million pairs.  Labels are 200-sweep solves from the
 adaptive guess.  Training: the device's BF16-MFMA forward / exact backward /
 Adam kernels, 30 epochs, mini-batches of 4,096, learning rate 1e-3, seed 1.
 
-Run on a GPU box from the repository root:  python -m clothcraft_amd.train_warm
+Run on a machine with an MI355X from the repository root:  python -m clothcraft_amd.train_warm
 """
 from __future__ import annotations
 
