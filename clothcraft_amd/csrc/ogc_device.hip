@@ -148,23 +148,28 @@ __global__ void k_reg_fill(const double* fbox, const double* ebox, uint32_t F, d
   keys[j] = key;
   prims[j] = prim;
   // Lanes of a wavefront with the same bucket (neighbouring primitives cover
-  // the same cells) take their ranks from one atomic: one returning atomic
-  // per distinct bucket and wavefront instead of one per entry on a hot
-  // counter.  The loop is wave-uniform over the active lanes.
+  // the same cells) take their ranks from one atomic: the lanes are grouped
+  // by bucket first (a wave-uniform loop over the active lanes, no memory
+  // traffic), then every group's leader adds its group's size in the same
+  // instruction, so a wavefront waits for one atomic round trip.
   const uint32_t bk = table + bucket_of(key, log2t), lane = threadIdx.x & 63;
   unsigned long long todo = __ballot(1);
-  uint32_t rank = 0;
+  uint32_t leader_of = lane, within = 0, group = 0;
   while (todo) {
     const uint32_t leader = __ffsll(todo) - 1;
     const uint32_t bl = __shfl(bk, leader, 64);
     const unsigned long long peers = __ballot(bk == bl);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&bcount[bl], (uint32_t)__popcll(peers));
-    base = __shfl(base, leader, 64);
-    if (bk == bl) rank = base + (uint32_t)__popcll(peers & ((1ull << lane) - 1));
+    if (bk == bl) {
+      leader_of = leader;
+      within = (uint32_t)__popcll(peers & ((1ull << lane) - 1));
+    }
+    if (lane == leader) group = (uint32_t)__popcll(peers);
     todo &= ~peers;
   }
-  ranks[j] = rank;  // one store per wavefront
+  uint32_t base = 0;
+  if (lane == leader_of) base = atomicAdd(&bcount[bk], group);
+  base = __shfl(base, leader_of, 64);
+  ranks[j] = base + within;
 }
 // Bucketed entries carry the primitive's (unpadded) box, so a query scans
 // contiguous memory and tests boxes without a dependent gather; each entry is
