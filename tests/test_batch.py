@@ -38,6 +38,30 @@ def test_batched_step_equals_individual_steps(gpu, flags):
     print("batched == individual over 4 steps for", [sc.name for sc in scs])
 
 
+def test_batched_f32_step_equals_individual_f32_steps(gpu):
+    """The f32 sweep (set_precision(32), the batched workload's precision):
+    the batched launch runs the same per-vertex arithmetic as each solver's
+    own launches.  Both contexts are this test's own, so the shared one keeps
+    its precision."""
+    from clothcraft_amd import GpuVbd, VbdSolver
+    scs = _instances()
+    settings = scenes.default_settings(dt=1 / 60, iterations=12, radius=0.002, stiffness=1e5)
+    one, batch = GpuVbd(0), GpuVbd(0)
+    one.set_precision(32)
+    batch.set_precision(32)
+    a_states = [sc.state.copy() for sc in scs]
+    b_states = [sc.state.copy() for sc in scs]
+    a_sol = [VbdSolver(sc.garment, sc.avatar, st) for sc, st in zip(scs, a_states)]
+    b_sol = [VbdSolver(sc.garment, sc.avatar, st) for sc, st in zip(scs, b_states)]
+    for k in range(3):
+        ma = [one.step(s, st, settings) for s, st in zip(a_sol, a_states)]
+        mb = batch.step_batch(b_sol, b_states, settings)
+        for i, (a, b) in enumerate(zip(a_states, b_states)):
+            assert np.array_equal(a.x, b.x) and np.array_equal(a.v, b.v), (k, i, np.abs(a.x - b.x).max())
+            assert ma[i]["iterations_run"] == mb[i]["iterations_run"]
+            assert ma[i]["contact_terms"] == mb[i]["contact_terms"]
+
+
 def test_full_batch_of_64_uneven_instances_equals_individual_steps(gpu):
     """The batched workload at its full size (64 seeded instances, 1k-50k
     vertices each, ~0.78 M vertices): two batched steps equal each
