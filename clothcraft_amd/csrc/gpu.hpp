@@ -56,8 +56,10 @@ struct DBuf {
 // Step scalars shared by kernels and read back by the host in one copy.
 // One hash-grid registration in bucket order: the cell key (hash
 // collisions are filtered by it), the primitive and its unpadded box, one
-// 64-byte line, so the counting sort's scatter writes whole lines and a query
-// reads each entry with one contiguous load.
+// 64-byte line, so the counting sort's scatter writes whole lines and the
+// broad pass reads each entry with one contiguous load.  Queries are
+// registered the same way (a point in its own cell, a moving edge in the
+// cells of its box), so the queries of one cell sit together.
 struct alignas(64) GridEntry {
   unsigned long long key;
   uint32_t prim, pad;
@@ -83,14 +85,13 @@ struct Scalars {
   unsigned int stop, stop_it;          // refresh needed after iteration stop_it (device decides)
   unsigned int iters_run;
   unsigned int sweep_blocks;           // blocks of the iteration's last sweep launch that have finished
-  unsigned long long n_over;           // edges served by the overflow broad-phase kernel
   unsigned long long cand_export_vf, cand_export_ee;
   unsigned long long vf_list_n, ee_list_n;  // candidates queued for the narrow phase
   unsigned long long grid_entries;          // hash-grid registrations of the last detection
   double diag_p95, diag_max;                // extension percent of the committed positions
   unsigned long long diag_gap_bits;         // max seam gap (bit pattern of a non-negative double)
   unsigned int err_degenerate, err_overflow, err_nonfinite, err_trust;
-  unsigned int span_first[8], span_last[8], span_next[8];  // broad-pass spans per XCD (k_span_bounds)
+  unsigned int span_next;  // the grid broad pass's next span of query entries
 };
 
 // MI355X dispatches the workgroups of a launch to its 8 XCDs round-robin
@@ -172,7 +173,7 @@ struct StepBufs {
   DBuf<double> fbox, ebox;  // [6F], [6E]
   BvhBufs bvh_faces, bvh_edges;          // LBVH broad phase (cv_gpu_set_broad_phase 1)
   BvhBufs bvh_points;                    // points in Morton order (the VF walk's query order)
-  uint64_t bvh_cap_vf = 0, bvh_cap_ee = 0;  // candidate list regions of the LBVH walks
+  uint64_t cand_cap_vf = 0, cand_cap_ee = 0;  // candidate list regions of the broad pass (grid or LBVH)
   bool bvh_rebuild = true;               // rebuild (not refit) at the next LBVH detection
   bool bvh_fallback = false;             // this detection runs the grid (an LBVH walk overflowed)
   uint64_t bvh_fallbacks = 0;
@@ -180,15 +181,12 @@ struct StepBufs {
   DBuf<unsigned long long> cell_key;
   DBuf<uint32_t> cell_val, cell_rank, cell_owner;
   DBuf<GridEntry> cell_entry;  // bucketed registrations
-  DBuf<uint32_t> bcount, bstart;  // hash-grid buckets [2T+1]
-  DBuf<uint32_t> item_len, item_off, item_start, over;  // flattened broad phase
-  DBuf<uint2> pair_list;  // candidates for the narrow phase: VF from the front, EE from the back
+  DBuf<uint32_t> bcount, bstart;  // hash-grid buckets [4T+1]: faces, edges, points, moving-edge cells
+  DBuf<uint2> pair_list;  // candidates for the narrow phase: VF region, then EE region
   DBuf<uint32_t> scan_sums;  // tile totals of the two-level scan
   DBuf<double> red_parts;    // per-block partials of the prologue reduction
   DBuf<unsigned long long> ext_key, ext_key_sorted;  // diagnostics: per-triangle extension
   DBuf<double> ext_area, ext_area_sorted, ext_prefix;
-  DBuf<unsigned long long> item_key;
-  uint64_t pair_cap = 0;
   uint64_t grid_cap = 0, grid_last = 0;  // registration capacity; last detection's count
   DBuf<uint32_t> raw_ids;   // [5*cap] kind, ids
   DBuf<uint32_t> raw_count;

@@ -15,12 +15,15 @@
 // bucket entry keeps its cell key so hash collisions are filtered exactly.  A
 // point looks only in its own cell; an edge scans the cells of its own box
 // and keeps a pair only in the cell holding the lower corner of the box
-// intersection, so each pair is examined once.  The (query cell, bucket
-// entry) pairs are flattened into one list of slots, one thread per slot
-// (k_pairs); edges covering more cells than the list holds per edge are
-// served by groups of G lanes striding over their cells' entries.
-// Pairs are emitted with atomics and then put in the reference's canonical
-// (kind, ids) order by a two-key radix sort, so results are deterministic.
+// intersection, so each pair is examined once.  The queries are registered
+// in the same counting sort (points and moving-edge cells in tables of their
+// own), so the queries of one bucket are contiguous: the broad pass (k_pairs)
+// holds a chunk of 64 bucket entries in a wavefront's lanes, one per lane, and
+// steps through the bucket's queries, each query's data read once from LDS
+// by all lanes.  Candidates go to per-wavefront LDS staging and then to the
+// narrow pass's list.  Pairs are emitted with atomics and then put in the
+// reference's canonical (kind, ids) order by a two-key radix sort, so results
+// are deterministic.
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -91,18 +94,55 @@ __device__ __forceinline__ void cell_range(const ogm::Box& b, double inv_h, long
 __device__ __forceinline__ uint32_t bucket_of(unsigned long long key, int log2t) {
   return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - log2t));
 }
-__device__ __forceinline__ ogm::Box prim_box(const double* fbox, const double* ebox, uint32_t F, uint32_t i) {
-  return i < F ? ldbox(fbox, i) : ldbox(ebox, i - F);
-}
+// The registered primitives, in index order: faces [0, F) and edges
+// [F, F+E) with their boxes inflated by the padded radius (the grid's
+// entries, tables 0 and 1), then the queries: points [F+E, F+E+P) in their
+// own cell (table 2) and moving edges [F+E+P, F+2E+P) in the cells of their
+// own box (table 3; a fixed edge registers no cell).
+struct RegPrims {
+  const double* fbox;
+  const double* ebox;
+  const double* x;
+  const uint8_t* edge_moving;
+  uint32_t F, E, P;
+  double pr;
+  __device__ __forceinline__ uint32_t count() const { return F + 2 * E + P; }
+  // Box, table and primitive id of registered primitive i; false: no cells.
+  __device__ __forceinline__ bool box(uint32_t i, ogm::Box& b, uint32_t& table, uint32_t& prim) const {
+    if (i < F) {
+      b = inflate(ldbox(fbox, i), pr);
+      table = 0;
+      prim = i;
+    } else if (i < F + E) {
+      prim = i - F;
+      b = inflate(ldbox(ebox, prim), pr);
+      table = 1;
+    } else if (i < F + E + P) {
+      prim = i - F - E;
+      b = ogm::box_of(ld3(x, prim));
+      table = 2;
+    } else {
+      prim = i - F - E - P;
+      if (!edge_moving[prim]) return false;
+      b = ldbox(ebox, prim);
+      table = 3;
+    }
+    return true;
+  }
+};
 
-// Cells covered by each primitive box inflated by pr: faces 0..F-1, then
-// edges F..F+E-1.
-__global__ void k_reg_count(const double* fbox, const double* ebox, uint32_t F, uint32_t E, double pr,
-                            double inv_h, uint32_t* count) {
+// Cells covered by each registered primitive.
+__global__ void k_reg_count(RegPrims rp, double inv_h, uint32_t* count) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= F + E) return;
+  if (i >= rp.count()) return;
+  ogm::Box bx;
+  uint32_t table, prim;
+  if (!rp.box(i, bx, table, prim)) {
+    count[i] = 0;
+    return;
+  }
   long long r[6];
-  cell_range(inflate(prim_box(fbox, ebox, F, i), pr), inv_h, r);
+  cell_range(bx, inv_h, r);
   long long c = (r[3] - r[0] + 1) * (r[4] - r[1] + 1) * (r[5] - r[2] + 1);
   count[i] = (uint32_t)(c > kMaxCells ? 1 : c);  // over the cap: one sentinel entry, see k_reg_fill
 }
@@ -119,21 +159,21 @@ __global__ void k_reg_owner(const uint32_t* off, uint32_t N, uint32_t* owner, ui
   }
   for (uint32_t o = off[i]; o < off[i + 1]; ++o) owner[o] = i;
 }
-// One thread per entry: its cell key, primitive id and rank in its bucket
-// (one returning atomic per thread, all in flight together).  Faces use
-// buckets [0, T), edges [T, 2T).
-// The grid covers the capacity; the entry count M = off[N] is read here.
-__global__ void k_reg_fill(const double* fbox, const double* ebox, uint32_t F, double pr, double inv_h,
-                           const uint32_t* off, uint32_t N, const uint32_t* owner, uint64_t cap, int log2t,
-                           unsigned long long* keys, uint32_t* prims, uint32_t* ranks, uint32_t* bcount,
+// One thread per entry: its cell key, primitive id and rank in its bucket.
+// Table t uses buckets [tT, (t+1)T).  The grid covers the capacity; the entry
+// count M = off[N] is read here.
+__global__ void k_reg_fill(RegPrims rp, double inv_h, const uint32_t* off, const uint32_t* owner, uint64_t cap,
+                           int log2t, unsigned long long* keys, uint32_t* prims, uint32_t* ranks, uint32_t* bcount,
                            Scalars* sc) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t M = off[N];
+  const uint32_t N = rp.count(), M = off[N];
   if (M > cap || j >= M) return;
   uint32_t i = owner[j];
-  uint32_t table = i < F ? 0u : (1u << log2t), prim = i < F ? i : i - F;
+  ogm::Box bx;
+  uint32_t table = 0, prim = 0;
+  rp.box(i, bx, table, prim);  // true: primitive i owns an entry
   long long r[6];
-  cell_range(inflate(prim_box(fbox, ebox, F, i), pr), inv_h, r);
+  cell_range(bx, inv_h, r);
   long long nx = r[3] - r[0] + 1, ny = r[4] - r[1] + 1, nz = r[5] - r[2] + 1;
   unsigned long long key;
   if (nx * ny * nz > kMaxCells) {  // over the per-primitive cap: report, never drop silently
@@ -152,7 +192,7 @@ __global__ void k_reg_fill(const double* fbox, const double* ebox, uint32_t F, d
   // by bucket first (a wave-uniform loop over the active lanes, no memory
   // traffic), then every group's leader adds its group's size in the same
   // instruction, so a wavefront waits for one atomic round trip.
-  const uint32_t bk = table + bucket_of(key, log2t), lane = threadIdx.x & 63;
+  const uint32_t bk = (table << log2t) + bucket_of(key, log2t), lane = threadIdx.x & 63;
   unsigned long long todo = __ballot(1);
   uint32_t leader_of = lane, within = 0, group = 0;
   while (todo) {
@@ -171,26 +211,34 @@ __global__ void k_reg_fill(const double* fbox, const double* ebox, uint32_t F, d
   base = __shfl(base, leader_of, 64);
   ranks[j] = base + within;
 }
-// Bucketed entries carry the primitive's (unpadded) box, so a query scans
-// contiguous memory and tests boxes without a dependent gather; each entry is
-// written as one 64-byte line (four 16-byte stores).
-__global__ void k_reg_scatter(const unsigned long long* keys, const uint32_t* prims, const uint32_t* ranks,
-                              const uint32_t* off, uint32_t F, uint32_t N, uint64_t cap, int log2t,
-                              const uint32_t* bstart, const double* fbox, const double* ebox, GridEntry* ent) {
+// Bucketed entries carry the primitive's (unpadded) box, a query entry its
+// own box (a point's: the point twice), so the broad pass reads contiguous
+// memory and tests boxes without a dependent gather; each entry is written as
+// one 64-byte line (four 16-byte stores).
+__global__ void k_reg_scatter(RegPrims rp, const unsigned long long* keys, const uint32_t* prims,
+                              const uint32_t* ranks, const uint32_t* off, uint64_t cap, int log2t,
+                              const uint32_t* bstart, GridEntry* ent) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t M = off[N], face_entries = off[F];
+  const uint32_t N = rp.count(), M = off[N];
   if (M > cap || j >= M) return;
-  bool face = j < face_entries;
+  // Tables in entry order: entries of faces, edges, points, moving edges.
+  const uint32_t table = (j >= off[rp.F]) + (j >= off[rp.F + rp.E]) + (j >= off[rp.F + rp.E + rp.P]);
   const unsigned long long key = keys[j];
-  uint32_t b = (face ? 0u : (1u << log2t)) + bucket_of(key, log2t);
-  uint32_t pos = bstart[b] + ranks[j];
+  uint32_t pos = bstart[(table << log2t) + bucket_of(key, log2t)] + ranks[j];
   uint32_t p = prims[j];
-  const double2* src = (const double2*)(face ? fbox + 6 * (size_t)p : ebox + 6 * (size_t)p);
   double2* dst = (double2*)(ent + pos);
   dst[0] = make_double2(__longlong_as_double((long long)key), __longlong_as_double((long long)p));
-  dst[1] = src[0];
-  dst[2] = src[1];
-  dst[3] = src[2];
+  if (table == 2) {
+    const D3 q = ld3(rp.x, p);
+    dst[1] = make_double2(q.x, q.y);
+    dst[2] = make_double2(q.z, q.x);
+    dst[3] = make_double2(q.y, q.z);
+  } else {
+    const double2* src = (const double2*)(table == 0 ? rp.fbox + 6 * (size_t)p : rp.ebox + 6 * (size_t)p);
+    dst[1] = src[0];
+    dst[2] = src[1];
+    dst[3] = src[2];
+  }
 }
 
 struct DetectArgs {
@@ -211,7 +259,7 @@ struct DetectArgs {
   const double* ebox;
   double rq, pr, pr2, inv_h;
   const GridEntry* ent;            // bucketed entries: cell key, primitive id and box
-  const uint32_t* bstart;          // [2T+1] bucket starts, faces then edges
+  const uint32_t* bstart;          // [4T+1] bucket starts: faces, edges, points, moving-edge cells
   int log2t;
   unsigned long long* dist_bits;
   uint32_t* raw_ids;
@@ -282,14 +330,35 @@ struct PairCounts {
 // depend on the order in which pairs are examined (contacts are sorted and
 // deduplicated afterwards), so the split leaves results unchanged.
 
+// The face data the vertex-face filter needs, and the edge data of the
+// edge-edge filter: loaded when the filter first needs them (the BVH walk's
+// leaves) or held per lane for a whole chunk of queries (the grid pass).
+struct FaceRec {
+  uint32_t v0, v1, v2;
+  bool moving;
+};
+struct EdgeRec {
+  uint32_t c, d;
+  bool moving;
+};
+__device__ __forceinline__ FaceRec load_face(const DetectArgs& a, uint32_t t) {
+  return {a.faces[3 * t], a.faces[3 * t + 1], a.faces[3 * t + 2], a.face_moving[t] != 0};
+}
+__device__ __forceinline__ EdgeRec load_edge(const DetectArgs& a, uint32_t f) {
+  return {a.edges[2 * f], a.edges[2 * f + 1], a.edge_moving[f] != 0};
+}
+
 // Vertex-face filter (src/ogc/world.rs:461-470): padded box test, moving
-// check, incidence and seam exclusion.
+// check, incidence and seam exclusion.  `face()` yields face t's FaceRec.
+template <class FaceOf>
 __device__ __forceinline__ bool vf_filter(const DetectArgs& a, uint32_t v, bool v_moving, bool v_ex,
-                                          const ogm::Box& pb, uint32_t t, const ogm::Box& tb, PairCounts& c) {
+                                          const ogm::Box& pb, uint32_t t, const ogm::Box& tb, PairCounts& c,
+                                          FaceOf face) {
   ++c.tests;
   if (ogm::box_dist2(pb, tb) > a.pr2) return false;
-  if (!v_moving && !a.face_moving[t]) return false;
-  uint32_t t0 = a.faces[3 * t], t1 = a.faces[3 * t + 1], t2 = a.faces[3 * t + 2];
+  const FaceRec fr = face();
+  if (!v_moving && !fr.moving) return false;
+  const uint32_t t0 = fr.v0, t1 = fr.v1, t2 = fr.v2;
   if (v == t0 || v == t1 || v == t2) return false;
   if (v_ex && (is_excluded(a, v, t0) || is_excluded(a, v, t1) || is_excluded(a, v, t2))) return false;
   ++c.vf_cand;
@@ -330,25 +399,22 @@ __device__ __forceinline__ void vf_narrow(const DetectArgs& a, uint32_t v, uint3
   }
 }
 
-constexpr unsigned long long kNoCell = ~0ull;  // a pair met once per query (LBVH): no cell check
-// Edge-edge filter for a pair met in the grid cell `cell`
-// (src/ogc/world.rs:498-512).  The moving edge e is the first operand;
-// moving/moving pairs are taken once with e < f, and only in the cell
-// holding the lower corner of box(e) ∩ inflated box(f), so each pair is
-// examined once.
+// Edge-edge filter (src/ogc/world.rs:498-512).  The moving edge e is the
+// first operand; moving/moving pairs are taken once with e < f.  `edge()`
+// yields edge f's EdgeRec; `in_cell()` is the grid's rule that a pair met in
+// several cells is kept only in the cell holding the lower corner of
+// box(e) ∩ inflated box(f) (a BVH walk meets each pair once: always true).
+template <class EdgeOf, class InCell>
 __device__ __forceinline__ bool ee_filter(const DetectArgs& a, uint32_t e, const ogm::Box& be, uint32_t ia,
                                           uint32_t ib, bool ia_ex, bool ib_ex, uint32_t f, const ogm::Box& bfr,
-                                          unsigned long long cell, PairCounts& c) {
+                                          PairCounts& c, EdgeOf edge, InCell in_cell) {
   ++c.tests;
   if (ogm::box_dist2(be, bfr) > a.pr2) return false;
   if (f == e) return false;
-  if (a.edge_moving[f] && f < e) return false;
-  if (cell != kNoCell) {
-    ogm::Box bf = inflate(bfr, a.pr);
-    double lx = fmax(be.lo.x, bf.lo.x), ly = fmax(be.lo.y, bf.lo.y), lz = fmax(be.lo.z, bf.lo.z);
-    if (cell_key(cell_of(lx, a.inv_h), cell_of(ly, a.inv_h), cell_of(lz, a.inv_h)) != cell) return false;
-  }
-  uint32_t jc = a.edges[2 * f], jd = a.edges[2 * f + 1];
+  const EdgeRec er = edge();
+  if (er.moving && f < e) return false;
+  if (!in_cell()) return false;
+  const uint32_t jc = er.c, jd = er.d;
   if (ia == jc || ia == jd || ib == jc || ib == jd) return false;
   if ((ia_ex && (is_excluded(a, ia, jc) || is_excluded(a, ia, jd))) ||
       (ib_ex && (is_excluded(a, ib, jc) || is_excluded(a, ib, jd))))
@@ -394,52 +460,6 @@ __device__ __forceinline__ void ee_narrow(const DetectArgs& a, uint32_t e, uint3
     add_contact(a, 3, ia, ib, jc, jd, 4, 1e-3 * a.rest_len2[e] * a.rest_len2[f]);
 }
 
-// Wavefront-local staging of the candidate list.  Each wavefront appends its
-// candidates to its own LDS buffer, VF from the front and EE from the back
-// (positions from a ballot prefix count; the counts are wave-uniform
-// registers), and writes them out in bulk with one global atomic per kind
-// when the buffer nears full and at the end: same-address global atomics per
-// test step would serialise across the whole device.  In the global list VF
-// entries grow from the front and EE entries from the back (the two never
-// meet: the list holds one slot per broad-phase pair, and each pair yields at
-// most one candidate).
-constexpr int kPairUnroll = 4;  // pairs per lane per broad-pass step
-constexpr int kWaveList = 512;  // staged candidates per wavefront, both kinds
-// LDS written by some lanes of a wavefront and read by others: a wavefront's
-// LDS operations execute in order, the fences keep the compiler from moving
-// them across this point.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-// Called by every lane of the wavefront; n is wave-uniform.
-__device__ __forceinline__ void wave_stage(uint2* buf, uint32_t& n, bool back, bool take, uint32_t q, uint32_t p) {
-  const unsigned long long mask = __ballot(take);
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t i = n + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
-  if (take) buf[back ? kWaveList - 1 - i : i] = make_uint2(q, p);
-  n += (uint32_t)__popcll(mask);
-}
-// Called by every lane of the wavefront; n is wave-uniform and reset to 0.
-__device__ __forceinline__ void wave_flush(const uint2* buf, uint32_t& n, bool back, unsigned long long* counter,
-                                           uint2* list, uint64_t W) {
-  if (!n) return;
-  wave_sync();
-  const uint32_t lane = threadIdx.x & 63;
-  unsigned long long base = 0;
-  if (lane == 0) base = atomicAdd(counter, (unsigned long long)n);
-  base = (unsigned long long)__shfl((long long)base, 0, 64);
-  for (uint32_t k = lane; k < n; k += 64) {
-    if (back)
-      list[W - 1 - (base + k)] = buf[kWaveList - 1 - k];
-    else
-      list[base + k] = buf[k];
-  }
-  wave_sync();
-  n = 0;
-}
-
 __device__ __forceinline__ void add_counts(const DetectArgs& a, const PairCounts& c) {
   wave_add(&a.sc->grid_tests, c.tests);
   wave_add(&a.sc->vf_cand, c.vf_cand);
@@ -448,256 +468,283 @@ __device__ __forceinline__ void add_counts(const DetectArgs& a, const PairCounts
   wave_add(&a.sc->ee_count, c.ee_near);
 }
 
-// The broad phase is flattened so each thread tests one (query, entry) pair
-// with a short chain of dependent loads; a thread per query walking its
-// cells would serialise dozens of memory round trips.
-//
-// Query items: item i < n_total is point i's own cell in the face table; item
-// n_total + kEdgeCells*e + c is the c-th cell of moving edge e's box in the
-// edge table (unused slots have length 0).  An edge covering more cells than
-// kEdgeCells goes to the overflow list, served by k_ee_overflow.
-constexpr int kEdgeCells = 8;
-__global__ void k_items(DetectArgs a, uint32_t* len, uint32_t* start, unsigned long long* ikey, uint32_t* over,
-                        unsigned long long* n_over) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < a.n_total) {
-    D3 p = ld3(a.x, i);
-    unsigned long long key = cell_key(cell_of(p.x, a.inv_h), cell_of(p.y, a.inv_h), cell_of(p.z, a.inv_h));
-    uint32_t bk = bucket_of(key, a.log2t);
-    uint32_t s0 = a.bstart[bk];
-    start[i] = s0;
-    len[i] = a.bstart[bk + 1] - s0;
-    ikey[i] = key;
-  } else if (i < a.n_total + a.E) {
-    uint32_t e = i - a.n_total;
-    size_t base = a.n_total + (size_t)kEdgeCells * e;
-    int c = 0;
-    if (a.edge_moving[e]) {
-      long long r[6];
-      cell_range(ldbox(a.ebox, e), a.inv_h, r);
-      long long cnt = (r[3] - r[0] + 1) * (r[4] - r[1] + 1) * (r[5] - r[2] + 1);
-      if (cnt > kEdgeCells) {
-        over[atomicAdd(n_over, 1ull)] = e;
-      } else {
-        uint32_t table = 1u << a.log2t;
-        for (long long x = r[0]; x <= r[3]; ++x)
-          for (long long y = r[1]; y <= r[4]; ++y)
-            for (long long z = r[2]; z <= r[5]; ++z) {
-              unsigned long long key = cell_key(x, y, z);
-              uint32_t bk = table + bucket_of(key, a.log2t);
-              uint32_t s0 = a.bstart[bk];
-              start[base + c] = s0;
-              len[base + c] = a.bstart[bk + 1] - s0;
-              ikey[base + c] = key;
-              ++c;
-            }
-      }
+// Per-wavefront candidate staging of the broad pass (grid or LBVH walks): a
+// lane with a candidate appends it to its wavefront's LDS buffer (positions
+// from a ballot prefix count), and a wavefront whose buffer could overflow on
+// its next step writes the buffer out with one global atomic: same-address
+// global atomics per test step would serialise across the whole device.  The
+// loops that stage are wavefront-uniform and need no block barrier.  Entries
+// beyond `cap` are counted but not written: the capacity flag (32) makes the
+// host grow the region and detect again.
+template <int NB>
+struct WaveStage {
+  static constexpr uint32_t kCap = NB;
+  uint2 buf[NB];
+  uint32_t n;
+};
+template <class WS>
+__device__ __forceinline__ void wave_stage(WS& ws, bool take, uint32_t q, uint32_t p) {
+  const uint64_t mask = __ballot(take);
+  if (!mask) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t base = ws.n;  // wavefront-uniform
+  if (take) ws.buf[base + __popcll(mask & ((1ull << lane) - 1))] = make_uint2(q, p);
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) ws.n = base + (uint32_t)__popcll(mask);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+template <class WS>
+__device__ __forceinline__ void wave_flush(WS& ws, unsigned long long* total, uint64_t cap, uint2* region,
+                                           Scalars* sc) {
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const uint32_t n = ws.n, lane = threadIdx.x & 63;
+  if (!n) return;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(total, (unsigned long long)n);
+  base = __shfl(base, 0, 64);
+  for (uint32_t k = lane; k < n; k += 64)
+    if (base + k < cap) region[base + k] = ws.buf[k];
+  if (lane == 0 && base + n > cap) atomicOr(&sc->err_overflow, 32u);
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) ws.n = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+// LDS written by some lanes of a wavefront and read by others: a wavefront's
+// LDS operations execute in order, the fences keep the compiler from moving
+// them across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Grid broad pass, bucket-major.  The query entries (tables 2 and 3 of the
+// registration: points, then the cells of moving edges) are taken in spans of
+// 64 consecutive entries per wavefront, from one counter.  A span's queries
+// are staged in LDS (box, cell key, the bucket's entry range, ids, flags) and
+// fall into runs of consecutive queries of one bucket, since the registration
+// put each bucket's queries together.  For each run the wavefront loads the
+// bucket's entries in chunks of 64, one per lane with its face or edge data,
+// and tests every query of the run against its lanes' entries, the query's
+// data read by all lanes at one LDS address: each entry is loaded once per
+// run instead of once per (query, entry) pair.  The pairs examined, and so the
+// candidates, are those of each query scanning its own bucket.
+constexpr uint32_t kSpan = 64;
+struct SpanQuery {
+  double box[6];          // a point's box is the point twice
+  unsigned long long key; // the query's cell
+  uint32_t s, e;          // the bucket's entries
+  uint32_t id;            // point or edge
+  uint32_t fl;            // point: moving | excluded list << 1; edge: a excluded | b excluded << 1
+  uint32_t qa, qb;        // an edge's vertices
+  long long c[3];         // an edge: the cell of its box's lower corner
+};
+// A query as the test loop holds it in registers (read from LDS one query
+// ahead of its use, so the read's latency overlaps the previous query's tests).
+struct QueryRegs {
+  ogm::Box box;
+  unsigned long long key;
+  uint32_t id, fl, qa, qb;
+  long long c0, c1, c2;
+};
+template <bool EDGE>
+__device__ __forceinline__ QueryRegs query_regs(const SpanQuery& q) {
+  QueryRegs r;
+  r.box = ogm::Box{{q.box[0], q.box[1], q.box[2]}, {q.box[3], q.box[4], q.box[5]}};
+  r.key = q.key;
+  r.id = q.id;
+  r.fl = q.fl;
+  if (EDGE) {
+    r.qa = q.qa;
+    r.qb = q.qb;
+    r.c0 = q.c[0];
+    r.c1 = q.c[1];
+    r.c2 = q.c[2];
+  } else {
+    r.qa = r.qb = 0;
+    r.c0 = r.c1 = r.c2 = 0;
+  }
+  return r;
+}
+// Candidate staging with the staged count in a register (wavefront-uniform):
+// a step's candidates are placed by a ballot prefix count, with no LDS read
+// on the test loop's path.
+__device__ __forceinline__ void stage_take(uint2* buf, uint32_t& n, bool take, uint32_t q, uint32_t p) {
+  const unsigned long long mask = __ballot(take);
+  const uint32_t lane = threadIdx.x & 63;
+  if (take) buf[n + (uint32_t)__popcll(mask & ((1ull << lane) - 1))] = make_uint2(q, p);
+  n += (uint32_t)__popcll(mask);
+}
+__device__ __forceinline__ void stage_flush(const uint2* buf, uint32_t& n, unsigned long long* total, uint64_t cap,
+                                            uint2* region, Scalars* sc) {
+  if (!n) return;
+  wave_sync();  // the staged entries are read by other lanes
+  const uint32_t lane = threadIdx.x & 63;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(total, (unsigned long long)n);
+  base = __shfl(base, 0, 64);
+  for (uint32_t k = lane; k < n; k += 64)
+    if (base + k < cap) region[base + k] = buf[k];
+  if (lane == 0 && base + n > cap) atomicOr(&sc->err_overflow, 32u);
+  wave_sync();  // read before the buffer is staged again
+  n = 0;
+}
+// The tests of one run's queries [k0, k1) against a chunk of the bucket's
+// entries, one entry per lane (`ok`: the lane holds one).
+template <bool EDGE>
+__device__ __forceinline__ void run_chunk(const DetectArgs& a, const SpanQuery* qs, uint32_t k0, uint32_t k1,
+                                          const GridEntry& en, bool ok, uint2* buf, uint32_t& n, uint32_t ncap,
+                                          unsigned long long* total, uint64_t cap, uint2* region, PairCounts& cnt) {
+  const ogm::Box eb{{en.box[0], en.box[1], en.box[2]}, {en.box[3], en.box[4], en.box[5]}};
+  const uint32_t prim = en.prim;
+  const unsigned long long ekey = ok ? en.key : ~0ull;  // a lane past the bucket matches no query
+  FaceRec fr{};
+  EdgeRec er{};
+  long long f0 = 0, f1 = 0, f2 = 0;
+  if (ok) {
+    if (EDGE) {
+      er = load_edge(a, prim);
+      // Cells of the inflated entry box's lower corner: the lower corner of
+      // box(e) ∩ inflated box(f) is the per-axis max of the two lower
+      // corners, and the cell map is monotone, so its cell is the per-axis
+      // max of the two corners' cells.
+      f0 = cell_of(eb.lo.x - a.pr, a.inv_h);
+      f1 = cell_of(eb.lo.y - a.pr, a.inv_h);
+      f2 = cell_of(eb.lo.z - a.pr, a.inv_h);
+    } else {
+      fr = load_face(a, prim);
     }
-    for (; c < kEdgeCells; ++c) len[base + c] = 0;
+  }
+  QueryRegs cur = query_regs<EDGE>(qs[k0]);
+  for (uint32_t k = k0; k < k1; ++k) {
+    const QueryRegs nxt = query_regs<EDGE>(qs[min(k + 1, k1 - 1)]);
+    bool take = false;
+    if (ekey == cur.key) {
+      if (EDGE)
+        take = ee_filter(a, cur.id, cur.box, cur.qa, cur.qb, cur.fl & 1u, cur.fl & 2u, prim, eb, cnt,
+                         [&] { return er; },
+                         [&] { return cell_key(max(cur.c0, f0), max(cur.c1, f1), max(cur.c2, f2)) == cur.key; });
+      else
+        take = vf_filter(a, cur.id, cur.fl & 1u, cur.fl & 2u, cur.box, prim, eb, cnt, [&] { return fr; });
+    }
+    stage_take(buf, n, take, cur.id, prim);
+    if (n > ncap - 64) stage_flush(buf, n, total, cap, region, a.sc);
+    cur = nxt;
   }
 }
-// Broad-pass work split.  The pair slots (item_off) are cut into 8 equal
-// ranges, one per XCD (blocks b with b % 8 == x run on XCD x), so
-// neighbouring items read the same buckets from one L2 and every XCD gets an
-// equal share however many slots are in use.  Items are taken in spans of
-// kSpan consecutive items; XCD x serves the spans that meet its slot range,
-// clipped to it: first[x] and last[x] are those spans, next[x] the counter its
-// wavefronts take spans from.  Slots past `cap` raise the capacity flag (the
-// host grows the list and detects again; the broad pass examines nothing).
-constexpr uint32_t kSpan = 64;
-__global__ void k_span_bounds(const uint32_t* off, uint32_t NI, uint64_t cap, Scalars* sc) {
-  const uint32_t x = threadIdx.x;
-  const uint64_t W = off[NI];
-  if (x == 0 && W > cap) atomicOr(&sc->err_overflow, 4u);
-  if (x >= 8) return;
-  const uint32_t NS = (NI + kSpan - 1) / kSpan;
-  const uint64_t lo = W * x / 8, hi = W * (x + 1) / 8;
-  // span holding slot t: the last span k with off[k kSpan] <= t
-  auto span_of = [&](uint64_t t) {
-    uint32_t a = 0, b = NS;  // off[a kSpan] <= t < off[b kSpan] (b = NS: W)
-    while (b - a > 1) {
-      uint32_t m = (a + b) >> 1;
-      if (off[(size_t)m * kSpan] <= t) a = m; else b = m;
-    }
-    return a;
-  };
-  const bool any = NS > 0 && hi > lo && W <= cap;
-  sc->span_first[x] = any ? span_of(lo) : 1u;
-  sc->span_last[x] = any ? span_of(hi - 1) : 0u;
-  sc->span_next[x] = 0u;
-}
-// Broad pass: one (query item, bucket entry) pair per lane and kPairUnroll
-// per lane per step, the candidates staged per wavefront for the
-// narrow-phase list.  A wavefront takes a span of items at a time and stages
-// in LDS what every pair of an item shares: its slot offset, bucket start and
-// cell key, and the query's box, vertex ids and flags.  A slot's item is found
-// by binary search over the offsets, so a pair test starts from its bucket
-// entry's loads.
-__global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, const uint32_t* off, const uint32_t* start,
-                                               const unsigned long long* ikey, uint32_t NI, uint64_t cap,
-                                               uint2* list) {
-  constexpr uint32_t WPB = TPB / 64;
+__global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, uint64_t cap_vf, uint2* vf, uint64_t cap_ee,
+                                               uint2* ee) {
+  constexpr uint32_t kOut = 256;  // staged candidates per wavefront (with the queries: 4 wavefronts per SIMD)
   struct Stage {
-    uint2 cand[kWaveList];
-    double qbox[kSpan][6];       // the query's box (a point's: the point twice)
-    unsigned long long key[kSpan];
-    uint32_t off[kSpan + 1], start[kSpan];
-    uint32_t qa[kSpan], qb[kSpan];  // an edge's vertices
-    uint32_t qf[kSpan];             // point: moving | excluded list << 1; edge: a excluded | b excluded << 1
+    SpanQuery q[kSpan];
+    uint2 out[kOut];
   };
-  __shared__ Stage stage[WPB];
-  const uint64_t W = off[NI];
-  if (W > cap) return;  // flagged by k_span_bounds
-  const uint32_t lane = threadIdx.x & 63, x = blockIdx.x % 8;
+  __shared__ Stage stage[TPB / 64];
   Stage& st = stage[threadIdx.x >> 6];
-  const uint64_t lo_x = W * x / 8, hi_x = W * (x + 1) / 8;
-  const uint32_t first = a.sc->span_first[x], last = a.sc->span_last[x];
-  uint32_t n_vf = 0, n_ee = 0;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t T = 1u << a.log2t;
+  const uint32_t P0 = a.bstart[2 * T], PE = a.bstart[3 * T], P1 = a.bstart[4 * T];
   PairCounts cnt;
+  uint32_t n_out = 0;   // staged candidates (wavefront-uniform)
+  bool out_ee = false;  // their kind
   for (;;) {
     uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(&a.sc->span_next[x], 1u);
-    const uint32_t span = first + (uint32_t)__shfl((int)t, 0, 64);
-    if (span > last) break;
-    const uint32_t base = span * kSpan, ni = min(kSpan, NI - base);
-    wave_sync();  // the previous span's staged values are no longer read
-    uint32_t my_off = 0, my_end = 0;
-    if (lane < ni) {
-      my_off = off[base + lane];
-      my_end = off[base + lane + 1];
-      st.off[lane] = my_off;
-      st.start[lane] = start[base + lane];
-      st.key[lane] = ikey[base + lane];
-    }
-    if (lane == 0) st.off[ni] = off[base + ni];
-    if (lane < ni && my_end > my_off) {  // the query of an item with pairs
-      const uint32_t item = base + lane;
-      double* qb = st.qbox[lane];
-      if (item < a.n_total) {
-        const D3 p = ld3(a.x, item);
-        qb[0] = qb[3] = p.x;
-        qb[1] = qb[4] = p.y;
-        qb[2] = qb[5] = p.z;
-        st.qf[lane] = (item < a.n ? 1u : 0u) | (a.ex_off[item] != a.ex_off[item + 1] ? 2u : 0u);
-      } else {
-        const uint32_t e = (item - a.n_total) / kEdgeCells;
-        const ogm::Box be = ldbox(a.ebox, e);
-        qb[0] = be.lo.x;
-        qb[1] = be.lo.y;
-        qb[2] = be.lo.z;
-        qb[3] = be.hi.x;
-        qb[4] = be.hi.y;
-        qb[5] = be.hi.z;
-        const uint32_t ia = a.edges[2 * e], ib = a.edges[2 * e + 1];
-        st.qa[lane] = ia;
-        st.qb[lane] = ib;
-        st.qf[lane] = (a.ex_off[ia] != a.ex_off[ia + 1] ? 1u : 0u) | (a.ex_off[ib] != a.ex_off[ib + 1] ? 2u : 0u);
+    if (lane == 0) t = atomicAdd(&a.sc->span_next, 1u);
+    const uint64_t p0 = P0 + (uint64_t)kSpan * (uint32_t)__shfl((int)t, 0, 64);
+    if (p0 >= P1) break;
+    wave_sync();  // the previous span's queries are no longer read
+    const uint64_t p = p0 + lane;
+    bool member = false;
+    uint32_t s = 0;
+    if (p < P1) {
+      const GridEntry qe = a.ent[p];
+      const bool edge = p >= PE;
+      uint32_t e = 0;
+      if (qe.key != ~0ull) {  // a sentinel (over the cell cap, flagged) queries nothing
+        const uint32_t bk = (edge ? T : 0u) + bucket_of(qe.key, a.log2t);
+        s = a.bstart[bk];
+        e = a.bstart[bk + 1];
+        member = e > s;
+      }
+      if (member) {
+        SpanQuery& q = st.q[lane];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) q.box[k] = qe.box[k];
+        q.key = qe.key;
+        q.s = s;
+        q.e = e;
+        q.id = qe.prim;
+        if (!edge) {
+          const uint32_t v = qe.prim;
+          q.fl = (v < a.n ? 1u : 0u) | (a.ex_off[v] != a.ex_off[v + 1] ? 2u : 0u);
+        } else {
+          const uint32_t ia = a.edges[2 * qe.prim], ib = a.edges[2 * qe.prim + 1];
+          q.qa = ia;
+          q.qb = ib;
+          q.fl = (a.ex_off[ia] != a.ex_off[ia + 1] ? 1u : 0u) | (a.ex_off[ib] != a.ex_off[ib + 1] ? 2u : 0u);
+          q.c[0] = cell_of(qe.box[0], a.inv_h);
+          q.c[1] = cell_of(qe.box[1], a.inv_h);
+          q.c[2] = cell_of(qe.box[2], a.inv_h);
+        }
       }
     }
     wave_sync();
-    const uint64_t s_lo = max((uint64_t)st.off[0], lo_x), s_hi = min((uint64_t)st.off[ni], hi_x);
-    for (uint64_t s0 = s_lo; s0 < s_hi; s0 += 64 * kPairUnroll) {
-      bool vf[kPairUnroll], ee[kPairUnroll];
-      uint32_t qs[kPairUnroll], prims[kPairUnroll];
-#pragma unroll
-      for (int u = 0; u < kPairUnroll; ++u) {
-        const uint64_t sl = s0 + (uint64_t)u * 64 + lane;
-        vf[u] = ee[u] = false;
-        uint32_t q = 0, prim = 0;
-        if (sl < s_hi) {
-          const uint32_t s32 = (uint32_t)sl;
-          uint32_t k = 0, kb = ni;  // st.off[k] <= s32 < st.off[kb]
-          while (kb - k > 1) {
-            const uint32_t m = (k + kb) >> 1;
-            if (st.off[m] <= s32) k = m; else kb = m;
-          }
-          const uint32_t item = base + k;
-          const uint32_t it = st.start[k] + (s32 - st.off[k]);
-          const unsigned long long key = st.key[k];
-          const GridEntry en = a.ent[it];
-          prim = en.prim;
-          if (en.key == key) {  // else another cell hashed to the same bucket
-            const ogm::Box eb{{en.box[0], en.box[1], en.box[2]}, {en.box[3], en.box[4], en.box[5]}};
-            const double* qx = st.qbox[k];
-            const ogm::Box qbx{{qx[0], qx[1], qx[2]}, {qx[3], qx[4], qx[5]}};
-            const uint32_t fl = st.qf[k];
-            if (item < a.n_total) {
-              q = item;
-              vf[u] = vf_filter(a, q, fl & 1u, fl & 2u, qbx, prim, eb, cnt);
-            } else {
-              q = (item - a.n_total) / kEdgeCells;
-              ee[u] = ee_filter(a, q, qbx, st.qa[k], st.qb[k], fl & 1u, fl & 2u, prim, eb, key, cnt);
-            }
-          }
-        }
-        qs[u] = q;
-        prims[u] = prim;
+    // Runs: a member starts one unless the lane before it is a member of the
+    // same bucket (the entry range's start identifies a non-empty bucket).
+    const uint32_t prev_s = (uint32_t)__shfl((int)s, (int)((lane + 63) & 63), 64);
+    const bool prev_member = __shfl((int)member, (int)((lane + 63) & 63), 64) != 0;
+    const bool head = member && (lane == 0 || !prev_member || prev_s != s);
+    unsigned long long heads = __ballot(head);
+    const unsigned long long members = __ballot(member);
+    while (heads) {
+      const uint32_t k0 = (uint32_t)__ffsll((long long)heads) - 1;
+      heads &= heads - 1;
+      const unsigned long long after = k0 == 63 ? 0ull : (~0ull << (k0 + 1));
+      const unsigned long long stop = (heads | ~members) & after;
+      const uint32_t k1 = stop ? (uint32_t)__ffsll((long long)stop) - 1 : kSpan;
+      const bool edge_run = p0 + k0 >= PE;
+      const uint32_t rs = st.q[k0].s, re = st.q[k0].e;
+      if (edge_run != out_ee) {
+        stage_flush(st.out, n_out, out_ee ? &a.sc->ee_list_n : &a.sc->vf_list_n, out_ee ? cap_ee : cap_vf,
+                    out_ee ? ee : vf, a.sc);
+        out_ee = edge_run;
       }
-#pragma unroll
-      for (int u = 0; u < kPairUnroll; ++u) {
-        wave_stage(st.cand, n_vf, false, vf[u], qs[u], prims[u]);
-        wave_stage(st.cand, n_ee, true, ee[u], qs[u], prims[u]);
-      }
-      if (n_vf + n_ee > kWaveList - 64 * kPairUnroll) {
-        wave_flush(st.cand, n_vf, false, &a.sc->vf_list_n, list, W);
-        wave_flush(st.cand, n_ee, true, &a.sc->ee_list_n, list, W);
+      unsigned long long* total = edge_run ? &a.sc->ee_list_n : &a.sc->vf_list_n;
+      const uint64_t cap = edge_run ? cap_ee : cap_vf;
+      uint2* region = edge_run ? ee : vf;
+      for (uint32_t c0 = rs; c0 < re; c0 += 64) {
+        const uint32_t j = c0 + lane;
+        const bool ok = j < re;
+        GridEntry en{};
+        if (ok) en = a.ent[j];
+        if (edge_run)
+          run_chunk<true>(a, st.q, k0, k1, en, ok, st.out, n_out, kOut, total, cap, region, cnt);
+        else
+          run_chunk<false>(a, st.q, k0, k1, en, ok, st.out, n_out, kOut, total, cap, region, cnt);
       }
     }
   }
-  wave_flush(st.cand, n_vf, false, &a.sc->vf_list_n, list, W);
-  wave_flush(st.cand, n_ee, true, &a.sc->ee_list_n, list, W);
-  add_counts(a, cnt);
+  stage_flush(st.out, n_out, out_ee ? &a.sc->ee_list_n : &a.sc->vf_list_n, out_ee ? cap_ee : cap_vf,
+              out_ee ? ee : vf, a.sc);
+  add_counts(a, cnt);  // every lane reaches it
 }
-// Narrow pass over the candidate list, VF and EE in separate grid-stride
-// ranges of one launch (no host read-back of the counts).
-__global__ __launch_bounds__(TPB) void k_narrow(DetectArgs a, const uint32_t* off, uint32_t NI, uint64_t cap,
-                                                const uint2* list) {
-  uint64_t W = off[NI];
-  if (W > cap) W = 0;
-  uint64_t nvf = W ? a.sc->vf_list_n : 0, nee = W ? a.sc->ee_list_n : 0;
+// Narrow pass over the two candidate regions (the counts are the device's; a
+// region that overflowed is redone by the host).
+__global__ __launch_bounds__(TPB) void k_narrow_lists(DetectArgs a, const uint2* vf, uint64_t cap_vf, const uint2* ee,
+                                                      uint64_t cap_ee) {
   PairCounts cnt;
-  uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t nvf = min((uint64_t)a.sc->vf_list_n, cap_vf), nee = min((uint64_t)a.sc->ee_list_n, cap_ee);
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nvf + nee; j += step) {
     if (j < nvf) {
-      uint2 e = list[j];
-      vf_narrow(a, e.x, e.y, cnt);
+      const uint2 p = vf[j];
+      vf_narrow(a, p.x, p.y, cnt);
     } else {
-      uint2 e = list[W - 1 - (j - nvf)];
-      ee_narrow(a, e.x, e.y, cnt);
+      const uint2 p = ee[j - nvf];
+      ee_narrow(a, p.x, p.y, cnt);
     }
-  }
-  add_counts(a, cnt);
-}
-// Edges whose box covers more than kEdgeCells cells: a group of G lanes per
-// listed edge walks its cells (rare: edges much longer than the cell size).
-template <int G>
-__global__ __launch_bounds__(TPB) void k_ee_overflow(DetectArgs a, const uint32_t* over,
-                                                     const unsigned long long* n_over) {
-  PairCounts cnt;
-  uint32_t lane = threadIdx.x % G;
-  uint64_t groups = (uint64_t)gridDim.x * blockDim.x / G;
-  for (uint64_t gi = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G; gi < *n_over; gi += groups) {
-    uint32_t e = over[gi];
-    ogm::Box be = ldbox(a.ebox, e);
-    long long r[6];
-    cell_range(be, a.inv_h, r);
-    uint32_t ia = a.edges[2 * e], ib = a.edges[2 * e + 1];
-    bool ia_ex = a.ex_off[ia] != a.ex_off[ia + 1], ib_ex = a.ex_off[ib] != a.ex_off[ib + 1];
-    uint32_t table = 1u << a.log2t;
-    for (long long ci = r[0]; ci <= r[3]; ++ci)
-      for (long long cj = r[1]; cj <= r[4]; ++cj)
-        for (long long ck = r[2]; ck <= r[5]; ++ck) {
-          unsigned long long key = cell_key(ci, cj, ck);
-          uint32_t bk = table + bucket_of(key, a.log2t);
-          uint32_t end = a.bstart[bk + 1];
-          for (uint32_t it = a.bstart[bk] + lane; it < end; it += G) {
-            const GridEntry en = a.ent[it];
-            if (en.key != key) continue;
-            ogm::Box bfr{{en.box[0], en.box[1], en.box[2]}, {en.box[3], en.box[4], en.box[5]}};
-            uint32_t f = en.prim;
-            if (ee_filter(a, e, be, ia, ib, ia_ex, ib_ex, f, bfr, key, cnt)) ee_narrow(a, e, f, cnt);
-          }
-        }
   }
   add_counts(a, cnt);
 }
@@ -853,46 +900,6 @@ __global__ void k_morton_points(const double* x, uint32_t N, double lo, double s
   idx[i] = i;
 }
 
-// Per-wavefront candidate staging of the LBVH walks: a lane with a candidate
-// appends it to its wavefront's LDS buffer (one LDS atomic per wavefront and
-// step), and a wavefront whose buffer could overflow on its next step writes
-// the buffer out with one global atomic.  The walks' loops are
-// wavefront-uniform and need no block barrier.  Entries beyond `cap` are
-// counted but not written: the capacity flag (32) makes the host grow the
-// region and detect again.
-constexpr int kWaveBuf = 512;
-struct WaveStage {
-  uint2 buf[kWaveBuf];
-  uint32_t n;
-};
-__device__ __forceinline__ void wave_stage(WaveStage& ws, bool take, uint32_t q, uint32_t p) {
-  const uint64_t mask = __ballot(take);
-  if (!mask) return;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t base = ws.n;  // wavefront-uniform
-  if (take) ws.buf[base + __popcll(mask & ((1ull << lane) - 1))] = make_uint2(q, p);
-  __builtin_amdgcn_wave_barrier();
-  if (lane == 0) ws.n = base + (uint32_t)__popcll(mask);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-__device__ __forceinline__ void wave_flush(WaveStage& ws, unsigned long long* total, uint64_t cap, uint2* region,
-                                           Scalars* sc) {
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const uint32_t n = ws.n, lane = threadIdx.x & 63;
-  if (!n) return;
-  unsigned long long base = 0;
-  if (lane == 0) base = atomicAdd(total, (unsigned long long)n);
-  base = __shfl(base, 0, 64);
-  for (uint32_t k = lane; k < n; k += 64)
-    if (base + k < cap) region[base + k] = ws.buf[k];
-  if (lane == 0 && base + n > cap) atomicOr(&sc->err_overflow, 32u);
-  __builtin_amdgcn_wave_barrier();
-  if (lane == 0) ws.n = 0;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
 // One LBVH walk per query, queries in Morton order (`order`), so the lanes
 // of a wavefront descend the same subtrees.  A popped internal node is known
 // to be within reach; its children are tested before they are pushed, and a
@@ -902,8 +909,8 @@ __device__ __forceinline__ void wave_flush(WaveStage& ws, unsigned long long* to
 template <int KIND>
 __global__ __launch_bounds__(TPB) void k_bvh_walk(DetectArgs a, Bvh t, const uint32_t* order, uint32_t nq,
                                                   uint64_t cap, uint2* region) {
-  __shared__ WaveStage stage[TPB / 64];
-  WaveStage& ws = stage[threadIdx.x >> 6];
+  __shared__ WaveStage<512> stage[TPB / 64];
+  WaveStage<512>& ws = stage[threadIdx.x >> 6];
   if ((threadIdx.x & 63) == 0) ws.n = 0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -947,8 +954,9 @@ __global__ __launch_bounds__(TPB) void k_bvh_walk(DetectArgs a, Bvh t, const uin
       } else if (nd >= leaf0) {
         prim = t.prim[nd - leaf0];
         const ogm::Box lb = ldbox(t.box, (uint32_t)nd);
-        take = KIND == 0 ? vf_filter(a, q, moving, ex, qb, prim, lb, cnt)
-                         : ee_filter(a, q, qb, ia, ib, ia_ex, ib_ex, prim, lb, kNoCell, cnt);
+        take = KIND == 0 ? vf_filter(a, q, moving, ex, qb, prim, lb, cnt, [&] { return load_face(a, prim); })
+                         : ee_filter(a, q, qb, ia, ib, ia_ex, ib_ex, prim, lb, cnt,
+                                     [&] { return load_edge(a, prim); }, [] { return true; });
       } else {
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
@@ -965,30 +973,11 @@ __global__ __launch_bounds__(TPB) void k_bvh_walk(DetectArgs a, Bvh t, const uin
       }
     }
     wave_stage(ws, take, q, prim);
-    if (ws.n > kWaveBuf - 64) wave_flush(ws, KIND == 0 ? &a.sc->vf_list_n : &a.sc->ee_list_n, cap, region, a.sc);
+    if (ws.n > ws.kCap - 64) wave_flush(ws, KIND == 0 ? &a.sc->vf_list_n : &a.sc->ee_list_n, cap, region, a.sc);
   }
   wave_flush(ws, KIND == 0 ? &a.sc->vf_list_n : &a.sc->ee_list_n, cap, region, a.sc);
   add_counts(a, cnt);  // every lane reaches it
 }
-// Narrow pass over the two candidate regions of the LBVH walks (the counts
-// are the device's; a region that overflowed is redone by the host).
-__global__ __launch_bounds__(TPB) void k_narrow_lists(DetectArgs a, const uint2* vf, uint64_t cap_vf, const uint2* ee,
-                                                      uint64_t cap_ee) {
-  PairCounts cnt;
-  const uint64_t nvf = min((uint64_t)a.sc->vf_list_n, cap_vf), nee = min((uint64_t)a.sc->ee_list_n, cap_ee);
-  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nvf + nee; j += step) {
-    if (j < nvf) {
-      const uint2 p = vf[j];
-      vf_narrow(a, p.x, p.y, cnt);
-    } else {
-      const uint2 p = ee[j - nvf];
-      ee_narrow(a, p.x, p.y, cnt);
-    }
-  }
-  add_counts(a, cnt);
-}
-
 __global__ void k_init_dist(unsigned long long* bits, uint32_t n, double rq) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) bits[i] = (unsigned long long)__double_as_longlong(rq);
@@ -1168,19 +1157,21 @@ void read_scalars(cv_gpu& g) {
   CV_HIP(hipStreamSynchronize(g.stream));
 }
 
-// Register face and edge boxes (inflated by pr) in the hash grid: entries
-// bucketed by cell-key hash, faces in buckets [0, T) and edges in [T, 2T).
-// Buffers are sized from the previous detection's entry count (with room to
-// grow), so no read-back is needed here; an overflow is flagged on the
-// device and the caller detects again with more room.  Returns log2(T).
-int build_grid(cv_gpu& g, uint32_t F, uint32_t E, double pr, double h) {
+// Register face and edge boxes (inflated by pr) in the hash grid, and the
+// queries (points in their own cell, moving edges in the cells of their box):
+// entries bucketed by cell-key hash, table t in buckets [tT, (t+1)T) (faces,
+// edges, points, moving edges).  Buffers are sized from the previous
+// detection's entry count (with room to grow), so no read-back is needed
+// here; an overflow is flagged on the device and the caller detects again
+// with more room.  Returns log2(T).
+int build_grid(cv_gpu& g, const RegPrims& rp, double h) {
   StepBufs& b = g.b;
   hipStream_t st = g.stream;
-  uint32_t N = F + E;
+  const uint32_t N = rp.F + 2 * rp.E + rp.P;
   b.cell_count.reserve(N + 1);
   b.cell_off.reserve(N + 1);
   const double inv_h = 1.0 / h;
-  k_reg_count<<<blocks(N), TPB, 0, st>>>(b.fbox.p, b.ebox.p, F, E, pr, inv_h, b.cell_count.p);
+  k_reg_count<<<blocks(N), TPB, 0, st>>>(rp, inv_h, b.cell_count.p);
   CV_HIP(hipGetLastError());
   CV_HIP(hipMemsetAsync(b.cell_count.p + N, 0, sizeof(uint32_t), st));
   scan_u32(g, b.cell_count.p, b.cell_off.p, (size_t)N + 1);
@@ -1196,22 +1187,22 @@ int build_grid(cv_gpu& g, uint32_t F, uint32_t E, double pr, double h) {
   if (cap > (1u << 30)) fail(CV_ERR_CAPACITY, "OGC grid registration exceeds 2^30 cells");
   int log2t = 10;
   while (log2t < 28 && (1ull << log2t) < M / 4) ++log2t;
-  size_t T2 = (size_t)2 << log2t;
+  size_t T4 = (size_t)4 << log2t;
   b.cell_key.reserve(cap);
   b.cell_val.reserve(cap);
   b.cell_rank.reserve(cap);
   b.cell_entry.reserve(cap);
   b.cell_owner.reserve(cap);
-  b.bcount.reserve(T2 + 1);
-  b.bstart.reserve(T2 + 1);
-  CV_HIP(hipMemsetAsync(b.bcount.p, 0, (T2 + 1) * sizeof(uint32_t), st));
+  b.bcount.reserve(T4 + 1);
+  b.bstart.reserve(T4 + 1);
+  CV_HIP(hipMemsetAsync(b.bcount.p, 0, (T4 + 1) * sizeof(uint32_t), st));
   k_reg_owner<<<blocks(N), TPB, 0, st>>>(b.cell_off.p, N, b.cell_owner.p, cap, g.d_scal);
-  k_reg_fill<<<blocks(cap), TPB, 0, st>>>(b.fbox.p, b.ebox.p, F, pr, inv_h, b.cell_off.p, N, b.cell_owner.p, cap,
-                                          log2t, b.cell_key.p, b.cell_val.p, b.cell_rank.p, b.bcount.p, g.d_scal);
+  k_reg_fill<<<blocks(cap), TPB, 0, st>>>(rp, inv_h, b.cell_off.p, b.cell_owner.p, cap, log2t, b.cell_key.p,
+                                          b.cell_val.p, b.cell_rank.p, b.bcount.p, g.d_scal);
   CV_HIP(hipGetLastError());
-  scan_u32(g, b.bcount.p, b.bstart.p, T2 + 1);
-  k_reg_scatter<<<blocks(cap), TPB, 0, st>>>(b.cell_key.p, b.cell_val.p, b.cell_rank.p, b.cell_off.p, F, N, cap,
-                                             log2t, b.bstart.p, b.fbox.p, b.ebox.p, b.cell_entry.p);
+  scan_u32(g, b.bcount.p, b.bstart.p, T4 + 1);
+  k_reg_scatter<<<blocks(cap), TPB, 0, st>>>(rp, b.cell_key.p, b.cell_val.p, b.cell_rank.p, b.cell_off.p, cap, log2t,
+                                             b.bstart.p, b.cell_entry.p);
   CV_HIP(hipGetLastError());
   return log2t;
 }
@@ -1331,7 +1322,7 @@ void lbvh_tree(cv_gpu& g, BvhBufs& t, const double* pbox, uint32_t N, double sca
   }
 }
 
-void lbvh_broad_narrow(cv_gpu& g, const DetectArgs& a, double scale) {
+void lbvh_broad_narrow(cv_gpu& g, const DetectArgs& a, double scale, uint2* vf, uint2* ee) {
   DevSolver& d = g.dev;
   StepBufs& b = g.b;
   hipStream_t st = g.stream;
@@ -1358,17 +1349,10 @@ void lbvh_broad_narrow(cv_gpu& g, const DetectArgs& a, double scale) {
   }
   Bvh tf{b.bvh_faces.box.p, b.bvh_faces.child.p, b.bvh_faces.prim.p, d.F};
   Bvh te{b.bvh_edges.box.p, b.bvh_edges.child.p, b.bvh_edges.prim.p, d.E};
-  // Candidate regions: VF [0, cap_vf), EE [cap_vf, cap_vf + cap_ee), sized
-  // from the last detection's counts (grown and redone on overflow).
-  if (b.bvh_cap_vf == 0) b.bvh_cap_vf = 8ull * d.n_total + (1u << 20);
-  if (b.bvh_cap_ee == 0) b.bvh_cap_ee = 8ull * d.E + (1u << 20);
-  b.pair_list.reserve(b.bvh_cap_vf + b.bvh_cap_ee);
-  uint2* vf = b.pair_list.p;
-  uint2* ee = b.pair_list.p + b.bvh_cap_vf;
-  if (d.F) k_bvh_walk<0><<<blocks(d.n_total), TPB, 0, st>>>(a, tf, pts.prim.p, d.n_total, b.bvh_cap_vf, vf);
-  if (d.E) k_bvh_walk<1><<<blocks(d.E), TPB, 0, st>>>(a, te, b.bvh_edges.prim.p, d.E, b.bvh_cap_ee, ee);
+  if (d.F) k_bvh_walk<0><<<blocks(d.n_total), TPB, 0, st>>>(a, tf, pts.prim.p, d.n_total, b.cand_cap_vf, vf);
+  if (d.E) k_bvh_walk<1><<<blocks(d.E), TPB, 0, st>>>(a, te, b.bvh_edges.prim.p, d.E, b.cand_cap_ee, ee);
   CV_HIP(hipGetLastError());
-  k_narrow_lists<<<8 * 256, TPB, 0, st>>>(a, vf, b.bvh_cap_vf, ee, b.bvh_cap_ee);
+  k_narrow_lists<<<8 * 256, TPB, 0, st>>>(a, vf, b.cand_cap_vf, ee, b.cand_cap_ee);
   CV_HIP(hipGetLastError());
 }
 
@@ -1391,6 +1375,7 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   hs.raw_contacts = hs.contacts = hs.active = hs.incidences = 0;
   hs.cand_export_vf = hs.cand_export_ee = 0;
   hs.vf_list_n = hs.ee_list_n = 0;
+  hs.span_next = 0;
   hs.err_degenerate = hs.err_overflow = 0;
   hs.rq = rq;
   hs.padded_radius2 = pr2;
@@ -1438,41 +1423,26 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   a.cand_ee = cand_cap ? b.cand_ee.p : nullptr;
   a.cand_cap = cand_cap;
   a.sc = g.d_scal;
-  uint32_t NI = 0;  // grid query items
+  // Candidate regions of the broad pass: VF [0, cap_vf), EE [cap_vf,
+  // cap_vf + cap_ee), sized from the last detection's counts (grown and
+  // redone on overflow).
+  if (b.cand_cap_vf == 0) b.cand_cap_vf = 8ull * d.n_total + (1u << 20);
+  if (b.cand_cap_ee == 0) b.cand_cap_ee = 8ull * d.E + (1u << 20);
+  b.pair_list.reserve(b.cand_cap_vf + b.cand_cap_ee);
+  uint2* vf = b.pair_list.p;
+  uint2* ee = b.pair_list.p + b.cand_cap_vf;
   if (g.broad_phase == 1 && !g.b.bvh_fallback) {
-    lbvh_broad_narrow(g, a, scale);
+    lbvh_broad_narrow(g, a, scale, vf, ee);
   } else {
     // Faces and edges registered with the padded inflation; points look up
     // their own cell, edges the cells of their own box.
-    a.log2t = build_grid(g, d.F, d.E, pr_reg, h);
+    const RegPrims rp{b.fbox.p, b.ebox.p, b.x64.p, d.edge_moving.p, d.F, d.E, d.n_total, pr_reg};
+    a.log2t = build_grid(g, rp, h);
     a.ent = b.cell_entry.p;
     a.bstart = b.bstart.p;
-    // Flattened broad phase: query items, their pair counts, owners, pairs.
-    NI = d.n_total + kEdgeCells * d.E;
-    b.item_len.reserve(NI + 1);
-    b.item_off.reserve(NI + 1);
-    b.item_start.reserve(NI);
-    b.item_key.reserve(NI);
-    b.over.reserve(std::max<uint32_t>(d.E, 1));
-    CV_HIP(hipMemsetAsync(b.item_len.p + NI, 0, sizeof(uint32_t), st));
-    CV_HIP(hipMemsetAsync(&g.d_scal->n_over, 0, sizeof(unsigned long long), st));
-    k_items<<<blocks(d.n_total + d.E), TPB, 0, st>>>(a, b.item_len.p, b.item_start.p, b.item_key.p, b.over.p,
-                                                     &g.d_scal->n_over);
+    k_pairs<<<8 * 256, TPB, 0, st>>>(a, b.cand_cap_vf, vf, b.cand_cap_ee, ee);
     CV_HIP(hipGetLastError());
-    scan_u32(g, b.item_len.p, b.item_off.p, (size_t)NI + 1);
-    // Pair-slot capacity: grown from the last detection's need; an overflow is
-    // flagged on the device and the detection is run again with more room.
-    if (b.pair_cap == 0) b.pair_cap = std::max<uint64_t>(4ull * b.grid_cap + (1u << 20), 1u << 22);
-    b.pair_list.reserve(b.pair_cap);
-    k_span_bounds<<<1, 64, 0, st>>>(b.item_off.p, NI, b.pair_cap, g.d_scal);
-    CV_HIP(hipGetLastError());
-    k_pairs<<<8 * 256, TPB, 0, st>>>(a, b.item_off.p, b.item_start.p, b.item_key.p, NI, b.pair_cap,
-                                     b.pair_list.p);
-    CV_HIP(hipGetLastError());
-    k_narrow<<<8 * 256, TPB, 0, st>>>(a, b.item_off.p, NI, b.pair_cap, b.pair_list.p);
-    CV_HIP(hipGetLastError());
-    constexpr int G = 16;
-    k_ee_overflow<G><<<64, TPB, 0, st>>>(a, b.over.p, &g.d_scal->n_over);
+    k_narrow_lists<<<8 * 256, TPB, 0, st>>>(a, vf, b.cand_cap_vf, ee, b.cand_cap_ee);
     CV_HIP(hipGetLastError());
   }
   read_scalars(g);
@@ -1485,26 +1455,14 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
     g.b.bvh_fallback = false;
     return;
   }
-  if (g.h_scal->err_overflow & 32u) {
-    // An LBVH candidate region was too small: grow it to the count the walks
-    // found and detect again.
-    const Scalars& hs = *g.h_scal;
-    if (hs.vf_list_n > b.bvh_cap_vf) b.bvh_cap_vf = hs.vf_list_n + hs.vf_list_n / 4 + 4096;
-    if (hs.ee_list_n > b.bvh_cap_ee) b.bvh_cap_ee = hs.ee_list_n + hs.ee_list_n / 4 + 4096;
-    gpu_detect(g, s, rq, ogc, cand_cap);
-    return;
-  }
   if (g.broad_phase != 1 || g.b.bvh_fallback) b.grid_last = g.h_scal->grid_entries;
-  if (g.h_scal->err_overflow & 12u) {
-    // More grid registrations (8) or (query, entry) pairs (4) than room:
-    // grow and detect again.
-    if (g.h_scal->err_overflow & 8u) b.grid_cap = b.grid_last + b.grid_last / 2 + 4096;
-    if (g.h_scal->err_overflow & 4u) {
-      uint32_t need = 0;
-      CV_HIP(hipMemcpyAsync(&need, b.item_off.p + NI, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-      CV_HIP(hipStreamSynchronize(st));
-      b.pair_cap = (uint64_t)need + need / 2 + 1024;
-    }
+  if (g.h_scal->err_overflow & 40u) {
+    // More grid registrations (8) than room, or a candidate region too small
+    // (32): grow to the counts this detection found and detect again.
+    const Scalars& hs = *g.h_scal;
+    if (hs.err_overflow & 8u) b.grid_cap = b.grid_last + b.grid_last / 2 + 4096;
+    if (hs.vf_list_n > b.cand_cap_vf) b.cand_cap_vf = hs.vf_list_n + hs.vf_list_n / 4 + 4096;
+    if (hs.ee_list_n > b.cand_cap_ee) b.cand_cap_ee = hs.ee_list_n + hs.ee_list_n / 4 + 4096;
     gpu_detect(g, s, rq, ogc, cand_cap);
     return;
   }
