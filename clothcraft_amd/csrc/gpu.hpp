@@ -86,12 +86,12 @@ struct Scalars {
   unsigned int iters_run;
   unsigned int sweep_blocks;           // blocks of the iteration's last sweep launch that have finished
   unsigned long long cand_export_vf, cand_export_ee;
-  unsigned long long vf_list_n, ee_list_n;  // candidates queued for the narrow phase
+  unsigned long long vf_list_n, ee_list_n;      // candidates queued for the narrow phase (all regions)
+  unsigned long long vf_list_max, ee_list_max;  // the most queued in one region
   unsigned long long grid_entries;          // hash-grid registrations of the last detection
   double diag_p95, diag_max;                // extension percent of the committed positions
   unsigned long long diag_gap_bits;         // max seam gap (bit pattern of a non-negative double)
   unsigned int err_degenerate, err_overflow, err_nonfinite, err_trust;
-  unsigned int span_next;  // the grid broad pass's next span of query entries
 };
 
 // MI355X dispatches the workgroups of a launch to its 8 XCDs round-robin
@@ -183,6 +183,7 @@ struct StepBufs {
   DBuf<GridEntry> cell_entry;  // bucketed registrations
   DBuf<uint32_t> bcount, bstart;  // hash-grid buckets [4T+1]: faces, edges, points, moving-edge cells
   DBuf<uint2> pair_list;  // candidates for the narrow phase: VF region, then EE region
+  DBuf<unsigned long long> list_ctr;  // broad-pass counters, one per 128-byte line (see ogc_device.hip)
   DBuf<uint32_t> scan_sums;  // tile totals of the two-level scan
   DBuf<double> red_parts;    // per-block partials of the prologue reduction
   DBuf<unsigned long long> ext_key, ext_key_sorted;  // diagnostics: per-triangle extension

@@ -520,9 +520,22 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Broad-pass counters, each on its own 128-byte line: same-address atomics
+// from the whole device serialise, so the grid pass keeps one candidate
+// counter per kind and XCD (and one span counter per XCD), and its candidate
+// regions are split into kSub parts, part x filled by the blocks on XCD x.
+// The LBVH walks use part 0 of each kind as one region.
+constexpr uint32_t kSub = 8;
+constexpr uint32_t kCtrStride = 16;  // u64 per counter line
+constexpr uint32_t kCtrCount = 3 * kSub;  // VF parts, EE parts, span counters
+__device__ __forceinline__ unsigned long long* list_ctr(unsigned long long* ctr, uint32_t kind, uint32_t sub) {
+  return ctr + (size_t)(kind * kSub + sub) * kCtrStride;
+}
+
 // Grid broad pass, bucket-major.  The query entries (tables 2 and 3 of the
 // registration: points, then the cells of moving edges) are taken in spans of
-// 64 consecutive entries per wavefront, from one counter.  A span's queries
+// 64 consecutive entries per wavefront, the spans split evenly over the
+// XCDs, each XCD's wavefronts taking them from its own counter.  A span's queries
 // are staged in LDS (box, cell key, the bucket's entry range, ids, flags) and
 // fall into runs of consecutive queries of one bucket, since the registration
 // put each bucket's queries together.  For each run the wavefront loads the
@@ -635,7 +648,7 @@ __device__ __forceinline__ void run_chunk(const DetectArgs& a, const SpanQuery* 
   }
 }
 __global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, uint64_t cap_vf, uint2* vf, uint64_t cap_ee,
-                                               uint2* ee) {
+                                               uint2* ee, unsigned long long* ctr) {
   constexpr uint32_t kOut = 256;  // staged candidates per wavefront (with the queries: 4 wavefronts per SIMD)
   struct Stage {
     SpanQuery q[kSpan];
@@ -643,17 +656,29 @@ __global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, uint64_t cap_vf, ui
   };
   __shared__ Stage stage[TPB / 64];
   Stage& st = stage[threadIdx.x >> 6];
-  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t lane = threadIdx.x & 63, x = blockIdx.x % kSub;  // blocks b with b % 8 == x run on XCD x
   const uint32_t T = 1u << a.log2t;
   const uint32_t P0 = a.bstart[2 * T], PE = a.bstart[3 * T], P1 = a.bstart[4 * T];
+  const uint32_t NS = (P1 - P0 + kSpan - 1) / kSpan;
+  const uint32_t span_lo = (uint32_t)((uint64_t)NS * x / kSub), span_hi = (uint32_t)((uint64_t)NS * (x + 1) / kSub);
+  // This XCD's candidate parts and counters.
+  const uint64_t sub_vf = cap_vf / kSub, sub_ee = cap_ee / kSub;
+  vf += sub_vf * x;
+  ee += sub_ee * x;
+  cap_vf = sub_vf;
+  cap_ee = sub_ee;
+  unsigned long long* const n_vf = list_ctr(ctr, 0, x);
+  unsigned long long* const n_ee = list_ctr(ctr, 1, x);
+  unsigned int* const span_next = (unsigned int*)list_ctr(ctr, 2, x);
   PairCounts cnt;
   uint32_t n_out = 0;   // staged candidates (wavefront-uniform)
   bool out_ee = false;  // their kind
   for (;;) {
     uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(&a.sc->span_next, 1u);
-    const uint64_t p0 = P0 + (uint64_t)kSpan * (uint32_t)__shfl((int)t, 0, 64);
-    if (p0 >= P1) break;
+    if (lane == 0) t = atomicAdd(span_next, 1u);
+    const uint32_t span = span_lo + (uint32_t)__shfl((int)t, 0, 64);
+    if (span >= span_hi) break;
+    const uint64_t p0 = P0 + (uint64_t)kSpan * span;
     wave_sync();  // the previous span's queries are no longer read
     const uint64_t p = p0 + lane;
     bool member = false;
@@ -707,11 +732,10 @@ __global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, uint64_t cap_vf, ui
       const bool edge_run = p0 + k0 >= PE;
       const uint32_t rs = st.q[k0].s, re = st.q[k0].e;
       if (edge_run != out_ee) {
-        stage_flush(st.out, n_out, out_ee ? &a.sc->ee_list_n : &a.sc->vf_list_n, out_ee ? cap_ee : cap_vf,
-                    out_ee ? ee : vf, a.sc);
+        stage_flush(st.out, n_out, out_ee ? n_ee : n_vf, out_ee ? cap_ee : cap_vf, out_ee ? ee : vf, a.sc);
         out_ee = edge_run;
       }
-      unsigned long long* total = edge_run ? &a.sc->ee_list_n : &a.sc->vf_list_n;
+      unsigned long long* total = edge_run ? n_ee : n_vf;
       const uint64_t cap = edge_run ? cap_ee : cap_vf;
       uint2* region = edge_run ? ee : vf;
       for (uint32_t c0 = rs; c0 < re; c0 += 64) {
@@ -726,23 +750,53 @@ __global__ __launch_bounds__(TPB) void k_pairs(DetectArgs a, uint64_t cap_vf, ui
       }
     }
   }
-  stage_flush(st.out, n_out, out_ee ? &a.sc->ee_list_n : &a.sc->vf_list_n, out_ee ? cap_ee : cap_vf,
-              out_ee ? ee : vf, a.sc);
+  stage_flush(st.out, n_out, out_ee ? n_ee : n_vf, out_ee ? cap_ee : cap_vf, out_ee ? ee : vf, a.sc);
   add_counts(a, cnt);  // every lane reaches it
 }
-// Narrow pass over the two candidate regions (the counts are the device's; a
-// region that overflowed is redone by the host).
+// Narrow pass over the candidate regions, each kind in `nsub` equal parts
+// with their own counters (the counts are the device's; a part that
+// overflowed is redone by the host).  Block 0 records the totals and the
+// largest part for the host.
 __global__ __launch_bounds__(TPB) void k_narrow_lists(DetectArgs a, const uint2* vf, uint64_t cap_vf, const uint2* ee,
-                                                      uint64_t cap_ee) {
+                                                      uint64_t cap_ee, const unsigned long long* ctr, uint32_t nsub) {
+  // Exclusive prefix of the parts' counts: VF parts, then EE parts.
+  __shared__ unsigned long long pre[2 * kSub + 1];
+  const uint64_t sub_vf = cap_vf / nsub, sub_ee = cap_ee / nsub;
+  if (threadIdx.x == 0) {
+    unsigned long long acc = 0, tot_v = 0, tot_e = 0, max_v = 0, max_e = 0;
+    for (uint32_t k = 0; k < 2 * nsub; ++k) {
+      const bool v = k < nsub;
+      const unsigned long long c = ctr[(size_t)(v ? k : kSub + (k - nsub)) * kCtrStride];
+      pre[k] = acc;
+      acc += min(c, (unsigned long long)(v ? sub_vf : sub_ee));
+      if (v) {
+        tot_v += c;
+        max_v = max(max_v, c);
+      } else {
+        tot_e += c;
+        max_e = max(max_e, c);
+      }
+    }
+    pre[2 * nsub] = acc;
+    if (blockIdx.x == 0) {
+      a.sc->vf_list_n = tot_v;
+      a.sc->ee_list_n = tot_e;
+      a.sc->vf_list_max = max_v;
+      a.sc->ee_list_max = max_e;
+    }
+  }
+  __syncthreads();
   PairCounts cnt;
-  const uint64_t nvf = min((uint64_t)a.sc->vf_list_n, cap_vf), nee = min((uint64_t)a.sc->ee_list_n, cap_ee);
-  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nvf + nee; j += step) {
-    if (j < nvf) {
-      const uint2 p = vf[j];
+  const uint64_t total = pre[2 * nsub], step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += step) {
+    uint32_t k = 0;  // the part holding list position j
+    while (pre[k + 1] <= j) ++k;
+    const uint64_t r = j - pre[k];
+    if (k < nsub) {
+      const uint2 p = vf[sub_vf * k + r];
       vf_narrow(a, p.x, p.y, cnt);
     } else {
-      const uint2 p = ee[j - nvf];
+      const uint2 p = ee[sub_ee * (k - nsub) + r];
       ee_narrow(a, p.x, p.y, cnt);
     }
   }
@@ -908,7 +962,7 @@ __global__ void k_morton_points(const double* x, uint32_t N, double lo, double s
 // Every wavefront runs the loop until all its lanes are done.
 template <int KIND>
 __global__ __launch_bounds__(TPB) void k_bvh_walk(DetectArgs a, Bvh t, const uint32_t* order, uint32_t nq,
-                                                  uint64_t cap, uint2* region) {
+                                                  uint64_t cap, uint2* region, unsigned long long* ctr) {
   __shared__ WaveStage<512> stage[TPB / 64];
   WaveStage<512>& ws = stage[threadIdx.x >> 6];
   if ((threadIdx.x & 63) == 0) ws.n = 0;
@@ -973,9 +1027,9 @@ __global__ __launch_bounds__(TPB) void k_bvh_walk(DetectArgs a, Bvh t, const uin
       }
     }
     wave_stage(ws, take, q, prim);
-    if (ws.n > ws.kCap - 64) wave_flush(ws, KIND == 0 ? &a.sc->vf_list_n : &a.sc->ee_list_n, cap, region, a.sc);
+    if (ws.n > ws.kCap - 64) wave_flush(ws, list_ctr(ctr, KIND, 0), cap, region, a.sc);
   }
-  wave_flush(ws, KIND == 0 ? &a.sc->vf_list_n : &a.sc->ee_list_n, cap, region, a.sc);
+  wave_flush(ws, list_ctr(ctr, KIND, 0), cap, region, a.sc);
   add_counts(a, cnt);  // every lane reaches it
 }
 __global__ void k_init_dist(unsigned long long* bits, uint32_t n, double rq) {
@@ -1349,10 +1403,12 @@ void lbvh_broad_narrow(cv_gpu& g, const DetectArgs& a, double scale, uint2* vf, 
   }
   Bvh tf{b.bvh_faces.box.p, b.bvh_faces.child.p, b.bvh_faces.prim.p, d.F};
   Bvh te{b.bvh_edges.box.p, b.bvh_edges.child.p, b.bvh_edges.prim.p, d.E};
-  if (d.F) k_bvh_walk<0><<<blocks(d.n_total), TPB, 0, st>>>(a, tf, pts.prim.p, d.n_total, b.cand_cap_vf, vf);
-  if (d.E) k_bvh_walk<1><<<blocks(d.E), TPB, 0, st>>>(a, te, b.bvh_edges.prim.p, d.E, b.cand_cap_ee, ee);
+  if (d.F)
+    k_bvh_walk<0><<<blocks(d.n_total), TPB, 0, st>>>(a, tf, pts.prim.p, d.n_total, b.cand_cap_vf, vf, b.list_ctr.p);
+  if (d.E)
+    k_bvh_walk<1><<<blocks(d.E), TPB, 0, st>>>(a, te, b.bvh_edges.prim.p, d.E, b.cand_cap_ee, ee, b.list_ctr.p);
   CV_HIP(hipGetLastError());
-  k_narrow_lists<<<8 * 256, TPB, 0, st>>>(a, vf, b.cand_cap_vf, ee, b.cand_cap_ee);
+  k_narrow_lists<<<8 * 256, TPB, 0, st>>>(a, vf, b.cand_cap_vf, ee, b.cand_cap_ee, b.list_ctr.p, 1);
   CV_HIP(hipGetLastError());
 }
 
@@ -1374,8 +1430,7 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   hs.vf_count = hs.ee_count = hs.vf_cand = hs.ee_cand = hs.grid_tests = 0;
   hs.raw_contacts = hs.contacts = hs.active = hs.incidences = 0;
   hs.cand_export_vf = hs.cand_export_ee = 0;
-  hs.vf_list_n = hs.ee_list_n = 0;
-  hs.span_next = 0;
+  hs.vf_list_n = hs.ee_list_n = hs.vf_list_max = hs.ee_list_max = 0;
   hs.err_degenerate = hs.err_overflow = 0;
   hs.rq = rq;
   hs.padded_radius2 = pr2;
@@ -1429,6 +1484,8 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
   if (b.cand_cap_vf == 0) b.cand_cap_vf = 8ull * d.n_total + (1u << 20);
   if (b.cand_cap_ee == 0) b.cand_cap_ee = 8ull * d.E + (1u << 20);
   b.pair_list.reserve(b.cand_cap_vf + b.cand_cap_ee);
+  b.list_ctr.reserve(kCtrCount * kCtrStride);
+  CV_HIP(hipMemsetAsync(b.list_ctr.p, 0, kCtrCount * kCtrStride * sizeof(unsigned long long), st));
   uint2* vf = b.pair_list.p;
   uint2* ee = b.pair_list.p + b.cand_cap_vf;
   if (g.broad_phase == 1 && !g.b.bvh_fallback) {
@@ -1440,9 +1497,9 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
     a.log2t = build_grid(g, rp, h);
     a.ent = b.cell_entry.p;
     a.bstart = b.bstart.p;
-    k_pairs<<<8 * 256, TPB, 0, st>>>(a, b.cand_cap_vf, vf, b.cand_cap_ee, ee);
+    k_pairs<<<8 * 256, TPB, 0, st>>>(a, b.cand_cap_vf, vf, b.cand_cap_ee, ee, b.list_ctr.p);
     CV_HIP(hipGetLastError());
-    k_narrow_lists<<<8 * 256, TPB, 0, st>>>(a, vf, b.cand_cap_vf, ee, b.cand_cap_ee);
+    k_narrow_lists<<<8 * 256, TPB, 0, st>>>(a, vf, b.cand_cap_vf, ee, b.cand_cap_ee, b.list_ctr.p, kSub);
     CV_HIP(hipGetLastError());
   }
   read_scalars(g);
@@ -1461,8 +1518,9 @@ void gpu_detect(cv_gpu& g, const cv_solver& s, double rq, const cv_ogc_settings&
     // (32): grow to the counts this detection found and detect again.
     const Scalars& hs = *g.h_scal;
     if (hs.err_overflow & 8u) b.grid_cap = b.grid_last + b.grid_last / 2 + 4096;
-    if (hs.vf_list_n > b.cand_cap_vf) b.cand_cap_vf = hs.vf_list_n + hs.vf_list_n / 4 + 4096;
-    if (hs.ee_list_n > b.cand_cap_ee) b.cand_cap_ee = hs.ee_list_n + hs.ee_list_n / 4 + 4096;
+    const uint64_t ns = g.broad_phase == 1 && !g.b.bvh_fallback ? 1 : kSub;  // parts per kind
+    if (hs.vf_list_max * ns > b.cand_cap_vf) b.cand_cap_vf = ns * (hs.vf_list_max + hs.vf_list_max / 4 + 4096);
+    if (hs.ee_list_max * ns > b.cand_cap_ee) b.cand_cap_ee = ns * (hs.ee_list_max + hs.ee_list_max / 4 + 4096);
     gpu_detect(g, s, rq, ogc, cand_cap);
     return;
   }
