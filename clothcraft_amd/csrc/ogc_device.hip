@@ -310,14 +310,6 @@ __device__ __forceinline__ void add_contact(const DetectArgs& a, int kind, uint3
   a.raw_eps[slot] = eps;
 }
 
-// Sum of v over the wavefront, added to *dst once per wavefront.  Every lane
-// of the wave must reach the call (kernels using it never return early), so
-// the scalar counters see a few atomics per block instead of one per pair.
-__device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long long v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
-}
-
 struct PairCounts {
   unsigned long long tests = 0, vf_cand = 0, vf_near = 0, ee_cand = 0, ee_near = 0;
 };
@@ -460,12 +452,29 @@ __device__ __forceinline__ void ee_narrow(const DetectArgs& a, uint32_t e, uint3
     add_contact(a, 3, ia, ib, jc, jd, 4, 1e-3 * a.rest_len2[e] * a.rest_len2[f]);
 }
 
+// The pair statistics summed over the block and added with one atomic per
+// counter and block: the counters are five addresses shared by the whole
+// device, where atomics serialise.  Every thread of the block calls it once,
+// at the end of the kernel.
 __device__ __forceinline__ void add_counts(const DetectArgs& a, const PairCounts& c) {
-  wave_add(&a.sc->grid_tests, c.tests);
-  wave_add(&a.sc->vf_cand, c.vf_cand);
-  wave_add(&a.sc->vf_count, c.vf_near);
-  wave_add(&a.sc->ee_cand, c.ee_cand);
-  wave_add(&a.sc->ee_count, c.ee_near);
+  constexpr int NW = TPB / 64;
+  __shared__ unsigned long long part[NW][5];
+  unsigned long long v[5] = {c.tests, c.vf_cand, c.vf_near, c.ee_cand, c.ee_near};
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+  const uint32_t w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) part[w][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    unsigned long long sum = 0;
+    for (uint32_t i = 0; i < blockDim.x / 64; ++i) sum += part[i][threadIdx.x];
+    unsigned long long* const dst[5] = {&a.sc->grid_tests, &a.sc->vf_cand, &a.sc->vf_count, &a.sc->ee_cand,
+                                        &a.sc->ee_count};
+    if (sum) atomicAdd(dst[threadIdx.x], sum);
+  }
 }
 
 // Per-wavefront candidate staging of the broad pass (grid or LBVH walks): a
