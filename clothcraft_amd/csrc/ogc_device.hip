@@ -295,19 +295,24 @@ __device__ __forceinline__ void dmin(const DetectArgs& a, uint32_t v, double d) 
   if (a.dist_bits[v] <= bits) return;
   atomicMin(&a.dist_bits[v], bits);
 }
-__device__ __forceinline__ void add_contact(const DetectArgs& a, int kind, uint32_t i0, uint32_t i1,
-                                            uint32_t i2, uint32_t i3, int count, double eps) {
+// A contact found by one narrow-phase test (at most one per candidate pair),
+// in the canonical id order; the narrow pass stages the found ones per
+// wavefront and allocates their raw-list slots with one atomic per flush.
+struct RawContact {
+  uint32_t kind, ids[4];
+  uint32_t count;
+  double eps;
+  bool valid;
+};
+__device__ __forceinline__ void set_contact(RawContact& out, int kind, uint32_t i0, uint32_t i1, uint32_t i2,
+                                            uint32_t i3, int count, double eps) {
   uint32_t ids[4] = {i0, i1, i2, i3};
   ogm::canonical(kind, ids);
-  unsigned long long slot = atomicAdd(&a.sc->raw_contacts, 1ull);
-  if (slot >= a.cap) {
-    atomicOr(&a.sc->err_overflow, 1u);
-    return;
-  }
-  a.raw_ids[5 * slot] = (uint32_t)kind;
-  for (int k = 0; k < 4; ++k) a.raw_ids[5 * slot + 1 + k] = ids[k];
-  a.raw_count[slot] = (uint32_t)count;
-  a.raw_eps[slot] = eps;
+  out.kind = (uint32_t)kind;
+  for (int k = 0; k < 4; ++k) out.ids[k] = ids[k];
+  out.count = (uint32_t)count;
+  out.eps = eps;
+  out.valid = true;
 }
 
 struct PairCounts {
@@ -365,7 +370,8 @@ __device__ __forceinline__ bool vf_filter(const DetectArgs& a, uint32_t v, bool 
 }
 // Vertex-face narrow phase (src/ogc/world.rs:471-497): closest point,
 // distance, minima, feasible feature, contact.
-__device__ __forceinline__ void vf_narrow(const DetectArgs& a, uint32_t v, uint32_t t, PairCounts& c) {
+__device__ __forceinline__ void vf_narrow(const DetectArgs& a, uint32_t v, uint32_t t, PairCounts& c,
+                                          RawContact& out) {
   uint32_t t0 = a.faces[3 * t], t1 = a.faces[3 * t + 1], t2 = a.faces[3 * t + 2];
   D3 p = ld3(a.x, v);
   D3 q;
@@ -380,14 +386,14 @@ __device__ __forceinline__ void vf_narrow(const DetectArgs& a, uint32_t v, uint3
   dmin(a, t2, d);
   if (feat == ogm::F_VERTEX) {
     uint32_t w = loc == 0 ? t0 : (loc == 1 ? t1 : t2);
-    if (ogm::vertex_feasible(p, (int)w, (const D3*)a.x, a.nb_off, a.nb)) add_contact(a, 0, v, w, 0, 0, 2, 0.0);
+    if (ogm::vertex_feasible(p, (int)w, (const D3*)a.x, a.nb_off, a.nb)) set_contact(out, 0, v, w, 0, 0, 2, 0.0);
   } else if (feat == ogm::F_EDGE) {
     uint32_t e = a.tri_edges[3 * t + loc];
     uint32_t ea = a.edges[2 * e], eb = a.edges[2 * e + 1];
     if (ogm::edge_feasible(p, ld3(a.x, ea), ld3(a.x, eb), &a.opp[2 * e], (const D3*)a.x))
-      add_contact(a, 1, v, ea, eb, 0, 3, 0.0);
+      set_contact(out, 1, v, ea, eb, 0, 3, 0.0);
   } else {
-    add_contact(a, 2, v, t0, t1, t2, 4, 0.0);
+    set_contact(out, 2, v, t0, t1, t2, 4, 0.0);
   }
 }
 
@@ -422,7 +428,8 @@ __device__ __forceinline__ bool ee_filter(const DetectArgs& a, uint32_t e, const
   return true;
 }
 // Edge-edge narrow phase (src/ogc/world.rs:513-555).
-__device__ __forceinline__ void ee_narrow(const DetectArgs& a, uint32_t e, uint32_t f, PairCounts& c) {
+__device__ __forceinline__ void ee_narrow(const DetectArgs& a, uint32_t e, uint32_t f, PairCounts& c,
+                                          RawContact& out) {
   uint32_t ia = a.edges[2 * e], ib = a.edges[2 * e + 1];
   uint32_t jc = a.edges[2 * f], jd = a.edges[2 * f + 1];
   D3 A = ld3(a.x, ia), B = ld3(a.x, ib), C = ld3(a.x, jc), Dd = ld3(a.x, jd);
@@ -443,13 +450,13 @@ __device__ __forceinline__ void ee_narrow(const DetectArgs& a, uint32_t e, uint3
       (end_b >= 0 && !ogm::vertex_feasible(P, end_b, (const D3*)a.x, a.nb_off, a.nb)))
     return;
   if (end_a >= 0 && end_b >= 0)
-    add_contact(a, 0, end_a, end_b, 0, 0, 2, 0.0);
+    set_contact(out, 0, end_a, end_b, 0, 0, 2, 0.0);
   else if (end_a >= 0)
-    add_contact(a, 1, end_a, jc, jd, 0, 3, 0.0);
+    set_contact(out, 1, end_a, jc, jd, 0, 3, 0.0);
   else if (end_b >= 0)
-    add_contact(a, 1, end_b, ia, ib, 0, 3, 0.0);
+    set_contact(out, 1, end_b, ia, ib, 0, 3, 0.0);
   else
-    add_contact(a, 3, ia, ib, jc, jd, 4, 1e-3 * a.rest_len2[e] * a.rest_len2[f]);
+    set_contact(out, 3, ia, ib, jc, jd, 4, 1e-3 * a.rest_len2[e] * a.rest_len2[f]);
 }
 
 // The pair statistics summed over the block and added with one atomic per
@@ -795,20 +802,58 @@ __global__ __launch_bounds__(TPB) void k_narrow_lists(DetectArgs a, const uint2*
     }
   }
   __syncthreads();
+  // Found contacts staged per wavefront (count in a register), written to the
+  // raw list with one slot allocation per flush: the raw-contact counter is
+  // one address for the whole device.
+  constexpr uint32_t kStage = 128;
+  __shared__ RawContact staged[TPB / 64][kStage];
+  RawContact* const buf = staged[threadIdx.x >> 6];
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t n = 0;
+  auto flush = [&] {
+    if (!n) return;
+    wave_sync();  // the staged records are read by other lanes
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&a.sc->raw_contacts, (unsigned long long)n);
+    base = __shfl(base, 0, 64);
+    for (uint32_t k = lane; k < n; k += 64) {
+      const unsigned long long slot = base + k;
+      if (slot >= a.cap) continue;
+      const RawContact& rc = buf[k];
+      a.raw_ids[5 * slot] = rc.kind;
+      for (int q = 0; q < 4; ++q) a.raw_ids[5 * slot + 1 + q] = rc.ids[q];
+      a.raw_count[slot] = rc.count;
+      a.raw_eps[slot] = rc.eps;
+    }
+    if (lane == 0 && base + n > a.cap) atomicOr(&a.sc->err_overflow, 1u);
+    wave_sync();  // read before the buffer is staged again
+    n = 0;
+  };
   PairCounts cnt;
   const uint64_t total = pre[2 * nsub], step = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += step) {
-    uint32_t k = 0;  // the part holding list position j
-    while (pre[k + 1] <= j) ++k;
-    const uint64_t r = j - pre[k];
-    if (k < nsub) {
-      const uint2 p = vf[sub_vf * k + r];
-      vf_narrow(a, p.x, p.y, cnt);
-    } else {
-      const uint2 p = ee[sub_ee * (k - nsub) + r];
-      ee_narrow(a, p.x, p.y, cnt);
+  // Wavefront-uniform trip count: every lane reaches the staging steps.
+  for (uint64_t j0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); j0 < total; j0 += step) {
+    const uint64_t j = j0 + lane;
+    RawContact rc;
+    rc.valid = false;
+    if (j < total) {
+      uint32_t k = 0;  // the part holding list position j
+      while (pre[k + 1] <= j) ++k;
+      const uint64_t r = j - pre[k];
+      if (k < nsub) {
+        const uint2 p = vf[sub_vf * k + r];
+        vf_narrow(a, p.x, p.y, cnt, rc);
+      } else {
+        const uint2 p = ee[sub_ee * (k - nsub) + r];
+        ee_narrow(a, p.x, p.y, cnt, rc);
+      }
     }
+    const unsigned long long mask = __ballot(rc.valid);
+    if (rc.valid) buf[n + (uint32_t)__popcll(mask & ((1ull << lane) - 1))] = rc;
+    n += (uint32_t)__popcll(mask);
+    if (n > kStage - 64) flush();
   }
+  flush();
   add_counts(a, cnt);
 }
 
